@@ -1,0 +1,223 @@
+/*
+ * ofrr.h -- C-ABI of the MI355X (gfx950) OFRR projection pipeline.
+ *
+ * Every entry point returns an int status whose values are the reference's
+ * MSVDStatus codes (CPP/core/utils/error_handling.hpp:24-62): 0 success,
+ * 1000 invalid argument, 1003 not implemented, 1005 NaN, 1006 Inf,
+ * 2005 invalid size, 4001 max iterations, 6000-6003 HIP runtime errors.
+ * The message of the last failure on a context is ofrr_ctx_last_error().
+ *
+ * Pointers named *_dev are device pointers; everything else documented as
+ * host.  Dense blocks are addressed with two strides: element (i, j) of X is
+ * X[i * rs + j * cs], so column-major (rs = 1, cs = ld, the reference's
+ * Matrix<T> layout, CPP/containers/matrix.cu:89-99) and row-major
+ * (rs = ld, cs = 1, the layout the hot path keeps in HBM) are both accepted.
+ * All work is enqueued on the context's stream; the call returns when the work
+ * is enqueued unless documented as synchronous.
+ *
+ * The reference C++ functions each entry point replaces are cited per
+ * declaration (paths relative to the reference repository root).
+ */
+#ifndef OFRR_H
+#define OFRR_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OFRR_API __attribute__((visibility("default")))
+
+typedef struct ofrr_ctx ofrr_ctx;
+
+/* element types (the reference's T_I/T_O/T_COMPUTE instantiations
+ * double/float/__half, CPP/containers/vector.cu:11-13, plus bf16 and OCP FP8 E4M3) */
+enum ofrr_dtype { OFRR_F64 = 0, OFRR_F32 = 1, OFRR_F16 = 2, OFRR_BF16 = 3, OFRR_FP8 = 4 };
+
+/* ---- context: replaces the CUDAHandler singleton
+ *      (CPP/core/utils/cuda_handler.cu:9-45): one per host thread and device */
+OFRR_API int ofrr_ctx_create(int device, ofrr_ctx **ctx);
+OFRR_API int ofrr_ctx_destroy(ofrr_ctx *ctx);
+OFRR_API int ofrr_ctx_sync(ofrr_ctx *ctx);
+/* use an external hipStream_t (NULL restores the context's own stream) */
+OFRR_API int ofrr_ctx_set_stream(ofrr_ctx *ctx, void *stream);
+OFRR_API void *ofrr_ctx_stream(ofrr_ctx *ctx);
+OFRR_API const char *ofrr_ctx_last_error(ofrr_ctx *ctx);
+OFRR_API const char *ofrr_status_string(int status);
+OFRR_API int ofrr_version(void);
+
+/* ---- random fill: replaces Vector<T>::fill_random (CPP/containers/vector.cu:188-225).
+ * x[i*inc] = N(0,1) draw number (offset + i) of the Philox4x32-10 stream
+ * (seed, stream), Box-Muller, rounded to dtype. */
+OFRR_API int ofrr_fill_randn(ofrr_ctx *ctx, int dtype, int64_t n, void *x_dev, int64_t inc,
+                             uint64_t seed, uint32_t stream, uint64_t offset);
+/* x[i*inc] = U[0,1) draw number (offset + i) of stream (seed, stream) */
+OFRR_API int ofrr_fill_randu(ofrr_ctx *ctx, int dtype, int64_t n, void *x_dev, int64_t inc,
+                             uint64_t seed, uint32_t stream, uint64_t offset);
+
+/* ---- operators.  CSR with 0-based int64 row_ptr and int32 col_idx.  The CSR
+ * format and SpMM semantics follow MATLAB/+src/+csr/sparse2csr.m:21-50 and
+ * csrmv.m:27-76 (y(i,:) = sum_p a_p x(j_p,:)); the reference has no C++ CSR path. */
+/* stencil operator on an N^dim grid (dim 2: 5-point, 3: 7-point), natural
+ * x-fastest ordering, Dirichlet; seed 0 = Laplacian (diag 2*dim, off -1),
+ * else face coefficients 10^(2u-1), u ~ U[0,1).  Returns nnz in *nnz (host). */
+OFRR_API int ofrr_stencil_nnz(int dim, int64_t N, int64_t *nnz);
+OFRR_API int ofrr_stencil_csr(ofrr_ctx *ctx, int dim, int64_t N, uint64_t seed,
+                              int64_t *row_ptr_dev, int32_t *col_idx_dev, int val_dtype,
+                              void *vals_dev);
+/* Y = alpha * A X + beta * Y, FP64 accumulation for FP64 operands, FP32
+ * otherwise (beta == 0: Y is not read).  X: n x k, Y: n x k (csrmv.m:48-59). */
+OFRR_API int ofrr_spmm_csr(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr_dev,
+                           const int32_t *col_idx_dev, const void *vals_dev, int val_dtype,
+                           int k, const void *X_dev, int x_dtype, int64_t x_rs, int64_t x_cs,
+                           double alpha, double beta, void *Y_dev, int y_dtype, int64_t y_rs,
+                           int64_t y_cs);
+
+/* ---- pivots and scaling */
+/* 1-based index of the first max |x_i| and its signed value, both written to
+ * device memory: iamax<T_O,T_COMPUTE>(x, result, result_val, kDEVICE)
+ * (CPP/linalg/blas/mvops.hpp:98-99, mvops.cu:607-946). */
+OFRR_API int ofrr_iamax(ofrr_ctx *ctx, int dtype, int64_t n, const void *x_dev, int64_t inc,
+                        int32_t *idx1_dev, double *val_dev);
+/* 0-based per-column argmax |A(:,j)| (mvops.hpp:113-114, mvops.cu:955-1199) */
+OFRR_API int ofrr_matrix_iamax(ofrr_ctx *ctx, int dtype, int64_t m, int n, const void *A_dev,
+                               int64_t rs, int64_t cs, int32_t *idx0_dev);
+/* divide each column by its signed max-abs entry, first index on ties, skip
+ * zero columns (MATLAB/+src/+mvops/scalecols.m:21-34); the pivot values are
+ * written to piv_dev (n doubles) when it is not NULL */
+OFRR_API int ofrr_scalecols(ofrr_ctx *ctx, int dtype, int64_t m, int n, void *A_dev, int64_t rs,
+                            int64_t cs, double *piv_dev);
+/* check_special_values (mvops.cu:1388-1599): synchronous; returns 0, 1005 (NaN
+ * first) or 1006 (Inf) */
+OFRR_API int ofrr_check_special(ofrr_ctx *ctx, int dtype, int64_t m, int n, const void *A_dev,
+                                int64_t rs, int64_t cs);
+
+/* ---- Hessenberg process (pivoted, max-abs scaled, no orthogonalization):
+ * hessenberg<T_I,T_O,T_COMPUTE> / hessenberg_v3 (CPP/linalg/factorization/
+ * hessenberg.hpp:22-108) with MATLAB/+src/+qr/hessenberg.m:55-74 semantics.
+ * In place on Q (m x n, dtype q_dtype, arithmetic in FP64 for F64, FP32
+ * otherwise).  R (n x n FP64, column-major, device) receives the upper
+ * triangular factor, perm_dev the 0-based pivot rows (-1 for dropped columns),
+ * skip_dev 1 for dropped columns.  variant 0 skips dropped columns in later
+ * eliminations (hessenberg.m:58, hessenberg.cu:520); a dropped column keeps its
+ * unscaled residual (hessenberg.cu:515-522). */
+OFRR_API int ofrr_hessenberg(ofrr_ctx *ctx, int q_dtype, int64_t m, int n, void *Q_dev,
+                             int64_t rs, int64_t cs, double select_tol, double *R_dev, int ldr,
+                             int32_t *perm_dev, int32_t *skip_dev);
+
+/* ---- projection products: G = alpha * (V^T W) + beta * G in FP64
+ * accumulation with a fixed-order (deterministic) split over rows.
+ * gemm(transA = true, ...) of mvops.hpp:79-82 / eig_rr.m:49,51 used as
+ * V^T V and V^T (A V); V: n x k1, W: n x k2, G: k1 x k2 FP64 column-major. */
+OFRR_API int ofrr_gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V_dev, int v_dtype,
+                       int64_t v_rs, int64_t v_cs, const void *W_dev, int w_dtype, int64_t w_rs,
+                       int64_t w_cs, double alpha, double beta, double *G_dev, int ldg);
+/* Y = sum_b V_b C_b over nb blocks (Y n x k FP64 or FP32; V_b n x kb of any
+ * dtype; C_b kb x k FP64 column-major): the back-transform V <- V U of
+ * eig_rr.m:56 for a basis whose column blocks have different dtypes. */
+OFRR_API int ofrr_basis_update(ofrr_ctx *ctx, int64_t n, int k, int nb, const void *const *V_dev,
+                               const int *v_dtype, const int *kb, const int64_t *v_rs,
+                               const int64_t *v_cs, const double *const *C_dev, const int *ldc,
+                               void *Y_dev, int y_dtype, int64_t y_rs, int64_t y_cs);
+
+/* ---- FP64 dense symmetric eigensolver and Rayleigh-Ritz on the GPU */
+/* batched syev: eigenvalues ascending (w_dev, n per matrix), eigenvectors in
+ * V (column-major, ldv), A read from its upper triangle (syev of
+ * mvops.hpp:184-197, LAPACKE dsyev 'V','U' at mvops.cu:1617).  One
+ * workgroup per matrix, parallel cyclic Jacobi. */
+OFRR_API int ofrr_syev_batched(ofrr_ctx *ctx, int batch, int n, const double *A_dev, int lda,
+                               int64_t strideA, double *w_dev, int64_t stridew, double *V_dev,
+                               int ldv, int64_t strideV);
+/* A x = lambda B x (itype 1), eigenvalues ascending, B-normalised vectors
+ * (sygv of mvops.hpp:184-197, LAPACKE dsygv_work at mvops.cu:1650); returns
+ * 3001 (kErrorSingularMatrix) when B is not positive definite */
+OFRR_API int ofrr_sygv(ofrr_ctx *ctx, int n, const double *A_dev, int lda, const double *B_dev,
+                       int ldb, double *w_dev, double *V_dev, int ldv);
+/* Truncated OFRR Rayleigh-Ritz (ofrr_eig of CPP/linalg/solvers/eig/rr_eig.hpp:
+ * 21-35 with select_tol = tau): S = diag(G)^{-1/2}; S G S = U L U^T; keep
+ * L_i > tau * max(L); B = S U_r L_r^{-1/2}; eig(B^T H B) -> theta (nkeep,
+ * descending) and Y = B Z (m x nkeep, column-major).  rank_dev: rank kept. */
+OFRR_API int ofrr_rr_truncated(ofrr_ctx *ctx, int m, const double *G_dev, int ldg,
+                               const double *H_dev, int ldh, double tau, int nkeep,
+                               double *theta_dev, double *Y_dev, int ldy, int32_t *rank_dev);
+/* the reference's generalised form eig(H, G + gtol I) (eig_rr.m:52), sorted
+ * descending; all m pairs */
+OFRR_API int ofrr_rr_generalized(ofrr_ctx *ctx, int m, const double *G_dev, int ldg,
+                                 const double *H_dev, int ldh, double gtol, double *theta_dev,
+                                 double *Y_dev, int ldy);
+
+/* ---- residuals: res[j] = ||A x_j - theta_j x_j||_2 / (normA ||x_j||_2) in
+ * FP64 (MATLAB/+tests/+test4/test.m:340-350, normalised); synchronous, host res */
+OFRR_API int ofrr_residuals(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr_dev,
+                            const int32_t *col_idx_dev, const double *vals_dev, int k,
+                            const double *X_dev, int64_t x_rs, int64_t x_cs,
+                            const double *theta_host, double normA, double *res_host);
+
+/* ---- drivers ------------------------------------------------------------ */
+/* Arnoldi-Hessenberg eigensolver (arnoldi_eig of CPP/linalg/solvers/eig/
+ * arnoldi_eig.hpp:25-28 over a CSR operator, semantics of MATLAB/+src/+krylov/
+ * arnoldi_hess_eig.m:43-107) in FP64.  v0_dev: start vector (n).  theta_host
+ * (k, descending) and X_dev (n x k column-major, ldx) out.  Synchronous.
+ * *built (host, nullable) receives the basis columns of the last sweep. */
+OFRR_API int ofrr_arnoldi_hess_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz,
+                                   const int64_t *row_ptr_dev, const int32_t *col_idx_dev,
+                                   const double *vals_dev, int k, int kdim, int maxiter,
+                                   double select_tol, double gtol, const double *v0_dev,
+                                   double *theta_host, double *X_dev, int64_t ldx, int *built);
+
+/* Block subspace iteration (si_eig of CPP/linalg/solvers/eig/si_eig.hpp:24-27
+ * over a CSR operator, semantics of MATLAB/+src/+subspace/subspace_iter_eig.m:
+ * 53-68 with function_orth = hessenberg and function_rr = eig_rr generalised)
+ * in FP64.  V_dev (n x m column-major, ldv) start block in / Ritz block out;
+ * theta_host (m) out, sorted descending.  Synchronous. */
+OFRR_API int ofrr_subspace_iter_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz,
+                                    const int64_t *row_ptr_dev, const int32_t *col_idx_dev,
+                                    const double *vals_dev, int m, int maxiter, int step_size,
+                                    double select_tol, double gtol, double *V_dev, int64_t ldv,
+                                    double *theta_host);
+
+/* Filtered OFRR: block subspace iteration whose growth blocks are Chebyshev-
+ * filtered residual corrections built in basis_dtype (FP64, FP32, FP16, BF16
+ * or FP8 E4M3) with FP32 accumulation, FP64 Ritz block X, FP64 Gram /
+ * projected products, FP64 truncated Rayleigh-Ritz on the GPU. */
+typedef struct {
+   int k;            /* block width (Ritz block columns) */
+   int nwant;        /* wanted largest eigenpairs */
+   int nd;           /* correction blocks in the basis (basis dim = k * (1 + nd)) */
+   int degree;       /* filter degree per outer iteration */
+   int maxiter;      /* outer iterations */
+   int basis_dtype;  /* ofrr_dtype of the growth blocks */
+   double tol;       /* ||A x - theta x|| / (||A|| ||x||) */
+   double tau;       /* Gram truncation tolerance */
+   double lower;     /* lower bound of the spectrum (damped interval start) */
+   double normA;     /* ||A||_2, 0: use the largest Ritz value */
+} ofrr_filter_opts;
+
+typedef struct {
+   int iterations;       /* outer iterations run */
+   int converged;        /* pairs meeting tol among the nwant */
+   int spmm_calls;       /* operator applications on k-column blocks */
+   int rank_min;         /* smallest rank kept by the truncation */
+   double t_total_ms;    /* device time of the solve (events on the stream) */
+   double t_spmm_ms;     /* filter steps + projection operator applications */
+   double t_gram_ms;     /* projection products */
+   double t_rr_ms;       /* Rayleigh-Ritz */
+   double t_update_ms;   /* back-transform */
+   double t_resid_ms;    /* FP64 operator application + residual */
+} ofrr_filter_stats;
+
+/* X_dev: n x k FP64 row-major (row stride k), start block in / Ritz block out;
+ * theta_host, res_host: k values out (descending Ritz values and their
+ * residuals).  vals_lo_dev: the operator values in lo_dtype for the growth
+ * SpMMs (NULL: use vals_dev).  Synchronous.  Returns 0 when the nwant pairs
+ * converged, 4001 otherwise. */
+OFRR_API int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *row_ptr_dev,
+                               const int32_t *col_idx_dev, const double *vals_dev,
+                               const void *vals_lo_dev, int lo_dtype, const ofrr_filter_opts *opt,
+                               double *X_dev, double *theta_host, double *res_host,
+                               ofrr_filter_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,98 @@
+"""ctypes declarations of the C-ABI in include/ofrr.h (libofrr.so, built in-tree).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback for any entry point.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libofrr.so")
+
+F64, F32, F16, BF16, FP8 = 0, 1, 2, 3, 4
+
+# every extern "C" entry point declared in include/ofrr.h
+EXPORTS = [
+    "ofrr_ctx_create", "ofrr_ctx_destroy", "ofrr_ctx_sync", "ofrr_ctx_set_stream",
+    "ofrr_ctx_stream", "ofrr_ctx_last_error", "ofrr_status_string", "ofrr_version",
+    "ofrr_fill_randn", "ofrr_fill_randu", "ofrr_stencil_nnz", "ofrr_stencil_csr",
+    "ofrr_spmm_csr", "ofrr_iamax", "ofrr_matrix_iamax", "ofrr_scalecols", "ofrr_check_special",
+    "ofrr_hessenberg", "ofrr_gram", "ofrr_basis_update", "ofrr_syev_batched", "ofrr_sygv",
+    "ofrr_rr_truncated", "ofrr_rr_generalized", "ofrr_residuals", "ofrr_arnoldi_hess_eig",
+    "ofrr_subspace_iter_eig", "ofrr_filtered_eig",
+]
+
+
+class FilterOpts(C.Structure):
+    _fields_ = [("k", C.c_int), ("nwant", C.c_int), ("nd", C.c_int), ("degree", C.c_int),
+                ("maxiter", C.c_int), ("basis_dtype", C.c_int), ("tol", C.c_double),
+                ("tau", C.c_double), ("lower", C.c_double), ("normA", C.c_double)]
+
+
+class FilterStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("spmm_calls", C.c_int),
+                ("rank_min", C.c_int), ("t_total_ms", C.c_double), ("t_spmm_ms", C.c_double),
+                ("t_gram_ms", C.c_double), ("t_rr_ms", C.c_double), ("t_update_ms", C.c_double),
+                ("t_resid_ms", C.c_double)]
+
+
+_lib = None
+vp = C.c_void_p
+i64 = C.c_int64
+i32 = C.c_int
+dbl = C.c_double
+
+
+def _decl(L):
+    L.ofrr_ctx_create.argtypes = [i32, C.POINTER(vp)]
+    L.ofrr_ctx_destroy.argtypes = [vp]
+    L.ofrr_ctx_sync.argtypes = [vp]
+    L.ofrr_ctx_set_stream.argtypes = [vp, vp]
+    L.ofrr_ctx_stream.argtypes = [vp]
+    L.ofrr_ctx_stream.restype = vp
+    L.ofrr_ctx_last_error.argtypes = [vp]
+    L.ofrr_ctx_last_error.restype = C.c_char_p
+    L.ofrr_status_string.argtypes = [i32]
+    L.ofrr_status_string.restype = C.c_char_p
+    L.ofrr_fill_randn.argtypes = [vp, i32, i64, vp, i64, C.c_uint64, C.c_uint32, C.c_uint64]
+    L.ofrr_fill_randu.argtypes = [vp, i32, i64, vp, i64, C.c_uint64, C.c_uint32, C.c_uint64]
+    L.ofrr_stencil_nnz.argtypes = [i32, i64, C.POINTER(i64)]
+    L.ofrr_stencil_csr.argtypes = [vp, i32, i64, C.c_uint64, vp, vp, i32, vp]
+    L.ofrr_spmm_csr.argtypes = [vp, i64, i64, vp, vp, vp, i32, i32, vp, i32, i64, i64, dbl, dbl,
+                                vp, i32, i64, i64]
+    L.ofrr_iamax.argtypes = [vp, i32, i64, vp, i64, vp, vp]
+    L.ofrr_matrix_iamax.argtypes = [vp, i32, i64, i32, vp, i64, i64, vp]
+    L.ofrr_scalecols.argtypes = [vp, i32, i64, i32, vp, i64, i64, vp]
+    L.ofrr_check_special.argtypes = [vp, i32, i64, i32, vp, i64, i64]
+    L.ofrr_hessenberg.argtypes = [vp, i32, i64, i32, vp, i64, i64, dbl, vp, i32, vp, vp]
+    L.ofrr_gram.argtypes = [vp, i64, i32, i32, vp, i32, i64, i64, vp, i32, i64, i64, dbl, dbl,
+                            vp, i32]
+    L.ofrr_basis_update.argtypes = [vp, i64, i32, i32, C.POINTER(vp), C.POINTER(i32),
+                                    C.POINTER(i32), C.POINTER(i64), C.POINTER(i64), C.POINTER(vp),
+                                    C.POINTER(i32), vp, i32, i64, i64]
+    L.ofrr_syev_batched.argtypes = [vp, i32, i32, vp, i32, i64, vp, i64, vp, i32, i64]
+    L.ofrr_sygv.argtypes = [vp, i32, vp, i32, vp, i32, vp, vp, i32]
+    L.ofrr_rr_truncated.argtypes = [vp, i32, vp, i32, vp, i32, dbl, i32, vp, vp, i32, vp]
+    L.ofrr_rr_generalized.argtypes = [vp, i32, vp, i32, vp, i32, dbl, vp, vp, i32]
+    L.ofrr_residuals.argtypes = [vp, i64, i64, vp, vp, vp, i32, vp, i64, i64,
+                                 C.POINTER(dbl), dbl, C.POINTER(dbl)]
+    L.ofrr_arnoldi_hess_eig.argtypes = [vp, i64, i64, vp, vp, vp, i32, i32, i32, dbl, dbl, vp,
+                                        C.POINTER(dbl), vp, i64, C.POINTER(i32)]
+    L.ofrr_subspace_iter_eig.argtypes = [vp, i64, i64, vp, vp, vp, i32, i32, i32, dbl, dbl, vp,
+                                         i64, C.POINTER(dbl)]
+    L.ofrr_filtered_eig.argtypes = [vp, i64, i64, vp, vp, vp, vp, i32, C.POINTER(FilterOpts), vp,
+                                    C.POINTER(dbl), C.POINTER(dbl), C.POINTER(FilterStats)]
+
+
+def load():
+    """Load libofrr.so; raises if it is missing (never falls back)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                f"g.build()'` (hipcc, gfx950); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        _decl(L)
+        _lib = L
+    return _lib

@@ -1,0 +1,347 @@
+"""Python host layer over the C-ABI (include/ofrr.h).
+
+PyTorch only supplies device memory and the stream; every operation runs in
+libofrr.so (hand-written HIP for gfx950).  Names and argument meanings follow
+the reference's operator interface: the MATLAB plugins (csrmv, scalecols,
+hessenberg, eig_rr, arnoldi_hess_eig, subspace_iter_eig) and the C++ free
+functions (iamax, matrix_iamax, gemm, syev, sygv, check_special_values,
+rr_eig/ofrr_eig).  Errors raise OfrrError carrying the MSVDStatus code.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import BF16, F16, F32, F64, FP8, FilterOpts, FilterStats
+
+_DT = {torch.float64: F64, torch.float32: F32, torch.float16: F16, torch.bfloat16: BF16,
+       torch.float8_e4m3fn: FP8}
+_TORCH = {v: k for k, v in _DT.items()}
+
+STATUS_SUCCESS = 0
+STATUS_MAX_ITER = 4001
+
+
+class OfrrError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"[{status}] {msg}")
+        self.status = status
+
+
+def dtype_code(t):
+    if isinstance(t, int):
+        return t
+    if isinstance(t, torch.dtype):
+        return _DT[t]
+    return _DT[t.dtype]
+
+
+def torch_dtype(code):
+    return _TORCH[code]
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+def colmajor(rows, cols, dtype=torch.float64, device="cuda"):
+    """Empty rows x cols tensor in column-major storage (ld = rows)."""
+    return torch.empty(cols, rows, dtype=dtype, device=device).t()
+
+
+class Context:
+    """Owns an ofrr_ctx bound to torch's current stream on `device`."""
+
+    def __init__(self, device=0):
+        self.lib = _lib.load()
+        self.device = device
+        h = C.c_void_p()
+        st = self.lib.ofrr_ctx_create(device, C.byref(h))
+        if st != 0:
+            raise OfrrError(st, "ofrr_ctx_create failed")
+        self.h = h
+        self.bind_stream()
+
+    def bind_stream(self, stream=None):
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        self.lib.ofrr_ctx_set_stream(self.h, C.c_void_p(s.cuda_stream))
+
+    def check(self, st, what=""):
+        if st != 0:
+            msg = self.lib.ofrr_ctx_last_error(self.h).decode() or \
+                self.lib.ofrr_status_string(st).decode()
+            raise OfrrError(st, f"{what}: {msg}")
+
+    def sync(self):
+        self.check(self.lib.ofrr_ctx_sync(self.h), "sync")
+
+    def close(self):
+        if self.h:
+            self.lib.ofrr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CSRMatrix:
+    """Device CSR operator: int64 row_ptr, int32 col_idx, FP64 values, and an
+    optional low-precision copy of the values for the growth SpMMs."""
+
+    def __init__(self, n, row_ptr, col_idx, vals):
+        self.n = int(n)
+        self.row_ptr = row_ptr
+        self.col_idx = col_idx
+        self.vals = vals
+        self.vals_lo = None
+
+    @property
+    def nnz(self):
+        return int(self.col_idx.numel())
+
+    @staticmethod
+    def stencil(ctx, dim, N, seed=0):
+        nnz = C.c_int64()
+        ctx.check(ctx.lib.ofrr_stencil_nnz(dim, N, C.byref(nnz)), "stencil_nnz")
+        n = N ** dim
+        dev = f"cuda:{ctx.device}"
+        rp = torch.empty(n + 1, dtype=torch.int64, device=dev)
+        ci = torch.empty(nnz.value, dtype=torch.int32, device=dev)
+        va = torch.empty(nnz.value, dtype=torch.float64, device=dev)
+        ctx.check(ctx.lib.ofrr_stencil_csr(ctx.h, dim, N, seed, _p(rp), _p(ci), F64, _p(va)),
+                  "stencil_csr")
+        return CSRMatrix(n, rp, ci, va)
+
+    @staticmethod
+    def from_host(ctx, row_ptr, col_idx, vals):
+        dev = f"cuda:{ctx.device}"
+        rp = torch.as_tensor(np.asarray(row_ptr, np.int64), device=dev)
+        ci = torch.as_tensor(np.asarray(col_idx, np.int32), device=dev)
+        va = torch.as_tensor(np.asarray(vals, np.float64), device=dev)
+        return CSRMatrix(len(row_ptr) - 1, rp, ci, va)
+
+    def with_low_precision(self, dtype=torch.float16):
+        self.vals_lo = self.vals.to(dtype)
+        return self
+
+
+# ---------------------------------------------------------------------------
+def spmm(ctx, A, X, out_dtype=None, alpha=1.0, beta=0.0, Y=None, vals=None):
+    """Y = alpha A X + beta Y (csrmv semantics, csrmv.m:48-59)."""
+    if X.dim() == 1:
+        X = X.view(-1, 1)
+    n, k = X.shape
+    if Y is None:
+        Y = torch.zeros(n, k, dtype=out_dtype or X.dtype, device=X.device)
+    va = A.vals if vals is None else vals
+    ctx.check(ctx.lib.ofrr_spmm_csr(ctx.h, A.n, A.nnz, _p(A.row_ptr), _p(A.col_idx), _p(va),
+                                    dtype_code(va), k, _p(X), dtype_code(X), X.stride(0),
+                                    X.stride(1), alpha, beta, _p(Y), dtype_code(Y), Y.stride(0),
+                                    Y.stride(1)), "spmm_csr")
+    return Y
+
+
+def fill_randn(ctx, t, seed, stream=0, offset=0):
+    ctx.check(ctx.lib.ofrr_fill_randn(ctx.h, dtype_code(t), t.numel(), _p(t), 1, seed, stream,
+                                      offset), "fill_randn")
+    return t
+
+
+def fill_randu(ctx, t, seed, stream=0, offset=0):
+    ctx.check(ctx.lib.ofrr_fill_randu(ctx.h, dtype_code(t), t.numel(), _p(t), 1, seed, stream,
+                                      offset), "fill_randu")
+    return t
+
+
+def iamax(ctx, x):
+    """(1-based index, signed value) of the first max |x_i| (mvops.hpp:98-99)."""
+    idx = torch.zeros(1, dtype=torch.int32, device=x.device)
+    val = torch.zeros(1, dtype=torch.float64, device=x.device)
+    ctx.check(ctx.lib.ofrr_iamax(ctx.h, dtype_code(x), x.numel(), _p(x), x.stride(0) if x.dim() else 1,
+                                 _p(idx), _p(val)), "iamax")
+    return int(idx.item()), float(val.item())
+
+
+def matrix_iamax(ctx, A):
+    """0-based per-column argmax |A(:,j)| (mvops.hpp:113-114)."""
+    m, n = A.shape
+    idx = torch.zeros(n, dtype=torch.int32, device=A.device)
+    ctx.check(ctx.lib.ofrr_matrix_iamax(ctx.h, dtype_code(A), m, n, _p(A), A.stride(0),
+                                        A.stride(1), _p(idx)), "matrix_iamax")
+    return idx
+
+
+def scalecols(ctx, A):
+    """In place: each column divided by its signed max-abs entry (scalecols.m:21-34)."""
+    m, n = A.shape
+    piv = torch.zeros(n, dtype=torch.float64, device=A.device)
+    ctx.check(ctx.lib.ofrr_scalecols(ctx.h, dtype_code(A), m, n, _p(A), A.stride(0), A.stride(1),
+                                     _p(piv)), "scalecols")
+    return piv
+
+
+def check_special(ctx, A):
+    """0, 1005 (NaN) or 1006 (Inf) -- check_special_values (mvops.cu:1388-1599)."""
+    if A.dim() == 1:
+        A = A.view(-1, 1)
+    m, n = A.shape
+    return ctx.lib.ofrr_check_special(ctx.h, dtype_code(A), m, n, _p(A), A.stride(0), A.stride(1))
+
+
+def hessenberg(ctx, A, select_tol=None):
+    """[Q, R, perm, skip] of the Hessenberg process (hessenberg.m:55-74) on a copy of A."""
+    Q = A.clone() if A.stride(0) == 1 else A.t().contiguous().t().clone()
+    m, n = Q.shape
+    if select_tol is None:
+        select_tol = {torch.float64: 2.0 ** -52, torch.float32: 2.0 ** -23,
+                      torch.float16: 2.0 ** -10, torch.bfloat16: 2.0 ** -7}[Q.dtype]
+    R = colmajor(n, n, torch.float64, Q.device)
+    perm = torch.zeros(n, dtype=torch.int32, device=Q.device)
+    skip = torch.zeros(n, dtype=torch.int32, device=Q.device)
+    ctx.check(ctx.lib.ofrr_hessenberg(ctx.h, dtype_code(Q), m, n, _p(Q), Q.stride(0), Q.stride(1),
+                                      select_tol, _p(R), n, _p(perm), _p(skip)), "hessenberg")
+    return Q, R, perm, skip
+
+
+def gram(ctx, V, W, alpha=1.0, beta=0.0, G=None):
+    """G = alpha V^T W + beta G, FP64 accumulation (eig_rr.m:49,51)."""
+    n, k1 = V.shape
+    k2 = W.shape[1]
+    if G is None:
+        G = colmajor(k1, k2, torch.float64, V.device)
+        G.zero_()
+    ctx.check(ctx.lib.ofrr_gram(ctx.h, n, k1, k2, _p(V), dtype_code(V), V.stride(0), V.stride(1),
+                                _p(W), dtype_code(W), W.stride(0), W.stride(1), alpha, beta,
+                                _p(G), G.stride(1)), "gram")
+    return G
+
+
+def basis_update(ctx, blocks, k, out_dtype=torch.float64, Y=None):
+    """Y = sum_b V_b C_b for [(V_b, C_b)] (C_b column-major FP64)."""
+    n = blocks[0][0].shape[0]
+    nb = len(blocks)
+    if Y is None:
+        Y = torch.empty(n, k, dtype=out_dtype, device=blocks[0][0].device)
+    Vp = (C.c_void_p * nb)(*[V.data_ptr() for V, _ in blocks])
+    dt = (C.c_int * nb)(*[dtype_code(V) for V, _ in blocks])
+    kb = (C.c_int * nb)(*[V.shape[1] for V, _ in blocks])
+    rs = (C.c_int64 * nb)(*[V.stride(0) for V, _ in blocks])
+    cs = (C.c_int64 * nb)(*[V.stride(1) for V, _ in blocks])
+    Cp = (C.c_void_p * nb)(*[Cb.data_ptr() for _, Cb in blocks])
+    ld = (C.c_int * nb)(*[Cb.stride(1) for _, Cb in blocks])
+    for _, Cb in blocks:
+        assert Cb.stride(0) == 1 and Cb.dtype == torch.float64
+    ctx.check(ctx.lib.ofrr_basis_update(ctx.h, n, k, nb, Vp, dt, kb, rs, cs, Cp, ld, _p(Y),
+                                        dtype_code(Y), Y.stride(0), Y.stride(1)), "basis_update")
+    return Y
+
+
+def syev_batched(ctx, A):
+    """Eigenvalues ascending and vectors of each symmetric matrix in A (batch, n, n),
+    read from the upper triangle in column-major storage (syev, mvops.hpp:184-197)."""
+    if A.dim() == 2:
+        A = A.unsqueeze(0)
+    b, n, _ = A.shape
+    Acm = A.transpose(1, 2).contiguous()  # element (i, j) at [j, i]: column-major per matrix
+    w = torch.empty(b, n, dtype=torch.float64, device=A.device)
+    V = torch.empty(b, n, n, dtype=torch.float64, device=A.device)
+    ctx.check(ctx.lib.ofrr_syev_batched(ctx.h, b, n, _p(Acm), n, n * n, _p(w), n, _p(V), n, n * n),
+              "syev_batched")
+    return w, V.transpose(1, 2)
+
+
+def sygv(ctx, A, B):
+    """A x = lambda B x (itype 1), ascending, B-normalised vectors (sygv, mvops.hpp:184-197)."""
+    n = A.shape[0]
+    Acm = A.t().contiguous()
+    Bcm = B.t().contiguous()
+    w = torch.empty(n, dtype=torch.float64, device=A.device)
+    V = colmajor(n, n, torch.float64, A.device)
+    ctx.check(ctx.lib.ofrr_sygv(ctx.h, n, _p(Acm), n, _p(Bcm), n, _p(w), _p(V), n), "sygv")
+    return w, V
+
+
+def rr_truncated(ctx, G, H, tau, nkeep):
+    """Truncated OFRR Rayleigh-Ritz (ofrr_eig, rr_eig.hpp:21-35): (theta desc, Y, rank)."""
+    m = G.shape[0]
+    Gc = G.t().contiguous().t() if G.stride(0) != 1 else G
+    Hc = H.t().contiguous().t() if H.stride(0) != 1 else H
+    th = torch.empty(nkeep, dtype=torch.float64, device=G.device)
+    Y = colmajor(m, nkeep, torch.float64, G.device)
+    rk = torch.zeros(1, dtype=torch.int32, device=G.device)
+    ctx.check(ctx.lib.ofrr_rr_truncated(ctx.h, m, _p(Gc), Gc.stride(1), _p(Hc), Hc.stride(1), tau,
+                                        nkeep, _p(th), _p(Y), m, _p(rk)), "rr_truncated")
+    return th, Y, int(rk.item())
+
+
+def rr_generalized(ctx, G, H, gtol=0.0):
+    """eig(H, G + gtol I) sorted descending (eig_rr.m:52)."""
+    m = G.shape[0]
+    Gc = G.t().contiguous().t() if G.stride(0) != 1 else G
+    Hc = H.t().contiguous().t() if H.stride(0) != 1 else H
+    th = torch.empty(m, dtype=torch.float64, device=G.device)
+    Y = colmajor(m, m, torch.float64, G.device)
+    ctx.check(ctx.lib.ofrr_rr_generalized(ctx.h, m, _p(Gc), Gc.stride(1), _p(Hc), Hc.stride(1),
+                                          gtol, _p(th), _p(Y), m), "rr_generalized")
+    return th, Y
+
+
+def residuals(ctx, A, X, theta, normA):
+    """||A x_j - theta_j x_j|| / (normA ||x_j||) in FP64 (test4/test.m:340-350)."""
+    k = X.shape[1]
+    th = (C.c_double * k)(*[float(t) for t in theta])
+    res = (C.c_double * k)()
+    ctx.check(ctx.lib.ofrr_residuals(ctx.h, A.n, A.nnz, _p(A.row_ptr), _p(A.col_idx), _p(A.vals),
+                                     k, _p(X), X.stride(0), X.stride(1), th, normA, res),
+              "residuals")
+    return np.array(res[:])
+
+
+def arnoldi_hess_eig(ctx, A, k, kdim, maxiter, v0, select_tol=2.0 ** -52, gtol=0.0):
+    """arnoldi_hess_eig.m in FP64 on the GPU: (theta desc, X col-major, built)."""
+    th = (C.c_double * k)()
+    X = colmajor(A.n, k, torch.float64, v0.device)
+    built = C.c_int()
+    ctx.check(ctx.lib.ofrr_arnoldi_hess_eig(ctx.h, A.n, A.nnz, _p(A.row_ptr), _p(A.col_idx),
+                                            _p(A.vals), k, kdim, maxiter, select_tol, gtol,
+                                            _p(v0.contiguous()), th, _p(X), A.n,
+                                            C.byref(built)), "arnoldi_hess_eig")
+    return np.array(th[:]), X, built.value
+
+
+def subspace_iter_eig(ctx, A, V0, maxiter, step_size=1, select_tol=2.0 ** -52, gtol=0.0):
+    """subspace_iter_eig.m with hessenberg + eig_rr(generalised), FP64 on the GPU."""
+    m = V0.shape[1]
+    V = colmajor(A.n, m, torch.float64, V0.device)
+    V.copy_(V0)
+    th = (C.c_double * m)()
+    ctx.check(ctx.lib.ofrr_subspace_iter_eig(ctx.h, A.n, A.nnz, _p(A.row_ptr), _p(A.col_idx),
+                                             _p(A.vals), m, maxiter, step_size, select_tol, gtol,
+                                             _p(V), A.n, th), "subspace_iter_eig")
+    return np.array(th[:]), V
+
+
+def filtered_eig(ctx, A, X0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=torch.float16,
+                 tol=1e-8, tau=1e-12, lower=0.0, normA=0.0, use_lo_values=True):
+    """Filtered OFRR solve; X0 (n x k, FP64) is copied.  Returns
+    (status, theta, X row-major, res, stats dict)."""
+    n, k = X0.shape
+    X = X0.to(torch.float64).contiguous().clone()
+    bdt = dtype_code(basis_dtype)
+    o = FilterOpts(k, nwant, nd, degree, maxiter, bdt, tol, tau, lower, normA)
+    th = (C.c_double * k)()
+    res = (C.c_double * k)()
+    stats = FilterStats()
+    vlo, lodt = C.c_void_p(0), F64
+    if use_lo_values and A.vals_lo is not None and bdt != F64:
+        vlo, lodt = _p(A.vals_lo), dtype_code(A.vals_lo)
+    st = ctx.lib.ofrr_filtered_eig(ctx.h, A.n, A.nnz, _p(A.row_ptr), _p(A.col_idx), _p(A.vals),
+                                   vlo, lodt, C.byref(o), _p(X), th, res, C.byref(stats))
+    if st not in (STATUS_SUCCESS, STATUS_MAX_ITER):
+        ctx.check(st, "filtered_eig")
+    sd = {f: getattr(stats, f) for f, _ in FilterStats._fields_}
+    return st, np.array(th[:]), X, np.array(res[:]), sd

@@ -1,0 +1,150 @@
+// Context, status strings and workspace of the gfx950 OFRR library.
+// The context replaces the reference's CUDAHandler singleton
+// (CPP/core/utils/cuda_handler.cu:9-45): it owns a stream, a grow-only device
+// workspace and a pinned host staging buffer, one context per host thread.
+#include "common.hpp"
+
+namespace ofrr {
+
+int set_error(ofrr_ctx *ctx, int status, const std::string &msg) {
+   if (ctx) ctx->last_error = msg;
+   return status;
+}
+
+int hip_status(ofrr_ctx *ctx, hipError_t e, const char *what) {
+   std::string m = std::string(what) + ": " + hipGetErrorString(e);
+   int s = kErrorCUDAGeneral;
+   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) s = kErrorCUDAMemory;
+   else if (e == hipErrorLaunchFailure || e == hipErrorInvalidConfiguration) s = kErrorCUDALaunch;
+   return set_error(ctx, s, m);
+}
+
+void *workspace(ofrr_ctx *ctx, size_t bytes) {
+   if (bytes == 0) bytes = 256;
+   if (ctx->ws_bytes >= bytes) return ctx->ws;
+   if (ctx->ws) {
+      // the old workspace may still be read by enqueued work
+      if (hipStreamSynchronize(ctx->stream) != hipSuccess) return nullptr;
+      (void)hipFree(ctx->ws);
+      ctx->ws = nullptr;
+      ctx->ws_bytes = 0;
+   }
+   size_t want = bytes + bytes / 4;
+   if (hipMalloc(&ctx->ws, want) != hipSuccess) {
+      ctx->ws = nullptr;
+      return nullptr;
+   }
+   ctx->ws_bytes = want;
+   return ctx->ws;
+}
+
+void *pinned(ofrr_ctx *ctx, size_t bytes) {
+   if (ctx->pinned_bytes >= bytes) return ctx->pinned;
+   if (ctx->pinned) {
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipHostFree(ctx->pinned);
+      ctx->pinned = nullptr;
+      ctx->pinned_bytes = 0;
+   }
+   if (hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+   ctx->pinned_bytes = bytes;
+   return ctx->pinned;
+}
+
+size_t dtype_size(int dt) {
+   switch (dt) {
+      case OFRR_F64: return 8;
+      case OFRR_F32: return 4;
+      case OFRR_F16: return 2;
+      case OFRR_BF16: return 2;
+      case OFRR_FP8: return 1;
+      default: return 0;
+   }
+}
+bool dtype_valid(int dt) { return dtype_size(dt) != 0; }
+
+}  // namespace ofrr
+
+using namespace ofrr;
+
+extern "C" {
+
+int ofrr_version(void) { return 100; }
+
+const char *ofrr_status_string(int s) {
+   switch (s) {
+      case kSuccess: return "Success";
+      case kErrorInvalidArgument: return "Invalid argument";
+      case kErrorInvalidOperation: return "Invalid operation";
+      case kErrorOutOfMemory: return "Out of memory";
+      case kErrorNotImplemented: return "Not implemented";
+      case kErrorInternalError: return "Internal error";
+      case kErrorNaN: return "NaN value detected";
+      case kErrorInf: return "Inf value detected";
+      case kErrorBLASInvalidLocation: return "Vectors in different locations";
+      case kErrorBLASUnsupportedType: return "Unsupported data type combination";
+      case kErrorBLASInvalidSize: return "Invalid size for operation";
+      case kErrorFactorizationFailed: return "Factorization failed";
+      case kErrorSingularMatrix: return "Singular matrix";
+      case kErrorSolverDivergence: return "Solver divergence";
+      case kErrorSolverMaxIterations: return "Solver reached maximum iterations";
+      case kErrorCUDAGeneral: return "HIP runtime error";
+      case kErrorCUDAMemory: return "HIP memory error";
+      case kErrorCUDALaunch: return "HIP kernel launch error";
+      case kErrorCUDASync: return "HIP synchronization error";
+      default: return "Unknown status";
+   }
+}
+
+int ofrr_ctx_create(int device, ofrr_ctx **out) {
+   if (!out) return kErrorInvalidArgument;
+   *out = nullptr;
+   int n = 0;
+   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return kErrorCUDAGeneral;
+   if (device < 0 || device >= n) return kErrorInvalidArgument;
+   if (hipSetDevice(device) != hipSuccess) return kErrorCUDAGeneral;
+   ofrr_ctx *c = new ofrr_ctx();
+   c->device = device;
+   if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return kErrorCUDAGeneral;
+   }
+   c->stream = c->own_stream;
+   hipDeviceProp_t prop;
+   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+   *out = c;
+   return kSuccess;
+}
+
+int ofrr_ctx_destroy(ofrr_ctx *c) {
+   if (!c) return kSuccess;
+   (void)hipSetDevice(c->device);
+   if (c->stream) (void)hipStreamSynchronize(c->stream);
+   if (c->ws) (void)hipFree(c->ws);
+   if (c->pinned) (void)hipHostFree(c->pinned);
+   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+   delete c;
+   return kSuccess;
+}
+
+int ofrr_ctx_sync(ofrr_ctx *c) {
+   OFRR_CHECK_CTX(c);
+   hipError_t e = hipStreamSynchronize(c->stream);
+   if (e != hipSuccess) {
+      hip_status(c, e, "hipStreamSynchronize");
+      return kErrorCUDASync;
+   }
+   return kSuccess;
+}
+
+int ofrr_ctx_set_stream(ofrr_ctx *c, void *s) {
+   OFRR_CHECK_CTX(c);
+   c->stream = s ? (hipStream_t)s : c->own_stream;
+   return kSuccess;
+}
+
+void *ofrr_ctx_stream(ofrr_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+const char *ofrr_ctx_last_error(ofrr_ctx *c) { return c ? c->last_error.c_str() : ""; }
+
+}  // extern "C"

@@ -1,0 +1,440 @@
+// FP64 dense symmetric eigensolver and Rayleigh-Ritz on the GPU.
+//
+// syev: parallel cyclic Jacobi, one workgroup per matrix (batched for the
+// serving path).  A round pairs all indices with the round-robin ("circle")
+// schedule; every 2x2 block (pair I, pair J) of A is then transformed
+// independently, A'_IJ = J_I^T A_IJ J_J, and the eigenvector block V <- V J,
+// so one round needs two barriers.  A rotation is skipped when
+// |a_pq| <= eps/2 sqrt(|a_pp a_qq|) (the Demmel-Veselic threshold), and the
+// solver stops after a sweep without rotations.  The working copy of A lives
+// in LDS when n <= 88 and in global memory (L2 resident) otherwise.
+// Replaces the host LAPACKE dsyev/dsygv of the reference
+// (CPP/linalg/blas/mvops.cu:1601-1664).
+//
+// rr_truncated: the OFRR projection step (ofrr_eig, CPP/linalg/solvers/eig/
+// rr_eig.hpp:21-35; MATLAB/+src/+rr/eig_rr.m:48-57 with the shift replaced by
+// truncation): S = diag(G)^{-1/2}; eig(S G S) = U L; keep L_i > tau max L;
+// B = S U_r L_r^{-1/2}; eig(B^T H B) = Z Theta; Y = B Z, descending.
+// The kept rank stays on the device: the kernels after the truncation read it.
+#include "internal.hpp"
+
+namespace ofrr {
+
+constexpr int kJacobiThreads = 1024;
+constexpr int kLdsMaxN = 88;  // n^2 doubles + the static arrays stay below 64 KiB
+
+__device__ inline void pair_of(int nn, int r, int s, int &p, int &q) {
+   if (s == 0) { p = nn - 1; q = r; }
+   else { p = (r + s) % (nn - 1); q = (r - s + nn - 1) % (nn - 1); }
+   if (p > q) { int t = p; p = q; q = t; }
+}
+
+// n_dev (nullable): per-matrix size read from device memory (<= n)
+template <bool LDS>
+__global__ __launch_bounds__(kJacobiThreads) void jacobi_kernel(
+    int n_max, const int32_t *__restrict__ n_dev, const double *__restrict__ A, int lda,
+    int64_t strideA, double *__restrict__ w, int64_t stridew, double *__restrict__ V, int ldv,
+    int64_t strideV, double *__restrict__ work, int64_t strideWork, int max_sweeps) {
+   extern __shared__ double lds[];
+   __shared__ double rc[128], rsn[128];
+   __shared__ int rotated;
+   __shared__ int s_n;
+   const int b = blockIdx.x;
+   const double *Ab = A + b * strideA;
+   double *wb = w + b * stridew;
+   double *Vb = V + b * strideV;
+   if (threadIdx.x == 0) s_n = n_dev ? min((int)n_dev[b], n_max) : n_max;
+   __syncthreads();
+   const int n = s_n;
+   if (n <= 0) return;
+   const int nn = n + (n & 1);
+   const int np = nn / 2;
+   double *a = LDS ? lds : work + b * strideWork;                 // n x n, ld n
+   double *v = (LDS ? work + b * strideWork : work + b * strideWork + (int64_t)n_max * n_max);
+   // working copies: A symmetrised from its upper triangle, V = I
+   for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+      int i = t % n, j = t / n;
+      a[t] = (i <= j) ? Ab[i + (int64_t)j * lda] : Ab[j + (int64_t)i * lda];
+      v[t] = (i == j) ? 1.0 : 0.0;
+   }
+   __syncthreads();
+   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+      if (threadIdx.x == 0) rotated = 0;
+      __syncthreads();
+      for (int r = 0; r < nn - 1; ++r) {
+         // rotation of each pair
+         for (int s = threadIdx.x; s < np; s += blockDim.x) {
+            int p, q;
+            pair_of(nn, r, s, p, q);
+            double c = 1.0, sn = 0.0;
+            if (q < n) {
+               double apq = a[p + q * n], app = a[p + p * n], aqq = a[q + q * n];
+               if (apq != 0.0 && fabs(apq) > 1.1102230246251565e-16 * sqrt(fabs(app) * fabs(aqq))) {
+                  double th = (aqq - app) / (2.0 * apq);
+                  double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                  c = 1.0 / sqrt(t * t + 1.0);
+                  sn = t * c;
+                  rotated = 1;
+               }
+            }
+            rc[s] = c;
+            rsn[s] = sn;
+         }
+         __syncthreads();
+         // A' = J^T A J over 2x2 blocks (I <= J), mirrored; V' = V J
+         const int nblocks = np * np;
+         for (int t = threadIdx.x; t < nblocks; t += blockDim.x) {
+            int I = t / np, J = t % np;
+            if (I > J) continue;
+            int p1, q1, p2, q2;
+            pair_of(nn, r, I, p1, q1);
+            pair_of(nn, r, J, p2, q2);
+            const double c1 = rc[I], s1 = rsn[I], c2 = rc[J], s2 = rsn[J];
+            const bool q1ok = q1 < n, q2ok = q2 < n;
+            double a11 = a[p1 + p2 * n];
+            double a12 = q2ok ? a[p1 + q2 * n] : 0.0;
+            double a21 = q1ok ? a[q1 + p2 * n] : 0.0;
+            double a22 = (q1ok && q2ok) ? a[q1 + q2 * n] : 0.0;
+            // M J2
+            double m11 = c2 * a11 - s2 * a12, m12 = s2 * a11 + c2 * a12;
+            double m21 = c2 * a21 - s2 * a22, m22 = s2 * a21 + c2 * a22;
+            // J1^T (M J2)
+            double b11 = c1 * m11 - s1 * m21, b12 = c1 * m12 - s1 * m22;
+            double b21 = s1 * m11 + c1 * m21, b22 = s1 * m12 + c1 * m22;
+            if (I == J) {
+               // diagonal block: the pair's off-diagonal entry is annihilated
+               a[p1 + p1 * n] = b11;
+               if (q1ok) {
+                  a[q1 + q1 * n] = b22;
+                  double off = (s1 != 0.0) ? 0.0 : 0.5 * (b12 + b21);
+                  a[p1 + q1 * n] = off;
+                  a[q1 + p1 * n] = off;
+               }
+            } else {
+               a[p1 + p2 * n] = b11; a[p2 + p1 * n] = b11;
+               if (q2ok) { a[p1 + q2 * n] = b12; a[q2 + p1 * n] = b12; }
+               if (q1ok) { a[q1 + p2 * n] = b21; a[p2 + q1 * n] = b21; }
+               if (q1ok && q2ok) { a[q1 + q2 * n] = b22; a[q2 + q1 * n] = b22; }
+            }
+         }
+         for (int t = threadIdx.x; t < n * np; t += blockDim.x) {
+            int row = t % n, s = t / n;
+            int p, q;
+            pair_of(nn, r, s, p, q);
+            if (q >= n) continue;
+            double c = rc[s], sn = rsn[s];
+            if (sn == 0.0) continue;
+            double vp = v[row + p * n], vq = v[row + q * n];
+            v[row + p * n] = c * vp - sn * vq;
+            v[row + q * n] = sn * vp + c * vq;
+         }
+         __syncthreads();
+      }
+      if (!rotated) break;
+      __syncthreads();
+   }
+   // eigenvalues ascending (stable rank), vectors permuted accordingly
+   for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      double di = a[i + i * n];
+      int rank = 0;
+      for (int j = 0; j < n; ++j) {
+         double dj = a[j + j * n];
+         rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
+      }
+      wb[rank] = di;
+      for (int row = 0; row < n; ++row) Vb[row + (int64_t)rank * ldv] = v[row + i * n];
+   }
+}
+
+static size_t jacobi_work_doubles(int n) { return 2 * (size_t)n * n; }
+
+int syev_batched(ofrr_ctx *ctx, int batch, int n, const int32_t *n_dev, const double *A, int lda,
+                 int64_t strideA, double *w, int64_t stridew, double *V, int ldv,
+                 int64_t strideV, double *work /* nullable: workspace */) {
+   if (n <= 0 || batch <= 0) return kSuccess;
+   const int64_t strideWork = (int64_t)jacobi_work_doubles(n);
+   if (!work) {
+      work = (double *)workspace(ctx, (size_t)batch * strideWork * sizeof(double));
+      if (!work) return set_error(ctx, kErrorCUDAMemory, "syev: workspace");
+   }
+   const int sweeps = 40;
+   if (n <= kLdsMaxN) {
+      size_t lds = (size_t)n * n * sizeof(double);
+      jacobi_kernel<true><<<batch, kJacobiThreads, lds, ctx->stream>>>(
+          n, n_dev, A, lda, strideA, w, stridew, V, ldv, strideV, work, strideWork, sweeps);
+   } else {
+      if (n > 256) return set_error(ctx, kErrorBLASInvalidSize, "syev: n > 256 not supported");
+      jacobi_kernel<false><<<batch, kJacobiThreads, 0, ctx->stream>>>(
+          n, n_dev, A, lda, strideA, w, stridew, V, ldv, strideV, work, strideWork, sweeps);
+   }
+   OFRR_LAUNCH(ctx);
+   return kSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// C = alpha op(A) B + beta C, small dense FP64 (one thread per output)
+__global__ void small_gemm_kernel(int transA, int M, int N, int K, const int32_t *__restrict__ kdev,
+                                  double alpha, const double *__restrict__ A, int lda,
+                                  const double *__restrict__ B, int ldb, double beta,
+                                  double *__restrict__ Cm, int ldc) {
+   int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+   if (i >= M || j >= N) return;
+   int kk = kdev ? min(K, (int)*kdev) : K;
+   double s = 0.0;
+   for (int p = 0; p < kk; ++p) {
+      double av = transA ? A[p + (int64_t)i * lda] : A[i + (int64_t)p * lda];
+      s = fma(av, B[p + (int64_t)j * ldb], s);
+   }
+   double *c = Cm + i + (int64_t)j * ldc;
+   *c = (beta == 0.0) ? alpha * s : fma(alpha, s, beta * *c);
+}
+
+static int small_gemm(ofrr_ctx *ctx, int transA, int M, int N, int K, const int32_t *kdev,
+                      double alpha, const double *A, int lda, const double *B, int ldb,
+                      double beta, double *Cm, int ldc) {
+   small_gemm_kernel<<<dim3((M + 63) / 64, N), 64, 0, ctx->stream>>>(transA, M, N, K, kdev, alpha,
+                                                                    A, lda, B, ldb, beta, Cm, ldc);
+   OFRR_LAUNCH(ctx);
+   return kSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// truncated RR helpers
+__global__ void rr_scale_kernel(int m, const double *__restrict__ G, int ldg,
+                                const double *__restrict__ H, int ldh, double *__restrict__ sc,
+                                double *__restrict__ Gs, double *__restrict__ Hs) {
+   int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= m * m) return;
+   int i = t % m, j = t / m;
+   double gi = G[i + (int64_t)i * ldg], gj = G[j + (int64_t)j * ldg];
+   double si = gi > 0.0 ? 1.0 / sqrt(gi) : 0.0, sj = gj > 0.0 ? 1.0 / sqrt(gj) : 0.0;
+   if (j == 0) sc[i] = si;
+   double g = 0.5 * (G[i + (int64_t)j * ldg] + G[j + (int64_t)i * ldg]);
+   double h = 0.5 * (H[i + (int64_t)j * ldh] + H[j + (int64_t)i * ldh]);
+   Gs[t] = g * si * sj;
+   Hs[t] = h * si * sj;
+}
+
+// rank r of the eigenvalues lam (ascending) above tau * max; Bm = U_r L_r^{-1/2}
+// (m x r, columns = the top r in ascending order)
+__global__ void rr_trunc_kernel(int m, double tau, const double *__restrict__ lam,
+                                const double *__restrict__ U, double *__restrict__ Bm,
+                                int32_t *__restrict__ rank) {
+   __shared__ int s_r;
+   if (threadIdx.x == 0) {
+      double lmax = lam[m - 1];
+      int r = 0;
+      for (int i = 0; i < m; ++i) r += (lam[i] > tau * lmax) ? 1 : 0;
+      s_r = r;
+      *rank = r;
+   }
+   __syncthreads();
+   const int r = s_r;
+   for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+      int i = t % m, c = t / m;
+      double val = 0.0;
+      if (c < r) {
+         int src = m - r + c;
+         val = U[i + (int64_t)src * m] / sqrt(lam[src]);
+      }
+      Bm[t] = val;
+   }
+}
+
+// theta (descending) and Y = S Bm Z for the top nkeep
+__global__ void rr_finish_kernel(int m, int nkeep, const int32_t *__restrict__ rank,
+                                 const double *__restrict__ sc, const double *__restrict__ Bm,
+                                 const double *__restrict__ th, const double *__restrict__ Z,
+                                 double *__restrict__ theta, double *__restrict__ Y, int ldy) {
+   const int r = *rank;
+   const int c = blockIdx.x;
+   for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      double s = 0.0;
+      if (c < r) {
+         int src = r - 1 - c;
+         for (int p = 0; p < r; ++p) s = fma(Bm[i + (int64_t)p * m], Z[p + (int64_t)src * m], s);
+         s *= sc[i];
+      }
+      Y[i + (int64_t)c * ldy] = s;
+   }
+   if (threadIdx.x == 0) theta[c] = (c < r) ? th[r - 1 - c] : __builtin_nan("");
+}
+
+int rr_truncated(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H, int ldh,
+                 double tau, int nkeep, double *theta, double *Y, int ldy, int32_t *rank) {
+   if (m <= 0 || nkeep <= 0 || nkeep > m) return set_error(ctx, kErrorInvalidArgument, "rr_truncated: sizes");
+   // workspace: sc(m) Gs Hs U Bm T Hr Z (m*m each) lam th (m) rank(1) + jacobi work
+   const size_t mm = (size_t)m * m;
+   const size_t jw = jacobi_work_doubles(m);
+   size_t need = (m + 6 * mm + 2 * m + 8 + jw) * sizeof(double);
+   double *base = (double *)workspace(ctx, need);
+   if (!base) return set_error(ctx, kErrorCUDAMemory, "rr_truncated: workspace");
+   double *sc = base, *Gs = sc + m, *Hs = Gs + mm, *U = Hs + mm, *Bm = U + mm, *T = Bm + mm;
+   double *Hr = T + mm, *lam = Hr + mm, *th = lam + m;
+   int32_t *rk = (int32_t *)(th + m);
+   double *jwork = th + m + 8;
+   double *Z = Gs;  // Gs is dead after its eigendecomposition
+   int32_t *rank_out = rank ? rank : rk;
+   rr_scale_kernel<<<(unsigned)((mm + 255) / 256), 256, 0, ctx->stream>>>(m, G, ldg, H, ldh, sc, Gs, Hs);
+   OFRR_LAUNCH(ctx);
+   int st = syev_batched(ctx, 1, m, nullptr, Gs, m, 0, lam, 0, U, m, 0, jwork);
+   if (st) return st;
+   rr_trunc_kernel<<<1, 256, 0, ctx->stream>>>(m, tau, lam, U, Bm, rank_out);
+   OFRR_LAUNCH(ctx);
+   // Hr = Bm^T Hs Bm over the first r columns (zero beyond)
+   if ((st = small_gemm(ctx, 0, m, m, m, nullptr, 1.0, Hs, m, Bm, m, 0.0, T, m))) return st;
+   if ((st = small_gemm(ctx, 1, m, m, m, nullptr, 1.0, Bm, m, T, m, 0.0, Hr, m))) return st;
+   // eig of the leading r x r block of Hr (ld m); sizes from the device
+   st = syev_batched(ctx, 1, m, rank_out, Hr, m, 0, th, 0, Z, m, 0, jwork);
+   if (st) return st;
+   rr_finish_kernel<<<nkeep, 256, 0, ctx->stream>>>(m, nkeep, rank_out, sc, Bm, th, Z, theta, Y, ldy);
+   OFRR_LAUNCH(ctx);
+   return kSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// generalised problem H x = lambda (G + gtol I) x via Cholesky (one workgroup)
+__global__ __launch_bounds__(1024) void chol_transform_kernel(int m, const double *__restrict__ G,
+                                                              int ldg, const double *__restrict__ H,
+                                                              int ldh, double gtol,
+                                                              double *__restrict__ L,
+                                                              double *__restrict__ Cm,
+                                                              int32_t *__restrict__ info) {
+   __shared__ int s_info;
+   if (threadIdx.x == 0) s_info = 0;
+   for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+      int i = t % m, j = t / m;
+      // lower triangle of B from G's upper triangle; C from H's upper triangle
+      L[t] = (i >= j) ? G[j + (int64_t)i * ldg] + (i == j ? gtol : 0.0) : 0.0;
+      Cm[t] = (i <= j) ? H[i + (int64_t)j * ldh] : H[j + (int64_t)i * ldh];
+   }
+   __syncthreads();
+   // right-looking Cholesky, column by column
+   for (int j = 0; j < m; ++j) {
+      double d = L[j + j * m];
+      if (!(d > 0.0)) {
+         if (threadIdx.x == 0 && s_info == 0) s_info = j + 1;
+         __syncthreads();
+         break;
+      }
+      d = sqrt(d);
+      __syncthreads();
+      for (int i = j + threadIdx.x; i < m; i += blockDim.x) L[i + j * m] = (i == j) ? d : L[i + j * m] / d;
+      __syncthreads();
+      for (int t = threadIdx.x; t < (m - j - 1) * (m - j - 1); t += blockDim.x) {
+         int i = j + 1 + t % (m - j - 1), c = j + 1 + t / (m - j - 1);
+         if (i >= c) L[i + c * m] -= L[i + j * m] * L[c + j * m];
+      }
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) *info = s_info;
+   if (s_info) return;
+   // C <- L^{-1} C (columns independent), then C <- C L^{-T} (rows independent)
+   for (int c = threadIdx.x; c < m; c += blockDim.x)
+      for (int i = 0; i < m; ++i) {
+         double s = Cm[i + c * m];
+         for (int p = 0; p < i; ++p) s -= L[i + p * m] * Cm[p + c * m];
+         Cm[i + c * m] = s / L[i + i * m];
+      }
+   __syncthreads();
+   for (int r = threadIdx.x; r < m; r += blockDim.x)
+      for (int j = 0; j < m; ++j) {
+         double s = Cm[r + j * m];
+         for (int p = 0; p < j; ++p) s -= L[j + p * m] * Cm[r + p * m];
+         Cm[r + j * m] = s / L[j + j * m];
+      }
+   __syncthreads();
+   for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+      int i = t % m, j = t / m;
+      if (i < j) {
+         double s = 0.5 * (Cm[i + j * m] + Cm[j + i * m]);
+         Cm[i + j * m] = s;
+         Cm[j + i * m] = s;
+      }
+   }
+}
+
+// V = L^{-T} Yv (back substitution per column); optional descending reorder
+__global__ void chol_back_kernel(int m, const double *__restrict__ L, const double *__restrict__ w,
+                                 const double *__restrict__ Yv, int descending,
+                                 double *__restrict__ wout, double *__restrict__ V, int ldv) {
+   int c = blockIdx.x * blockDim.x + threadIdx.x;
+   if (c >= m) return;
+   int src = descending ? m - 1 - c : c;
+   wout[c] = w[src];
+   for (int i = m - 1; i >= 0; --i) {
+      double s = Yv[i + (int64_t)src * m];
+      for (int p = i + 1; p < m; ++p) s -= L[p + i * m] * V[p + (int64_t)c * ldv];
+      V[i + (int64_t)c * ldv] = s / L[i + i * m];
+   }
+}
+
+int sygv_impl(ofrr_ctx *ctx, int m, const double *A, int lda, const double *B, int ldb,
+              double gtol, int descending, double *w, double *V, int ldv) {
+   const size_t mm = (size_t)m * m;
+   size_t need = (3 * mm + m + 8 + jacobi_work_doubles(m)) * sizeof(double);
+   double *base = (double *)workspace(ctx, need);
+   if (!base) return set_error(ctx, kErrorCUDAMemory, "sygv: workspace");
+   double *L = base, *Cm = L + mm, *Yv = Cm + mm, *wl = Yv + mm;
+   int32_t *info = (int32_t *)(wl + m);
+   double *jwork = wl + m + 8;
+   int32_t *hinfo = (int32_t *)pinned(ctx, sizeof(int32_t));
+   if (!hinfo) return set_error(ctx, kErrorCUDAMemory, "sygv: pinned");
+   chol_transform_kernel<<<1, 1024, 0, ctx->stream>>>(m, B, ldb, A, lda, gtol, L, Cm, info);
+   OFRR_LAUNCH(ctx);
+   OFRR_HIP(ctx, hipMemcpyAsync(hinfo, info, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+   OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+   if (*hinfo) return set_error(ctx, kErrorSingularMatrix, "sygv: B is not positive definite");
+   int st = syev_batched(ctx, 1, m, nullptr, Cm, m, 0, wl, 0, Yv, m, 0, jwork);
+   if (st) return st;
+   chol_back_kernel<<<(m + 63) / 64, 64, 0, ctx->stream>>>(m, L, wl, Yv, descending, w, V, ldv);
+   OFRR_LAUNCH(ctx);
+   return kSuccess;
+}
+
+int rr_generalized(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H, int ldh,
+                   double gtol, double *theta, double *Y, int ldy) {
+   return sygv_impl(ctx, m, H, ldh, G, ldg, gtol, 1, theta, Y, ldy);
+}
+
+}  // namespace ofrr
+
+using namespace ofrr;
+
+extern "C" {
+
+int ofrr_syev_batched(ofrr_ctx *ctx, int batch, int n, const double *A, int lda, int64_t strideA,
+                      double *w, int64_t stridew, double *V, int ldv, int64_t strideV) {
+   OFRR_CHECK_CTX(ctx);
+   if (batch < 0 || n < 0 || lda < n || ldv < n) return set_error(ctx, kErrorInvalidArgument, "syev: bad sizes");
+   if (batch == 0 || n == 0) return kSuccess;
+   if (!A || !w || !V) return set_error(ctx, kErrorInvalidArgument, "syev: null pointer");
+   return syev_batched(ctx, batch, n, nullptr, A, lda, strideA, w, stridew, V, ldv, strideV, nullptr);
+}
+
+int ofrr_sygv(ofrr_ctx *ctx, int n, const double *A, int lda, const double *B, int ldb, double *w,
+              double *V, int ldv) {
+   OFRR_CHECK_CTX(ctx);
+   if (n <= 0 || lda < n || ldb < n || ldv < n || !A || !B || !w || !V)
+      return set_error(ctx, kErrorInvalidArgument, "sygv: bad arguments");
+   if (n > 256) return set_error(ctx, kErrorBLASInvalidSize, "sygv: n > 256 not supported");
+   return sygv_impl(ctx, n, A, lda, B, ldb, 0.0, 0, w, V, ldv);
+}
+
+int ofrr_rr_truncated(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H, int ldh,
+                      double tau, int nkeep, double *theta, double *Y, int ldy, int32_t *rank) {
+   OFRR_CHECK_CTX(ctx);
+   if (m <= 0 || m > 256 || ldg < m || ldh < m || ldy < m || !G || !H || !theta || !Y)
+      return set_error(ctx, kErrorInvalidArgument, "rr_truncated: bad arguments");
+   return rr_truncated(ctx, m, G, ldg, H, ldh, tau, nkeep, theta, Y, ldy, rank);
+}
+
+int ofrr_rr_generalized(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H, int ldh,
+                        double gtol, double *theta, double *Y, int ldy) {
+   OFRR_CHECK_CTX(ctx);
+   if (m <= 0 || m > 256 || ldg < m || ldh < m || ldy < m || !G || !H || !theta || !Y)
+      return set_error(ctx, kErrorInvalidArgument, "rr_generalized: bad arguments");
+   return rr_generalized(ctx, m, G, ldg, H, ldh, gtol, theta, Y, ldy);
+}
+
+}  // extern "C"

@@ -1,0 +1,44 @@
+// Host-side launchers shared between the library's translation units.
+#pragma once
+#include "common.hpp"
+
+namespace ofrr {
+
+// Y = alpha A X + beta Y (any layout; row-major contiguous blocks take the fast path)
+int spmm_csr(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const void *va,
+             int vdt, int k, const void *X, int xdt, int64_t xrs, int64_t xcs, double alpha,
+             double beta, void *Y, int ydt, int64_t yrs, int64_t ycs);
+// W = A X, R = W - X diag(theta) quantised by inv_sR into Rq (rdt), column max
+// |R| folded into colmax (atomic max), sums[2j] = ||r_j||^2, sums[2j+1] = ||x_j||^2.
+// X, W, Rq row-major with row stride k.
+int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const double *va,
+               int k, const double *X, const double *theta, double *W, void *Rq, int rdt,
+               const double *inv_sR, float *colmax, double *sums);
+// one Chebyshev step on row-major blocks of width k (see spmm.hip)
+int cheb_step(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const void *va,
+              int vdt, int k, const void *Dc, const void *Dp, const void *R, int bdt,
+              const double *coef, const double *inv_s, void *Dn, float *colmax);
+// G = alpha V^T W + beta G (FP64), strided operands
+int gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64_t vrs,
+         int64_t vcs, const void *W, int wdt, int64_t wrs, int64_t wcs, double alpha, double beta,
+         double *G, int ldg);
+// Y = sum_b V_b C_b
+int basis_update(ofrr_ctx *ctx, int64_t n, int k, int nb, const void *const *V, const int *vdt,
+                 const int *kb, const int64_t *vrs, const int64_t *vcs, const double *const *C,
+                 const int *ldc, void *Y, int ydt, int64_t yrs, int64_t ycs);
+// truncated Rayleigh-Ritz (device pointers, FP64)
+int rr_truncated(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H, int ldh,
+                 double tau, int nkeep, double *theta, double *Y, int ldy, int32_t *rank);
+int rr_generalized(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H, int ldh,
+                   double gtol, double *theta, double *Y, int ldy);
+// argmax |x| (1-based index + signed value, device outputs)
+int iamax(ofrr_ctx *ctx, int dtype, int64_t n, const void *x, int64_t inc, int32_t *idx1,
+          double *val);
+
+// per-column argmax |A(:,j)| with index base `base` (0 or 1) and signed value
+int argmax_columns(ofrr_ctx *ctx, int dtype, int64_t m, int n, const void *A, int64_t rs,
+                   int64_t cs, int base, int32_t *idx, double *val);
+// out[j] = sum over b of part[b * width + j], fixed summation order
+int reduce_partials(ofrr_ctx *ctx, int64_t nblk, int width, const double *part, double *out);
+
+}  // namespace ofrr

@@ -1,0 +1,611 @@
+// Eigen-solver drivers over a CSR operator, composed from the library's
+// kernels.  The reference declares these solvers as empty C++ stubs
+// (CPP/linalg/solvers/eig/si_eig.cu:10-18, arnoldi_eig.cu:10-18,
+// rr_eig.cu:9-34); their semantics come from the MATLAB prototype:
+//   ofrr_arnoldi_hess_eig   MATLAB/+src/+krylov/arnoldi_hess_eig.m:43-107
+//   ofrr_subspace_iter_eig  MATLAB/+src/+subspace/subspace_iter_eig.m:53-68
+//   ofrr_filtered_eig       block subspace iteration with Chebyshev-filtered,
+//                           low-precision residual corrections, FP64 Ritz block,
+//                           FP64 projections and truncated Rayleigh-Ritz
+//   ofrr_residuals          MATLAB/+tests/+test4/test.m:340-350 (normalised)
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace ofrr {
+
+// ---- small device kernels used by the drivers ------------------------------
+__global__ void scale_vec_kernel(int64_t n, double *__restrict__ x, const double *__restrict__ piv) {
+   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) x[i] = x[i] / *piv;
+}
+
+// Arnoldi-Hessenberg elimination coefficients for the new column w against
+// columns 0..i of V (pivot rows perm): c_j = w[perm_j] - sum_{j'<j} c_j' V(perm_j, j'),
+// accumulated with fused multiply-adds in the order the sequential process uses
+__global__ void arnoldi_coef_kernel(int i, int64_t n, const double *__restrict__ V,
+                                    const double *__restrict__ w, const int32_t *__restrict__ perm,
+                                    double *__restrict__ c) {
+   if (threadIdx.x != 0) return;
+   for (int j = 0; j <= i; ++j) {
+      const int64_t pj = perm[j];
+      double t = w[pj];
+      for (int jj = 0; jj < j; ++jj) t = fma(-c[jj], V[pj + (int64_t)jj * n], t);
+      c[j] = t;
+   }
+}
+
+__global__ void arnoldi_elim_kernel(int i, int64_t n, const double *__restrict__ V,
+                                    double *__restrict__ w, const double *__restrict__ c) {
+   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (r >= n) return;
+   double t = w[r];
+   for (int j = 0; j <= i; ++j) t = fma(-c[j], V[r + (int64_t)j * n], t);
+   w[r] = t;
+}
+
+// per-column sums of (Ax - theta x)^2 and x^2 for strided X, partial per block
+__global__ __launch_bounds__(256) void resid_cols_kernel(int64_t n, const double *__restrict__ AX,
+                                                         const double *__restrict__ X,
+                                                         int64_t xrs, int64_t xcs,
+                                                         const double *__restrict__ theta,
+                                                         int64_t chunk, double *__restrict__ part) {
+   __shared__ double s1[256], s2[256];
+   const int col = blockIdx.y;
+   const int64_t r0 = (int64_t)blockIdx.x * chunk, r1 = min(n, r0 + chunk);
+   double a = 0.0, b = 0.0;
+   const double th = theta[col];
+   for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) {
+      double x = X[r * xrs + (int64_t)col * xcs];
+      double d = AX[r + (int64_t)col * n] - th * x;
+      a = fma(d, d, a);
+      b = fma(x, x, b);
+   }
+   s1[threadIdx.x] = a;
+   s2[threadIdx.x] = b;
+   __syncthreads();
+   for (int o = 128; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) { s1[threadIdx.x] += s1[threadIdx.x + o]; s2[threadIdx.x] += s2[threadIdx.x + o]; }
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) {
+      part[(int64_t)blockIdx.x * 2 * gridDim.y + 2 * col] = s1[0];
+      part[(int64_t)blockIdx.x * 2 * gridDim.y + 2 * col + 1] = s2[0];
+   }
+}
+
+// mirror the upper triangle of an m x m column-major matrix into its lower one
+__global__ void mirror_upper_kernel(int m, double *__restrict__ G, int ldg) {
+   int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= m * m) return;
+   int i = t % m, j = t / m;
+   if (i > j) G[i + (int64_t)j * ldg] = G[j + (int64_t)i * ldg];
+}
+
+// Chebyshev step coefficients are formed inside the step launcher kernel below
+// from device-side scales; this kernel prepares them.
+//   D_{j+1} = (2/e)(A D_j - c0 D_j) - D_{j-1} + (2/e) R z_j
+// stored blocks: D_j = Dc * sc, D_{j-1} = Dp * sp, R = Rq * sR; new scale
+// sn = 2^(ceil(log2(2 max|D_j|)) - target_exp) places the predicted range of
+// D_{j+1} near 2^target_exp in the storage format.
+__global__ void cheb_coef_kernel(int k, double e, double c0, const double *__restrict__ z,
+                                 const double *__restrict__ sR, const double *__restrict__ sc,
+                                 const double *__restrict__ sp, const float *__restrict__ maxc,
+                                 int target_exp, double *__restrict__ coef,
+                                 double *__restrict__ inv_sn, double *__restrict__ sn) {
+   int j = blockIdx.x * blockDim.x + threadIdx.x;
+   if (j >= k) return;
+   double m = (double)maxc[j];
+   double s = (m > 0.0) ? exp2(ceil(log2(2.0 * m)) - (double)target_exp) : 1.0;
+   coef[j] = 2.0 * sc[j] / e;
+   coef[k + j] = -2.0 * c0 * sc[j] / e;
+   coef[2 * k + j] = sp ? -sp[j] : 0.0;
+   coef[3 * k + j] = 2.0 * sR[j] * z[j] / e;
+   inv_sn[j] = 1.0 / s;
+   sn[j] = s;
+}
+
+// ---- device arena for the drivers (grow-only, reused between solves) -------
+struct Arena {
+   char *base = nullptr;
+   size_t bytes = 0, off = 0;
+};
+static thread_local Arena g_arena;
+static thread_local int g_arena_dev = -1;
+
+static int arena_reserve(ofrr_ctx *ctx, size_t bytes) {
+   if (g_arena.base && g_arena.bytes >= bytes && g_arena_dev == ctx->device) {
+      g_arena.off = 0;
+      return kSuccess;
+   }
+   if (g_arena.base) {
+      OFRR_HIP(ctx, hipDeviceSynchronize());
+      (void)hipFree(g_arena.base);
+      g_arena = Arena();
+   }
+   OFRR_HIP(ctx, hipMalloc((void **)&g_arena.base, bytes));
+   g_arena.bytes = bytes;
+   g_arena.off = 0;
+   g_arena_dev = ctx->device;
+   return kSuccess;
+}
+static void *arena_take(size_t bytes) {
+   size_t a = (g_arena.off + 255) & ~(size_t)255;
+   g_arena.off = a + bytes;
+   return g_arena.base + a;
+}
+
+static inline size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace ofrr
+
+using namespace ofrr;
+
+extern "C" {
+
+int ofrr_residuals(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, const int32_t *ci,
+                   const double *va, int k, const double *X, int64_t xrs, int64_t xcs,
+                   const double *theta_host, double normA, double *res_host) {
+   OFRR_CHECK_CTX(ctx);
+   if (n <= 0 || k <= 0 || !rp || !ci || !va || !X || !theta_host || !res_host)
+      return set_error(ctx, kErrorInvalidArgument, "residuals: bad arguments");
+   (void)nnz;
+   int64_t chunk = 16384;
+   int nch = (int)((n + chunk - 1) / chunk);
+   size_t need = al((size_t)n * k * 8) + al((size_t)k * 8) + al((size_t)nch * 2 * k * 8) + al((size_t)2 * k * 8);
+   char *ws = (char *)workspace(ctx, need);
+   double *hs = (double *)pinned(ctx, (size_t)2 * k * 8);
+   if (!ws || !hs) return set_error(ctx, kErrorCUDAMemory, "residuals: workspace");
+   double *AX = (double *)ws;
+   double *th = (double *)(ws + al((size_t)n * k * 8));
+   double *part = (double *)((char *)th + al((size_t)k * 8));
+   double *sums = (double *)((char *)part + al((size_t)nch * 2 * k * 8));
+   OFRR_HIP(ctx, hipMemcpyAsync(th, theta_host, (size_t)k * 8, hipMemcpyHostToDevice, ctx->stream));
+   int st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, k, X, OFRR_F64, xrs, xcs, 1.0, 0.0, AX, OFRR_F64, 1, n);
+   if (st) return st;
+   resid_cols_kernel<<<dim3(nch, k), 256, 0, ctx->stream>>>(n, AX, X, xrs, xcs, th, chunk, part);
+   OFRR_LAUNCH(ctx);
+   if ((st = reduce_partials(ctx, nch, 2 * k, part, sums))) return st;
+   OFRR_HIP(ctx, hipMemcpyAsync(hs, sums, (size_t)2 * k * 8, hipMemcpyDeviceToHost, ctx->stream));
+   OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+   for (int j = 0; j < k; ++j) res_host[j] = std::sqrt(hs[2 * j]) / (normA * std::sqrt(hs[2 * j + 1]));
+   return kSuccess;
+}
+
+// ---------------------------------------------------------------------------
+int ofrr_arnoldi_hess_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp,
+                          const int32_t *ci, const double *va, int k, int kdim, int maxiter,
+                          double select_tol, double gtol, const double *v0, double *theta_host,
+                          double *X, int64_t ldx, int *built_out) {
+   OFRR_CHECK_CTX(ctx);
+   (void)nnz;
+   if (n <= 0 || k <= 0 || kdim <= 0 || kdim + 1 > 256 || !rp || !ci || !va || !v0 || !theta_host || !X || ldx < n)
+      return set_error(ctx, kErrorInvalidArgument, "arnoldi_hess_eig: bad arguments");
+   if (k > n) k = (int)n;
+   const int m = kdim + 1;
+   size_t need = al((size_t)n * m * 8) * 3 + al((size_t)m * m * 8) * 3 + al((size_t)m * 8) * 3 + al((size_t)m * 4);
+   int st = arena_reserve(ctx, need);
+   if (st) return st;
+   double *V = (double *)arena_take((size_t)n * m * 8);
+   double *AV = (double *)arena_take((size_t)n * m * 8);
+   double *U = (double *)arena_take((size_t)n * m * 8);
+   double *G = (double *)arena_take((size_t)m * m * 8);
+   double *H = (double *)arena_take((size_t)m * m * 8);
+   double *Y = (double *)arena_take((size_t)m * m * 8);
+   double *th = (double *)arena_take((size_t)m * 8);
+   double *piv = (double *)arena_take((size_t)m * 8);
+   double *coef = (double *)arena_take((size_t)m * 8);
+   int32_t *perm = (int32_t *)arena_take((size_t)m * 4);
+   double *hp = (double *)pinned(ctx, (size_t)m * 8 + 64);
+   if (!hp) return set_error(ctx, kErrorCUDAMemory, "arnoldi: pinned");
+   OFRR_HIP(ctx, hipMemcpyAsync(V, v0, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+   const unsigned nb = (unsigned)((n + 255) / 256);
+   int tits = 1, built = 0, have = 0;
+   while (tits < maxiter) {
+      if ((st = argmax_columns(ctx, OFRR_F64, n, 1, V, 1, 0, 0, perm, piv))) return st;
+      OFRR_HIP(ctx, hipMemcpyAsync(hp, piv, 8, hipMemcpyDeviceToHost, ctx->stream));
+      OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      if (std::fabs(hp[0]) < select_tol) break;
+      scale_vec_kernel<<<nb, 256, 0, ctx->stream>>>(n, V, piv);
+      int ncols = 1;
+      for (int i = 0; i < kdim; ++i) {
+         double *w = V + (int64_t)(i + 1) * n;
+         if ((st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, 1, V + (int64_t)i * n, OFRR_F64, 1, n, 1.0, 0.0, w, OFRR_F64, 1, n))) return st;
+         arnoldi_coef_kernel<<<1, 64, 0, ctx->stream>>>(i, n, V, w, perm, coef);
+         arnoldi_elim_kernel<<<nb, 256, 0, ctx->stream>>>(i, n, V, w, coef);
+         OFRR_LAUNCH(ctx);
+         if ((st = argmax_columns(ctx, OFRR_F64, n, 1, w, 1, 0, 0, perm + i + 1, piv + i + 1))) return st;
+         OFRR_HIP(ctx, hipMemcpyAsync(hp, piv + i + 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+         OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+         if (std::fabs(hp[0]) < select_tol) { tits = maxiter; break; }
+         scale_vec_kernel<<<nb, 256, 0, ctx->stream>>>(n, w, piv + i + 1);
+         OFRR_LAUNCH(ctx);
+         ++ncols;
+         ++tits;
+      }
+      built = ncols;
+      if ((st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, ncols, V, OFRR_F64, 1, n, 1.0, 0.0, AV, OFRR_F64, 1, n))) return st;
+      if ((st = gram(ctx, n, ncols, ncols, V, OFRR_F64, 1, n, AV, OFRR_F64, 1, n, 1.0, 0.0, H, ncols))) return st;
+      if ((st = gram(ctx, n, ncols, ncols, V, OFRR_F64, 1, n, V, OFRR_F64, 1, n, 1.0, 0.0, G, ncols))) return st;
+      st = rr_generalized(ctx, ncols, G, ncols, H, ncols, gtol, th, Y, ncols);
+      if (st == kErrorSingularMatrix) break;
+      if (st) return st;
+      {
+         const void *vb[1] = {V};
+         int dt[1] = {OFRR_F64}, kb[1] = {ncols}, ldc[1] = {ncols};
+         int64_t rs[1] = {1}, cs[1] = {n};
+         const double *cb[1] = {Y};
+         if ((st = basis_update(ctx, n, ncols, 1, vb, dt, kb, rs, cs, cb, ldc, U, OFRR_F64, 1, n))) return st;
+      }
+      have = ncols;
+      if (tits < maxiter)
+         OFRR_HIP(ctx, hipMemcpyAsync(V, U, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+   }
+   int kk = std::min(k, have);
+   if (kk > 0) {
+      OFRR_HIP(ctx, hipMemcpyAsync(hp, th, (size_t)kk * 8, hipMemcpyDeviceToHost, ctx->stream));
+      OFRR_HIP(ctx, hipMemcpy2DAsync(X, (size_t)ldx * 8, U, (size_t)n * 8, (size_t)n * 8, kk,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+   }
+   OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+   for (int c = 0; c < k; ++c) theta_host[c] = (c < kk) ? hp[c] : NAN;
+   if (built_out) *built_out = built;
+   return kSuccess;
+}
+
+// ---------------------------------------------------------------------------
+int ofrr_subspace_iter_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp,
+                           const int32_t *ci, const double *va, int m, int maxiter, int step_size,
+                           double select_tol, double gtol, double *Vout, int64_t ldv,
+                           double *theta_host) {
+   OFRR_CHECK_CTX(ctx);
+   (void)nnz;
+   if (n <= 0 || m <= 0 || m > 256 || m > n || !rp || !ci || !va || !Vout || ldv < n || !theta_host)
+      return set_error(ctx, kErrorInvalidArgument, "subspace_iter_eig: bad arguments");
+   size_t need = al((size_t)n * m * 8) * 2 + al((size_t)m * m * 8) * 4 + al((size_t)m * 8) * 2 + al((size_t)m * 4) * 2;
+   int st = arena_reserve(ctx, need);
+   if (st) return st;
+   double *T = (double *)arena_take((size_t)n * m * 8);
+   double *AV = (double *)arena_take((size_t)n * m * 8);
+   double *R = (double *)arena_take((size_t)m * m * 8);
+   double *G = (double *)arena_take((size_t)m * m * 8);
+   double *H = (double *)arena_take((size_t)m * m * 8);
+   double *Y = (double *)arena_take((size_t)m * m * 8);
+   double *th = (double *)arena_take((size_t)m * 8);
+   double *piv = (double *)arena_take((size_t)m * 8);
+   int32_t *perm = (int32_t *)arena_take((size_t)m * 4);
+   int32_t *skip = (int32_t *)arena_take((size_t)m * 4);
+   int32_t *hskip = (int32_t *)pinned(ctx, (size_t)m * 4 + (size_t)m * 8);
+   if (!hskip) return set_error(ctx, kErrorCUDAMemory, "subspace: pinned");
+   double *hth = (double *)(hskip + m);
+   OFRR_HIP(ctx, hipMemcpy2DAsync(T, (size_t)n * 8, Vout, (size_t)ldv * 8, (size_t)n * 8, m,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+   if ((st = ofrr_scalecols(ctx, OFRR_F64, n, m, T, 1, n, piv))) return st;
+   int ms = m;
+   for (int it = 0; it < maxiter; ++it) {
+      for (int s = 0; s < step_size; ++s) {
+         if ((st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, ms, T, OFRR_F64, 1, n, 1.0, 0.0, AV, OFRR_F64, 1, n))) return st;
+         std::swap(T, AV);
+         if ((st = ofrr_scalecols(ctx, OFRR_F64, n, ms, T, 1, n, piv))) return st;
+      }
+      if ((st = ofrr_hessenberg(ctx, OFRR_F64, n, ms, T, 1, n, select_tol, R, m, perm, skip))) return st;
+      OFRR_HIP(ctx, hipMemcpyAsync(hskip, skip, (size_t)ms * 4, hipMemcpyDeviceToHost, ctx->stream));
+      OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      // economic Q: kept columns only (hessenberg.m:76-78)
+      int keep = 0;
+      for (int j = 0; j < ms; ++j)
+         if (!hskip[j]) {
+            if (keep != j)
+               OFRR_HIP(ctx, hipMemcpyAsync(T + (int64_t)keep * n, T + (int64_t)j * n, (size_t)n * 8,
+                                            hipMemcpyDeviceToDevice, ctx->stream));
+            ++keep;
+         }
+      ms = keep;
+      if (ms == 0) return set_error(ctx, kErrorFactorizationFailed, "subspace_iter_eig: every column dropped");
+      if ((st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, ms, T, OFRR_F64, 1, n, 1.0, 0.0, AV, OFRR_F64, 1, n))) return st;
+      if ((st = gram(ctx, n, ms, ms, T, OFRR_F64, 1, n, AV, OFRR_F64, 1, n, 1.0, 0.0, H, ms))) return st;
+      if ((st = gram(ctx, n, ms, ms, T, OFRR_F64, 1, n, T, OFRR_F64, 1, n, 1.0, 0.0, G, ms))) return st;
+      if ((st = rr_generalized(ctx, ms, G, ms, H, ms, gtol, th, Y, ms))) return st;
+      const void *vb[1] = {T};
+      int dt[1] = {OFRR_F64}, kb[1] = {ms}, ldc[1] = {ms};
+      int64_t rs[1] = {1}, cs[1] = {n};
+      const double *cb[1] = {Y};
+      if ((st = basis_update(ctx, n, ms, 1, vb, dt, kb, rs, cs, cb, ldc, AV, OFRR_F64, 1, n))) return st;
+      std::swap(T, AV);
+   }
+   OFRR_HIP(ctx, hipMemcpy2DAsync(Vout, (size_t)ldv * 8, T, (size_t)n * 8, (size_t)n * 8, ms,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+   if (ms < m) {
+      for (int j = ms; j < m; ++j)
+         OFRR_HIP(ctx, hipMemsetAsync(Vout + (int64_t)j * ldv, 0, (size_t)n * 8, ctx->stream));
+   }
+   OFRR_HIP(ctx, hipMemcpyAsync(hth, th, (size_t)ms * 8, hipMemcpyDeviceToHost, ctx->stream));
+   OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+   for (int j = 0; j < m; ++j) theta_host[j] = (j < ms) ? hth[j] : NAN;
+   return kSuccess;
+}
+
+// ---------------------------------------------------------------------------
+int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, const int32_t *ci,
+                      const double *va, const void *va_lo, int lo_dt, const ofrr_filter_opts *o,
+                      double *Xio, double *theta_host, double *res_host, ofrr_filter_stats *stats) {
+   OFRR_CHECK_CTX(ctx);
+   (void)nnz;
+   if (!o || n <= 0 || !rp || !ci || !va || !Xio || !theta_host || !res_host)
+      return set_error(ctx, kErrorInvalidArgument, "filtered_eig: bad arguments");
+   const int k = o->k, nd = o->nd, d = o->degree, bdt = o->basis_dtype;
+   if (k < 2 || k > 128 || (k & (k - 1)) || nd < 1 || nd > 2 || d < 2 || o->nwant < 1 ||
+       o->nwant > k || !dtype_valid(bdt) || (int64_t)k > n)
+      return set_error(ctx, kErrorInvalidArgument,
+                       "filtered_eig: need k a power of two in [2,128], nd in {1,2}, degree >= 2, 1 <= nwant <= k");
+   const int m = k * (1 + nd);
+   if (m > 256) return set_error(ctx, kErrorInvalidArgument, "filtered_eig: basis dimension > 256");
+   const int zdt = (bdt == OFRR_F64) ? OFRR_F64 : OFRR_F32;
+   const void *vlo = va_lo ? va_lo : (const void *)va;
+   const int vlo_dt = va_lo ? lo_dt : OFRR_F64;
+   const size_t bsz = dtype_size(bdt), zsz = dtype_size(zdt);
+   const int mid = d / 2;
+   // ---- buffers
+   size_t need = al((size_t)n * k * 8) * 3 + al((size_t)n * k * bsz) * 5 + al((size_t)n * k * nd * zsz) +
+                 al((size_t)m * m * 8) * 2 + al((size_t)m * k * 8) + al((size_t)k * 8) * 14 +
+                 al((size_t)d * k * 8) + al((size_t)k * 4) * 2 + 4096;
+   int st = arena_reserve(ctx, need);
+   if (st) return st;
+   double *X = (double *)arena_take((size_t)n * k * 8);
+   double *Xn = (double *)arena_take((size_t)n * k * 8);
+   double *W = (double *)arena_take((size_t)n * k * 8);
+   void *Rq = arena_take((size_t)n * k * bsz);
+   void *Db[4];
+   for (int i = 0; i < 4; ++i) Db[i] = arena_take((size_t)n * k * bsz);
+   void *Z = arena_take((size_t)n * k * nd * zsz);
+   double *G = (double *)arena_take((size_t)m * m * 8);
+   double *H = (double *)arena_take((size_t)m * m * 8);
+   double *C = (double *)arena_take((size_t)m * k * 8);
+   double *th = (double *)arena_take((size_t)k * 8);
+   double *sums = (double *)arena_take((size_t)2 * k * 8);
+   double *invsR = (double *)arena_take((size_t)k * 8);
+   double *sR = (double *)arena_take((size_t)k * 8);
+   double *coef = (double *)arena_take((size_t)4 * k * 8);
+   double *scale[4];
+   for (int i = 0; i < 4; ++i) scale[i] = (double *)arena_take((size_t)k * 8);
+   double *invs = (double *)arena_take((size_t)k * 8);
+   double *sD1 = (double *)arena_take((size_t)k * 8);
+   double *zall = (double *)arena_take((size_t)d * k * 8);
+   float *cmax[2];
+   cmax[0] = (float *)arena_take((size_t)k * 4);
+   cmax[1] = (float *)arena_take((size_t)k * 4);
+   int32_t *rank = (int32_t *)arena_take(64);
+   float *cmaxR = (float *)arena_take((size_t)k * 4);
+   double *sRd = sR;
+   // pinned host mirrors
+   const size_t hbytes = (size_t)(2 * k + k + k + d * k + 4 * k + 16) * 8;
+   double *hp = (double *)pinned(ctx, hbytes);
+   if (!hp) return set_error(ctx, kErrorCUDAMemory, "filtered_eig: pinned");
+   double *h_sums = hp, *h_th = hp + 2 * k, *h_sR = h_th + k, *h_z = h_sR + k;
+   float *h_cmaxR = (float *)(h_z + (size_t)d * k);
+   int32_t *h_rank = (int32_t *)(h_cmaxR + 2 * k);
+
+   hipEvent_t ev[8];
+   for (int i = 0; i < 8; ++i) OFRR_HIP(ctx, hipEventCreate(&ev[i]));
+   float t_acc[6] = {0, 0, 0, 0, 0, 0};
+   auto rec = [&](int i) { return hipEventRecord(ev[i], ctx->stream); };
+   OFRR_HIP(ctx, rec(0));
+
+   int spmm_calls = 0, rank_min = m;
+   // ---- initial Rayleigh-Ritz on the start block
+   OFRR_HIP(ctx, hipMemcpyAsync(X, Xio, (size_t)n * k * 8, hipMemcpyDeviceToDevice, ctx->stream));
+   if ((st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, k, X, OFRR_F64, k, 1, 1.0, 0.0, W, OFRR_F64, k, 1))) return st;
+   ++spmm_calls;
+   if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, X, OFRR_F64, k, 1, 1.0, 0.0, G, k))) return st;
+   if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, W, OFRR_F64, k, 1, 1.0, 0.0, H, k))) return st;
+   if ((st = rr_truncated(ctx, k, G, k, H, k, o->tau, k, th, C, k, rank))) return st;
+   {
+      const void *vb[1] = {X};
+      int dt[1] = {OFRR_F64}, kb[1] = {k}, ldc[1] = {k};
+      int64_t rs[1] = {k}, cs[1] = {1};
+      const double *cb[1] = {C};
+      if ((st = basis_update(ctx, n, k, 1, vb, dt, kb, rs, cs, cb, ldc, Xn, OFRR_F64, k, 1))) return st;
+      std::swap(X, Xn);
+   }
+   OFRR_HIP(ctx, hipMemcpyAsync(h_th, th, (size_t)k * 8, hipMemcpyDeviceToHost, ctx->stream));
+   OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+
+   int it = 0, converged = 0, nconv = 0;
+   std::vector<double> Lth(k), res(k, NAN), theta(k);
+   // host staging kept alive across the asynchronous uploads of an iteration
+   std::vector<double> inv(k), s1(k), pred(k, 1.0);
+   std::vector<float> m1(k);
+   // quantisation targets: the stored maximum of R sits near 2^tgt_resid and
+   // the filter blocks near 2^tgt_step (FP8 E4M3 uses the top of its range)
+   const bool scaled = (bdt == OFRR_F16 || bdt == OFRR_FP8);
+   const int tgt_resid = (bdt == OFRR_FP8) ? 8 : 0;
+   const int tgt_step = (bdt == OFRR_FP8) ? 6 : 0;
+   const double lo_ratio = (bdt == OFRR_FP8) ? 16.0 : 1.0 / 256.0;
+   for (int j = 0; j < k; ++j) h_sR[j] = 1.0;
+   for (int j = 0; j < k; ++j) theta[j] = h_th[j];
+   for (it = 0;; ++it) {
+      // ---- FP64 operator application + residual, quantised R
+      OFRR_HIP(ctx, rec(1));
+      for (int pass = 0; pass < 2; ++pass) {
+         for (int j = 0; j < k; ++j) inv[j] = 1.0 / h_sR[j];
+         OFRR_HIP(ctx, hipMemcpyAsync(invsR, inv.data(), (size_t)k * 8, hipMemcpyHostToDevice, ctx->stream));
+         OFRR_HIP(ctx, hipMemcpyAsync(sRd, h_sR, (size_t)k * 8, hipMemcpyHostToDevice, ctx->stream));
+         OFRR_HIP(ctx, hipMemsetAsync(cmaxR, 0, (size_t)k * 4, ctx->stream));
+         if ((st = spmm_resid(ctx, n, rp, ci, va, k, X, th, W, Rq, bdt, invsR, cmaxR, sums))) return st;
+         ++spmm_calls;
+         OFRR_HIP(ctx, hipMemcpyAsync(h_sums, sums, (size_t)2 * k * 8, hipMemcpyDeviceToHost, ctx->stream));
+         OFRR_HIP(ctx, hipMemcpyAsync(h_cmaxR, cmaxR, (size_t)k * 4, hipMemcpyDeviceToHost, ctx->stream));
+         OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+         if (!scaled) break;  // FP64 / FP32 / BF16 hold R unscaled
+         // R was quantised with the predicted scale h_sR; redo once when the
+         // stored range left the window of the format
+         bool redo = false;
+         for (int j = 0; j < k; ++j) {
+            const double cmj = (double)h_cmaxR[j];
+            const double ratio = cmj / h_sR[j];
+            const double want = cmj > 0 ? std::exp2(std::ceil(std::log2(cmj)) - tgt_resid) : 1.0;
+            if (ratio > 0.5 * dtype_max(bdt) || ratio < lo_ratio) redo = true;
+            pred[j] = want;
+         }
+         if (!redo || pass == 1) break;
+         for (int j = 0; j < k; ++j) h_sR[j] = pred[j];
+      }
+      OFRR_HIP(ctx, rec(2));
+      const double normA = o->normA > 0 ? o->normA : theta[0];
+      nconv = 0;
+      for (int j = 0; j < k; ++j) {
+         res[j] = std::sqrt(h_sums[2 * j]) / (normA * std::sqrt(h_sums[2 * j + 1]));
+         if (j < o->nwant && res[j] <= o->tol) ++nconv;
+      }
+      {
+         float ms_;
+         OFRR_HIP(ctx, hipEventElapsedTime(&ms_, ev[1], ev[2]));
+         t_acc[5] += ms_;
+      }
+      if (nconv == o->nwant) { converged = 1; break; }
+      if (it >= o->maxiter) break;
+
+      // ---- Chebyshev filter on the correction, damping [lower, theta_k]
+      const double cut = theta[k - 1], lo = o->lower;
+      const double e = 0.5 * (cut - lo), c0 = 0.5 * (cut + lo);
+      for (int j = 0; j < k; ++j) Lth[j] = (theta[j] - c0) / e;
+      // z_j = T_j(L(theta)) for j = 1..d-1 (row j-1 of h_z)
+      for (int j = 0; j < k; ++j) {
+         double zp = 1.0, zc = Lth[j];
+         for (int s = 1; s < d; ++s) {
+            h_z[(size_t)(s - 1) * k + j] = zc;
+            double zn = 2.0 * Lth[j] * zc - zp;
+            zp = zc;
+            zc = zn;
+         }
+      }
+      OFRR_HIP(ctx, hipMemcpyAsync(zall, h_z, (size_t)(d - 1) * k * 8, hipMemcpyHostToDevice, ctx->stream));
+      // D_1 = R / e, stored in Rq: scale sR / e, range max|R| / e
+      for (int j = 0; j < k; ++j) s1[j] = h_sR[j] / e;
+      OFRR_HIP(ctx, hipMemcpyAsync(sD1, s1.data(), (size_t)k * 8, hipMemcpyHostToDevice, ctx->stream));
+      for (int j = 0; j < k; ++j) m1[j] = (float)((double)h_cmaxR[j] / e);
+      OFRR_HIP(ctx, hipMemcpyAsync(cmax[0], m1.data(), (size_t)k * 4, hipMemcpyHostToDevice, ctx->stream));
+      OFRR_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+      // recurrence state: current / previous stored block and their scales;
+      // Db[b] has scale scale[b]; D_1 lives in Rq with scale sD1
+      const void *Dc = Rq, *Dp = nullptr;
+      double *sc = sD1, *sp = nullptr;
+      int cm = 0, curIdx = -1, prevIdx = -1, snap = -1;
+      const void *Dmid = (nd == 2 && mid == 1) ? Rq : nullptr;
+      for (int s = 1; s < d; ++s) {
+         int out = -1;
+         for (int b = 0; b < 4; ++b)
+            if (b != curIdx && b != prevIdx && b != snap) { out = b; break; }
+         cheb_coef_kernel<<<(k + 63) / 64, 64, 0, ctx->stream>>>(k, e, c0, zall + (size_t)(s - 1) * k,
+                                                                 sRd, sc, sp, cmax[cm], tgt_step,
+                                                                 coef, invs, scale[out]);
+         OFRR_LAUNCH(ctx);
+         OFRR_HIP(ctx, hipMemsetAsync(cmax[cm ^ 1], 0, (size_t)k * 4, ctx->stream));
+         if ((st = cheb_step(ctx, n, rp, ci, vlo, vlo_dt, k, Dc, Dp, Rq, bdt, coef, invs, Db[out],
+                             cmax[cm ^ 1])))
+            return st;
+         ++spmm_calls;
+         prevIdx = curIdx;
+         Dp = Dc;
+         sp = sc;
+         curIdx = out;
+         Dc = Db[out];
+         sc = scale[out];
+         cm ^= 1;
+         if (nd == 2 && s + 1 == mid) { snap = out; Dmid = Db[out]; }
+      }
+      OFRR_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
+      // ---- projection: V = [X | Dd (| Dmid)], AV = [W | Z]
+      const void *Dd = Dc;
+      if ((st = spmm_csr(ctx, n, rp, ci, vlo, vlo_dt, k, Dd, bdt, k, 1, 1.0, 0.0, Z, zdt, (int64_t)k * nd, 1))) return st;
+      ++spmm_calls;
+      if (nd == 2) {
+         if ((st = spmm_csr(ctx, n, rp, ci, vlo, vlo_dt, k, Dmid, bdt, k, 1, 1.0, 0.0,
+                            (char *)Z + (size_t)k * zsz, zdt, (int64_t)k * nd, 1))) return st;
+         ++spmm_calls;
+      }
+      OFRR_HIP(ctx, hipEventRecord(ev[5], ctx->stream));
+      const int64_t zs = (int64_t)k * nd;
+      // G blocks (upper), column-major ld m
+      if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, X, OFRR_F64, k, 1, 1.0, 0.0, G, m))) return st;
+      if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, Dd, bdt, k, 1, 1.0, 0.0, G + (size_t)k * m, m))) return st;
+      if ((st = gram(ctx, n, k, k, Dd, bdt, k, 1, Dd, bdt, k, 1, 1.0, 0.0, G + k + (size_t)k * m, m))) return st;
+      if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, W, OFRR_F64, k, 1, 1.0, 0.0, H, m))) return st;
+      if ((st = gram(ctx, n, k, k, W, OFRR_F64, k, 1, Dd, bdt, k, 1, 1.0, 0.0, H + (size_t)k * m, m))) return st;
+      if ((st = gram(ctx, n, k, k, Dd, bdt, k, 1, Z, zdt, zs, 1, 1.0, 0.0, H + k + (size_t)k * m, m))) return st;
+      if (nd == 2) {
+         const size_t c2 = (size_t)2 * k * m;
+         const void *Zm = (const char *)Z + (size_t)k * zsz;
+         if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, Dmid, bdt, k, 1, 1.0, 0.0, G + c2, m))) return st;
+         if ((st = gram(ctx, n, k, k, Dd, bdt, k, 1, Dmid, bdt, k, 1, 1.0, 0.0, G + k + c2, m))) return st;
+         if ((st = gram(ctx, n, k, k, Dmid, bdt, k, 1, Dmid, bdt, k, 1, 1.0, 0.0, G + 2 * k + c2, m))) return st;
+         if ((st = gram(ctx, n, k, k, W, OFRR_F64, k, 1, Dmid, bdt, k, 1, 1.0, 0.0, H + c2, m))) return st;
+         if ((st = gram(ctx, n, k, k, Dd, bdt, k, 1, Zm, zdt, zs, 1, 1.0, 0.0, H + k + c2, m))) return st;
+         if ((st = gram(ctx, n, k, k, Dmid, bdt, k, 1, Zm, zdt, zs, 1, 1.0, 0.0, H + 2 * k + c2, m))) return st;
+      }
+      mirror_upper_kernel<<<(m * m + 255) / 256, 256, 0, ctx->stream>>>(m, G, m);
+      mirror_upper_kernel<<<(m * m + 255) / 256, 256, 0, ctx->stream>>>(m, H, m);
+      OFRR_LAUNCH(ctx);
+      OFRR_HIP(ctx, hipEventRecord(ev[6], ctx->stream));
+      if ((st = rr_truncated(ctx, m, G, m, H, m, o->tau, k, th, C, m, rank))) return st;
+      OFRR_HIP(ctx, hipEventRecord(ev[7], ctx->stream));
+      {
+         const void *vb[3] = {X, Dd, Dmid};
+         int dt[3] = {OFRR_F64, bdt, bdt}, kb[3] = {k, k, k}, ldc[3] = {m, m, m};
+         int64_t rs[3] = {k, k, k}, cs[3] = {1, 1, 1};
+         const double *cb[3] = {C, C + k, C + 2 * k};
+         if ((st = basis_update(ctx, n, k, 1 + nd, vb, dt, kb, rs, cs, cb, ldc, Xn, OFRR_F64, k, 1))) return st;
+         std::swap(X, Xn);
+      }
+      OFRR_HIP(ctx, hipMemcpyAsync(h_th, th, (size_t)k * 8, hipMemcpyDeviceToHost, ctx->stream));
+      OFRR_HIP(ctx, hipMemcpyAsync(h_rank, rank, 4, hipMemcpyDeviceToHost, ctx->stream));
+      OFRR_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+      OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      {
+         float a, b, c, dd, e2;
+         OFRR_HIP(ctx, hipEventElapsedTime(&a, ev[3], ev[4]));
+         OFRR_HIP(ctx, hipEventElapsedTime(&b, ev[4], ev[5]));
+         OFRR_HIP(ctx, hipEventElapsedTime(&c, ev[5], ev[6]));
+         OFRR_HIP(ctx, hipEventElapsedTime(&dd, ev[6], ev[7]));
+         OFRR_HIP(ctx, hipEventElapsedTime(&e2, ev[7], ev[2]));
+         t_acc[0] += a + b;
+         t_acc[1] += c;
+         t_acc[2] += dd;
+         t_acc[3] += e2;
+      }
+      rank_min = std::min(rank_min, (int)h_rank[0]);
+      if (h_rank[0] < k) {
+         st = set_error(ctx, kErrorSolverDivergence, "filtered_eig: truncated rank below the block width");
+         break;
+      }
+      for (int j = 0; j < k; ++j) theta[j] = h_th[j];
+   }
+   OFRR_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+   OFRR_HIP(ctx, hipMemcpyAsync(Xio, X, (size_t)n * k * 8, hipMemcpyDeviceToDevice, ctx->stream));
+   OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+   float total;
+   OFRR_HIP(ctx, hipEventElapsedTime(&total, ev[0], ev[2]));
+   for (int i = 0; i < 8; ++i) (void)hipEventDestroy(ev[i]);
+   for (int j = 0; j < k; ++j) {
+      theta_host[j] = theta[j];
+      res_host[j] = res[j];
+   }
+   if (stats) {
+      stats->iterations = it;
+      stats->converged = nconv;
+      stats->spmm_calls = spmm_calls;
+      stats->rank_min = rank_min;
+      stats->t_total_ms = total;
+      stats->t_spmm_ms = t_acc[0];
+      stats->t_gram_ms = t_acc[1];
+      stats->t_rr_ms = t_acc[2];
+      stats->t_update_ms = t_acc[3];
+      stats->t_resid_ms = t_acc[5];
+   }
+   if (st) return st;
+   return converged ? kSuccess : kErrorSolverMaxIterations;
+}
+
+}  // extern "C"

@@ -1,0 +1,359 @@
+// CSR SpMM kernels for gfx950 (semantics of MATLAB/+src/+csr/csrmv.m:48-59:
+// y(i,:) = sum over the stored entries p of row i of a_p * x(j_p,:)).
+//
+// Layout in HBM: the hot path keeps n x k blocks row-major (a row of k values
+// is contiguous), so one operator entry gathers one contiguous row of X.  A
+// group of G lanes of a wave64 serves one row, each lane loading 16 bytes
+// (VEC elements) of the gathered row: for k = 64 that is 8 rows per wave in
+// FP16 (128 B per row) and 2 rows per wave in FP64 (512 B per row).
+// Accumulation is FP64 for FP64 operands and FP32 otherwise; entries of a row
+// are accumulated with fused multiply-adds in storage order.  Each workgroup
+// owns a contiguous range of rows and the block index is remapped so that each
+// XCD walks a contiguous part of the matrix (its L2 then serves the +-N and
+// +-N^2 stencil neighbours).
+//
+// Kernels:
+//   spmm_rm      Y = alpha A X + beta Y
+//   spmm_resid   W = A X, R = W - X diag(theta) -> quantised R, column max |R|,
+//                per-block FP64 sums of r^2 and x^2 (residual norms)
+//   cheb_step    T = a.(A Dc) + b.Dc + g.Dp + d.R (per-column coefficients),
+//                quantised T / s, column max |T|: one Chebyshev-filter step
+//   spmm_strided generic strides (any layout), one thread per output
+#include "internal.hpp"
+
+namespace ofrr {
+
+template <typename T, int N> struct alignas(sizeof(T) * N) VecT { T v[N]; };
+
+template <typename T, int N> __device__ inline VecT<T, N> vload(const T *p) {
+   return *reinterpret_cast<const VecT<T, N> *>(p);
+}
+template <typename T, int N> __device__ inline void vstore(T *p, const VecT<T, N> &v) {
+   *reinterpret_cast<VecT<T, N> *>(p) = v;
+}
+
+// acc[v] += sum_p a_p * X(col_p, c0 + v), entries in storage order, 4 in flight
+template <typename TV, typename TX, int VEC, typename ACC>
+__device__ inline void row_gather(const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                  const TV *__restrict__ va, const TX *__restrict__ X,
+                                  int64_t ldx, int64_t row, int c0, ACC acc[VEC]) {
+   const int64_t b = rp[row], e = rp[row + 1];
+   for (int64_t p = b; p < e; p += 4) {
+      int32_t c[4];
+      ACC a[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+         bool ok = p + u < e;
+         c[u] = ok ? ci[p + u] : (int32_t)row;
+         a[u] = ok ? (ACC)Elem<TV>::load(va[p + u]) : (ACC)0;
+      }
+      VecT<TX, VEC> xv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xv[u] = vload<TX, VEC>(X + (int64_t)c[u] * ldx + c0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+         for (int v = 0; v < VEC; ++v)
+            acc[v] = fma(a[u], (ACC)Elem<TX>::load(xv[u].v[v]), acc[v]);
+   }
+}
+
+// ---------------------------------------------------------------------------
+template <typename TV, typename TX, typename TY, int VEC>
+__global__ __launch_bounds__(256) void spmm_rm_kernel(
+    int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+    const TV *__restrict__ va, int k, const TX *__restrict__ X, int64_t ldx, double alpha,
+    double beta, TY *__restrict__ Y, int64_t ldy, int lg, int64_t rows_per_block, int64_t nblk) {
+   using ACC = typename std::conditional<std::is_same<TX, double>::value, double, float>::type;
+   const int64_t blk = xcd_block(nblk);
+   if (blk < 0) return;
+   const int G = 1 << lg;
+   const int lane_g = threadIdx.x & (G - 1), grp = threadIdx.x >> lg;
+   const int GROUPS = 256 >> lg;
+   const int64_t r0 = blk * rows_per_block;
+   const int64_t r1 = min(n, r0 + rows_per_block);
+   for (int c0 = lane_g * VEC; c0 < k; c0 += G * VEC) {
+      for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
+         ACC acc[VEC];
+#pragma unroll
+         for (int v = 0; v < VEC; ++v) acc[v] = 0;
+         row_gather<TV, TX, VEC, ACC>(rp, ci, va, X, ldx, row, c0, acc);
+         TY *y = Y + row * ldy + c0;
+#pragma unroll
+         for (int v = 0; v < VEC; ++v) {
+            double r = (double)alpha * (double)acc[v];
+            if (beta != 0.0) r += beta * (double)Elem<TY>::load(y[v]);
+            y[v] = Elem<TY>::store(r);
+         }
+      }
+   }
+}
+
+// ---------------------------------------------------------------------------
+// W = A X (FP64), R = W - X theta, Rq = quantise(R * inv_sR), colmax |R|,
+// partial sums of r^2 and x^2 per block (fixed order -> deterministic)
+template <typename TB>
+__global__ __launch_bounds__(256) void spmm_resid_kernel(
+    int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+    const double *__restrict__ va, int k, const double *__restrict__ X,
+    const double *__restrict__ theta, double *__restrict__ W, TB *__restrict__ Rq,
+    const double *__restrict__ inv_sR, float *__restrict__ colmax, double *__restrict__ part,
+    int lg, int64_t rows_per_block, int64_t nblk) {
+   constexpr int VEC = 2;
+   __shared__ double red[256][VEC * 2];
+   __shared__ float redm[256][VEC];
+   const int64_t blk = xcd_block(nblk);
+   if (blk < 0) return;
+   const int G = 1 << lg;
+   const int GROUPS = 256 >> lg;
+   const int lane_g = threadIdx.x & (G - 1), grp = threadIdx.x >> lg;
+   const int c0 = lane_g * VEC;  // host guarantees k == G * VEC
+   const int64_t r0 = blk * rows_per_block;
+   const int64_t r1 = min(n, r0 + rows_per_block);
+   double th[VEC], isr[VEC], s_r[VEC] = {0, 0}, s_x[VEC] = {0, 0};
+   float mx[VEC] = {0.f, 0.f};
+#pragma unroll
+   for (int v = 0; v < VEC; ++v) { th[v] = theta[c0 + v]; isr[v] = inv_sR[c0 + v]; }
+   for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
+      double acc[VEC] = {0.0, 0.0};
+      row_gather<double, double, VEC, double>(rp, ci, va, X, k, row, c0, acc);
+      VecT<double, VEC> xv = vload<double, VEC>(X + row * k + c0);
+      VecT<double, VEC> wv;
+      VecT<TB, VEC> rq;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+         wv.v[v] = acc[v];
+         double r = acc[v] - th[v] * xv.v[v];
+         s_r[v] = fma(r, r, s_r[v]);
+         s_x[v] = fma(xv.v[v], xv.v[v], s_x[v]);
+         mx[v] = fmaxf(mx[v], (float)fabs(r));
+         rq.v[v] = Elem<TB>::store(r * isr[v]);
+      }
+      vstore<double, VEC>(W + row * k + c0, wv);
+      vstore<TB, VEC>(Rq + row * k + c0, rq);
+   }
+#pragma unroll
+   for (int v = 0; v < VEC; ++v) {
+      red[threadIdx.x][2 * v] = s_r[v];
+      red[threadIdx.x][2 * v + 1] = s_x[v];
+      redm[threadIdx.x][v] = mx[v];
+   }
+   __syncthreads();
+   // column c = lane * VEC + v; sums over the row groups in a fixed order
+   for (int t = threadIdx.x; t < G * VEC * 2; t += blockDim.x) {
+      const int c = t >> 1, which = t & 1, lane = c / VEC, v = c % VEC;
+      double s = 0.0;
+      for (int g = 0; g < GROUPS; ++g) s += red[g * G + lane][2 * v + which];
+      part[blk * (int64_t)(2 * k) + t] = s;  // [col*2 + {0: r^2, 1: x^2}]
+   }
+   for (int t = threadIdx.x; t < G * VEC; t += blockDim.x) {
+      const int lane = t / VEC, v = t % VEC;
+      float m = 0.f;
+      for (int g = 0; g < GROUPS; ++g) m = fmaxf(m, redm[g * G + lane][v]);
+      atomic_max_nonneg(colmax + t, m);
+   }
+}
+
+// ---------------------------------------------------------------------------
+// one Chebyshev step on the correction block:
+// T = ca.(A Dc) + cb.Dc + cg.Dp + cd.R,  Dn = quantise(T * inv_s), colmax |T|
+template <typename TV, typename TB, int VEC>
+__global__ __launch_bounds__(256) void cheb_step_kernel(
+    int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+    const TV *__restrict__ va, int k, const TB *__restrict__ Dc, const TB *__restrict__ Dp,
+    const TB *__restrict__ R, const double *__restrict__ coef /* 4 x k */,
+    const double *__restrict__ inv_s, TB *__restrict__ Dn, float *__restrict__ colmax, int lg,
+    int64_t rows_per_block, int64_t nblk) {
+   using ACC = typename std::conditional<std::is_same<TB, double>::value, double, float>::type;
+   __shared__ float redm[256][VEC];
+   const int64_t blk = xcd_block(nblk);
+   if (blk < 0) return;
+   const int G = 1 << lg;
+   const int GROUPS = 256 >> lg;
+   const int lane_g = threadIdx.x & (G - 1), grp = threadIdx.x >> lg;
+   const int c0 = lane_g * VEC;  // host guarantees k == G * VEC
+   const int64_t r0 = blk * rows_per_block;
+   const int64_t r1 = min(n, r0 + rows_per_block);
+   ACC ca[VEC], cb[VEC], cg[VEC], cd[VEC], is[VEC];
+   float mx[VEC];
+#pragma unroll
+   for (int v = 0; v < VEC; ++v) {
+      ca[v] = (ACC)coef[c0 + v];
+      cb[v] = (ACC)coef[k + c0 + v];
+      cg[v] = (ACC)coef[2 * k + c0 + v];
+      cd[v] = (ACC)coef[3 * k + c0 + v];
+      is[v] = (ACC)inv_s[c0 + v];
+      mx[v] = 0.f;
+   }
+   for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
+      ACC acc[VEC];
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) acc[v] = 0;
+      row_gather<TV, TB, VEC, ACC>(rp, ci, va, Dc, k, row, c0, acc);
+      VecT<TB, VEC> dc = vload<TB, VEC>(Dc + row * k + c0);
+      VecT<TB, VEC> rr = vload<TB, VEC>(R + row * k + c0);
+      VecT<TB, VEC> dp;
+      if (Dp) dp = vload<TB, VEC>(Dp + row * k + c0);
+      VecT<TB, VEC> out;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+         ACC t = ca[v] * acc[v];
+         t = fma(cb[v], (ACC)Elem<TB>::load(dc.v[v]), t);
+         if (Dp) t = fma(cg[v], (ACC)Elem<TB>::load(dp.v[v]), t);
+         t = fma(cd[v], (ACC)Elem<TB>::load(rr.v[v]), t);
+         mx[v] = fmaxf(mx[v], (float)fabs((double)t));
+         out.v[v] = Elem<TB>::store((double)(t * is[v]));
+      }
+      vstore<TB, VEC>(Dn + row * k + c0, out);
+   }
+#pragma unroll
+   for (int v = 0; v < VEC; ++v) redm[threadIdx.x][v] = mx[v];
+   __syncthreads();
+   for (int t = threadIdx.x; t < G * VEC; t += blockDim.x) {
+      const int lane = t / VEC, v = t % VEC;
+      float m = 0.f;
+      for (int g = 0; g < GROUPS; ++g) m = fmaxf(m, redm[g * G + lane][v]);
+      atomic_max_nonneg(colmax + t, m);
+   }
+}
+
+// ---------------------------------------------------------------------------
+template <typename TV, typename TX, typename TY>
+__global__ void spmm_strided_kernel(int64_t n, const int64_t *__restrict__ rp,
+                                    const int32_t *__restrict__ ci, const TV *__restrict__ va,
+                                    int k, const TX *__restrict__ X, int64_t xrs, int64_t xcs,
+                                    double alpha, double beta, TY *__restrict__ Y, int64_t yrs,
+                                    int64_t ycs) {
+   using ACC = typename std::conditional<std::is_same<TX, double>::value, double, float>::type;
+   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= n * (int64_t)k) return;
+   int64_t row = t % n;
+   int j = (int)(t / n);
+   ACC acc = 0;
+   for (int64_t p = rp[row]; p < rp[row + 1]; ++p)
+      acc = fma((ACC)Elem<TV>::load(va[p]), (ACC)Elem<TX>::load(X[(int64_t)ci[p] * xrs + j * xcs]), acc);
+   TY *y = Y + row * yrs + (int64_t)j * ycs;
+   double r = alpha * (double)acc;
+   if (beta != 0.0) r += beta * (double)Elem<TY>::load(*y);
+   *y = Elem<TY>::store(r);
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static inline bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+template <typename TV, typename TX, typename TY>
+static int launch_spmm(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci,
+                       const TV *va, int k, const TX *X, int64_t xrs, int64_t xcs, double alpha,
+                       double beta, TY *Y, int64_t yrs, int64_t ycs) {
+   constexpr int VEC = 16 / sizeof(TX) > 0 ? 16 / sizeof(TX) : 1;
+   bool fast = xcs == 1 && ycs == 1 && k % VEC == 0 && aligned16(X) &&
+               (xrs * (int64_t)sizeof(TX)) % 16 == 0;
+   if (fast) {
+      int groups = k / VEC;
+      int G = 1;
+      while (G < groups && G < 64) G <<= 1;
+      int64_t rows_per_block = 256;
+      int64_t nblk = (n + rows_per_block - 1) / rows_per_block;
+      dim3 grid((unsigned)xcd_grid(nblk));
+      int lg = 0;
+      while ((1 << lg) < G) ++lg;
+      spmm_rm_kernel<TV, TX, TY, VEC><<<grid, 256, 0, ctx->stream>>>(
+          n, rp, ci, va, k, X, xrs, alpha, beta, Y, yrs, lg, rows_per_block, nblk);
+   } else {
+      int64_t total = n * (int64_t)k;
+      spmm_strided_kernel<TV, TX, TY><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(
+          n, rp, ci, va, k, X, xrs, xcs, alpha, beta, Y, yrs, ycs);
+   }
+   OFRR_LAUNCH(ctx);
+   return kSuccess;
+}
+
+int spmm_csr(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const void *va,
+             int vdt, int k, const void *X, int xdt, int64_t xrs, int64_t xcs, double alpha,
+             double beta, void *Y, int ydt, int64_t yrs, int64_t ycs) {
+   if (vdt != OFRR_F64 && vdt != OFRR_F32 && vdt != OFRR_F16)
+      return set_error(ctx, kErrorBLASUnsupportedType, "spmm: operator values must be FP64, FP32 or FP16");
+   if (ydt == OFRR_FP8)
+      return set_error(ctx, kErrorBLASUnsupportedType, "spmm: FP8 output is produced by the filter step only");
+   return dispatch_dtype(vdt, [&](auto tv) -> int {
+      using TV = decltype(tv);
+      return dispatch_dtype(xdt, [&](auto tx) -> int {
+         using TX = decltype(tx);
+         return dispatch_dtype(ydt, [&](auto ty) -> int {
+            using TY = decltype(ty);
+            return launch_spmm<TV, TX, TY>(ctx, n, rp, ci, (const TV *)va, k, (const TX *)X, xrs,
+                                           xcs, alpha, beta, (TY *)Y, yrs, ycs);
+         });
+      });
+   });
+}
+
+int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const double *va,
+               int k, const double *X, const double *theta, double *W, void *Rq, int rdt,
+               const double *inv_sR, float *colmax, double *sums /* 2k */) {
+   if (k % 2 != 0 || k > 128) return set_error(ctx, kErrorInvalidArgument, "spmm_resid: k must be even and <= 128");
+   int G = k / 2;
+   if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16 && G != 32 && G != 64)
+      return set_error(ctx, kErrorInvalidArgument, "spmm_resid: k/2 must be a power of two");
+   int64_t rows_per_block = 512;
+   int64_t nblk = (n + rows_per_block - 1) / rows_per_block;
+   double *part = (double *)workspace(ctx, (size_t)nblk * 2 * k * sizeof(double));
+   if (!part) return set_error(ctx, kErrorCUDAMemory, "spmm_resid: workspace");
+   dim3 grid((unsigned)xcd_grid(nblk));
+   int st = dispatch_dtype(rdt, [&](auto tb) -> int {
+      using TB = decltype(tb);
+      int lg = 0;
+      while ((1 << lg) < G) ++lg;
+      spmm_resid_kernel<TB><<<grid, 256, 0, ctx->stream>>>(n, rp, ci, va, k, X, theta, W, (TB *)Rq,
+                                                           inv_sR, colmax, part, lg,
+                                                           rows_per_block, nblk);
+      OFRR_LAUNCH(ctx);
+      return kSuccess;
+   });
+   if (st) return st;
+   return reduce_partials(ctx, nblk, 2 * k, part, sums);
+}
+
+int cheb_step(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const void *va,
+              int vdt, int k, const void *Dc, const void *Dp, const void *R, int bdt,
+              const double *coef, const double *inv_s, void *Dn, float *colmax) {
+   return dispatch_dtype(vdt, [&](auto tv) -> int {
+      using TV = decltype(tv);
+      return dispatch_dtype(bdt, [&](auto tb) -> int {
+         using TB = decltype(tb);
+         constexpr int VEC = 16 / sizeof(TB);
+         if (k % VEC != 0) return set_error(ctx, kErrorInvalidArgument, "cheb_step: k not a multiple of the vector width");
+         int G = k / VEC;
+         if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16 && G != 32 && G != 64)
+            return set_error(ctx, kErrorInvalidArgument, "cheb_step: k / vector width must be a power of two <= 64");
+         int64_t rows_per_block = std::is_same<TB, double>::value ? 256 : 512;
+         int64_t nblk = (n + rows_per_block - 1) / rows_per_block;
+         dim3 grid((unsigned)xcd_grid(nblk));
+         int lg = 0;
+         while ((1 << lg) < G) ++lg;
+         cheb_step_kernel<TV, TB, VEC><<<grid, 256, 0, ctx->stream>>>(
+             n, rp, ci, (const TV *)va, k, (const TB *)Dc, (const TB *)Dp, (const TB *)R, coef,
+             inv_s, (TB *)Dn, colmax, lg, rows_per_block, nblk);
+         OFRR_LAUNCH(ctx);
+         return kSuccess;
+      });
+   });
+}
+
+}  // namespace ofrr
+
+using namespace ofrr;
+
+extern "C" int ofrr_spmm_csr(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp,
+                             const int32_t *ci, const void *va, int vdt, int k, const void *X,
+                             int xdt, int64_t xrs, int64_t xcs, double alpha, double beta,
+                             void *Y, int ydt, int64_t yrs, int64_t ycs) {
+   OFRR_CHECK_CTX(ctx);
+   if (n < 0 || k < 0 || nnz < 0) return set_error(ctx, kErrorInvalidArgument, "spmm_csr: negative size");
+   if (n == 0 || k == 0) return kSuccess;
+   if (!rp || !ci || !va || !X || !Y) return set_error(ctx, kErrorInvalidArgument, "spmm_csr: null pointer");
+   if (xrs < 1 && xcs < 1) return set_error(ctx, kErrorInvalidArgument, "spmm_csr: bad X strides");
+   return spmm_csr(ctx, n, rp, ci, va, vdt, k, X, xdt, xrs, xcs, alpha, beta, Y, ydt, yrs, ycs);
+}

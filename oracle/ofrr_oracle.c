@@ -1,0 +1,661 @@
+/*
+ * ofrr_oracle.c -- CPU FP64 restatement of the OFRR projection pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Nothing in the product links it.
+ *
+ * What pins each function (tests/test_oracle_*.py):
+ *   orc_philox4x32_10   Random123 published known-answer vectors (Philox4x32-10)
+ *   orc_randn/randu     moments + Kolmogorov-Smirnov against the normal/uniform laws
+ *   orc_stencil_csr     closed-form nnz counts (cfg1 20,224 / cfg2 14,581,760), symmetry,
+ *                       sine-mode eigenvectors A s = lambda s (closed form), scipy kron build
+ *   orc_spmm_csr        brute-force dense A X, integer-valued exact results
+ *   orc_scalecols       hand cases incl. ties and zero columns
+ *   orc_iamax           reference KATs mvops_test.cpp:774-850 (index 72345 -> -8.0, 1-based)
+ *   orc_matrix_iamax    reference KATs mvops_test.cpp:1151-1497 (0-based)
+ *   orc_gemm_tn/nn      reference gemm KAT mvops_test.cpp:455-721 and numpy on random data
+ *   orc_hessenberg      A = QR, Q(perm(i),i) = 1, Q(perm(j),i) = 0 (j<i), |Q| <= 1, and the
+ *                       duplicated-column skip pattern of hessenberg_test.cpp:73-91
+ *   orc_syev            reference KAT mvops_test.cpp:2292-2447 (tridiag(-1,2,-1), n = 10)
+ *   orc_sygv            reference KAT mvops_test.cpp:2450-2626 (B = diag(1..10))
+ *   orc_rr_truncated    exact Ritz values for sine-mode bases, planted rank drop
+ *   orc_rr_generalized  scipy.linalg.eigh(H, G) on random SPD pencils
+ *   orc_arnoldi_hess_eig exact termination: v0 = sum of j sine modes -> closed-form eigenvalues
+ *   orc_subspace_iter_eig / orc_filtered_ofrr  closed-form Laplacian spectra, residual bound
+ *   orc_residuals       hand-computed residual of exact eigenpairs (zero) and perturbed ones
+ */
+#include "ofrr_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------- */
+/* Philox4x32-10                                                              */
+/* ------------------------------------------------------------------------- */
+static inline void mulhilo32(uint32_t a, uint32_t b, uint32_t *hi, uint32_t *lo) {
+   uint64_t p = (uint64_t)a * (uint64_t)b;
+   *hi = (uint32_t)(p >> 32);
+   *lo = (uint32_t)p;
+}
+
+void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+   const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+   const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+   uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+   uint32_t k0 = key_in[0], k1 = key_in[1];
+   for (int r = 0; r < 10; ++r) {
+      uint32_t hi0, lo0, hi1, lo1;
+      mulhilo32(M0, c0, &hi0, &lo0);
+      mulhilo32(M1, c2, &hi1, &lo1);
+      uint32_t n0 = hi1 ^ c1 ^ k0;
+      uint32_t n2 = hi0 ^ c3 ^ k1;
+      c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+      k0 += W0; k1 += W1;
+   }
+   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* block p of stream (seed, stream): counter (p_lo, p_hi, stream, 0), key (seed_lo, seed_hi) */
+static void philox_block(uint64_t seed, uint32_t stream, uint64_t p, uint32_t out[4]) {
+   uint32_t ctr[4] = {(uint32_t)p, (uint32_t)(p >> 32), stream, 0u};
+   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+   orc_philox4x32_10(ctr, key, out);
+}
+
+/* 53-bit uniform in [0,1) from two words */
+static inline double u53(uint32_t a, uint32_t b) {
+   return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+void orc_randn(uint64_t seed, uint32_t stream, uint64_t offset, int64_t n, double *out) {
+   const double two_pi = 6.283185307179586476925286766559;
+   for (int64_t i = 0; i < n; ++i) {
+      uint64_t e = offset + (uint64_t)i;
+      uint32_t w[4];
+      philox_block(seed, stream, e >> 1, w);
+      double u1 = u53(w[0], w[1]);
+      double u2 = u53(w[2], w[3]);
+      double r = sqrt(-2.0 * log(1.0 - u1)); /* 1-u1 in (0,1] */
+      double a = two_pi * u2;
+      out[i] = (e & 1) ? r * sin(a) : r * cos(a);
+   }
+}
+
+void orc_randu(uint64_t seed, uint32_t stream, uint64_t offset, int64_t n, double *out) {
+   for (int64_t i = 0; i < n; ++i) {
+      uint64_t e = offset + (uint64_t)i;
+      uint32_t w[4];
+      philox_block(seed, stream, e >> 1, w);
+      out[i] = (e & 1) ? u53(w[2], w[3]) : u53(w[0], w[1]);
+   }
+}
+
+/* ------------------------------------------------------------------------- */
+/* stencil operators                                                          */
+/* ------------------------------------------------------------------------- */
+int64_t orc_stencil_nnz(int dim, int64_t N) {
+   if (dim == 2) return 5 * N * N - 4 * N;
+   return 7 * N * N * N - 6 * N * N;
+}
+
+/* coefficient of the face between cells c-1 and c along direction d (c = 0..N),
+ * other coordinates (o1, o2): index c + (N+1)*(o1 + N*o2) of stream d. */
+static double face_coef(uint64_t seed, int d, int64_t N, int64_t c, int64_t o1, int64_t o2) {
+   if (seed == 0) return 1.0;
+   uint64_t idx = (uint64_t)(c + (N + 1) * (o1 + N * o2));
+   double u;
+   orc_randu(seed, (uint32_t)d, idx, 1, &u);
+   return exp((2.0 * u - 1.0) * 2.302585092994045684017991454684);
+}
+
+void orc_stencil_csr(int dim, int64_t N, uint64_t seed, int64_t *row_ptr, int32_t *col_idx,
+                     double *vals) {
+   int64_t n = (dim == 2) ? N * N : N * N * N;
+   int64_t pos = 0;
+   for (int64_t p = 0; p < n; ++p) {
+      int64_t x = p % N, y = (p / N) % N, z = (dim == 3) ? p / (N * N) : 0;
+      /* faces: lower/upper along x, y, z; the "other" coordinates per direction */
+      double kxl = face_coef(seed, 0, N, x, y, z), kxu = face_coef(seed, 0, N, x + 1, y, z);
+      double kyl = face_coef(seed, 1, N, y, x, z), kyu = face_coef(seed, 1, N, y + 1, x, z);
+      double kzl = 0, kzu = 0;
+      if (dim == 3) {
+         kzl = face_coef(seed, 2, N, z, x, y);
+         kzu = face_coef(seed, 2, N, z + 1, x, y);
+      }
+      double diag = kxl + kxu + kyl + kyu + kzl + kzu;
+      row_ptr[p] = pos;
+      if (dim == 3 && z > 0) { col_idx[pos] = (int32_t)(p - N * N); vals[pos++] = -kzl; }
+      if (y > 0) { col_idx[pos] = (int32_t)(p - N); vals[pos++] = -kyl; }
+      if (x > 0) { col_idx[pos] = (int32_t)(p - 1); vals[pos++] = -kxl; }
+      col_idx[pos] = (int32_t)p; vals[pos++] = diag;
+      if (x < N - 1) { col_idx[pos] = (int32_t)(p + 1); vals[pos++] = -kxu; }
+      if (y < N - 1) { col_idx[pos] = (int32_t)(p + N); vals[pos++] = -kyu; }
+      if (dim == 3 && z < N - 1) { col_idx[pos] = (int32_t)(p + N * N); vals[pos++] = -kzu; }
+   }
+   row_ptr[n] = pos;
+}
+
+/* ------------------------------------------------------------------------- */
+/* kernels                                                                    */
+/* ------------------------------------------------------------------------- */
+void orc_spmm_csr(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *vals,
+                  int k, const double *X, int64_t ldx, double *Y, int64_t ldy) {
+#pragma omp parallel for schedule(static)
+   for (int64_t i = 0; i < n; ++i) {
+      for (int j = 0; j < k; ++j) {
+         double acc = 0.0;
+         for (int64_t p = row_ptr[i]; p < row_ptr[i + 1]; ++p)
+            acc = fma(vals[p], X[(int64_t)col_idx[p] + j * ldx], acc);
+         Y[i + j * ldy] = acc;
+      }
+   }
+}
+
+int orc_iamax(int64_t n, const double *x, double *val) {
+   int64_t best = 0;
+   double bv = -1.0;
+   for (int64_t i = 0; i < n; ++i) {
+      double a = fabs(x[i]);
+      if (a > bv) { bv = a; best = i; } /* strict: first index on ties */
+   }
+   if (val) *val = (n > 0) ? x[best] : 0.0;
+   return (int)(best + 1);
+}
+
+void orc_matrix_iamax(int64_t m, int n, const double *A, int64_t lda, int *idx) {
+   for (int j = 0; j < n; ++j) idx[j] = orc_iamax(m, A + (int64_t)j * lda, NULL) - 1;
+}
+
+void orc_scalecols(int64_t m, int n, double *A, int64_t lda) {
+   for (int j = 0; j < n; ++j) {
+      double *a = A + (int64_t)j * lda;
+      double t;
+      orc_iamax(m, a, &t);
+      if (t != 0.0)
+         for (int64_t i = 0; i < m; ++i) a[i] = a[i] / t;
+   }
+}
+
+void orc_gemm_tn(int64_t n, int k1, int k2, const double *V, int64_t ldv, const double *W,
+                 int64_t ldw, double *G, int ldg) {
+#pragma omp parallel for collapse(2) schedule(static)
+   for (int a = 0; a < k1; ++a)
+      for (int b = 0; b < k2; ++b) {
+         const double *v = V + (int64_t)a * ldv, *w = W + (int64_t)b * ldw;
+         double acc = 0.0;
+         for (int64_t i = 0; i < n; ++i) acc = fma(v[i], w[i], acc);
+         G[a + (int64_t)b * ldg] = acc;
+      }
+}
+
+void orc_gemm_nn(int64_t m, int n, int k, const double *A, int64_t lda, const double *B,
+                 int64_t ldb, double *C, int64_t ldc) {
+#pragma omp parallel for schedule(static)
+   for (int64_t i = 0; i < m; ++i)
+      for (int j = 0; j < n; ++j) {
+         double acc = 0.0;
+         for (int p = 0; p < k; ++p) acc = fma(A[i + p * lda], B[p + j * ldb], acc);
+         C[i + j * ldc] = acc;
+      }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Hessenberg process (hessenberg.m:55-74; C++ v1 keeps the unscaled residual */
+/* in a dropped column, hessenberg.cu:515-522)                                 */
+/* ------------------------------------------------------------------------- */
+void orc_hessenberg(int64_t m, int n, double *Q, int64_t ldq, double *R, int ldr,
+                    double select_tol, int variant, int *select, int *perm) {
+   for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) R[i + (int64_t)j * ldr] = 0.0;
+   for (int i = 0; i < n; ++i) {
+      double *v = Q + (int64_t)i * ldq;
+      for (int j = 0; j < i; ++j) {
+         if (!select[j] && variant == 0) continue;
+         if (!select[j]) continue; /* a dropped column has no pivot row */
+         double c = v[perm[j]];
+         R[j + (int64_t)i * ldr] = c;
+         const double *q = Q + (int64_t)j * ldq;
+         for (int64_t r = 0; r < m; ++r) v[r] = fma(-c, q[r], v[r]);
+      }
+      double piv;
+      int k1 = orc_iamax(m, v, &piv);
+      R[i + (int64_t)i * ldr] = piv;
+      if (fabs(piv) < select_tol) {
+         select[i] = 0;
+         perm[i] = -1;
+      } else {
+         select[i] = 1;
+         perm[i] = k1 - 1;
+         for (int64_t r = 0; r < m; ++r) v[r] = v[r] / piv;
+      }
+   }
+}
+
+/* ------------------------------------------------------------------------- */
+/* symmetric eigensolver: cyclic Jacobi with the Demmel-Veselic threshold      */
+/* ------------------------------------------------------------------------- */
+static int jacobi_eig(int n, double *a /* n x n, destroyed */, double *w, double *V, int ldv) {
+   for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) V[i + (int64_t)j * ldv] = (i == j) ? 1.0 : 0.0;
+   int sweep;
+   for (sweep = 0; sweep < 80; ++sweep) {
+      int rotated = 0;
+      for (int p = 0; p < n - 1; ++p)
+         for (int q = p + 1; q < n; ++q) {
+            double apq = a[p + q * n];
+            double app = a[p + p * n], aqq = a[q + q * n];
+            if (apq == 0.0) continue;
+            if (fabs(apq) <= DBL_EPSILON * 0.5 * sqrt(fabs(app) * fabs(aqq))) {
+               a[p + q * n] = a[q + p * n] = 0.0;
+               continue;
+            }
+            rotated = 1;
+            double th = (aqq - app) / (2.0 * apq);
+            double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+            double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+            for (int r = 0; r < n; ++r) { /* columns p, q */
+               double arp = a[r + p * n], arq = a[r + q * n];
+               a[r + p * n] = c * arp - s * arq;
+               a[r + q * n] = s * arp + c * arq;
+            }
+            for (int r = 0; r < n; ++r) { /* rows p, q */
+               double apr = a[p + r * n], aqr = a[q + r * n];
+               a[p + r * n] = c * apr - s * aqr;
+               a[q + r * n] = s * apr + c * aqr;
+            }
+            a[p + q * n] = a[q + p * n] = 0.0;
+            for (int r = 0; r < n; ++r) {
+               double vrp = V[r + (int64_t)p * ldv], vrq = V[r + (int64_t)q * ldv];
+               V[r + (int64_t)p * ldv] = c * vrp - s * vrq;
+               V[r + (int64_t)q * ldv] = s * vrp + c * vrq;
+            }
+         }
+      if (!rotated) break;
+   }
+   for (int i = 0; i < n; ++i) w[i] = a[i + i * n];
+   /* sort ascending (stable selection), permute vectors */
+   double *tmp = (double *)malloc(sizeof(double) * (size_t)n);
+   for (int i = 0; i < n - 1; ++i) {
+      int mi = i;
+      for (int j = i + 1; j < n; ++j)
+         if (w[j] < w[mi]) mi = j;
+      if (mi != i) {
+         double t = w[i]; w[i] = w[mi]; w[mi] = t;
+         memcpy(tmp, V + (int64_t)i * ldv, sizeof(double) * (size_t)n);
+         memcpy(V + (int64_t)i * ldv, V + (int64_t)mi * ldv, sizeof(double) * (size_t)n);
+         memcpy(V + (int64_t)mi * ldv, tmp, sizeof(double) * (size_t)n);
+      }
+   }
+   free(tmp);
+   return sweep;
+}
+
+int orc_syev(int n, const double *A, int lda, double *w, double *V, int ldv) {
+   double *a = (double *)malloc(sizeof(double) * (size_t)n * n);
+   for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) /* symmetrise from the upper triangle ('U') */
+         a[i + j * n] = (i <= j) ? A[i + (int64_t)j * lda] : A[j + (int64_t)i * lda];
+   jacobi_eig(n, a, w, V, ldv);
+   free(a);
+   return 0;
+}
+
+/* Cholesky B = L L^T (lower, in place on a copy). returns 0 or the failing order */
+static int cholesky(int n, double *L) {
+   for (int j = 0; j < n; ++j) {
+      double d = L[j + j * n];
+      for (int p = 0; p < j; ++p) d -= L[j + p * n] * L[j + p * n];
+      if (!(d > 0.0)) return j + 1;
+      d = sqrt(d);
+      L[j + j * n] = d;
+      for (int i = j + 1; i < n; ++i) {
+         double s = L[i + j * n];
+         for (int p = 0; p < j; ++p) s -= L[i + p * n] * L[j + p * n];
+         L[i + j * n] = s / d;
+      }
+      for (int i = 0; i < j; ++i) L[i + j * n] = 0.0;
+   }
+   return 0;
+}
+
+int orc_sygv(int n, const double *A, int lda, const double *B, int ldb, double *w, double *V,
+             int ldv) {
+   double *L = (double *)malloc(sizeof(double) * (size_t)n * n);
+   double *C = (double *)malloc(sizeof(double) * (size_t)n * n);
+   double *Y = (double *)malloc(sizeof(double) * (size_t)n * n);
+   for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+         L[i + j * n] = (i >= j) ? B[j + (int64_t)i * ldb] : B[i + (int64_t)j * ldb];
+         C[i + j * n] = (i <= j) ? A[i + (int64_t)j * lda] : A[j + (int64_t)i * lda];
+      }
+   int info = cholesky(n, L);
+   if (info) { free(L); free(C); free(Y); return info; }
+   /* C <- L^{-1} C L^{-T}: forward-substitute columns, then rows */
+   for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+         double s = C[i + j * n];
+         for (int p = 0; p < i; ++p) s -= L[i + p * n] * C[p + j * n];
+         C[i + j * n] = s / L[i + i * n];
+      }
+   for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+         double s = C[i + j * n];
+         for (int p = 0; p < j; ++p) s -= L[j + p * n] * C[i + p * n];
+         C[i + j * n] = s / L[j + j * n];
+      }
+   for (int j = 0; j < n; ++j) /* exact symmetry */
+      for (int i = j + 1; i < n; ++i) {
+         double s = 0.5 * (C[i + j * n] + C[j + i * n]);
+         C[i + j * n] = C[j + i * n] = s;
+      }
+   jacobi_eig(n, C, w, Y, n);
+   /* V = L^{-T} Y: back-substitution */
+   for (int c = 0; c < n; ++c)
+      for (int i = n - 1; i >= 0; --i) {
+         double s = Y[i + c * n];
+         for (int p = i + 1; p < n; ++p) s -= L[p + i * n] * V[p + (int64_t)c * ldv];
+         V[i + (int64_t)c * ldv] = s / L[i + i * n];
+      }
+   free(L); free(C); free(Y);
+   return 0;
+}
+
+int orc_rr_truncated(int m, const double *G, int ldg, const double *H, int ldh, double tau,
+                     int nkeep, double *theta, double *Y, int ldy) {
+   double *sc = (double *)malloc(sizeof(double) * (size_t)m);
+   double *Gs = (double *)malloc(sizeof(double) * (size_t)m * m);
+   double *Hs = (double *)malloc(sizeof(double) * (size_t)m * m);
+   double *U = (double *)malloc(sizeof(double) * (size_t)m * m);
+   double *lam = (double *)malloc(sizeof(double) * (size_t)m);
+   for (int i = 0; i < m; ++i) {
+      double d = G[i + (int64_t)i * ldg];
+      sc[i] = d > 0 ? 1.0 / sqrt(d) : 0.0;
+   }
+   for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) {
+         double g = 0.5 * (G[i + (int64_t)j * ldg] + G[j + (int64_t)i * ldg]);
+         double h = 0.5 * (H[i + (int64_t)j * ldh] + H[j + (int64_t)i * ldh]);
+         Gs[i + j * m] = g * sc[i] * sc[j];
+         Hs[i + j * m] = h * sc[i] * sc[j];
+      }
+   jacobi_eig(m, Gs, lam, U, m);
+   double lmax = lam[m - 1];
+   int r = 0;
+   for (int i = 0; i < m; ++i)
+      if (lam[i] > tau * lmax) ++r;
+   /* B (m x r) = U_r L_r^{-1/2}, the r largest */
+   double *B = (double *)malloc(sizeof(double) * (size_t)m * (r ? r : 1));
+   for (int c = 0; c < r; ++c) {
+      int src = m - r + c;
+      double f = 1.0 / sqrt(lam[src]);
+      for (int i = 0; i < m; ++i) B[i + c * m] = U[i + src * m] * f;
+   }
+   /* Hr = B^T Hs B */
+   double *T = (double *)malloc(sizeof(double) * (size_t)m * (r ? r : 1));
+   double *Hr = (double *)malloc(sizeof(double) * (size_t)(r ? r : 1) * (r ? r : 1));
+   double *Z = (double *)malloc(sizeof(double) * (size_t)(r ? r : 1) * (r ? r : 1));
+   double *th = (double *)malloc(sizeof(double) * (size_t)(r ? r : 1));
+   orc_gemm_nn(m, r, m, Hs, m, B, m, T, m);
+   orc_gemm_tn(m, r, r, B, m, T, m, Hr, r);
+   for (int j = 0; j < r; ++j)
+      for (int i = j + 1; i < r; ++i) {
+         double s = 0.5 * (Hr[i + j * r] + Hr[j + i * r]);
+         Hr[i + j * r] = Hr[j + i * r] = s;
+      }
+   if (r > 0) jacobi_eig(r, Hr, th, Z, r);
+   int nout = nkeep < r ? nkeep : r;
+   for (int c = 0; c < nout; ++c) {
+      int src = r - 1 - c; /* descending */
+      theta[c] = th[src];
+      for (int i = 0; i < m; ++i) {
+         double s = 0.0;
+         for (int p = 0; p < r; ++p) s += B[i + p * m] * Z[p + src * r];
+         Y[i + (int64_t)c * ldy] = s * sc[i];
+      }
+   }
+   for (int c = nout; c < nkeep; ++c) {
+      theta[c] = NAN;
+      for (int i = 0; i < m; ++i) Y[i + (int64_t)c * ldy] = 0.0;
+   }
+   free(sc); free(Gs); free(Hs); free(U); free(lam); free(B); free(T); free(Hr); free(Z); free(th);
+   return r;
+}
+
+int orc_rr_generalized(int m, const double *G, int ldg, const double *H, int ldh, double gtol,
+                       double *theta, double *Y, int ldy) {
+   double *B = (double *)calloc((size_t)m * m + 1, sizeof(double));
+   double *V = (double *)malloc(sizeof(double) * (size_t)m * m);
+   double *w = (double *)malloc(sizeof(double) * (size_t)m);
+   for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) B[i + j * m] = G[i + (int64_t)j * ldg] + (i == j ? gtol : 0.0);
+   int info = orc_sygv(m, H, ldh, B, m, w, V, m);
+   if (!info)
+      for (int c = 0; c < m; ++c) {
+         theta[c] = w[m - 1 - c];
+         memcpy(Y + (int64_t)c * ldy, V + (int64_t)(m - 1 - c) * m, sizeof(double) * (size_t)m);
+      }
+   free(B); free(V); free(w);
+   return info;
+}
+
+/* ------------------------------------------------------------------------- */
+/* drivers                                                                    */
+/* ------------------------------------------------------------------------- */
+int orc_arnoldi_hess_eig(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                         const double *vals, int k, int kdim, int maxiter, double select_tol,
+                         double gtol, const double *v0, double *theta, double *X, int64_t ldx) {
+   int m = kdim + 1;
+   if (k > n) k = (int)n;
+   double *V = (double *)calloc((size_t)n * m, sizeof(double));
+   double *AV = (double *)malloc(sizeof(double) * (size_t)n * m);
+   double *G = (double *)malloc(sizeof(double) * (size_t)m * m);
+   double *H = (double *)malloc(sizeof(double) * (size_t)m * m);
+   double *Y = (double *)malloc(sizeof(double) * (size_t)m * m);
+   double *th = (double *)malloc(sizeof(double) * (size_t)m);
+   double *U = (double *)malloc(sizeof(double) * (size_t)n * m);
+   int *perm = (int *)malloc(sizeof(int) * (size_t)m);
+   memcpy(V, v0, sizeof(double) * (size_t)n);
+   int tits = 1, built = 0, have = 0;
+   while (tits < maxiter) {
+      double piv;
+      int l = orc_iamax(n, V, &piv) - 1;
+      perm[0] = l;
+      if (fabs(piv) < select_tol) break;
+      for (int64_t r = 0; r < n; ++r) V[r] = V[r] / piv;
+      int ncols = 1;
+      for (int i = 0; i < kdim; ++i) {
+         double *w = V + (int64_t)(i + 1) * n;
+         orc_spmm_csr(n, row_ptr, col_idx, vals, 1, V + (int64_t)i * n, n, w, n);
+         for (int j = 0; j <= i; ++j) {
+            double c = w[perm[j]];
+            const double *q = V + (int64_t)j * n;
+            for (int64_t r = 0; r < n; ++r) w[r] = fma(-c, q[r], w[r]);
+         }
+         l = orc_iamax(n, w, &piv) - 1;
+         perm[i + 1] = l;
+         if (fabs(piv) < select_tol) { tits = maxiter; break; }
+         for (int64_t r = 0; r < n; ++r) w[r] = w[r] / piv;
+         ++ncols;
+         ++tits;
+      }
+      /* Rayleigh-Ritz on the columns built in this sweep (eig_rr.m:48-57) */
+      built = ncols;
+      orc_spmm_csr(n, row_ptr, col_idx, vals, ncols, V, n, AV, n);
+      orc_gemm_tn(n, ncols, ncols, V, n, AV, n, H, ncols);
+      orc_gemm_tn(n, ncols, ncols, V, n, V, n, G, ncols);
+      int info = orc_rr_generalized(ncols, G, ncols, H, ncols, gtol, th, Y, ncols);
+      if (info) break;
+      orc_gemm_nn(n, ncols, ncols, V, n, Y, ncols, U, n);
+      have = ncols;
+      if (tits < maxiter) {
+         memcpy(V, U, sizeof(double) * (size_t)n); /* restart on the top Ritz vector */
+      }
+   }
+   int kk = k < have ? k : have;
+   for (int c = 0; c < kk; ++c) {
+      theta[c] = th[c];
+      memcpy(X + (int64_t)c * ldx, U + (int64_t)c * n, sizeof(double) * (size_t)n);
+   }
+   for (int c = kk; c < k; ++c) theta[c] = NAN;
+   free(V); free(AV); free(G); free(H); free(Y); free(th); free(U); free(perm);
+   return built;
+}
+
+void orc_subspace_iter_eig(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                           const double *vals, int m, int maxiter, int step_size,
+                           double select_tol, double gtol, double *V, int64_t ldv, double *theta) {
+   double *T = (double *)malloc(sizeof(double) * (size_t)n * m);
+   double *AV = (double *)malloc(sizeof(double) * (size_t)n * m);
+   double *R = (double *)malloc(sizeof(double) * (size_t)m * m);
+   double *G = (double *)malloc(sizeof(double) * (size_t)m * m);
+   double *H = (double *)malloc(sizeof(double) * (size_t)m * m);
+   double *Y = (double *)malloc(sizeof(double) * (size_t)m * m);
+   int *sel = (int *)malloc(sizeof(int) * (size_t)m), *perm = (int *)malloc(sizeof(int) * (size_t)m);
+   for (int j = 0; j < m; ++j) memcpy(T + (int64_t)j * n, V + (int64_t)j * ldv, sizeof(double) * (size_t)n);
+   orc_scalecols(n, m, T, n);
+   for (int it = 0; it < maxiter; ++it) {
+      for (int s = 0; s < step_size; ++s) {
+         orc_spmm_csr(n, row_ptr, col_idx, vals, m, T, n, AV, n);
+         memcpy(T, AV, sizeof(double) * (size_t)n * m);
+         orc_scalecols(n, m, T, n);
+      }
+      orc_hessenberg(n, m, T, n, R, m, select_tol, 0, sel, perm);
+      /* economic Q: selected columns only (hessenberg.m:76-78) */
+      int ms = 0;
+      for (int j = 0; j < m; ++j)
+         if (sel[j]) {
+            if (ms != j) memcpy(T + (int64_t)ms * n, T + (int64_t)j * n, sizeof(double) * (size_t)n);
+            ++ms;
+         }
+      orc_spmm_csr(n, row_ptr, col_idx, vals, ms, T, n, AV, n);
+      orc_gemm_tn(n, ms, ms, T, n, AV, n, H, ms);
+      orc_gemm_tn(n, ms, ms, T, n, T, n, G, ms);
+      orc_rr_generalized(ms, G, ms, H, ms, gtol, theta, Y, ms);
+      orc_gemm_nn(n, ms, ms, T, n, Y, ms, AV, n);
+      memcpy(T, AV, sizeof(double) * (size_t)n * ms);
+      for (int j = ms; j < m; ++j) { theta[j] = NAN; memset(T + (int64_t)j * n, 0, sizeof(double) * (size_t)n); }
+   }
+   for (int j = 0; j < m; ++j) memcpy(V + (int64_t)j * ldv, T + (int64_t)j * n, sizeof(double) * (size_t)n);
+   free(T); free(AV); free(R); free(G); free(H); free(Y); free(sel); free(perm);
+}
+
+void orc_residuals(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *vals,
+                   int k, const double *X, int64_t ldx, const double *theta, double normA,
+                   double *res) {
+   double *AX = (double *)malloc(sizeof(double) * (size_t)n * k);
+   orc_spmm_csr(n, row_ptr, col_idx, vals, k, X, ldx, AX, n);
+   for (int j = 0; j < k; ++j) {
+      double s = 0.0, t = 0.0;
+      for (int64_t i = 0; i < n; ++i) {
+         double r = AX[i + j * n] - theta[j] * X[i + j * ldx];
+         s += r * r;
+         t += X[i + j * ldx] * X[i + j * ldx];
+      }
+      res[j] = sqrt(s) / (normA * sqrt(t));
+   }
+   free(AX);
+}
+
+/* ---- filtered OFRR ------------------------------------------------------- */
+static double colmaxabs(int64_t n, const double *a) {
+   double m = 0.0;
+   for (int64_t i = 0; i < n; ++i) m = fmax(m, fabs(a[i]));
+   return m;
+}
+
+int orc_filtered_ofrr(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                      const double *vals, const orc_ofrr_opts *o, double *X, int64_t ldx,
+                      double *theta, double *res) {
+   const int k = o->k, nd = o->nd, d = o->degree;
+   const int m = k * (1 + nd);
+   const int mid = d / 2;
+   double *V = (double *)malloc(sizeof(double) * (size_t)n * m);  /* [X | Dd | Dmid] */
+   double *W = (double *)malloc(sizeof(double) * (size_t)n * m);  /* A V */
+   double *Rr = (double *)malloc(sizeof(double) * (size_t)n * k);
+   double *Dp = (double *)malloc(sizeof(double) * (size_t)n * k);
+   double *T = (double *)malloc(sizeof(double) * (size_t)n * k);
+   double *G = (double *)malloc(sizeof(double) * (size_t)m * m);
+   double *H = (double *)malloc(sizeof(double) * (size_t)m * m);
+   double *C = (double *)malloc(sizeof(double) * (size_t)m * k);
+   double *zc = (double *)malloc(sizeof(double) * k), *zp = (double *)malloc(sizeof(double) * k);
+   double *Lth = (double *)malloc(sizeof(double) * k);
+   for (int j = 0; j < k; ++j) memcpy(V + (int64_t)j * n, X + (int64_t)j * ldx, sizeof(double) * (size_t)n);
+   /* initial Rayleigh-Ritz on the start block */
+   orc_spmm_csr(n, row_ptr, col_idx, vals, k, V, n, W, n);
+   orc_gemm_tn(n, k, k, V, n, V, n, G, k);
+   orc_gemm_tn(n, k, k, V, n, W, n, H, k);
+   orc_rr_truncated(k, G, k, H, k, o->tau, k, theta, C, k);
+   orc_gemm_nn(n, k, k, V, n, C, k, T, n);
+   memcpy(V, T, sizeof(double) * (size_t)n * k);
+   int it, converged = 0;
+   for (it = 0; it < o->maxiter; ++it) {
+      double *Xc = V, *Dd = V + (int64_t)k * n, *Dm = V + (int64_t)2 * k * n;
+      orc_spmm_csr(n, row_ptr, col_idx, vals, k, Xc, n, W, n);
+      double normA = o->normA > 0 ? o->normA : theta[0];
+      converged = 1;
+      for (int j = 0; j < k; ++j) {
+         double s = 0.0, t = 0.0;
+         for (int64_t i = 0; i < n; ++i) {
+            double r = W[i + j * n] - theta[j] * Xc[i + j * n];
+            Rr[i + j * n] = r;
+            s += r * r;
+            t += Xc[i + j * n] * Xc[i + j * n];
+         }
+         res[j] = sqrt(s) / (normA * sqrt(t));
+         if (j < o->nwant && !(res[j] <= o->tol)) converged = 0;
+      }
+      if (converged) break;
+      /* Chebyshev filter on the correction: damp [lower, theta_k] */
+      double cut = theta[k - 1], lo = o->lower;
+      double e = 0.5 * (cut - lo), c0 = 0.5 * (cut + lo);
+      for (int j = 0; j < k; ++j) {
+         double s = colmaxabs(n, Rr + (int64_t)j * n) / e;
+         double f = s > 0 ? 1.0 / s : 1.0;
+         Lth[j] = (theta[j] - c0) / e;
+         for (int64_t i = 0; i < n; ++i) {
+            Dd[i + j * n] = Rr[i + j * n] / e * f; /* D_1 */
+            Dp[i + j * n] = 0.0;                 /* D_0 */
+         }
+         zp[j] = f;            /* z_0 * f */
+         zc[j] = Lth[j] * f;   /* z_1 * f */
+      }
+      if (nd == 2 && mid == 1) memcpy(Dm, Dd, sizeof(double) * (size_t)n * k);
+      for (int step = 1; step < d; ++step) {
+         orc_spmm_csr(n, row_ptr, col_idx, vals, k, Dd, n, T, n);
+         for (int j = 0; j < k; ++j) {
+            double mx = 0.0;
+            for (int64_t i = 0; i < n; ++i) {
+               double t = (2.0 / e) * (T[i + j * n] - c0 * Dd[i + j * n]) - Dp[i + j * n] +
+                          (2.0 / e) * Rr[i + j * n] * zc[j];
+               T[i + j * n] = t;
+               mx = fmax(mx, fabs(t));
+            }
+            double f = mx > 0 ? 1.0 / mx : 1.0;
+            double znext = 2.0 * Lth[j] * zc[j] - zp[j];
+            for (int64_t i = 0; i < n; ++i) {
+               Dp[i + j * n] = Dd[i + j * n] * f;
+               Dd[i + j * n] = T[i + j * n] * f;
+            }
+            zp[j] = zc[j] * f;
+            zc[j] = znext * f;
+         }
+         if (nd == 2 && step + 1 == mid) memcpy(Dm, Dd, sizeof(double) * (size_t)n * k);
+      }
+      /* projection products over V = [X | Dd (| Dmid)] */
+      orc_spmm_csr(n, row_ptr, col_idx, vals, m - k, Dd, n, W + (int64_t)k * n, n);
+      orc_gemm_tn(n, m, m, V, n, V, n, G, m);
+      orc_gemm_tn(n, m, m, V, n, W, n, H, m);
+      int r = orc_rr_truncated(m, G, m, H, m, o->tau, k, theta, C, m);
+      if (r < k) { it = -(it + 1); break; }
+      orc_gemm_nn(n, k, m, V, n, C, m, T, n);
+      memcpy(V, T, sizeof(double) * (size_t)n * k);
+   }
+   if (!converged && it >= 0) {
+      orc_residuals(n, row_ptr, col_idx, vals, k, V, n, theta, o->normA > 0 ? o->normA : theta[0], res);
+   }
+   for (int j = 0; j < k; ++j) memcpy(X + (int64_t)j * ldx, V + (int64_t)j * n, sizeof(double) * (size_t)n);
+   free(V); free(W); free(Rr); free(Dp); free(T); free(G); free(H); free(C); free(zc); free(zp); free(Lth);
+   if (it < 0) return it;
+   return converged ? it : -(it + 1);
+}

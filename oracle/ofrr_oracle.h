@@ -1,0 +1,123 @@
+/*
+ * ofrr_oracle.h -- CPU FP64 restatement of the OFRR projection pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The
+ * product (mixedprecisionofrr_amd/) never links, loads or calls it.
+ *
+ * Each function cites the reference file:line whose semantics it restates
+ * (paths relative to the reference repository root).  Every function is
+ * pinned by a check in tests/test_oracle_*.py; see the table at the top of
+ * ofrr_oracle.c for which check pins which function.
+ *
+ * Conventions: column-major dense matrices with explicit leading dimension,
+ * 0-based CSR with int32 row_ptr/col_idx (int64 row_ptr for the large
+ * generators), FP64 everywhere.
+ */
+#ifndef OFRR_ORACLE_H
+#define OFRR_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- counter-based RNG: Philox4x32-10 (Salmon et al., SC'11) + Box-Muller */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* out[i] = N(0,1) number (offset+i) of stream (seed, stream) */
+void orc_randn(uint64_t seed, uint32_t stream, uint64_t offset, int64_t n, double *out);
+/* out[i] = U[0,1) number (offset+i) of stream (seed, stream) */
+void orc_randu(uint64_t seed, uint32_t stream, uint64_t offset, int64_t n, double *out);
+
+/* ---- CSR stencil operators (natural x-fastest ordering, Dirichlet) ------- */
+/* number of nonzeros of the dim-D (2 or 3) stencil on an N^D grid */
+int64_t orc_stencil_nnz(int dim, int64_t N);
+/* seed == 0 -> constant coefficients (the 2D 5-point / 3D 7-point Laplacian);
+ * otherwise face coefficients 10^(2u-1), u ~ U[0,1) from Philox stream d. */
+void orc_stencil_csr(int dim, int64_t N, uint64_t seed, int64_t *row_ptr, int32_t *col_idx,
+                     double *vals);
+
+/* ---- operator and BLAS-like kernels -------------------------------------- */
+/* Y = A X (csrmv.m:48-59 semantics, FP64, fused multiply-add in row order) */
+void orc_spmm_csr(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *vals,
+                  int k, const double *X, int64_t ldx, double *Y, int64_t ldy);
+/* divide each column by its signed max-abs entry (scalecols.m:26-33) */
+void orc_scalecols(int64_t m, int n, double *A, int64_t lda);
+/* 1-based index of the first max |x|, signed value (mvops.cu:607-946) */
+int orc_iamax(int64_t n, const double *x, double *val);
+/* 0-based per-column argmax |A(:,j)| (mvops.cu:955-1199) */
+void orc_matrix_iamax(int64_t m, int n, const double *A, int64_t lda, int *idx);
+/* G = V^T W, G is k1 x k2 (gemm transA=true, mvops.cu:423) */
+void orc_gemm_tn(int64_t n, int k1, int k2, const double *V, int64_t ldv, const double *W,
+                 int64_t ldw, double *G, int ldg);
+/* C = A B, A m x k, B k x n (gemm transA=false) */
+void orc_gemm_nn(int64_t m, int n, int k, const double *A, int64_t lda, const double *B,
+                 int64_t ldb, double *C, int64_t ldc);
+
+/* ---- Hessenberg process (hessenberg.m:55-74), in place on Q ---------------
+ * Q: m x n in (the matrix A) / out (the basis Q); R: n x n (zeroed here);
+ * select[i] = 1 kept, 0 dropped; perm[i] = 0-based pivot row (-1 if dropped).
+ * variant 0: skip dropped columns in later eliminations (MATLAB, C++ v1)
+ * variant 1: C++ v3 order, dropped columns still eliminated (hessenberg.cu:771)
+ */
+void orc_hessenberg(int64_t m, int n, double *Q, int64_t ldq, double *R, int ldr,
+                    double select_tol, int variant, int *select, int *perm);
+
+/* ---- dense symmetric eigensolvers (FP64) --------------------------------- */
+/* eigenvalues ascending in w, eigenvectors in V (LAPACK dsyev 'V','U' order) */
+int orc_syev(int n, const double *A, int lda, double *w, double *V, int ldv);
+/* A x = lambda B x, itype 1, B-normalised vectors (LAPACK dsygv order).
+ * returns 0, or i>0 if the leading minor of order i of B is not positive */
+int orc_sygv(int n, const double *A, int lda, const double *B, int ldb, double *w, double *V,
+             int ldv);
+/* truncated OFRR Rayleigh-Ritz: G = U L U^T, keep L_i > tau * max(L),
+ * B = S U_r L_r^{-1/2} with S = diag(G)^{-1/2} scaling, eig(B^T H B) -> top nkeep.
+ * theta descending, Y (m x nkeep) coefficient vectors. returns the rank kept. */
+int orc_rr_truncated(int m, const double *G, int ldg, const double *H, int ldh, double tau,
+                     int nkeep, double *theta, double *Y, int ldy);
+/* eig_rr.m:48-57 generalised form: eig(H, G + gtol I), sorted descending */
+int orc_rr_generalized(int m, const double *G, int ldg, const double *H, int ldh, double gtol,
+                       double *theta, double *Y, int ldy);
+
+/* ---- drivers -------------------------------------------------------------- */
+/* arnoldi_hess_eig.m:43-107: Hessenberg-type Krylov builder + generalised RR.
+ * v0: start vector (n).  Outputs the top k Ritz values (descending) and X (n x k).
+ * returns the number of basis columns built in the last sweep. */
+int orc_arnoldi_hess_eig(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                         const double *vals, int k, int kdim, int maxiter, double select_tol,
+                         double gtol, const double *v0, double *theta, double *X, int64_t ldx);
+
+/* subspace_iter_eig.m:53-68 with function_orth = hessenberg (select_tol) and
+ * function_rr = eig_rr(generalised, gtol).  V (n x m) start block in, Ritz block out. */
+void orc_subspace_iter_eig(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                           const double *vals, int m, int maxiter, int step_size,
+                           double select_tol, double gtol, double *V, int64_t ldv, double *theta);
+
+/* Filtered OFRR (block subspace iteration with a Chebyshev filter on the
+ * residual correction, FP64 throughout) -- the algorithm the FP64 GPU path runs.
+ * X (n x k) start block in / Ritz block out; theta (k) out descending;
+ * res (k) out relative residuals; returns iterations run (negative if not converged). */
+typedef struct {
+   int k;            /* block width */
+   int nwant;        /* wanted pairs (largest) */
+   int nd;           /* correction blocks in the basis: 1 or 2 */
+   int degree;       /* filter degree per outer iteration */
+   int maxiter;      /* outer iterations */
+   double tol;       /* residual tolerance ||Ax - theta x|| / (||A|| ||x||) */
+   double tau;       /* Gram truncation tolerance */
+   double lower;     /* lower bound of the spectrum */
+   double normA;     /* ||A||_2 (0: use the largest Ritz value) */
+} orc_ofrr_opts;
+int orc_filtered_ofrr(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
+                      const double *vals, const orc_ofrr_opts *opt, double *X, int64_t ldx,
+                      double *theta, double *res);
+
+/* r_j = ||A x_j - theta_j x_j||_2 / (normA ||x_j||_2)  (test4/test.m:340-350, normalised) */
+void orc_residuals(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *vals,
+                   int k, const double *X, int64_t ldx, const double *theta, double normA,
+                   double *res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
