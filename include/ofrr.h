@@ -198,12 +198,15 @@ typedef struct {
    int converged;        /* pairs meeting tol among the nwant */
    int spmm_calls;       /* operator applications on k-column blocks */
    int rank_min;         /* smallest rank kept by the truncation */
+   int filter_steps;     /* Chebyshev step launches (one kernel each) */
+   int resid_passes;     /* FP64 residual launches */
    double t_total_ms;    /* device time of the solve (events on the stream) */
-   double t_spmm_ms;     /* filter steps + projection operator applications */
+   double t_filter_ms;   /* Chebyshev steps only */
+   double t_zspmm_ms;    /* operator applied to the growth blocks for V^T A V */
    double t_gram_ms;     /* projection products */
    double t_rr_ms;       /* Rayleigh-Ritz */
    double t_update_ms;   /* back-transform */
-   double t_resid_ms;    /* FP64 operator application + residual */
+   double t_resid_ms;    /* FP64 operator application + residual (incl. host checks) */
 } ofrr_filter_stats;
 
 /* X_dev: n x k FP64 row-major (row stride k), start block in / Ritz block out;

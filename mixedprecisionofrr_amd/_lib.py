@@ -31,9 +31,10 @@ class FilterOpts(C.Structure):
 
 class FilterStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("spmm_calls", C.c_int),
-                ("rank_min", C.c_int), ("t_total_ms", C.c_double), ("t_spmm_ms", C.c_double),
-                ("t_gram_ms", C.c_double), ("t_rr_ms", C.c_double), ("t_update_ms", C.c_double),
-                ("t_resid_ms", C.c_double)]
+                ("rank_min", C.c_int), ("filter_steps", C.c_int), ("resid_passes", C.c_int),
+                ("t_total_ms", C.c_double), ("t_filter_ms", C.c_double),
+                ("t_zspmm_ms", C.c_double), ("t_gram_ms", C.c_double), ("t_rr_ms", C.c_double),
+                ("t_update_ms", C.c_double), ("t_resid_ms", C.c_double)]
 
 
 _lib = None

@@ -14,10 +14,23 @@ int spmm_csr(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, con
 int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const double *va,
                int k, const double *X, const double *theta, double *W, void *Rq, int rdt,
                const double *inv_sR, float *colmax, double *sums);
-// one Chebyshev step on row-major blocks of width k (see spmm.hip)
+// one Chebyshev step on row-major blocks of width k (see spmm.hip):
+// D_{j+1} = (2/e)(A D_j - c D_j) - D_{j-1} + (2/e) R z_j with stored blocks
+// D_j = Dc sc, D_{j-1} = Dp sp, R = Rq sR; the output is stored as D_{j+1} / sn with
+// sn = 2^(ceil(log2(2 max|D_j|)) - target_exp) written to sn_out
+struct ChebArgs {
+   double e, c;               // half-width and centre of the damped interval
+   const double *z;           // z_j = T_j((theta - c) / e), k values
+   const double *sR, *sc, *sp;// scales of R, Dc, Dp (sp may be null: no Dp)
+   const float *maxc;         // max |D_j| per column
+   float *maxn;               // max |D_{j+1}| per column (zeroed beforehand)
+   float *zero_next;          // cleared for the next step (nullable)
+   double *sn_out;            // scale of the stored D_{j+1}
+   int target_exp;
+};
 int cheb_step(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const void *va,
               int vdt, int k, const void *Dc, const void *Dp, const void *R, int bdt,
-              const double *coef, const double *inv_s, void *Dn, float *colmax);
+              const ChebArgs &args, void *Dn);
 // G = alpha V^T W + beta G (FP64), strided operands
 int gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64_t vrs,
          int64_t vcs, const void *W, int wdt, int64_t wrs, int64_t wcs, double alpha, double beta,

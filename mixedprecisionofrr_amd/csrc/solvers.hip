@@ -84,29 +84,6 @@ __global__ void mirror_upper_kernel(int m, double *__restrict__ G, int ldg) {
    if (i > j) G[i + (int64_t)j * ldg] = G[j + (int64_t)i * ldg];
 }
 
-// Chebyshev step coefficients are formed inside the step launcher kernel below
-// from device-side scales; this kernel prepares them.
-//   D_{j+1} = (2/e)(A D_j - c0 D_j) - D_{j-1} + (2/e) R z_j
-// stored blocks: D_j = Dc * sc, D_{j-1} = Dp * sp, R = Rq * sR; new scale
-// sn = 2^(ceil(log2(2 max|D_j|)) - target_exp) places the predicted range of
-// D_{j+1} near 2^target_exp in the storage format.
-__global__ void cheb_coef_kernel(int k, double e, double c0, const double *__restrict__ z,
-                                 const double *__restrict__ sR, const double *__restrict__ sc,
-                                 const double *__restrict__ sp, const float *__restrict__ maxc,
-                                 int target_exp, double *__restrict__ coef,
-                                 double *__restrict__ inv_sn, double *__restrict__ sn) {
-   int j = blockIdx.x * blockDim.x + threadIdx.x;
-   if (j >= k) return;
-   double m = (double)maxc[j];
-   double s = (m > 0.0) ? exp2(ceil(log2(2.0 * m)) - (double)target_exp) : 1.0;
-   coef[j] = 2.0 * sc[j] / e;
-   coef[k + j] = -2.0 * c0 * sc[j] / e;
-   coef[2 * k + j] = sp ? -sp[j] : 0.0;
-   coef[3 * k + j] = 2.0 * sR[j] * z[j] / e;
-   inv_sn[j] = 1.0 / s;
-   sn[j] = s;
-}
-
 // ---- device arena for the drivers (grow-only, reused between solves) -------
 struct Arena {
    char *base = nullptr;
@@ -350,7 +327,7 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
    // ---- buffers
    size_t need = al((size_t)n * k * 8) * 3 + al((size_t)n * k * bsz) * 5 + al((size_t)n * k * nd * zsz) +
                  al((size_t)m * m * 8) * 2 + al((size_t)m * k * 8) + al((size_t)k * 8) * 14 +
-                 al((size_t)d * k * 8) + al((size_t)k * 4) * 2 + 4096;
+                 al((size_t)d * k * 8) + al((size_t)k * 4) * 4 + 4096;
    int st = arena_reserve(ctx, need);
    if (st) return st;
    double *X = (double *)arena_take((size_t)n * k * 8);
@@ -373,9 +350,8 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
    double *invs = (double *)arena_take((size_t)k * 8);
    double *sD1 = (double *)arena_take((size_t)k * 8);
    double *zall = (double *)arena_take((size_t)d * k * 8);
-   float *cmax[2];
-   cmax[0] = (float *)arena_take((size_t)k * 4);
-   cmax[1] = (float *)arena_take((size_t)k * 4);
+   float *cmax[3];
+   for (int i = 0; i < 3; ++i) cmax[i] = (float *)arena_take((size_t)k * 4);
    int32_t *rank = (int32_t *)arena_take(64);
    float *cmaxR = (float *)arena_take((size_t)k * 4);
    double *sRd = sR;
@@ -389,7 +365,8 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
 
    hipEvent_t ev[8];
    for (int i = 0; i < 8; ++i) OFRR_HIP(ctx, hipEventCreate(&ev[i]));
-   float t_acc[6] = {0, 0, 0, 0, 0, 0};
+   float t_acc[7] = {0, 0, 0, 0, 0, 0, 0};
+   int filter_steps = 0, resid_passes = 0;
    auto rec = [&](int i) { return hipEventRecord(ev[i], ctx->stream); };
    OFRR_HIP(ctx, rec(0));
 
@@ -435,6 +412,7 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
          OFRR_HIP(ctx, hipMemsetAsync(cmaxR, 0, (size_t)k * 4, ctx->stream));
          if ((st = spmm_resid(ctx, n, rp, ci, va, k, X, th, W, Rq, bdt, invsR, cmaxR, sums))) return st;
          ++spmm_calls;
+         ++resid_passes;
          OFRR_HIP(ctx, hipMemcpyAsync(h_sums, sums, (size_t)2 * k * 8, hipMemcpyDeviceToHost, ctx->stream));
          OFRR_HIP(ctx, hipMemcpyAsync(h_cmaxR, cmaxR, (size_t)k * 4, hipMemcpyDeviceToHost, ctx->stream));
          OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -492,28 +470,37 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
       // Db[b] has scale scale[b]; D_1 lives in Rq with scale sD1
       const void *Dc = Rq, *Dp = nullptr;
       double *sc = sD1, *sp = nullptr;
-      int cm = 0, curIdx = -1, prevIdx = -1, snap = -1;
+      int curIdx = -1, prevIdx = -1, snap = -1;
       const void *Dmid = (nd == 2 && mid == 1) ? Rq : nullptr;
+      // the column maxima rotate over three buffers: step s reads cmax[(s-1)%3],
+      // accumulates into cmax[s%3] (cleared by step s-1, or here for s = 1) and
+      // clears cmax[(s+1)%3] for step s+1
+      OFRR_HIP(ctx, hipMemsetAsync(cmax[1], 0, (size_t)k * 4, ctx->stream));
       for (int s = 1; s < d; ++s) {
          int out = -1;
          for (int b = 0; b < 4; ++b)
             if (b != curIdx && b != prevIdx && b != snap) { out = b; break; }
-         cheb_coef_kernel<<<(k + 63) / 64, 64, 0, ctx->stream>>>(k, e, c0, zall + (size_t)(s - 1) * k,
-                                                                 sRd, sc, sp, cmax[cm], tgt_step,
-                                                                 coef, invs, scale[out]);
-         OFRR_LAUNCH(ctx);
-         OFRR_HIP(ctx, hipMemsetAsync(cmax[cm ^ 1], 0, (size_t)k * 4, ctx->stream));
-         if ((st = cheb_step(ctx, n, rp, ci, vlo, vlo_dt, k, Dc, Dp, Rq, bdt, coef, invs, Db[out],
-                             cmax[cm ^ 1])))
-            return st;
+         ChebArgs ca;
+         ca.e = e;
+         ca.c = c0;
+         ca.z = zall + (size_t)(s - 1) * k;
+         ca.sR = sRd;
+         ca.sc = sc;
+         ca.sp = sp;
+         ca.maxc = cmax[(s - 1) % 3];
+         ca.maxn = cmax[s % 3];
+         ca.zero_next = (s + 1 < d) ? cmax[(s + 1) % 3] : nullptr;
+         ca.sn_out = scale[out];
+         ca.target_exp = tgt_step;
+         if ((st = cheb_step(ctx, n, rp, ci, vlo, vlo_dt, k, Dc, Dp, Rq, bdt, ca, Db[out]))) return st;
          ++spmm_calls;
+         ++filter_steps;
          prevIdx = curIdx;
          Dp = Dc;
          sp = sc;
          curIdx = out;
          Dc = Db[out];
          sc = scale[out];
-         cm ^= 1;
          if (nd == 2 && s + 1 == mid) { snap = out; Dmid = Db[out]; }
       }
       OFRR_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
@@ -570,7 +557,8 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
          OFRR_HIP(ctx, hipEventElapsedTime(&c, ev[5], ev[6]));
          OFRR_HIP(ctx, hipEventElapsedTime(&dd, ev[6], ev[7]));
          OFRR_HIP(ctx, hipEventElapsedTime(&e2, ev[7], ev[2]));
-         t_acc[0] += a + b;
+         t_acc[0] += a;
+         t_acc[6] += b;
          t_acc[1] += c;
          t_acc[2] += dd;
          t_acc[3] += e2;
@@ -597,8 +585,11 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
       stats->converged = nconv;
       stats->spmm_calls = spmm_calls;
       stats->rank_min = rank_min;
+      stats->filter_steps = filter_steps;
+      stats->resid_passes = resid_passes;
       stats->t_total_ms = total;
-      stats->t_spmm_ms = t_acc[0];
+      stats->t_filter_ms = t_acc[0];
+      stats->t_zspmm_ms = t_acc[6];
       stats->t_gram_ms = t_acc[1];
       stats->t_rr_ms = t_acc[2];
       stats->t_update_ms = t_acc[3];

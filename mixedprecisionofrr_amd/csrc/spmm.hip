@@ -16,8 +16,10 @@
 //   spmm_rm      Y = alpha A X + beta Y
 //   spmm_resid   W = A X, R = W - X diag(theta) -> quantised R, column max |R|,
 //                per-block FP64 sums of r^2 and x^2 (residual norms)
-//   cheb_step    T = a.(A Dc) + b.Dc + g.Dp + d.R (per-column coefficients),
-//                quantised T / s, column max |T|: one Chebyshev-filter step
+//   cheb_step    T = a.(A Dc) + b.Dc + g.Dp + d.R, one Chebyshev-filter step;
+//                the per-column coefficients and the output scale s are formed
+//                in the kernel from device-side scales, T / s is stored in the
+//                basis format and column max |T| is folded in with atomics
 //   spmm_strided generic strides (any layout), one thread per output
 #include "internal.hpp"
 
@@ -161,9 +163,8 @@ template <typename TV, typename TB, int VEC>
 __global__ __launch_bounds__(256) void cheb_step_kernel(
     int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
     const TV *__restrict__ va, int k, const TB *__restrict__ Dc, const TB *__restrict__ Dp,
-    const TB *__restrict__ R, const double *__restrict__ coef /* 4 x k */,
-    const double *__restrict__ inv_s, TB *__restrict__ Dn, float *__restrict__ colmax, int lg,
-    int64_t rows_per_block, int64_t nblk) {
+    const TB *__restrict__ R, ChebArgs ca_, TB *__restrict__ Dn, int lg, int64_t rows_per_block,
+    int64_t nblk) {
    using ACC = typename std::conditional<std::is_same<TB, double>::value, double, float>::type;
    __shared__ float redm[256][VEC];
    const int64_t blk = xcd_block(nblk);
@@ -174,16 +175,26 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
    const int c0 = lane_g * VEC;  // host guarantees k == G * VEC
    const int64_t r0 = blk * rows_per_block;
    const int64_t r1 = min(n, r0 + rows_per_block);
+   // per-column coefficients from the device-side scales:
+   // T = (2/e)(A D_j - c D_j) - D_{j-1} + (2/e) R z_j over stored blocks
    ACC ca[VEC], cb[VEC], cg[VEC], cd[VEC], is[VEC];
    float mx[VEC];
 #pragma unroll
    for (int v = 0; v < VEC; ++v) {
-      ca[v] = (ACC)coef[c0 + v];
-      cb[v] = (ACC)coef[k + c0 + v];
-      cg[v] = (ACC)coef[2 * k + c0 + v];
-      cd[v] = (ACC)coef[3 * k + c0 + v];
-      is[v] = (ACC)inv_s[c0 + v];
+      const int j = c0 + v;
+      const double m = (double)ca_.maxc[j];
+      const double s = (m > 0.0) ? exp2(ceil(log2(2.0 * m)) - (double)ca_.target_exp) : 1.0;
+      const double sc = ca_.sc[j];
+      ca[v] = (ACC)(2.0 * sc / ca_.e);
+      cb[v] = (ACC)(-2.0 * ca_.c * sc / ca_.e);
+      cg[v] = (ACC)(ca_.sp ? -ca_.sp[j] : 0.0);
+      cd[v] = (ACC)(2.0 * ca_.sR[j] * ca_.z[j] / ca_.e);
+      is[v] = (ACC)(1.0 / s);
       mx[v] = 0.f;
+      if (blk == 0 && grp == 0) {
+         ca_.sn_out[j] = s;
+         if (ca_.zero_next) ca_.zero_next[j] = 0.f;
+      }
    }
    for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
       ACC acc[VEC];
@@ -213,7 +224,7 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
       const int lane = t / VEC, v = t % VEC;
       float m = 0.f;
       for (int g = 0; g < GROUPS; ++g) m = fmaxf(m, redm[g * G + lane][v]);
-      atomic_max_nonneg(colmax + t, m);
+      atomic_max_nonneg(ca_.maxn + t, m);
    }
 }
 
@@ -318,7 +329,7 @@ int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, c
 
 int cheb_step(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const void *va,
               int vdt, int k, const void *Dc, const void *Dp, const void *R, int bdt,
-              const double *coef, const double *inv_s, void *Dn, float *colmax) {
+              const ChebArgs &args, void *Dn) {
    return dispatch_dtype(vdt, [&](auto tv) -> int {
       using TV = decltype(tv);
       return dispatch_dtype(bdt, [&](auto tb) -> int {
@@ -334,8 +345,8 @@ int cheb_step(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, co
          int lg = 0;
          while ((1 << lg) < G) ++lg;
          cheb_step_kernel<TV, TB, VEC><<<grid, 256, 0, ctx->stream>>>(
-             n, rp, ci, (const TV *)va, k, (const TB *)Dc, (const TB *)Dp, (const TB *)R, coef,
-             inv_s, (TB *)Dn, colmax, lg, rows_per_block, nblk);
+             n, rp, ci, (const TV *)va, k, (const TB *)Dc, (const TB *)Dp, (const TB *)R, args,
+             (TB *)Dn, lg, rows_per_block, nblk);
          OFRR_LAUNCH(ctx);
          return kSuccess;
       });

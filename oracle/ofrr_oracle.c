@@ -29,6 +29,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* ------------------------------------------------------------------------- */
 /* Philox4x32-10                                                              */
@@ -179,14 +182,49 @@ void orc_scalecols(int64_t m, int n, double *A, int64_t lda) {
 
 void orc_gemm_tn(int64_t n, int k1, int k2, const double *V, int64_t ldv, const double *W,
                  int64_t ldw, double *G, int ldg) {
-#pragma omp parallel for collapse(2) schedule(static)
-   for (int a = 0; a < k1; ++a)
-      for (int b = 0; b < k2; ++b) {
-         const double *v = V + (int64_t)a * ldv, *w = W + (int64_t)b * ldw;
-         double acc = 0.0;
-         for (int64_t i = 0; i < n; ++i) acc = fma(v[i], w[i], acc);
-         G[a + (int64_t)b * ldg] = acc;
+   /* rows are split into one contiguous range per thread; each thread
+    * accumulates a private k1 x k2 block over row slabs transposed into
+    * row-major scratch; the private blocks are summed in thread order */
+   enum { SLAB = 32 };
+   int nth = 1;
+#ifdef _OPENMP
+   nth = omp_get_max_threads();
+#endif
+   double *part = (double *)calloc((size_t)nth * k1 * k2, sizeof(double));
+#pragma omp parallel num_threads(nth)
+   {
+      int t = 0;
+#ifdef _OPENMP
+      t = omp_get_thread_num();
+#endif
+      int64_t r0 = n * t / nth, r1 = n * (t + 1) / nth;
+      double *g = part + (size_t)t * k1 * k2;
+      double *vb = (double *)malloc(sizeof(double) * SLAB * (size_t)k1);
+      double *wb = (double *)malloc(sizeof(double) * SLAB * (size_t)k2);
+      for (int64_t rb = r0; rb < r1; rb += SLAB) {
+         int rows = (int)((r1 - rb) < SLAB ? (r1 - rb) : SLAB);
+         for (int a = 0; a < k1; ++a)
+            for (int r = 0; r < rows; ++r) vb[r * k1 + a] = V[rb + r + (int64_t)a * ldv];
+         for (int b2 = 0; b2 < k2; ++b2)
+            for (int r = 0; r < rows; ++r) wb[r * k2 + b2] = W[rb + r + (int64_t)b2 * ldw];
+         for (int r = 0; r < rows; ++r)
+            for (int a = 0; a < k1; ++a) {
+               const double va = vb[r * k1 + a];
+               double *ga = g + (size_t)a * k2;
+               const double *wr = wb + (size_t)r * k2;
+               for (int b2 = 0; b2 < k2; ++b2) ga[b2] = fma(va, wr[b2], ga[b2]);
+            }
       }
+      free(vb);
+      free(wb);
+   }
+   for (int a = 0; a < k1; ++a)
+      for (int b2 = 0; b2 < k2; ++b2) {
+         double s = 0.0;
+         for (int t = 0; t < nth; ++t) s += part[(size_t)t * k1 * k2 + (size_t)a * k2 + b2];
+         G[a + (int64_t)b2 * ldg] = s;
+      }
+   free(part);
 }
 
 void orc_gemm_nn(int64_t m, int n, int k, const double *A, int64_t lda, const double *B,

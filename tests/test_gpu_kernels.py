@@ -55,7 +55,11 @@ def test_spmm_fp64_bitwise(ctx, layout, k):
     Xd = _dev(X) if layout == "row" else _dev(X.T).t()
     print(f"spmm fp64 layout={layout} k={k}", flush=True)
     Y = api.spmm(ctx, A, Xd).cpu().numpy()
-    assert np.array_equal(Y, O.spmm(Ah, X))
+    ref = O.spmm(Ah, X)
+    bad = np.argwhere(Y != ref)
+    assert bad.shape[0] == 0, (f"{bad.shape[0]} mismatches, first {bad[:4].tolist()}, "
+                               f"max |diff| {np.max(np.abs(Y - ref)):.3e}, "
+                               f"max |ref| {np.max(np.abs(ref)):.3e}")
 
 
 @pytest.mark.parametrize("xdt,ydt", [(torch.float32, torch.float32), (torch.float16, torch.float32),
