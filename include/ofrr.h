@@ -39,8 +39,10 @@ enum ofrr_dtype { OFRR_F64 = 0, OFRR_F32 = 1, OFRR_F16 = 2, OFRR_BF16 = 3, OFRR_
 OFRR_API int ofrr_ctx_create(int device, ofrr_ctx **ctx);
 OFRR_API int ofrr_ctx_destroy(ofrr_ctx *ctx);
 OFRR_API int ofrr_ctx_sync(ofrr_ctx *ctx);
-/* use an external hipStream_t (NULL restores the context's own stream) */
+/* enqueue on an external hipStream_t; NULL selects the null (default) stream */
 OFRR_API int ofrr_ctx_set_stream(ofrr_ctx *ctx, void *stream);
+/* enqueue on the context's own stream (the state after ofrr_ctx_create) */
+OFRR_API int ofrr_ctx_use_own_stream(ofrr_ctx *ctx);
 OFRR_API void *ofrr_ctx_stream(ofrr_ctx *ctx);
 OFRR_API const char *ofrr_ctx_last_error(ofrr_ctx *ctx);
 OFRR_API const char *ofrr_status_string(int status);

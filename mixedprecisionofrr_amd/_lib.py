@@ -14,6 +14,7 @@ F64, F32, F16, BF16, FP8 = 0, 1, 2, 3, 4
 # every extern "C" entry point declared in include/ofrr.h
 EXPORTS = [
     "ofrr_ctx_create", "ofrr_ctx_destroy", "ofrr_ctx_sync", "ofrr_ctx_set_stream",
+    "ofrr_ctx_use_own_stream",
     "ofrr_ctx_stream", "ofrr_ctx_last_error", "ofrr_status_string", "ofrr_version",
     "ofrr_fill_randn", "ofrr_fill_randu", "ofrr_stencil_nnz", "ofrr_stencil_csr",
     "ofrr_spmm_csr", "ofrr_iamax", "ofrr_matrix_iamax", "ofrr_scalecols", "ofrr_check_special",
@@ -49,6 +50,7 @@ def _decl(L):
     L.ofrr_ctx_destroy.argtypes = [vp]
     L.ofrr_ctx_sync.argtypes = [vp]
     L.ofrr_ctx_set_stream.argtypes = [vp, vp]
+    L.ofrr_ctx_use_own_stream.argtypes = [vp]
     L.ofrr_ctx_stream.argtypes = [vp]
     L.ofrr_ctx_stream.restype = vp
     L.ofrr_ctx_last_error.argtypes = [vp]

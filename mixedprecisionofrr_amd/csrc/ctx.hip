@@ -105,7 +105,8 @@ int ofrr_ctx_create(int device, ofrr_ctx **out) {
    if (hipSetDevice(device) != hipSuccess) return kErrorCUDAGeneral;
    ofrr_ctx *c = new ofrr_ctx();
    c->device = device;
-   if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+   // a blocking stream: ordered with the null stream that frameworks default to
+   if (hipStreamCreateWithFlags(&c->own_stream, hipStreamDefault) != hipSuccess) {
       delete c;
       return kErrorCUDAGeneral;
    }
@@ -119,7 +120,7 @@ int ofrr_ctx_create(int device, ofrr_ctx **out) {
 int ofrr_ctx_destroy(ofrr_ctx *c) {
    if (!c) return kSuccess;
    (void)hipSetDevice(c->device);
-   if (c->stream) (void)hipStreamSynchronize(c->stream);
+   (void)hipStreamSynchronize(c->stream);
    if (c->ws) (void)hipFree(c->ws);
    if (c->pinned) (void)hipHostFree(c->pinned);
    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -139,7 +140,13 @@ int ofrr_ctx_sync(ofrr_ctx *c) {
 
 int ofrr_ctx_set_stream(ofrr_ctx *c, void *s) {
    OFRR_CHECK_CTX(c);
-   c->stream = s ? (hipStream_t)s : c->own_stream;
+   c->stream = (hipStream_t)s;  // NULL selects the null (default) stream
+   return kSuccess;
+}
+
+int ofrr_ctx_use_own_stream(ofrr_ctx *c) {
+   OFRR_CHECK_CTX(c);
+   c->stream = c->own_stream;
    return kSuccess;
 }
 

@@ -431,6 +431,7 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
          for (int j = 0; j < k; ++j) h_sR[j] = pred[j];
       }
       OFRR_HIP(ctx, rec(2));
+      OFRR_HIP(ctx, hipEventSynchronize(ev[2]));
       const double normA = o->normA > 0 ? o->normA : theta[0];
       nconv = 0;
       for (int j = 0; j < k; ++j) {
