@@ -2,12 +2,14 @@
 //
 // syev: parallel cyclic Jacobi, one workgroup per matrix (batched for the
 // serving path).  A round pairs all indices with the round-robin ("circle")
-// schedule; every 2x2 block (pair I, pair J) of A is then transformed
-// independently, A'_IJ = J_I^T A_IJ J_J, and the eigenvector block V <- V J,
-// so one round needs two barriers.  A rotation is skipped when
-// |a_pq| <= eps/2 sqrt(|a_pp a_qq|) (the Demmel-Veselic threshold), and the
-// solver stops after a sweep without rotations.  The working copy of A lives
-// in LDS when n <= 88 and in global memory (L2 resident) otherwise.
+// schedule; every 2x2 block (pair I, pair J) of the packed upper triangle of A
+// is then transformed independently, A'_IJ = J_I^T A_IJ J_J, so one round
+// needs two barriers.  A rotation is skipped when |a_pq| <= eps/2
+// sqrt(|a_pp a_qq|) (the Demmel-Veselic threshold), and the solver stops after
+// a sweep without rotations.  The packed triangle lives in LDS for n <= 192
+// (145 KiB of the 160 KiB per CU).  The rotations are logged, and a second,
+// many-workgroup kernel forms V = J_1 J_2 ... row by row (rows are
+// independent), so the sequential kernel touches only A.
 // Replaces the host LAPACKE dsyev/dsygv of the reference
 // (CPP/linalg/blas/mvops.cu:1601-1664).
 //
@@ -21,7 +23,9 @@
 namespace ofrr {
 
 constexpr int kJacobiThreads = 1024;
-constexpr int kLdsMaxN = 88;  // n^2 doubles + the static arrays stay below 64 KiB
+constexpr int kLdsMaxN = 192;     // packed upper triangle: n(n+1)/2 doubles <= 144.8 KiB
+constexpr int kMaxSweeps = 30;
+constexpr int kApplyRows = 8;     // rows of V per workgroup in the accumulation kernel
 
 __device__ inline void pair_of(int nn, int r, int s, int &p, int &q) {
    if (s == 0) { p = nn - 1; q = r; }
@@ -29,49 +33,51 @@ __device__ inline void pair_of(int nn, int r, int s, int &p, int &q) {
    if (p > q) { int t = p; p = q; q = t; }
 }
 
-// n_dev (nullable): per-matrix size read from device memory (<= n)
+// packed upper triangle index of the unordered pair {i, j}
+__device__ inline int pk(int i, int j) { return (i <= j) ? (j * (j + 1)) / 2 + i : (i * (i + 1)) / 2 + j; }
+
+// Rotations: A is diagonalised in place (packed storage, LDS or global) and the
+// (c, s) of every pair of every round is logged; the eigenvectors are formed
+// afterwards by jacobi_apply_kernel.  Outputs per matrix: eigenvalues ascending
+// (w), the rank of each diagonal index (rank), and the number of logged sweeps.
 template <bool LDS>
-__global__ __launch_bounds__(kJacobiThreads) void jacobi_kernel(
+__global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
     int n_max, const int32_t *__restrict__ n_dev, const double *__restrict__ A, int lda,
-    int64_t strideA, double *__restrict__ w, int64_t stridew, double *__restrict__ V, int ldv,
-    int64_t strideV, double *__restrict__ work, int64_t strideWork, int max_sweeps) {
+    int64_t strideA, double *__restrict__ w, int64_t stridew, double2 *__restrict__ rotlog,
+    int64_t strideLog, int32_t *__restrict__ rank, int32_t *__restrict__ nsweeps,
+    int64_t strideInts, double *__restrict__ work, int64_t strideWork, int max_sweeps) {
    extern __shared__ double lds[];
    __shared__ double rc[128], rsn[128];
-   __shared__ int rotated;
-   __shared__ int s_n;
+   __shared__ int rotated, s_n;
    const int b = blockIdx.x;
    const double *Ab = A + b * strideA;
-   double *wb = w + b * stridew;
-   double *Vb = V + b * strideV;
+   double2 *log = rotlog + b * strideLog;
    if (threadIdx.x == 0) s_n = n_dev ? min((int)n_dev[b], n_max) : n_max;
    __syncthreads();
    const int n = s_n;
-   if (n <= 0) return;
-   const int nn = n + (n & 1);
-   const int np = nn / 2;
-   double *a = LDS ? lds : work + b * strideWork;                 // n x n, ld n
-   double *v = (LDS ? work + b * strideWork : work + b * strideWork + (int64_t)n_max * n_max);
-   // working copies: A symmetrised from its upper triangle, V = I
-   for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
-      int i = t % n, j = t / n;
-      a[t] = (i <= j) ? Ab[i + (int64_t)j * lda] : Ab[j + (int64_t)i * lda];
-      v[t] = (i == j) ? 1.0 : 0.0;
+   if (n <= 0) {
+      if (threadIdx.x == 0) nsweeps[b * strideInts] = 0;
+      return;
    }
+   const int nn = n + (n & 1), np = nn / 2;
+   double *a = LDS ? lds : work + b * strideWork;
+   for (int j = threadIdx.x; j < n; j += blockDim.x)
+      for (int i = 0; i <= j; ++i) a[pk(i, j)] = Ab[i + (int64_t)j * lda];
    __syncthreads();
+   int done = 0;
    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
       if (threadIdx.x == 0) rotated = 0;
       __syncthreads();
       for (int r = 0; r < nn - 1; ++r) {
-         // rotation of each pair
          for (int s = threadIdx.x; s < np; s += blockDim.x) {
             int p, q;
             pair_of(nn, r, s, p, q);
             double c = 1.0, sn = 0.0;
             if (q < n) {
-               double apq = a[p + q * n], app = a[p + p * n], aqq = a[q + q * n];
+               const double apq = a[pk(p, q)], app = a[pk(p, p)], aqq = a[pk(q, q)];
                if (apq != 0.0 && fabs(apq) > 1.1102230246251565e-16 * sqrt(fabs(app) * fabs(aqq))) {
-                  double th = (aqq - app) / (2.0 * apq);
-                  double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                  const double th = (aqq - app) / (2.0 * apq);
+                  const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
                   c = 1.0 / sqrt(t * t + 1.0);
                   sn = t * c;
                   rotated = 1;
@@ -79,94 +85,159 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_kernel(
             }
             rc[s] = c;
             rsn[s] = sn;
+            log[((int64_t)sweep * (nn - 1) + r) * np + s] = make_double2(c, sn);
          }
          __syncthreads();
-         // A' = J^T A J over 2x2 blocks (I <= J), mirrored; V' = V J
+         // A' = J^T A J over the 2x2 blocks (I <= J) of the packed triangle
          const int nblocks = np * np;
          for (int t = threadIdx.x; t < nblocks; t += blockDim.x) {
-            int I = t / np, J = t % np;
+            const int I = t / np, J = t % np;
             if (I > J) continue;
+            const double s1 = rsn[I], s2 = rsn[J];
+            if (s1 == 0.0 && s2 == 0.0) continue;  // both rotations are the identity
             int p1, q1, p2, q2;
             pair_of(nn, r, I, p1, q1);
             pair_of(nn, r, J, p2, q2);
-            const double c1 = rc[I], s1 = rsn[I], c2 = rc[J], s2 = rsn[J];
+            const double c1 = rc[I], c2 = rc[J];
             const bool q1ok = q1 < n, q2ok = q2 < n;
-            double a11 = a[p1 + p2 * n];
-            double a12 = q2ok ? a[p1 + q2 * n] : 0.0;
-            double a21 = q1ok ? a[q1 + p2 * n] : 0.0;
-            double a22 = (q1ok && q2ok) ? a[q1 + q2 * n] : 0.0;
-            // M J2
-            double m11 = c2 * a11 - s2 * a12, m12 = s2 * a11 + c2 * a12;
-            double m21 = c2 * a21 - s2 * a22, m22 = s2 * a21 + c2 * a22;
-            // J1^T (M J2)
-            double b11 = c1 * m11 - s1 * m21, b12 = c1 * m12 - s1 * m22;
-            double b21 = s1 * m11 + c1 * m21, b22 = s1 * m12 + c1 * m22;
             if (I == J) {
-               // diagonal block: the pair's off-diagonal entry is annihilated
-               a[p1 + p1 * n] = b11;
-               if (q1ok) {
-                  a[q1 + q1 * n] = b22;
-                  double off = (s1 != 0.0) ? 0.0 : 0.5 * (b12 + b21);
-                  a[p1 + q1 * n] = off;
-                  a[q1 + p1 * n] = off;
-               }
+               if (!q1ok) continue;
+               const double app = a[pk(p1, p1)], apq = a[pk(p1, q1)], aqq = a[pk(q1, q1)];
+               // J^T [[app apq][apq aqq]] J with J = [[c s][-s c]]
+               const double m11 = c1 * app - s1 * apq, m12 = s1 * app + c1 * apq;
+               const double m21 = c1 * apq - s1 * aqq, m22 = s1 * apq + c1 * aqq;
+               a[pk(p1, p1)] = c1 * m11 - s1 * m21;
+               a[pk(q1, q1)] = s1 * m12 + c1 * m22;
+               a[pk(p1, q1)] = 0.0;
             } else {
-               a[p1 + p2 * n] = b11; a[p2 + p1 * n] = b11;
-               if (q2ok) { a[p1 + q2 * n] = b12; a[q2 + p1 * n] = b12; }
-               if (q1ok) { a[q1 + p2 * n] = b21; a[p2 + q1 * n] = b21; }
-               if (q1ok && q2ok) { a[q1 + q2 * n] = b22; a[q2 + q1 * n] = b22; }
+               const double a11 = a[pk(p1, p2)];
+               const double a12 = q2ok ? a[pk(p1, q2)] : 0.0;
+               const double a21 = q1ok ? a[pk(q1, p2)] : 0.0;
+               const double a22 = (q1ok && q2ok) ? a[pk(q1, q2)] : 0.0;
+               const double m11 = c2 * a11 - s2 * a12, m12 = s2 * a11 + c2 * a12;
+               const double m21 = c2 * a21 - s2 * a22, m22 = s2 * a21 + c2 * a22;
+               a[pk(p1, p2)] = c1 * m11 - s1 * m21;
+               if (q2ok) a[pk(p1, q2)] = c1 * m12 - s1 * m22;
+               if (q1ok) a[pk(q1, p2)] = s1 * m11 + c1 * m21;
+               if (q1ok && q2ok) a[pk(q1, q2)] = s1 * m12 + c1 * m22;
             }
-         }
-         for (int t = threadIdx.x; t < n * np; t += blockDim.x) {
-            int row = t % n, s = t / n;
-            int p, q;
-            pair_of(nn, r, s, p, q);
-            if (q >= n) continue;
-            double c = rc[s], sn = rsn[s];
-            if (sn == 0.0) continue;
-            double vp = v[row + p * n], vq = v[row + q * n];
-            v[row + p * n] = c * vp - sn * vq;
-            v[row + q * n] = sn * vp + c * vq;
          }
          __syncthreads();
       }
       if (!rotated) break;
+      done = sweep + 1;
       __syncthreads();
    }
-   // eigenvalues ascending (stable rank), vectors permuted accordingly
+   if (threadIdx.x == 0) nsweeps[b * strideInts] = done;
+   double *wb = w + b * stridew;
+   int32_t *rk = rank + b * strideInts;
    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      double di = a[i + i * n];
-      int rank = 0;
+      const double di = a[pk(i, i)];
+      int rr = 0;
       for (int j = 0; j < n; ++j) {
-         double dj = a[j + j * n];
-         rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
+         const double dj = a[pk(j, j)];
+         rr += (dj < di || (dj == di && j < i)) ? 1 : 0;
       }
-      wb[rank] = di;
-      for (int row = 0; row < n; ++row) Vb[row + (int64_t)rank * ldv] = v[row + i * n];
+      wb[rr] = di;
+      rk[i] = rr;
    }
 }
 
-static size_t jacobi_work_doubles(int n) { return 2 * (size_t)n * n; }
+// V = J_1 J_2 ... (all logged rotations) applied to the identity, rows
+// independent: each workgroup keeps kApplyRows rows of V in LDS; the columns
+// are written to their ascending-eigenvalue positions
+__global__ __launch_bounds__(256) void jacobi_apply_kernel(
+    int n_max, const int32_t *__restrict__ n_dev, const double2 *__restrict__ rotlog,
+    int64_t strideLog, const int32_t *__restrict__ rank, const int32_t *__restrict__ nsweeps,
+    int64_t strideInts, double *__restrict__ V, int ldv, int64_t strideV) {
+   __shared__ double rows[kApplyRows][256];
+   const int b = blockIdx.y;
+   const int n = n_dev ? min((int)n_dev[b], n_max) : n_max;
+   const int r0 = blockIdx.x * kApplyRows;
+   if (n <= 0 || r0 >= n) return;
+   const int nn = n + (n & 1), np = nn / 2;
+   const int rounds = nsweeps[b * strideInts] * (nn - 1);
+   const double2 *log = rotlog + b * strideLog;
+   for (int t = threadIdx.x; t < kApplyRows * nn; t += blockDim.x) {
+      const int rr = t / nn, c = t % nn;
+      rows[rr][c] = (r0 + rr == c) ? 1.0 : 0.0;
+   }
+   __syncthreads();
+   const int items = kApplyRows * np;
+   for (int rd = 0; rd < rounds; ++rd) {
+      const int r = rd % (nn - 1);
+      const double2 *lr = log + (int64_t)rd * np;
+      for (int t = threadIdx.x; t < items; t += blockDim.x) {
+         const int rr = t / np, s = t % np;
+         const double2 cs = lr[s];
+         if (cs.y == 0.0) continue;
+         int p, q;
+         pair_of(nn, r, s, p, q);
+         const double vp = rows[rr][p], vq = rows[rr][q];
+         rows[rr][p] = cs.x * vp - cs.y * vq;
+         rows[rr][q] = cs.y * vp + cs.x * vq;
+      }
+      __syncthreads();
+   }
+   double *Vb = V + b * strideV;
+   const int32_t *rk = rank + b * strideInts;
+   for (int t = threadIdx.x; t < kApplyRows * n; t += blockDim.x) {
+      const int rr = t / n, c = t % n;
+      if (r0 + rr < n) Vb[(r0 + rr) + (int64_t)rk[c] * ldv] = rows[rr][c];
+   }
+}
+
+// per-matrix scratch (doubles): [packed A | rotation log | ranks | sweep count]
+struct JacobiLayout {
+   size_t packed, log, ints, total;
+};
+static JacobiLayout jacobi_layout(int n) {
+   const int nn = n + (n & 1);
+   JacobiLayout L;
+   L.packed = ((size_t)n * (n + 1) / 2 + 1) & ~(size_t)1;  // even: the log is double2
+   L.log = (size_t)kMaxSweeps * (nn - 1) * (nn / 2) * 2;
+   L.ints = ((size_t)n + 2 + 1) / 2;
+   L.total = (L.packed + L.log + L.ints + 1) & ~(size_t)1;
+   return L;
+}
+static size_t jacobi_work_doubles(int n) { return jacobi_layout(n).total; }
 
 int syev_batched(ofrr_ctx *ctx, int batch, int n, const int32_t *n_dev, const double *A, int lda,
                  int64_t strideA, double *w, int64_t stridew, double *V, int ldv,
                  int64_t strideV, double *work /* nullable: workspace */) {
    if (n <= 0 || batch <= 0) return kSuccess;
-   const int64_t strideWork = (int64_t)jacobi_work_doubles(n);
+   if (n > 256) return set_error(ctx, kErrorBLASInvalidSize, "syev: n > 256 not supported");
+   const JacobiLayout L = jacobi_layout(n);
+   const int64_t strideWork = (int64_t)L.total;
    if (!work) {
       work = (double *)workspace(ctx, (size_t)batch * strideWork * sizeof(double));
       if (!work) return set_error(ctx, kErrorCUDAMemory, "syev: workspace");
    }
-   const int sweeps = 40;
+   double2 *rotlog = reinterpret_cast<double2 *>(work + L.packed);
+   int32_t *rank = reinterpret_cast<int32_t *>(work + L.packed + L.log);
+   int32_t *nsw = rank + n;
+   const int64_t strideLog = strideWork / 2, strideInts = strideWork * 2;
    if (n <= kLdsMaxN) {
-      size_t lds = (size_t)n * n * sizeof(double);
-      jacobi_kernel<true><<<batch, kJacobiThreads, lds, ctx->stream>>>(
-          n, n_dev, A, lda, strideA, w, stridew, V, ldv, strideV, work, strideWork, sweeps);
+      static bool attr_set = false;
+      const size_t lds = (size_t)n * (n + 1) / 2 * sizeof(double);
+      if (!attr_set) {
+         OFRR_HIP(ctx, hipFuncSetAttribute((const void *)jacobi_rot_kernel<true>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)((size_t)kLdsMaxN * (kLdsMaxN + 1) / 2 * sizeof(double))));
+         attr_set = true;
+      }
+      jacobi_rot_kernel<true><<<batch, kJacobiThreads, lds, ctx->stream>>>(
+          n, n_dev, A, lda, strideA, w, stridew, rotlog, strideLog, rank, nsw, strideInts, work,
+          strideWork, kMaxSweeps);
    } else {
-      if (n > 256) return set_error(ctx, kErrorBLASInvalidSize, "syev: n > 256 not supported");
-      jacobi_kernel<false><<<batch, kJacobiThreads, 0, ctx->stream>>>(
-          n, n_dev, A, lda, strideA, w, stridew, V, ldv, strideV, work, strideWork, sweeps);
+      jacobi_rot_kernel<false><<<batch, kJacobiThreads, 0, ctx->stream>>>(
+          n, n_dev, A, lda, strideA, w, stridew, rotlog, strideLog, rank, nsw, strideInts, work,
+          strideWork, kMaxSweeps);
    }
+   OFRR_LAUNCH(ctx);
+   dim3 grid((n + kApplyRows - 1) / kApplyRows, batch);
+   jacobi_apply_kernel<<<grid, 256, 0, ctx->stream>>>(n, n_dev, rotlog, strideLog, rank, nsw,
+                                                     strideInts, V, ldv, strideV);
    OFRR_LAUNCH(ctx);
    return kSuccess;
 }
@@ -266,13 +337,13 @@ int rr_truncated(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H
    // workspace: sc(m) Gs Hs U Bm T Hr Z (m*m each) lam th (m) rank(1) + jacobi work
    const size_t mm = (size_t)m * m;
    const size_t jw = jacobi_work_doubles(m);
-   size_t need = (m + 6 * mm + 2 * m + 8 + jw) * sizeof(double);
+   size_t need = (m + 6 * mm + 2 * m + 10 + jw) * sizeof(double);
    double *base = (double *)workspace(ctx, need);
    if (!base) return set_error(ctx, kErrorCUDAMemory, "rr_truncated: workspace");
    double *sc = base, *Gs = sc + m, *Hs = Gs + mm, *U = Hs + mm, *Bm = U + mm, *T = Bm + mm;
    double *Hr = T + mm, *lam = Hr + mm, *th = lam + m;
    int32_t *rk = (int32_t *)(th + m);
-   double *jwork = th + m + 8;
+   double *jwork = base + ((3 * (size_t)m + 6 * mm + 8 + 1) & ~(size_t)1);  // 16-byte aligned
    double *Z = Gs;  // Gs is dead after its eigendecomposition
    int32_t *rank_out = rank ? rank : rk;
    rr_scale_kernel<<<(unsigned)((mm + 255) / 256), 256, 0, ctx->stream>>>(m, G, ldg, H, ldh, sc, Gs, Hs);
@@ -323,7 +394,7 @@ __global__ __launch_bounds__(1024) void chol_transform_kernel(int m, const doubl
       __syncthreads();
       for (int t = threadIdx.x; t < (m - j - 1) * (m - j - 1); t += blockDim.x) {
          int i = j + 1 + t % (m - j - 1), c = j + 1 + t / (m - j - 1);
-         if (i >= c) L[i + c * m] -= L[i + j * m] * L[c + j * m];
+         if (i >= c) L[i + c * m] = fma(-L[i + j * m], L[c + j * m], L[i + c * m]);
       }
       __syncthreads();
    }
@@ -333,14 +404,14 @@ __global__ __launch_bounds__(1024) void chol_transform_kernel(int m, const doubl
    for (int c = threadIdx.x; c < m; c += blockDim.x)
       for (int i = 0; i < m; ++i) {
          double s = Cm[i + c * m];
-         for (int p = 0; p < i; ++p) s -= L[i + p * m] * Cm[p + c * m];
+         for (int p = 0; p < i; ++p) s = fma(-L[i + p * m], Cm[p + c * m], s);
          Cm[i + c * m] = s / L[i + i * m];
       }
    __syncthreads();
    for (int r = threadIdx.x; r < m; r += blockDim.x)
       for (int j = 0; j < m; ++j) {
          double s = Cm[r + j * m];
-         for (int p = 0; p < j; ++p) s -= L[j + p * m] * Cm[r + p * m];
+         for (int p = 0; p < j; ++p) s = fma(-L[j + p * m], Cm[r + p * m], s);
          Cm[r + j * m] = s / L[j + j * m];
       }
    __syncthreads();
@@ -364,7 +435,7 @@ __global__ void chol_back_kernel(int m, const double *__restrict__ L, const doub
    wout[c] = w[src];
    for (int i = m - 1; i >= 0; --i) {
       double s = Yv[i + (int64_t)src * m];
-      for (int p = i + 1; p < m; ++p) s -= L[p + i * m] * V[p + (int64_t)c * ldv];
+      for (int p = i + 1; p < m; ++p) s = fma(-L[p + i * m], V[p + (int64_t)c * ldv], s);
       V[i + (int64_t)c * ldv] = s / L[i + i * m];
    }
 }

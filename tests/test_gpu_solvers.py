@@ -70,7 +70,7 @@ def test_filtered_fp64_matches_cpu(ctx, nd):
         st, th, X, res, stats = api.filtered_eig(ctx, _csr(ctx, Ah), torch.as_tensor(X0, device="cuda:0"),
                                                  nwant, nd=nd, degree=10, maxiter=iters,
                                                  basis_dtype=torch.float64, tol=1e-30)
-        assert st == 4001 and stats["iterations"] == iters
+        assert st == 4001 and stats["iterations"] == iters, (st, stats)
         assert np.max(np.abs(th - th_o) / np.abs(th_o)) < 1e-12
         assert np.allclose(res[:nwant], res_o[:nwant], rtol=1e-6, atol=1e-15)
 
