@@ -48,6 +48,7 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
     int64_t strideInts, double *__restrict__ work, int64_t strideWork, int max_sweeps) {
    extern __shared__ double lds[];
    __shared__ double rc[128], rsn[128];
+   __shared__ int sp_[128], sq_[128];
    __shared__ int rotated, s_n;
    const int b = blockIdx.x;
    const double *Ab = A + b * strideA;
@@ -77,14 +78,16 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
                const double apq = a[pk(p, q)], app = a[pk(p, p)], aqq = a[pk(q, q)];
                if (apq != 0.0 && fabs(apq) > 1.1102230246251565e-16 * sqrt(fabs(app) * fabs(aqq))) {
                   const double th = (aqq - app) / (2.0 * apq);
-                  const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-                  c = 1.0 / sqrt(t * t + 1.0);
+                  const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(fma(th, th, 1.0)));
+                  c = 1.0 / sqrt(fma(t, t, 1.0));
                   sn = t * c;
                   rotated = 1;
                }
             }
             rc[s] = c;
             rsn[s] = sn;
+            sp_[s] = p;
+            sq_[s] = q;
             log[((int64_t)sweep * (nn - 1) + r) * np + s] = make_double2(c, sn);
          }
          __syncthreads();
@@ -95,31 +98,29 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
             if (I > J) continue;
             const double s1 = rsn[I], s2 = rsn[J];
             if (s1 == 0.0 && s2 == 0.0) continue;  // both rotations are the identity
-            int p1, q1, p2, q2;
-            pair_of(nn, r, I, p1, q1);
-            pair_of(nn, r, J, p2, q2);
+            const int p1 = sp_[I], q1 = sq_[I], p2 = sp_[J], q2 = sq_[J];
             const double c1 = rc[I], c2 = rc[J];
             const bool q1ok = q1 < n, q2ok = q2 < n;
+            // J^T A_IJ J with J = [[c s][-s c]], explicit fused multiply-adds
             if (I == J) {
                if (!q1ok) continue;
                const double app = a[pk(p1, p1)], apq = a[pk(p1, q1)], aqq = a[pk(q1, q1)];
-               // J^T [[app apq][apq aqq]] J with J = [[c s][-s c]]
-               const double m11 = c1 * app - s1 * apq, m12 = s1 * app + c1 * apq;
-               const double m21 = c1 * apq - s1 * aqq, m22 = s1 * apq + c1 * aqq;
-               a[pk(p1, p1)] = c1 * m11 - s1 * m21;
-               a[pk(q1, q1)] = s1 * m12 + c1 * m22;
+               const double m11 = fma(c1, app, -(s1 * apq)), m12 = fma(s1, app, c1 * apq);
+               const double m21 = fma(c1, apq, -(s1 * aqq)), m22 = fma(s1, apq, c1 * aqq);
+               a[pk(p1, p1)] = fma(c1, m11, -(s1 * m21));
+               a[pk(q1, q1)] = fma(s1, m12, c1 * m22);
                a[pk(p1, q1)] = 0.0;
             } else {
                const double a11 = a[pk(p1, p2)];
                const double a12 = q2ok ? a[pk(p1, q2)] : 0.0;
                const double a21 = q1ok ? a[pk(q1, p2)] : 0.0;
                const double a22 = (q1ok && q2ok) ? a[pk(q1, q2)] : 0.0;
-               const double m11 = c2 * a11 - s2 * a12, m12 = s2 * a11 + c2 * a12;
-               const double m21 = c2 * a21 - s2 * a22, m22 = s2 * a21 + c2 * a22;
-               a[pk(p1, p2)] = c1 * m11 - s1 * m21;
-               if (q2ok) a[pk(p1, q2)] = c1 * m12 - s1 * m22;
-               if (q1ok) a[pk(q1, p2)] = s1 * m11 + c1 * m21;
-               if (q1ok && q2ok) a[pk(q1, q2)] = s1 * m12 + c1 * m22;
+               const double m11 = fma(c2, a11, -(s2 * a12)), m12 = fma(s2, a11, c2 * a12);
+               const double m21 = fma(c2, a21, -(s2 * a22)), m22 = fma(s2, a21, c2 * a22);
+               a[pk(p1, p2)] = fma(c1, m11, -(s1 * m21));
+               if (q2ok) a[pk(p1, q2)] = fma(c1, m12, -(s1 * m22));
+               if (q1ok) a[pk(q1, p2)] = fma(s1, m11, c1 * m21);
+               if (q1ok && q2ok) a[pk(q1, q2)] = fma(s1, m12, c1 * m22);
             }
          }
          __syncthreads();
@@ -174,8 +175,8 @@ __global__ __launch_bounds__(256) void jacobi_apply_kernel(
          int p, q;
          pair_of(nn, r, s, p, q);
          const double vp = rows[rr][p], vq = rows[rr][q];
-         rows[rr][p] = cs.x * vp - cs.y * vq;
-         rows[rr][q] = cs.y * vp + cs.x * vq;
+         rows[rr][p] = fma(cs.x, vp, -(cs.y * vq));
+         rows[rr][q] = fma(cs.y, vp, cs.x * vq);
       }
       __syncthreads();
    }

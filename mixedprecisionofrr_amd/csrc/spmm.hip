@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void spmm_resid_kernel(
 #pragma unroll
       for (int v = 0; v < VEC; ++v) {
          wv.v[v] = acc[v];
-         double r = acc[v] - th[v] * xv.v[v];
+         double r = fma(-th[v], xv.v[v], acc[v]);
          s_r[v] = fma(r, r, s_r[v]);
          s_x[v] = fma(xv.v[v], xv.v[v], s_x[v]);
          mx[v] = fmaxf(mx[v], (float)fabs(r));

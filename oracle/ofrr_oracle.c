@@ -275,61 +275,107 @@ void orc_hessenberg(int64_t m, int n, double *Q, int64_t ldq, double *R, int ldr
 }
 
 /* ------------------------------------------------------------------------- */
-/* symmetric eigensolver: cyclic Jacobi with the Demmel-Veselic threshold      */
+/* symmetric eigensolver: parallel-ordered cyclic Jacobi (round-robin pairing,  */
+/* Brent & Luk 1985), Demmel-Veselic threshold, explicit fused multiply-adds.   */
+/* Every round rotates all disjoint pairs at once; the 2x2 blocks of A are      */
+/* transformed independently and the rotations are then applied to V in round   */
+/* order -- the same arithmetic, in the same order, as the device solver.       */
 /* ------------------------------------------------------------------------- */
-static int jacobi_eig(int n, double *a /* n x n, destroyed */, double *w, double *V, int ldv) {
+static void pair_of(int nn, int r, int s, int *p, int *q) {
+   int a, b;
+   if (s == 0) { a = nn - 1; b = r; }
+   else { a = (r + s) % (nn - 1); b = (r - s + nn - 1) % (nn - 1); }
+   if (a > b) { int t = a; a = b; b = t; }
+   *p = a; *q = b;
+}
+
+static int jacobi_eig(int n, double *a /* n x n symmetric, destroyed */, double *w, double *V,
+                      int ldv) {
+   const int nn = n + (n & 1), np = nn / 2;
+   double *rc = (double *)malloc(sizeof(double) * (size_t)np);
+   double *rs = (double *)malloc(sizeof(double) * (size_t)np);
+   int *pp = (int *)malloc(sizeof(int) * (size_t)np), *qq = (int *)malloc(sizeof(int) * (size_t)np);
+   double *Vw = (double *)malloc(sizeof(double) * (size_t)n * n);
    for (int j = 0; j < n; ++j)
-      for (int i = 0; i < n; ++i) V[i + (int64_t)j * ldv] = (i == j) ? 1.0 : 0.0;
+      for (int i = 0; i < n; ++i) Vw[i + (size_t)j * n] = (i == j) ? 1.0 : 0.0;
+#define A_(i, j) a[(i) + (size_t)(j) * n]
    int sweep;
-   for (sweep = 0; sweep < 80; ++sweep) {
+   for (sweep = 0; sweep < 30; ++sweep) {
       int rotated = 0;
-      for (int p = 0; p < n - 1; ++p)
-         for (int q = p + 1; q < n; ++q) {
-            double apq = a[p + q * n];
-            double app = a[p + p * n], aqq = a[q + q * n];
-            if (apq == 0.0) continue;
-            if (fabs(apq) <= DBL_EPSILON * 0.5 * sqrt(fabs(app) * fabs(aqq))) {
-               a[p + q * n] = a[q + p * n] = 0.0;
-               continue;
+      for (int r = 0; r < nn - 1; ++r) {
+         for (int s = 0; s < np; ++s) {
+            int p, q;
+            pair_of(nn, r, s, &p, &q);
+            pp[s] = p; qq[s] = q;
+            double c = 1.0, sn = 0.0;
+            if (q < n) {
+               const double apq = A_(p, q), app = A_(p, p), aqq = A_(q, q);
+               if (apq != 0.0 && fabs(apq) > 1.1102230246251565e-16 * sqrt(fabs(app) * fabs(aqq))) {
+                  const double th = (aqq - app) / (2.0 * apq);
+                  const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(fma(th, th, 1.0)));
+                  c = 1.0 / sqrt(fma(t, t, 1.0));
+                  sn = t * c;
+                  rotated = 1;
+               }
             }
-            rotated = 1;
-            double th = (aqq - app) / (2.0 * apq);
-            double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-            double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-            for (int r = 0; r < n; ++r) { /* columns p, q */
-               double arp = a[r + p * n], arq = a[r + q * n];
-               a[r + p * n] = c * arp - s * arq;
-               a[r + q * n] = s * arp + c * arq;
+            rc[s] = c; rs[s] = sn;
+         }
+         for (int I = 0; I < np; ++I)
+            for (int J = I; J < np; ++J) {
+               const double s1 = rs[I], s2 = rs[J];
+               if (s1 == 0.0 && s2 == 0.0) continue;
+               const int p1 = pp[I], q1 = qq[I], p2 = pp[J], q2 = qq[J];
+               const double c1 = rc[I], c2 = rc[J];
+               const int q1ok = q1 < n, q2ok = q2 < n;
+               if (I == J) {
+                  if (!q1ok) continue;
+                  const double app = A_(p1, p1), apq = A_(p1, q1), aqq = A_(q1, q1);
+                  const double m11 = fma(c1, app, -(s1 * apq)), m12 = fma(s1, app, c1 * apq);
+                  const double m21 = fma(c1, apq, -(s1 * aqq)), m22 = fma(s1, apq, c1 * aqq);
+                  A_(p1, p1) = fma(c1, m11, -(s1 * m21));
+                  A_(q1, q1) = fma(s1, m12, c1 * m22);
+                  A_(p1, q1) = A_(q1, p1) = 0.0;
+               } else {
+                  const double a11 = A_(p1, p2);
+                  const double a12 = q2ok ? A_(p1, q2) : 0.0;
+                  const double a21 = q1ok ? A_(q1, p2) : 0.0;
+                  const double a22 = (q1ok && q2ok) ? A_(q1, q2) : 0.0;
+                  const double m11 = fma(c2, a11, -(s2 * a12)), m12 = fma(s2, a11, c2 * a12);
+                  const double m21 = fma(c2, a21, -(s2 * a22)), m22 = fma(s2, a21, c2 * a22);
+                  const double b11 = fma(c1, m11, -(s1 * m21)), b12 = fma(c1, m12, -(s1 * m22));
+                  const double b21 = fma(s1, m11, c1 * m21), b22 = fma(s1, m12, c1 * m22);
+                  A_(p1, p2) = A_(p2, p1) = b11;
+                  if (q2ok) A_(p1, q2) = A_(q2, p1) = b12;
+                  if (q1ok) A_(q1, p2) = A_(p2, q1) = b21;
+                  if (q1ok && q2ok) A_(q1, q2) = A_(q2, q1) = b22;
+               }
             }
-            for (int r = 0; r < n; ++r) { /* rows p, q */
-               double apr = a[p + r * n], aqr = a[q + r * n];
-               a[p + r * n] = c * apr - s * aqr;
-               a[q + r * n] = s * apr + c * aqr;
-            }
-            a[p + q * n] = a[q + p * n] = 0.0;
-            for (int r = 0; r < n; ++r) {
-               double vrp = V[r + (int64_t)p * ldv], vrq = V[r + (int64_t)q * ldv];
-               V[r + (int64_t)p * ldv] = c * vrp - s * vrq;
-               V[r + (int64_t)q * ldv] = s * vrp + c * vrq;
+         for (int s = 0; s < np; ++s) {
+            if (rs[s] == 0.0 || qq[s] >= n) continue;
+            const double c = rc[s], sn = rs[s];
+            double *vp = Vw + (size_t)pp[s] * n, *vq = Vw + (size_t)qq[s] * n;
+            for (int row = 0; row < n; ++row) {
+               const double x = vp[row], y = vq[row];
+               vp[row] = fma(c, x, -(sn * y));
+               vq[row] = fma(sn, x, c * y);
             }
          }
+      }
       if (!rotated) break;
    }
-   for (int i = 0; i < n; ++i) w[i] = a[i + i * n];
-   /* sort ascending (stable selection), permute vectors */
-   double *tmp = (double *)malloc(sizeof(double) * (size_t)n);
-   for (int i = 0; i < n - 1; ++i) {
-      int mi = i;
-      for (int j = i + 1; j < n; ++j)
-         if (w[j] < w[mi]) mi = j;
-      if (mi != i) {
-         double t = w[i]; w[i] = w[mi]; w[mi] = t;
-         memcpy(tmp, V + (int64_t)i * ldv, sizeof(double) * (size_t)n);
-         memcpy(V + (int64_t)i * ldv, V + (int64_t)mi * ldv, sizeof(double) * (size_t)n);
-         memcpy(V + (int64_t)mi * ldv, tmp, sizeof(double) * (size_t)n);
+   /* eigenvalues ascending (stable rank), vectors permuted accordingly */
+   for (int i = 0; i < n; ++i) {
+      const double di = A_(i, i);
+      int rk = 0;
+      for (int j = 0; j < n; ++j) {
+         const double dj = A_(j, j);
+         rk += (dj < di || (dj == di && j < i)) ? 1 : 0;
       }
+      w[rk] = di;
+      memcpy(V + (int64_t)rk * ldv, Vw + (size_t)i * n, sizeof(double) * (size_t)n);
    }
-   free(tmp);
+#undef A_
+   free(rc); free(rs); free(pp); free(qq); free(Vw);
    return sweep;
 }
 
@@ -429,8 +475,8 @@ int orc_rr_truncated(int m, const double *G, int ldg, const double *H, int ldh, 
    double *B = (double *)malloc(sizeof(double) * (size_t)m * (r ? r : 1));
    for (int c = 0; c < r; ++c) {
       int src = m - r + c;
-      double f = 1.0 / sqrt(lam[src]);
-      for (int i = 0; i < m; ++i) B[i + c * m] = U[i + src * m] * f;
+      const double f = sqrt(lam[src]);
+      for (int i = 0; i < m; ++i) B[i + c * m] = U[i + src * m] / f;
    }
    /* Hr = B^T Hs B */
    double *T = (double *)malloc(sizeof(double) * (size_t)m * (r ? r : 1));
@@ -438,12 +484,14 @@ int orc_rr_truncated(int m, const double *G, int ldg, const double *H, int ldh, 
    double *Z = (double *)malloc(sizeof(double) * (size_t)(r ? r : 1) * (r ? r : 1));
    double *th = (double *)malloc(sizeof(double) * (size_t)(r ? r : 1));
    orc_gemm_nn(m, r, m, Hs, m, B, m, T, m);
-   orc_gemm_tn(m, r, r, B, m, T, m, Hr, r);
-   for (int j = 0; j < r; ++j)
-      for (int i = j + 1; i < r; ++i) {
-         double s = 0.5 * (Hr[i + j * r] + Hr[j + i * r]);
-         Hr[i + j * r] = Hr[j + i * r] = s;
+   for (int a2 = 0; a2 < r; ++a2) /* Hr = B^T T, one sequential dot per entry */
+      for (int b2 = 0; b2 < r; ++b2) {
+         double acc = 0.0;
+         for (int p = 0; p < m; ++p) acc = fma(B[p + a2 * m], T[p + b2 * m], acc);
+         Hr[a2 + b2 * r] = acc;
       }
+   for (int j = 0; j < r; ++j) /* the eigensolver reads the upper triangle */
+      for (int i = j + 1; i < r; ++i) Hr[i + j * r] = Hr[j + i * r];
    if (r > 0) jacobi_eig(r, Hr, th, Z, r);
    int nout = nkeep < r ? nkeep : r;
    for (int c = 0; c < nout; ++c) {
@@ -451,7 +499,7 @@ int orc_rr_truncated(int m, const double *G, int ldg, const double *H, int ldh, 
       theta[c] = th[src];
       for (int i = 0; i < m; ++i) {
          double s = 0.0;
-         for (int p = 0; p < r; ++p) s += B[i + p * m] * Z[p + src * r];
+         for (int p = 0; p < r; ++p) s = fma(B[i + p * m], Z[p + src * r], s);
          Y[i + (int64_t)c * ldy] = s * sc[i];
       }
    }
@@ -589,7 +637,7 @@ void orc_residuals(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, co
    for (int j = 0; j < k; ++j) {
       double s = 0.0, t = 0.0;
       for (int64_t i = 0; i < n; ++i) {
-         double r = AX[i + j * n] - theta[j] * X[i + j * ldx];
+         double r = fma(-theta[j], X[i + j * ldx], AX[i + j * n]);
          s += r * r;
          t += X[i + j * ldx] * X[i + j * ldx];
       }
@@ -638,7 +686,7 @@ int orc_filtered_ofrr(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
       for (int j = 0; j < k; ++j) {
          double s = 0.0, t = 0.0;
          for (int64_t i = 0; i < n; ++i) {
-            double r = W[i + j * n] - theta[j] * Xc[i + j * n];
+            double r = fma(-theta[j], Xc[i + j * n], W[i + j * n]);
             Rr[i + j * n] = r;
             s += r * r;
             t += Xc[i + j * n] * Xc[i + j * n];

@@ -28,7 +28,8 @@ def test_arnoldi_cfg1_matches_cpu(ctx):
         th, X, built = api.arnoldi_hess_eig(ctx, A, 20, 60, maxiter, v0)
         th_o, X_o, built_o = O.arnoldi_hess_eig(Ah, 20, 60, maxiter, v0h)
         assert built == built_o == 61
-        assert np.max(np.abs(th - th_o) / np.abs(th_o)) < 1e-12
+        # the FP64 pipeline runs the same operations in the same order on both sides
+        assert np.max(np.abs(th - th_o) / np.abs(th_o)) <= 1e-14
         res = O.residuals(Ah, X.cpu().numpy(), th, O.laplacian_eigs(2, 64)[0])
         res_o = O.residuals(Ah, X_o, th_o, O.laplacian_eigs(2, 64)[0])
         assert np.allclose(res, res_o, rtol=1e-6, atol=1e-13)
@@ -53,7 +54,7 @@ def test_subspace_iteration_matches_cpu(ctx):
     th_o, V_o = O.subspace_iter_eig(Ah, V0, maxiter=40, step_size=2)
     th, V = api.subspace_iter_eig(ctx, _csr(ctx, Ah), torch.as_tensor(V0, device="cuda:0"),
                                   maxiter=40, step_size=2)
-    assert np.max(np.abs(th - th_o) / np.abs(th_o)) < 1e-12
+    assert np.max(np.abs(th - th_o) / np.abs(th_o)) <= 1e-14
 
 
 @pytest.mark.parametrize("nd", [1, 2])
@@ -72,7 +73,10 @@ def test_filtered_fp64_matches_cpu(ctx, nd):
                                                  basis_dtype=torch.float64, tol=1e-30)
         assert st == 4001 and stats["iterations"] == iters, (st, stats)
         assert np.max(np.abs(th - th_o) / np.abs(th_o)) < 1e-12
-        assert np.allclose(res[:nwant], res_o[:nwant], rtol=1e-6, atol=1e-15)
+        # individual Ritz vectors are not unique inside the Laplacian's degenerate
+        # clusters; the residual of the whole block is
+        fro, fro_o = np.sqrt(np.sum(res[:nwant] ** 2)), np.sqrt(np.sum(res_o[:nwant] ** 2))
+        assert abs(fro - fro_o) <= 1e-6 * fro_o
 
 
 @pytest.mark.parametrize("bdt", [torch.float16, torch.bfloat16, torch.float32, torch.float64])
