@@ -4,9 +4,9 @@
 // serving path).  A round pairs all indices with the round-robin ("circle")
 // schedule; every 2x2 block (pair I, pair J) of the packed upper triangle of A
 // is then transformed independently, A'_IJ = J_I^T A_IJ J_J, so one round
-// needs two barriers.  A rotation is skipped when |a_pq| <= eps/2
-// sqrt(|a_pp a_qq|) (the Demmel-Veselic threshold), and the solver stops after
-// a sweep without rotations.  The packed triangle lives in LDS for n <= 192
+// needs two barriers.  A rotation is skipped when |a_pq| <= max(eps/2
+// sqrt(|a_pp a_qq|), eps max|a_ij|) (the Demmel-Veselic threshold with an
+// absolute floor), and the solver stops after a sweep without rotations.  The packed triangle lives in LDS for n <= 192
 // (145 KiB of the 160 KiB per CU).  The rotations are logged, and a second,
 // many-workgroup kernel forms V = J_1 J_2 ... row by row (rows are
 // independent), so the sequential kernel touches only A.
@@ -50,6 +50,7 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
    __shared__ double rc[128], rsn[128];
    __shared__ int sp_[128], sq_[128];
    __shared__ int rotated, s_n;
+   __shared__ double s_amax[16];
    const int b = blockIdx.x;
    const double *Ab = A + b * strideA;
    double2 *log = rotlog + b * strideLog;
@@ -62,9 +63,22 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
    }
    const int nn = n + (n & 1), np = nn / 2;
    double *a = LDS ? lds : work + b * strideWork;
+   double lmax = 0.0;
    for (int j = threadIdx.x; j < n; j += blockDim.x)
-      for (int i = 0; i <= j; ++i) a[pk(i, j)] = Ab[i + (int64_t)j * lda];
+      for (int i = 0; i <= j; ++i) {
+         const double v = Ab[i + (int64_t)j * lda];
+         a[pk(i, j)] = v;
+         lmax = fmax(lmax, fabs(v));
+      }
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o, 64));
+   if ((threadIdx.x & 63) == 0) s_amax[threadIdx.x >> 6] = lmax;
    __syncthreads();
+   double amax = 0.0;
+   for (int i = 0; i < (int)(blockDim.x + 63) / 64; ++i) amax = fmax(amax, s_amax[i]);
+   // rotations below eps * max|a_ij| cannot change any eigenvalue beyond the
+   // accuracy the matrix carries: they end the sweep like the relative test
+   const double tol_abs = 2.220446049250313e-16 * amax;
    int done = 0;
    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
       if (threadIdx.x == 0) rotated = 0;
@@ -76,7 +90,7 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
             double c = 1.0, sn = 0.0;
             if (q < n) {
                const double apq = a[pk(p, q)], app = a[pk(p, p)], aqq = a[pk(q, q)];
-               if (apq != 0.0 && fabs(apq) > 1.1102230246251565e-16 * sqrt(fabs(app) * fabs(aqq))) {
+               if (fabs(apq) > fmax(1.1102230246251565e-16 * sqrt(fabs(app) * fabs(aqq)), tol_abs)) {
                   const double th = (aqq - app) / (2.0 * apq);
                   const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(fma(th, th, 1.0)));
                   c = 1.0 / sqrt(fma(t, t, 1.0));

@@ -98,8 +98,8 @@ int gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64
          double *G, int ldg) {
    if (k1 <= 0 || k2 <= 0) return kSuccess;
    const int ti = (k1 + GT - 1) / GT, tj = (k2 + GT - 1) / GT;
-   // about 2048 workgroups in total, each with a whole number of 32-row slabs
-   int64_t want = std::max<int64_t>(1, 2048 / (ti * tj));
+   // about 512 workgroups in total, each with a whole number of 32-row slabs
+   int64_t want = std::max<int64_t>(1, 512 / (ti * tj));
    int64_t rows = (n + want - 1) / want;
    rows = std::max<int64_t>(GR * 4, ((rows + GR - 1) / GR) * GR);
    const int nchunks = (int)((n + rows - 1) / rows);

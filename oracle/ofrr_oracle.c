@@ -181,7 +181,7 @@ void orc_scalecols(int64_t m, int n, double *A, int64_t lda) {
 }
 
 /* G = V^T W with a fixed summation order: rows split into chunks (the same
- * split the GPU Gram kernel uses: about 2048 / (tiles of 64 x 64) chunks of a
+ * split the GPU Gram kernel uses: about 512 / (tiles of 64 x 64) chunks of a
  * whole number of 32-row slabs, at least 128 rows); within a chunk every entry
  * accumulates its rows in order with fused multiply-adds; the chunk partials
  * are then added in chunk order.  The result does not depend on the thread count. */
@@ -189,7 +189,7 @@ void orc_gemm_tn(int64_t n, int k1, int k2, const double *V, int64_t ldv, const 
                  int64_t ldw, double *G, int ldg) {
    enum { SLAB = 32 };
    const int64_t ti = (k1 + 63) / 64, tj = (k2 + 63) / 64;
-   int64_t want = 2048 / (ti * tj);
+   int64_t want = 512 / (ti * tj);
    if (want < 1) want = 1;
    int64_t rows = (n + want - 1) / want;
    rows = ((rows + SLAB - 1) / SLAB) * SLAB;
@@ -276,7 +276,8 @@ void orc_hessenberg(int64_t m, int n, double *Q, int64_t ldq, double *R, int ldr
 
 /* ------------------------------------------------------------------------- */
 /* symmetric eigensolver: parallel-ordered cyclic Jacobi (round-robin pairing,  */
-/* Brent & Luk 1985), Demmel-Veselic threshold, explicit fused multiply-adds.   */
+/* Brent & Luk 1985), Demmel-Veselic threshold with an absolute floor           */
+/* eps max|a_ij|, explicit fused multiply-adds.                                 */
 /* Every round rotates all disjoint pairs at once; the 2x2 blocks of A are      */
 /* transformed independently and the rotations are then applied to V in round   */
 /* order -- the same arithmetic, in the same order, as the device solver.       */
@@ -299,6 +300,9 @@ static int jacobi_eig(int n, double *a /* n x n symmetric, destroyed */, double 
    for (int j = 0; j < n; ++j)
       for (int i = 0; i < n; ++i) Vw[i + (size_t)j * n] = (i == j) ? 1.0 : 0.0;
 #define A_(i, j) a[(i) + (size_t)(j) * n]
+   double amax = 0.0;
+   for (size_t t = 0; t < (size_t)n * n; ++t) amax = fmax(amax, fabs(a[t]));
+   const double tol_abs = 2.220446049250313e-16 * amax;
    int sweep;
    for (sweep = 0; sweep < 30; ++sweep) {
       int rotated = 0;
@@ -310,7 +314,7 @@ static int jacobi_eig(int n, double *a /* n x n symmetric, destroyed */, double 
             double c = 1.0, sn = 0.0;
             if (q < n) {
                const double apq = A_(p, q), app = A_(p, p), aqq = A_(q, q);
-               if (apq != 0.0 && fabs(apq) > 1.1102230246251565e-16 * sqrt(fabs(app) * fabs(aqq))) {
+               if (fabs(apq) > fmax(1.1102230246251565e-16 * sqrt(fabs(app) * fabs(aqq)), tol_abs)) {
                   const double th = (aqq - app) / (2.0 * apq);
                   const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(fma(th, th, 1.0)));
                   c = 1.0 / sqrt(fma(t, t, 1.0));
