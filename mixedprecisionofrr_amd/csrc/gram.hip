@@ -2,13 +2,13 @@
 //
 //   gram          G = alpha V^T W + beta G: the projections V^T V and V^T (A V)
 //                 of eig_rr.m:49,51 (gemm with transA = true, mvops.cu:423-605).
-//                 Tall-skinny: each workgroup owns a 64 x 64 output tile and a
-//                 contiguous range of rows, stages 32-row slabs of V and W in
-//                 LDS (converted to FP64), and accumulates a 4 x 4 register
-//                 block per thread with FP64 fused multiply-adds.  The per-range
-//                 partial tiles are summed in a fixed order by a second kernel,
-//                 so the result is bitwise reproducible and independent of other
-//                 work on the device.
+//                 Tall-skinny: one wave per (contiguous range of rows, 64 x 64
+//                 output tile) streams its rows from HBM straight into FP64
+//                 MFMA accumulators (v_mfma_f64_16x16x4, operands converted
+//                 to FP64 in registers).  The per-range partial tiles are
+//                 summed in a fixed order by a second kernel, so the result is
+//                 bitwise reproducible and independent of other work on the
+//                 device.
 //   basis_update  Y = sum_b V_b C_b: the back-transform V <- V U (eig_rr.m:56)
 //                 for a basis whose column blocks are stored in different
 //                 precisions (FP64 Ritz block, low-precision growth blocks).
@@ -18,79 +18,97 @@
 namespace ofrr {
 
 constexpr int GT = 64;   // output tile
-constexpr int GR = 32;   // rows staged per slab
+constexpr int GR = 32;   // rows staged per slab (back-transform)
+constexpr int kGramChunks = 2048;  // row chunks of a one-tile product
+constexpr int kRedGroups = 8;      // first-level groups of the chunk sum
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// One wave per (row chunk, 64 x 64 output tile).  v_mfma_f64_16x16x4 takes
+// A = V^T (16 columns x 4 rows) and B = W (4 rows x 16 columns): lane l holds
+// V(r + l/16, i + l%16) and W(r + l/16, j + l%16), so a 16-lane group reads 16
+// consecutive entries of one row.  Each of the 16 accumulator tiles keeps the
+// rows of the chunk in order.  Entry (i, j) of a 16 x 16 result sits in lane
+// j + 16 (i / 4), register i % 4.
 template <typename TV, typename TW>
-__global__ __launch_bounds__(256) void gram_partial_kernel(
+__global__ __launch_bounds__(64) void gram_mfma_kernel(
     int64_t n, int k1, int k2, const TV *__restrict__ V, int64_t vrs, int64_t vcs,
     const TW *__restrict__ W, int64_t wrs, int64_t wcs, int64_t rows_per_chunk,
     double *__restrict__ part) {
-   __shared__ double sV[GR][GT + 1];
-   __shared__ double sW[GR][GT + 1];
-   const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+   const int lane = threadIdx.x;
+   const int li = lane & 15, lk = lane >> 4;
    const int i0 = blockIdx.y * GT, j0 = blockIdx.z * GT;
    const int64_t r0 = (int64_t)blockIdx.x * rows_per_chunk;
    const int64_t r1 = min(n, r0 + rows_per_chunk);
-   double acc[4][4];
+   dbl4 acc[4][4];
 #pragma unroll
    for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-   const bool vrow = (vcs == 1), wrow = (wcs == 1);
-   for (int64_t rb = r0; rb < r1; rb += GR) {
-      // stage V(rb:rb+GR, i0:i0+GT) and W(rb:rb+GR, j0:j0+GT)
-      for (int t = threadIdx.x; t < GR * GT; t += 256) {
-         int rr, cc;
-         if (vrow) { rr = t / GT; cc = t % GT; } else { cc = t / GR; rr = t % GR; }
-         int64_t row = rb + rr;
-         int col = i0 + cc;
-         sV[rr][cc] = (row < r1 && col < k1) ? (double)Elem<TV>::load(V[row * vrs + (int64_t)col * vcs]) : 0.0;
-         if (wrow) { rr = t / GT; cc = t % GT; } else { cc = t / GR; rr = t % GR; }
-         row = rb + rr;
-         col = j0 + cc;
-         sW[rr][cc] = (row < r1 && col < k2) ? (double)Elem<TW>::load(W[row * wrs + (int64_t)col * wcs]) : 0.0;
-      }
-      __syncthreads();
-#pragma unroll 4
-      for (int r = 0; r < GR; ++r) {
-         double va[4], wb[4];
+      for (int b = 0; b < 4; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+   bool cv[4], cw[4];
 #pragma unroll
-         for (int a = 0; a < 4; ++a) va[a] = sV[r][ty * 4 + a];
-#pragma unroll
-         for (int b = 0; b < 4; ++b) wb[b] = sW[r][tx * 4 + b];
-#pragma unroll
-         for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = fma(va[a], wb[b], acc[a][b]);
-      }
-      __syncthreads();
+   for (int t = 0; t < 4; ++t) {
+      cv[t] = i0 + 16 * t + li < k1;
+      cw[t] = j0 + 16 * t + li < k2;
    }
-   // partial tile layout: part[chunk][tile_i][tile_j][GT*GT]
+   const TV *vb = V + (int64_t)(i0 + li) * vcs;
+   const TW *wb = W + (int64_t)(j0 + li) * wcs;
+#pragma unroll 2
+   for (int64_t r = r0; r < r1; r += 4) {
+      const int64_t row = r + lk;
+      const bool rok = row < r1;
+      double a[4], b[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+         a[t] = (rok && cv[t]) ? (double)Elem<TV>::load(vb[row * vrs + (int64_t)(16 * t) * vcs]) : 0.0;
+         b[t] = (rok && cw[t]) ? (double)Elem<TW>::load(wb[row * wrs + (int64_t)(16 * t) * wcs]) : 0.0;
+      }
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+         for (int tj = 0; tj < 4; ++tj)
+            acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+   }
    const int64_t tile = (int64_t)blockIdx.y * gridDim.z + blockIdx.z;
    double *p = part + ((int64_t)blockIdx.x * gridDim.y * gridDim.z + tile) * (GT * GT);
 #pragma unroll
-   for (int a = 0; a < 4; ++a)
+   for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
-      for (int b = 0; b < 4; ++b) p[(ty * 4 + a) * GT + tx * 4 + b] = acc[a][b];
+      for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+         for (int rg = 0; rg < 4; ++rg)
+            p[(16 * ti + 4 * lk + rg) * GT + 16 * tj + li] = acc[ti][tj][rg];
 }
 
+// G(i, j) = alpha * sum over chunks of the partial tiles: the chunks are summed in
+// kRedGroups consecutive groups (each in chunk order), then the group sums in order
 __global__ __launch_bounds__(256) void gram_reduce_kernel(int nchunks, int ntiles_i, int ntiles_j,
                                                           int k1, int k2,
                                                           const double *__restrict__ part,
                                                           double alpha, double beta,
                                                           double *__restrict__ G, int ldg) {
-   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   __shared__ double gs[256];
    const int64_t per_chunk = (int64_t)ntiles_i * ntiles_j * GT * GT;
-   if (e >= per_chunk) return;
+   const int el = threadIdx.x % 32, grp = threadIdx.x / 32;  // 32 entries x 8 groups
+   const int64_t e = (int64_t)blockIdx.x * 32 + el;
+   const int gsz = (nchunks + kRedGroups - 1) / kRedGroups;
+   double s = 0.0;
+   if (e < per_chunk) {
+      const int c0 = grp * gsz, c1 = min(nchunks, c0 + gsz);
+      for (int c = c0; c < c1; ++c) s += part[c * per_chunk + e];
+   }
+   gs[threadIdx.x] = s;
+   __syncthreads();
+   if (grp != 0 || e >= per_chunk) return;
+   double t = 0.0;
+   for (int g = 0; g < kRedGroups; ++g) t += gs[g * 32 + el];
    const int64_t tile = e / (GT * GT);
    const int w = (int)(e % (GT * GT));
    const int i = (int)(tile / ntiles_j) * GT + w / GT;
    const int j = (int)(tile % ntiles_j) * GT + w % GT;
    if (i >= k1 || j >= k2) return;
-   double s = 0.0;
-   for (int c = 0; c < nchunks; ++c) s += part[c * per_chunk + e];
    double *g = G + i + (int64_t)j * ldg;
-   *g = (beta == 0.0) ? alpha * s : fma(alpha, s, beta * *g);
+   *g = (beta == 0.0) ? alpha * t : fma(alpha, t, beta * *g);
 }
 
 int gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64_t vrs,
@@ -98,10 +116,10 @@ int gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64
          double *G, int ldg) {
    if (k1 <= 0 || k2 <= 0) return kSuccess;
    const int ti = (k1 + GT - 1) / GT, tj = (k2 + GT - 1) / GT;
-   // about 512 workgroups in total, each with a whole number of 32-row slabs
-   int64_t want = std::max<int64_t>(1, 512 / (ti * tj));
+   // about kGramChunks waves per tile pair, each with a multiple of 4 rows
+   int64_t want = std::max<int64_t>(1, kGramChunks / (ti * tj));
    int64_t rows = (n + want - 1) / want;
-   rows = std::max<int64_t>(GR * 4, ((rows + GR - 1) / GR) * GR);
+   rows = std::max<int64_t>(64, ((rows + 3) / 4) * 4);
    const int nchunks = (int)((n + rows - 1) / rows);
    const int64_t per_chunk = (int64_t)ti * tj * GT * GT;
    double *part = (double *)workspace(ctx, (size_t)nchunks * per_chunk * sizeof(double));
@@ -110,14 +128,14 @@ int gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64
       using TV = decltype(tv);
       return dispatch_dtype(wdt, [&](auto tw) -> int {
          using TW = decltype(tw);
-         gram_partial_kernel<TV, TW><<<dim3(nchunks, ti, tj), 256, 0, ctx->stream>>>(
+         gram_mfma_kernel<TV, TW><<<dim3(nchunks, ti, tj), 64, 0, ctx->stream>>>(
              n, k1, k2, (const TV *)V, vrs, vcs, (const TW *)W, wrs, wcs, rows, part);
          OFRR_LAUNCH(ctx);
          return kSuccess;
       });
    });
    if (st) return st;
-   gram_reduce_kernel<<<(unsigned)((per_chunk + 255) / 256), 256, 0, ctx->stream>>>(
+   gram_reduce_kernel<<<(unsigned)((per_chunk + 31) / 32), 256, 0, ctx->stream>>>(
        nchunks, ti, tj, k1, k2, part, alpha, beta, G, ldg);
    OFRR_LAUNCH(ctx);
    return kSuccess;

@@ -180,20 +180,21 @@ void orc_scalecols(int64_t m, int n, double *A, int64_t lda) {
    }
 }
 
-/* G = V^T W with a fixed summation order: rows split into chunks (the same
- * split the GPU Gram kernel uses: about 512 / (tiles of 64 x 64) chunks of a
- * whole number of 32-row slabs, at least 128 rows); within a chunk every entry
- * accumulates its rows in order with fused multiply-adds; the chunk partials
- * are then added in chunk order.  The result does not depend on the thread count. */
+/* G = V^T W with a fixed summation order: rows split into chunks (the split the
+ * device Gram kernel uses: about 2048 / (tiles of 64 x 64) chunks of a multiple
+ * of 4 rows, at least 64 rows); within a chunk every entry accumulates its rows
+ * in order with fused multiply-adds; the chunk partials are added in 8
+ * consecutive groups (each in chunk order) and the group sums in order.  The
+ * result does not depend on the thread count. */
 void orc_gemm_tn(int64_t n, int k1, int k2, const double *V, int64_t ldv, const double *W,
                  int64_t ldw, double *G, int ldg) {
    enum { SLAB = 32 };
    const int64_t ti = (k1 + 63) / 64, tj = (k2 + 63) / 64;
-   int64_t want = 512 / (ti * tj);
+   int64_t want = 2048 / (ti * tj);
    if (want < 1) want = 1;
    int64_t rows = (n + want - 1) / want;
-   rows = ((rows + SLAB - 1) / SLAB) * SLAB;
-   if (rows < 4 * SLAB) rows = 4 * SLAB;
+   rows = ((rows + 3) / 4) * 4;
+   if (rows < 64) rows = 64;
    const int64_t nch = n > 0 ? (n + rows - 1) / rows : 0;
    double *part = (double *)calloc((size_t)(nch ? nch : 1) * k1 * k2, sizeof(double));
 #pragma omp parallel
@@ -222,11 +223,17 @@ void orc_gemm_tn(int64_t n, int k1, int k2, const double *V, int64_t ldv, const 
       free(vb);
       free(wb);
    }
+   const int64_t gsz = (nch + 7) / 8;
    for (int a = 0; a < k1; ++a)
       for (int b2 = 0; b2 < k2; ++b2) {
-         double s = 0.0;
-         for (int64_t c = 0; c < nch; ++c) s += part[(size_t)c * k1 * k2 + (size_t)a * k2 + b2];
-         G[a + (int64_t)b2 * ldg] = s;
+         double t = 0.0;
+         for (int g = 0; g < 8; ++g) {
+            double s = 0.0;
+            const int64_t c0 = g * gsz, c1 = (c0 + gsz < nch) ? c0 + gsz : nch;
+            for (int64_t c = c0; c < c1; ++c) s += part[(size_t)c * k1 * k2 + (size_t)a * k2 + b2];
+            t += s;
+         }
+         G[a + (int64_t)b2 * ldg] = t;
       }
    free(part);
 }
@@ -388,9 +395,9 @@ int orc_syev(int n, const double *A, int lda, double *w, double *V, int ldv) {
    for (int j = 0; j < n; ++j)
       for (int i = 0; i < n; ++i) /* symmetrise from the upper triangle ('U') */
          a[i + j * n] = (i <= j) ? A[i + (int64_t)j * lda] : A[j + (int64_t)i * lda];
-   jacobi_eig(n, a, w, V, ldv);
+   int sweeps = jacobi_eig(n, a, w, V, ldv);
    free(a);
-   return 0;
+   return sweeps; /* number of Jacobi sweeps run */
 }
 
 /* Cholesky B = L L^T, right-looking (column j scaled, then the trailing lower

@@ -64,7 +64,8 @@ void orc_hessenberg(int64_t m, int n, double *Q, int64_t ldq, double *R, int ldr
                     double select_tol, int variant, int *select, int *perm);
 
 /* ---- dense symmetric eigensolvers (FP64) --------------------------------- */
-/* eigenvalues ascending in w, eigenvectors in V (LAPACK dsyev 'V','U' order) */
+/* eigenvalues ascending in w, eigenvectors in V (LAPACK dsyev 'V','U' order);
+ * returns the number of Jacobi sweeps run */
 int orc_syev(int n, const double *A, int lda, double *w, double *V, int ldv);
 /* A x = lambda B x, itype 1, B-normalised vectors (LAPACK dsygv order).
  * returns 0, or i>0 if the leading minor of order i of B is not positive */

@@ -191,6 +191,9 @@ def test_gram_fp64_accumulation(ctx, vdt, wdt, tol):
     assert np.max(np.abs(G - ref) / scale) < tol
     # bitwise reproducible (fixed-order split over rows)
     assert np.array_equal(G, api.gram(ctx, V, W).cpu().numpy())
+    # the device accumulates each chunk's rows in order with FMAs (MFMA f64) and
+    # sums the chunks in the same fixed order as the oracle: equal bits
+    assert np.array_equal(G, ref)
 
 
 def test_gram_reference_kat_and_layouts(ctx, kats):
