@@ -105,11 +105,16 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
             log[((int64_t)sweep * (nn - 1) + r) * np + s] = make_double2(c, sn);
          }
          __syncthreads();
-         // A' = J^T A J over the 2x2 blocks (I <= J) of the packed triangle
-         const int nblocks = np * np;
+         // A' = J^T A J over the 2x2 blocks (I <= J) of the packed triangle,
+         // enumerated row by row of the upper block triangle
+         const int nblocks = np * (np + 1) / 2;
          for (int t = threadIdx.x; t < nblocks; t += blockDim.x) {
-            const int I = t / np, J = t % np;
-            if (I > J) continue;
+            // I = largest row with I*np - I*(I-1)/2 <= t
+            const float b2 = 2.0f * np + 1.0f;
+            int I = (int)((b2 - sqrtf(b2 * b2 - 8.0f * (float)t)) * 0.5f);
+            while (I > 0 && I * np - I * (I - 1) / 2 > t) --I;
+            while ((I + 1) * np - (I + 1) * I / 2 <= t) ++I;
+            const int J = I + (t - (I * np - I * (I - 1) / 2));
             const double s1 = rsn[I], s2 = rsn[J];
             if (s1 == 0.0 && s2 == 0.0) continue;  // both rotations are the identity
             const int p1 = sp_[I], q1 = sq_[I], p2 = sp_[J], q2 = sq_[J];
@@ -125,16 +130,17 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
                a[pk(q1, q1)] = fma(s1, m12, c1 * m22);
                a[pk(p1, q1)] = 0.0;
             } else {
-               const double a11 = a[pk(p1, p2)];
-               const double a12 = q2ok ? a[pk(p1, q2)] : 0.0;
-               const double a21 = q1ok ? a[pk(q1, p2)] : 0.0;
-               const double a22 = (q1ok && q2ok) ? a[pk(q1, q2)] : 0.0;
+               const int i11 = pk(p1, p2), i12 = pk(p1, q2), i21 = pk(q1, p2), i22 = pk(q1, q2);
+               const double a11 = a[i11];
+               const double a12 = q2ok ? a[i12] : 0.0;
+               const double a21 = q1ok ? a[i21] : 0.0;
+               const double a22 = (q1ok && q2ok) ? a[i22] : 0.0;
                const double m11 = fma(c2, a11, -(s2 * a12)), m12 = fma(s2, a11, c2 * a12);
                const double m21 = fma(c2, a21, -(s2 * a22)), m22 = fma(s2, a21, c2 * a22);
-               a[pk(p1, p2)] = fma(c1, m11, -(s1 * m21));
-               if (q2ok) a[pk(p1, q2)] = fma(c1, m12, -(s1 * m22));
-               if (q1ok) a[pk(q1, p2)] = fma(s1, m11, c1 * m21);
-               if (q1ok && q2ok) a[pk(q1, q2)] = fma(s1, m12, c1 * m22);
+               a[i11] = fma(c1, m11, -(s1 * m21));
+               if (q2ok) a[i12] = fma(c1, m12, -(s1 * m22));
+               if (q1ok) a[i21] = fma(s1, m11, c1 * m21);
+               if (q1ok && q2ok) a[i22] = fma(s1, m12, c1 * m22);
             }
          }
          __syncthreads();
