@@ -23,9 +23,9 @@
 namespace ofrr {
 
 constexpr int kJacobiThreads = 1024;
+constexpr int kLdsFullN = 128;    // full storage, row stride n + 1: <= 129 KiB of LDS
 constexpr int kLdsMaxN = 192;     // packed upper triangle: n(n+1)/2 doubles <= 144.8 KiB
 constexpr int kMaxSweeps = 30;
-constexpr int kApplyRows = 8;     // rows of V per workgroup in the accumulation kernel
 
 __device__ inline void pair_of(int nn, int r, int s, int &p, int &q) {
    if (s == 0) { p = nn - 1; q = r; }
@@ -33,14 +33,25 @@ __device__ inline void pair_of(int nn, int r, int s, int &p, int &q) {
    if (p > q) { int t = p; p = q; q = t; }
 }
 
-// packed upper triangle index of the unordered pair {i, j}
-__device__ inline int pk(int i, int j) { return (i <= j) ? (j * (j + 1)) / 2 + i : (i * (i + 1)) / 2 + j; }
+// storage of the symmetric working matrix: FULL keeps both triangles with row
+// stride ld (conflict-free LDS rows), otherwise the packed upper triangle
+template <bool FULL> struct SymIdx;
+template <> struct SymIdx<true> {
+   int ld;
+   __device__ int operator()(int i, int j) const { return i * ld + j; }
+};
+template <> struct SymIdx<false> {
+   int ld;
+   __device__ int operator()(int i, int j) const {
+      return (i <= j) ? (j * (j + 1)) / 2 + i : (i * (i + 1)) / 2 + j;
+   }
+};
 
 // Rotations: A is diagonalised in place (packed storage, LDS or global) and the
 // (c, s) of every pair of every round is logged; the eigenvectors are formed
 // afterwards by jacobi_apply_kernel.  Outputs per matrix: eigenvalues ascending
 // (w), the rank of each diagonal index (rank), and the number of logged sweeps.
-template <bool LDS>
+template <bool LDS, bool FULL>
 __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
     int n_max, const int32_t *__restrict__ n_dev, const double *__restrict__ A, int lda,
     int64_t strideA, double *__restrict__ w, int64_t stridew, double2 *__restrict__ rotlog,
@@ -63,11 +74,13 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
    }
    const int nn = n + (n & 1), np = nn / 2;
    double *a = LDS ? lds : work + b * strideWork;
+   const SymIdx<FULL> pk{n + 1};
    double lmax = 0.0;
    for (int j = threadIdx.x; j < n; j += blockDim.x)
       for (int i = 0; i <= j; ++i) {
          const double v = Ab[i + (int64_t)j * lda];
          a[pk(i, j)] = v;
+         if (FULL) a[pk(j, i)] = v;
          lmax = fmax(lmax, fabs(v));
       }
 #pragma unroll
@@ -129,6 +142,7 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
                a[pk(p1, p1)] = fma(c1, m11, -(s1 * m21));
                a[pk(q1, q1)] = fma(s1, m12, c1 * m22);
                a[pk(p1, q1)] = 0.0;
+               if (FULL) a[pk(q1, p1)] = 0.0;
             } else {
                const int i11 = pk(p1, p2), i12 = pk(p1, q2), i21 = pk(q1, p2), i22 = pk(q1, q2);
                const double a11 = a[i11];
@@ -137,10 +151,13 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
                const double a22 = (q1ok && q2ok) ? a[i22] : 0.0;
                const double m11 = fma(c2, a11, -(s2 * a12)), m12 = fma(s2, a11, c2 * a12);
                const double m21 = fma(c2, a21, -(s2 * a22)), m22 = fma(s2, a21, c2 * a22);
-               a[i11] = fma(c1, m11, -(s1 * m21));
-               if (q2ok) a[i12] = fma(c1, m12, -(s1 * m22));
-               if (q1ok) a[i21] = fma(s1, m11, c1 * m21);
-               if (q1ok && q2ok) a[i22] = fma(s1, m12, c1 * m22);
+               const double b11 = fma(c1, m11, -(s1 * m21)), b12 = fma(c1, m12, -(s1 * m22));
+               const double b21 = fma(s1, m11, c1 * m21), b22 = fma(s1, m12, c1 * m22);
+               a[i11] = b11;
+               if (FULL) a[pk(p2, p1)] = b11;
+               if (q2ok) { a[i12] = b12; if (FULL) a[pk(q2, p1)] = b12; }
+               if (q1ok) { a[i21] = b21; if (FULL) a[pk(p2, q1)] = b21; }
+               if (q1ok && q2ok) { a[i22] = b22; if (FULL) a[pk(q2, q1)] = b22; }
             }
          }
          __syncthreads();
@@ -164,48 +181,70 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
    }
 }
 
-// V = J_1 J_2 ... (all logged rotations) applied to the identity, rows
-// independent: each workgroup keeps kApplyRows rows of V in LDS; the columns
-// are written to their ascending-eigenvalue positions
-__global__ __launch_bounds__(256) void jacobi_apply_kernel(
+// V = J_1 J_2 ... (all logged rotations) applied to the identity.  Rows of V
+// are independent: one wave owns one row in LDS and applies every round in
+// order (the pairs of a round are disjoint).  The log is staged through LDS
+// kApplyChunk rounds at a time, so one global-memory latency covers many rounds,
+// and each lane steps its pair (p, q) from round to round instead of recomputing
+// it: in the circle schedule p = r + s and q = r - s (mod nn - 1) both advance
+// by one per round (pair 0 keeps p = nn - 1 and has q = r).  The columns are written to their
+// ascending-eigenvalue positions.
+constexpr int kApplyChunk = 16;
+__global__ __launch_bounds__(64) void jacobi_apply_kernel(
     int n_max, const int32_t *__restrict__ n_dev, const double2 *__restrict__ rotlog,
     int64_t strideLog, const int32_t *__restrict__ rank, const int32_t *__restrict__ nsweeps,
     int64_t strideInts, double *__restrict__ V, int ldv, int64_t strideV) {
-   __shared__ double rows[kApplyRows][256];
+   __shared__ double row[256];
+   __shared__ double2 lg[kApplyChunk * 128];
    const int b = blockIdx.y;
    const int n = n_dev ? min((int)n_dev[b], n_max) : n_max;
-   const int r0 = blockIdx.x * kApplyRows;
+   const int r0 = blockIdx.x;
    if (n <= 0 || r0 >= n) return;
-   const int nn = n + (n & 1), np = nn / 2;
-   const int rounds = nsweeps[b * strideInts] * (nn - 1);
+   const int nn = n + (n & 1), np = nn / 2, M = nn - 1;
+   const int rounds = nsweeps[b * strideInts] * M;
    const double2 *log = rotlog + b * strideLog;
-   for (int t = threadIdx.x; t < kApplyRows * nn; t += blockDim.x) {
-      const int rr = t / nn, c = t % nn;
-      rows[rr][c] = (r0 + rr == c) ? 1.0 : 0.0;
-   }
-   __syncthreads();
-   const int items = kApplyRows * np;
-   for (int rd = 0; rd < rounds; ++rd) {
-      const int r = rd % (nn - 1);
-      const double2 *lr = log + (int64_t)rd * np;
-      for (int t = threadIdx.x; t < items; t += blockDim.x) {
-         const int rr = t / np, s = t % np;
-         const double2 cs = lr[s];
-         if (cs.y == 0.0) continue;
-         int p, q;
-         pair_of(nn, r, s, p, q);
-         const double vp = rows[rr][p], vq = rows[rr][q];
-         rows[rr][p] = fma(cs.x, vp, -(cs.y * vq));
-         rows[rr][q] = fma(cs.y, vp, cs.x * vq);
-      }
+   const int lane = threadIdx.x;
+   for (int c = lane; c < nn; c += 64) row[c] = (r0 == c) ? 1.0 : 0.0;
+   // pair counters of s = lane and s = lane + 64 at round 0
+   int pa = lane == 0 ? M : lane, qa = lane == 0 ? 0 : M - lane;
+   int pb = lane + 64, qb = M - (lane + 64);
+   const bool hasA = lane < np, hasB = lane + 64 < np;
+   for (int c0 = 0; c0 < rounds; c0 += kApplyChunk) {
+      const int nr = min(kApplyChunk, rounds - c0);
+      const double2 *src = log + (int64_t)c0 * np;
+      __syncthreads();  // the previous chunk is consumed
+      for (int i = lane; i < nr * np; i += 64) lg[i] = src[i];
       __syncthreads();
+      for (int k = 0; k < nr; ++k) {
+         const double2 *lr = lg + k * np;
+         if (hasA) {
+            const double2 cs = lr[lane];
+            if (cs.y != 0.0) {
+               const int p = min(pa, qa), q = max(pa, qa);
+               const double vp = row[p], vq = row[q];
+               row[p] = fma(cs.x, vp, -(cs.y * vq));
+               row[q] = fma(cs.y, vp, cs.x * vq);
+            }
+         }
+         if (hasB) {
+            const double2 cs = lr[lane + 64];
+            if (cs.y != 0.0) {
+               const int p = min(pb, qb), q = max(pb, qb);
+               const double vp = row[p], vq = row[q];
+               row[p] = fma(cs.x, vp, -(cs.y * vq));
+               row[q] = fma(cs.y, vp, cs.x * vq);
+            }
+         }
+         if (lane != 0) pa = (pa + 1 == M) ? 0 : pa + 1;
+         qa = (qa + 1 == M) ? 0 : qa + 1;
+         pb = (pb + 1 == M) ? 0 : pb + 1;
+         qb = (qb + 1 == M) ? 0 : qb + 1;
+         __syncthreads();  // one wave: orders this round's LDS updates before the next
+      }
    }
    double *Vb = V + b * strideV;
    const int32_t *rk = rank + b * strideInts;
-   for (int t = threadIdx.x; t < kApplyRows * n; t += blockDim.x) {
-      const int rr = t / n, c = t % n;
-      if (r0 + rr < n) Vb[(r0 + rr) + (int64_t)rk[c] * ldv] = rows[rr][c];
-   }
+   for (int c = lane; c < n; c += 64) Vb[r0 + (int64_t)rk[c] * ldv] = row[c];
 }
 
 // per-matrix scratch (doubles): [packed A | rotation log | ranks | sweep count]
@@ -238,27 +277,35 @@ int syev_batched(ofrr_ctx *ctx, int batch, int n, const int32_t *n_dev, const do
    int32_t *rank = reinterpret_cast<int32_t *>(work + L.packed + L.log);
    int32_t *nsw = rank + n;
    const int64_t strideLog = strideWork / 2, strideInts = strideWork * 2;
-   if (n <= kLdsMaxN) {
-      static bool attr_set = false;
+   static bool attr_set = false;
+   if (!attr_set) {
+      const int full = (int)((size_t)kLdsFullN * (kLdsFullN + 1) * sizeof(double));
+      const int packed = (int)((size_t)kLdsMaxN * (kLdsMaxN + 1) / 2 * sizeof(double));
+      OFRR_HIP(ctx, hipFuncSetAttribute((const void *)jacobi_rot_kernel<true, true>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, full));
+      OFRR_HIP(ctx, hipFuncSetAttribute((const void *)jacobi_rot_kernel<true, false>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, packed));
+      attr_set = true;
+   }
+   if (n <= kLdsFullN) {
+      const size_t lds = (size_t)n * (n + 1) * sizeof(double);
+      jacobi_rot_kernel<true, true><<<batch, kJacobiThreads, lds, ctx->stream>>>(
+          n, n_dev, A, lda, strideA, w, stridew, rotlog, strideLog, rank, nsw, strideInts, work,
+          strideWork, kMaxSweeps);
+   } else if (n <= kLdsMaxN) {
       const size_t lds = (size_t)n * (n + 1) / 2 * sizeof(double);
-      if (!attr_set) {
-         OFRR_HIP(ctx, hipFuncSetAttribute((const void *)jacobi_rot_kernel<true>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)((size_t)kLdsMaxN * (kLdsMaxN + 1) / 2 * sizeof(double))));
-         attr_set = true;
-      }
-      jacobi_rot_kernel<true><<<batch, kJacobiThreads, lds, ctx->stream>>>(
+      jacobi_rot_kernel<true, false><<<batch, kJacobiThreads, lds, ctx->stream>>>(
           n, n_dev, A, lda, strideA, w, stridew, rotlog, strideLog, rank, nsw, strideInts, work,
           strideWork, kMaxSweeps);
    } else {
-      jacobi_rot_kernel<false><<<batch, kJacobiThreads, 0, ctx->stream>>>(
+      jacobi_rot_kernel<false, false><<<batch, kJacobiThreads, 0, ctx->stream>>>(
           n, n_dev, A, lda, strideA, w, stridew, rotlog, strideLog, rank, nsw, strideInts, work,
           strideWork, kMaxSweeps);
    }
    OFRR_LAUNCH(ctx);
-   dim3 grid((n + kApplyRows - 1) / kApplyRows, batch);
-   jacobi_apply_kernel<<<grid, 256, 0, ctx->stream>>>(n, n_dev, rotlog, strideLog, rank, nsw,
-                                                     strideInts, V, ldv, strideV);
+   dim3 grid(n, batch);
+   jacobi_apply_kernel<<<grid, 64, 0, ctx->stream>>>(n, n_dev, rotlog, strideLog, rank, nsw,
+                                                    strideInts, V, ldv, strideV);
    OFRR_LAUNCH(ctx);
    return kSuccess;
 }

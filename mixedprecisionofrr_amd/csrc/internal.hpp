@@ -14,13 +14,14 @@ int spmm_csr(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, con
 int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const double *va,
                int k, const double *X, const double *theta, double *W, void *Rq, int rdt,
                const double *inv_sR, float *colmax, double *sums);
-// one Chebyshev step on row-major blocks of width k (see spmm.hip):
-// D_{j+1} = (2/e)(A D_j - c D_j) - D_{j-1} + (2/e) R z_j with stored blocks
-// D_j = Dc sc, D_{j-1} = Dp sp, R = Rq sR; the output is stored as D_{j+1} / sn with
-// sn = 2^(ceil(log2(2 max|D_j|)) - target_exp) written to sn_out
+// one step of the normalised Chebyshev recurrence on row-major blocks of width k
+// (see spmm.hip and solvers.hip):
+// Dn = alpha (A Dc - c Dc) - beta Dp + w . R with stored blocks Dc = Dch sc,
+// Dp = Dph sp, R = Rq sR; the output is stored as Dn / sn with
+// sn = 2^(ceil(log2(2 max|Dc|)) - target_exp) written to sn_out
 struct ChebArgs {
-   double e, c;               // half-width and centre of the damped interval
-   const double *z;           // z_j = T_j((theta - c) / e), k values
+   double alpha, beta, c;     // step scalars and the centre of the damped interval
+   const double *w;           // per-column weight of the residual block (k values)
    const double *sR, *sc, *sp;// scales of R, Dc, Dp (sp may be null: no Dp)
    const float *maxc;         // max |D_j| per column
    float *maxn;               // max |D_{j+1}| per column (zeroed beforehand)

@@ -446,25 +446,33 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
       if (nconv == o->nwant) { converged = 1; break; }
       if (it >= o->maxiter) break;
 
-      // ---- Chebyshev filter on the correction, damping [lower, theta_k]
+      // ---- Chebyshev filter on the correction, damping [lower, theta_k].
+      // D_{j+1} = (2/e)(A - c)D_j - D_{j-1} + (2/e) z_j R with z_j = T_j(L(theta)),
+      // L(t) = (t - c)/e, run on Dh_j = D_j / mu_j, mu_j = T_j(L(b)), b = theta_1:
+      // Dh_{j+1} = (2/e) q_j (A - c) Dh_j - q_{j-1} q_j Dh_{j-1} + (2/e) zeta_j q_j R
+      // with q_j = mu_j / mu_{j+1} and zeta_j = z_j / mu_j (all O(1): no overflow
+      // at any degree).
       const double cut = theta[k - 1], lo = o->lower;
       const double e = 0.5 * (cut - lo), c0 = 0.5 * (cut + lo);
       for (int j = 0; j < k; ++j) Lth[j] = (theta[j] - c0) / e;
-      // z_j = T_j(L(theta)) for j = 1..d-1 (row j-1 of h_z)
+      const double Lb = std::max(1.0, (theta[0] - c0) / e);
+      std::vector<double> q(d + 1);
+      q[0] = 1.0 / Lb;
+      for (int s = 1; s <= d; ++s) q[s] = 1.0 / (2.0 * Lb - q[s - 1]);
       for (int j = 0; j < k; ++j) {
-         double zp = 1.0, zc = Lth[j];
+         double zp = 1.0, zc = Lth[j] / Lb;  // zeta_0, zeta_1
          for (int s = 1; s < d; ++s) {
-            h_z[(size_t)(s - 1) * k + j] = zc;
-            double zn = 2.0 * Lth[j] * zc - zp;
+            h_z[(size_t)(s - 1) * k + j] = (2.0 / e) * zc * q[s];  // weight of R in step s
+            const double zn = 2.0 * Lth[j] * zc * q[s] - zp * q[s - 1] * q[s];
             zp = zc;
             zc = zn;
          }
       }
       OFRR_HIP(ctx, hipMemcpyAsync(zall, h_z, (size_t)(d - 1) * k * 8, hipMemcpyHostToDevice, ctx->stream));
-      // D_1 = R / e, stored in Rq: scale sR / e, range max|R| / e
-      for (int j = 0; j < k; ++j) s1[j] = h_sR[j] / e;
+      // Dh_1 = R / (e L(b)), stored in Rq: scale sR / (e Lb), range max|R| / (e Lb)
+      for (int j = 0; j < k; ++j) s1[j] = h_sR[j] / (e * Lb);
       OFRR_HIP(ctx, hipMemcpyAsync(sD1, s1.data(), (size_t)k * 8, hipMemcpyHostToDevice, ctx->stream));
-      for (int j = 0; j < k; ++j) m1[j] = (float)((double)h_cmaxR[j] / e);
+      for (int j = 0; j < k; ++j) m1[j] = (float)((double)h_cmaxR[j] / (e * Lb));
       OFRR_HIP(ctx, hipMemcpyAsync(cmax[0], m1.data(), (size_t)k * 4, hipMemcpyHostToDevice, ctx->stream));
       OFRR_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
       // recurrence state: current / previous stored block and their scales;
@@ -482,9 +490,10 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
          for (int b = 0; b < 4; ++b)
             if (b != curIdx && b != prevIdx && b != snap) { out = b; break; }
          ChebArgs ca;
-         ca.e = e;
+         ca.alpha = (2.0 / e) * q[s];
+         ca.beta = q[s - 1] * q[s];
          ca.c = c0;
-         ca.z = zall + (size_t)(s - 1) * k;
+         ca.w = zall + (size_t)(s - 1) * k;
          ca.sR = sRd;
          ca.sc = sc;
          ca.sp = sp;

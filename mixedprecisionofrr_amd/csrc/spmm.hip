@@ -176,7 +176,7 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
    const int64_t r0 = blk * rows_per_block;
    const int64_t r1 = min(n, r0 + rows_per_block);
    // per-column coefficients from the device-side scales:
-   // T = (2/e)(A D_j - c D_j) - D_{j-1} + (2/e) R z_j over stored blocks
+   // T = alpha (A Dc - c Dc) - beta Dp + w R over the stored blocks
    ACC ca[VEC], cb[VEC], cg[VEC], cd[VEC], is[VEC];
    float mx[VEC];
 #pragma unroll
@@ -185,10 +185,10 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
       const double m = (double)ca_.maxc[j];
       const double s = (m > 0.0) ? exp2(ceil(log2(2.0 * m)) - (double)ca_.target_exp) : 1.0;
       const double sc = ca_.sc[j];
-      ca[v] = (ACC)(2.0 * sc / ca_.e);
-      cb[v] = (ACC)(-2.0 * ca_.c * sc / ca_.e);
-      cg[v] = (ACC)(ca_.sp ? -ca_.sp[j] : 0.0);
-      cd[v] = (ACC)(2.0 * ca_.sR[j] * ca_.z[j] / ca_.e);
+      ca[v] = (ACC)(ca_.alpha * sc);
+      cb[v] = (ACC)(-ca_.alpha * ca_.c * sc);
+      cg[v] = (ACC)(ca_.sp ? -ca_.beta * ca_.sp[j] : 0.0);
+      cd[v] = (ACC)(ca_.w[j] * ca_.sR[j]);
       is[v] = (ACC)(1.0 / s);
       mx[v] = 0.f;
       if (blk == 0 && grp == 0) {
