@@ -116,6 +116,8 @@ def main():
     p.add_argument("--fp64-steps", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="run a plain 1 GiB device copy first (calibrates HBM byte counters)")
     args = p.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -130,6 +132,14 @@ def main():
 
     dev = "cuda:0"
     ctx = api.Context(0)
+    if args.copy_probe:
+        # a plain copy of known size: 1 GiB read + 1 GiB written per launch
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     N = args.grid
     log(f"building 3D 7-point Laplacian {N}^3 on the GPU")
     A = api.CSRMatrix.stencil(ctx, 3, N).with_low_precision(torch.float16)

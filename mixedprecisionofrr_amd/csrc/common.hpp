@@ -207,6 +207,15 @@ __device__ inline void atomic_max_nonneg(float *addr, float v) {
 }
 
 // dispatch a functor on a runtime dtype: f(T{})
+// operator values: FP64, FP32 or FP16 only
+template <typename F> inline int dispatch_value_dtype(int dt, F &&f) {
+   switch (dt) {
+      case OFRR_F64: return f(double{});
+      case OFRR_F32: return f(float{});
+      case OFRR_F16: return f(__half{});
+      default: return kErrorBLASUnsupportedType;
+   }
+}
 template <typename F> inline int dispatch_dtype(int dt, F &&f) {
    switch (dt) {
       case OFRR_F64: return f(double{});

@@ -10,6 +10,7 @@
 //   ofrr_residuals          MATLAB/+tests/+test4/test.m:340-350 (normalised)
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "internal.hpp"
@@ -82,6 +83,36 @@ __global__ void mirror_upper_kernel(int m, double *__restrict__ G, int ldg) {
    if (t >= m * m) return;
    int i = t % m, j = t / m;
    if (i > j) G[i + (int64_t)j * ldg] = G[j + (int64_t)i * ldg];
+}
+
+// order-preserving map of a double onto uint64 (for atomicMax over any sign)
+__device__ inline unsigned long long ord_bits(double v) {
+   const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+   return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+static double ord_value(unsigned long long u) {
+   const unsigned long long b = (u >> 63) ? (u & 0x7fffffffffffffffull) : ~u;
+   double v;
+   std::memcpy(&v, &b, 8);
+   return v;
+}
+
+// upper Gershgorin bound max_i (a_ii + sum_{j != i} |a_ij|) of a symmetric CSR
+// matrix: the top of the window the Chebyshev filter is normalised on
+__global__ void gershgorin_upper_kernel(int64_t n, const int64_t *__restrict__ rp,
+                                        const int32_t *__restrict__ ci,
+                                        const double *__restrict__ va,
+                                        unsigned long long *__restrict__ out) {
+   double m = -INFINITY;
+   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+        i += (int64_t)gridDim.x * blockDim.x) {
+      double s = 0.0;
+      for (int64_t p = rp[i]; p < rp[i + 1]; ++p) s += (ci[p] == i) ? va[p] : fabs(va[p]);
+      m = fmax(m, s);
+   }
+#pragma unroll
+   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+   if ((threadIdx.x & 63) == 0) atomicMax(out, ord_bits(m));
 }
 
 // ---- device arena for the drivers (grow-only, reused between solves) -------
@@ -354,6 +385,7 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
    for (int i = 0; i < 3; ++i) cmax[i] = (float *)arena_take((size_t)k * 4);
    int32_t *rank = (int32_t *)arena_take(64);
    float *cmaxR = (float *)arena_take((size_t)k * 4);
+   unsigned long long *gub = (unsigned long long *)arena_take(8);
    double *sRd = sR;
    // pinned host mirrors
    const size_t hbytes = (size_t)(2 * k + k + k + d * k + 4 * k + 16) * 8;
@@ -371,6 +403,17 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
    OFRR_HIP(ctx, rec(0));
 
    int spmm_calls = 0, rank_min = m;
+   // ---- top of the filter window: the upper Gershgorin bound of A
+   double upper;
+   {
+      OFRR_HIP(ctx, hipMemsetAsync(gub, 0, 8, ctx->stream));
+      gershgorin_upper_kernel<<<1024, 256, 0, ctx->stream>>>(n, rp, ci, va, gub);
+      OFRR_LAUNCH(ctx);
+      unsigned long long u = 0;
+      OFRR_HIP(ctx, hipMemcpyAsync(&u, gub, 8, hipMemcpyDeviceToHost, ctx->stream));
+      OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      upper = ord_value(u);
+   }
    // ---- initial Rayleigh-Ritz on the start block
    OFRR_HIP(ctx, hipMemcpyAsync(X, Xio, (size_t)n * k * 8, hipMemcpyDeviceToDevice, ctx->stream));
    if ((st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, k, X, OFRR_F64, k, 1, 1.0, 0.0, W, OFRR_F64, k, 1))) return st;
@@ -448,14 +491,16 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
 
       // ---- Chebyshev filter on the correction, damping [lower, theta_k].
       // D_{j+1} = (2/e)(A - c)D_j - D_{j-1} + (2/e) z_j R with z_j = T_j(L(theta)),
-      // L(t) = (t - c)/e, run on Dh_j = D_j / mu_j, mu_j = T_j(L(b)), b = theta_1:
+      // L(t) = (t - c)/e, run on Dh_j = D_j / mu_j, mu_j = T_j(L(b)), b = the
+      // upper Gershgorin bound (no eigenvalue lies above it, so |Dh_j| stays
+      // within a small multiple of |R| / e whatever the degree):
       // Dh_{j+1} = (2/e) q_j (A - c) Dh_j - q_{j-1} q_j Dh_{j-1} + (2/e) zeta_j q_j R
       // with q_j = mu_j / mu_{j+1} and zeta_j = z_j / mu_j (all O(1): no overflow
       // at any degree).
       const double cut = theta[k - 1], lo = o->lower;
       const double e = 0.5 * (cut - lo), c0 = 0.5 * (cut + lo);
       for (int j = 0; j < k; ++j) Lth[j] = (theta[j] - c0) / e;
-      const double Lb = std::max(1.0, (theta[0] - c0) / e);
+      const double Lb = std::max(1.0, (std::max(theta[0], upper) - c0) / e);
       std::vector<double> q(d + 1);
       q[0] = 1.0 / Lb;
       for (int s = 1; s <= d; ++s) q[s] = 1.0 / (2.0 * Lb - q[s - 1]);

@@ -34,6 +34,76 @@ template <typename T, int N> __device__ inline void vstore(T *p, const VecT<T, N
    *reinterpret_cast<VecT<T, N> *>(p) = v;
 }
 
+// streamed (read-once / write-once) 16-byte accesses bypass the cache
+// hierarchy's retention so the gathered block keeps its L2 / MALL residency
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+template <int B> struct RawVec;
+template <> struct RawVec<16> { using type = u32x4; };
+template <> struct RawVec<8> { using type = u32x2; };
+template <typename T, int N> __device__ inline VecT<T, N> vload_nt(const T *p) {
+   using R = typename RawVec<sizeof(VecT<T, N>)>::type;
+   R u = __builtin_nontemporal_load(reinterpret_cast<const R *>(p));
+   VecT<T, N> r;
+   __builtin_memcpy(&r, &u, sizeof(r));
+   return r;
+}
+template <typename T, int N> __device__ inline void vstore_nt(T *p, const VecT<T, N> &v) {
+   using R = typename RawVec<sizeof(VecT<T, N>)>::type;
+   R u;
+   __builtin_memcpy(&u, &v, sizeof(u));
+   __builtin_nontemporal_store(u, reinterpret_cast<R *>(p));
+}
+
+// store an accumulator value in a block format without a detour through FP64
+template <typename TB, typename ACC> __device__ inline TB store_acc(ACC v) {
+   if constexpr (std::is_same<TB, __half>::value) return __float2half((float)v);
+   else if constexpr (std::is_same<TB, __hip_bfloat16>::value) return __float2bfloat16((float)v);
+   else if constexpr (std::is_same<TB, float>::value) return (float)v;
+   else return Elem<TB>::store((double)v);
+}
+
+// power-of-two exponent e with 2^e >= 2 m > 2^(e-1), i.e. ceil(log2(2 m)), m > 0
+__device__ inline int ceil_log2_2m(float m) {
+   int e;
+   const float f = frexpf(2.0f * m, &e);  // 2 m = f 2^e, f in [0.5, 1)
+   return (f == 0.5f) ? e - 1 : e;
+}
+
+// acc[v] += sum_p a_p * X(col_p, c0 + v) over entries [b, e) of the index /
+// value arrays cs / vs (LDS or global), in storage order, 8 gathers in flight;
+// the padding slots of the last group of 8 add 0 * X(self) (finite).
+// OVERREAD: cs / vs may be read up to 7 entries past e (padded LDS copies), so
+// the index loads are unconditional and only the selects depend on the bound.
+template <typename TV, typename TX, int VEC, typename ACC, bool OVERREAD, typename CI, typename VA>
+__device__ inline void row_gather8(CI cs, VA vs, int64_t b, int64_t e, const TX *__restrict__ X,
+                                   int64_t ldx, int64_t self, int c0, ACC acc[VEC]) {
+   for (int64_t p = b; p < e; p += 8) {
+      int32_t c[8];
+      ACC a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+         const bool ok = p + u < e;
+         if (OVERREAD) {
+            const int32_t cl = cs[p + u];
+            const ACC al = (ACC)Elem<TV>::load(vs[p + u]);
+            c[u] = ok ? cl : (int32_t)self;
+            a[u] = ok ? al : (ACC)0;
+         } else {
+            c[u] = ok ? cs[p + u] : (int32_t)self;
+            a[u] = ok ? (ACC)Elem<TV>::load(vs[p + u]) : (ACC)0;
+         }
+      }
+      VecT<TX, VEC> xv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xv[u] = vload<TX, VEC>(X + (int64_t)c[u] * ldx + c0);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+         for (int v = 0; v < VEC; ++v) acc[v] = fma(a[u], (ACC)Elem<TX>::load(xv[u].v[v]), acc[v]);
+   }
+}
+
 // acc[v] += sum_p a_p * X(col_p, c0 + v), entries in storage order, 4 in flight
 template <typename TV, typename TX, int VEC, typename ACC>
 __device__ inline void row_gather(const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
@@ -159,66 +229,116 @@ __global__ __launch_bounds__(256) void spmm_resid_kernel(
 // ---------------------------------------------------------------------------
 // one Chebyshev step on the correction block:
 // T = ca.(A Dc) + cb.Dc + cg.Dp + cd.R,  Dn = quantise(T * inv_s), colmax |T|
-template <typename TV, typename TB, int VEC>
+// A block owns kChebRows consecutive rows.  Their row pointers, column indices
+// and values are first copied to LDS with coalesced loads, so a row costs one
+// global latency (its gathers, all in flight together) instead of the
+// row-pointer -> index -> block chain; the streamed blocks (Dp, R, Dn) use
+// non-temporal accesses.  A block whose rows hold more than kChebCap entries
+// reads them from global memory instead (same arithmetic order).
+constexpr int kChebRows = 256;
+constexpr int kChebCap = 8 * kChebRows;
+constexpr int kChebMaxK = 256;
+
+// the row loop of the Chebyshev step; HAS_DP and the index source are compile-
+// time so that no branch sits between the streamed loads and the gathers
+// (a join there makes the compiler wait for every outstanding load)
+template <typename TV, typename TB, int VEC, typename ACC, bool HAS_DP, bool OVERREAD, typename CI,
+          typename VA>
+__device__ inline void cheb_rows(int64_t r0, int nr, int grp, int groups, int c0, int k, CI cs,
+                                 VA vs, const int32_t *s_rp, const ACC (*s_coef)[kChebMaxK],
+                                 ACC cshift, const TB *__restrict__ Dc,
+                                 const TB *__restrict__ Dp, const TB *__restrict__ R,
+                                 TB *__restrict__ Dn, float mx[VEC]) {
+   for (int lr = grp; lr < nr; lr += groups) {
+      const int64_t row = r0 + lr;
+      // the streamed operands first: independent of the gathers
+      const VecT<TB, VEC> rr = vload_nt<TB, VEC>(R + row * k + c0);
+      VecT<TB, VEC> dp;
+      if (HAS_DP) dp = vload_nt<TB, VEC>(Dp + row * k + c0);
+      const VecT<TB, VEC> dc = vload<TB, VEC>(Dc + row * k + c0);
+      ACC acc[VEC];
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) acc[v] = 0;
+      row_gather8<TV, TB, VEC, ACC, OVERREAD>(cs, vs, s_rp[lr], s_rp[lr + 1], Dc, k, row, c0, acc);
+      asm volatile("" ::: "memory");  // keep the coefficient reads inside the loop
+      const VecT<ACC, VEC> ca = *reinterpret_cast<const VecT<ACC, VEC> *>(&s_coef[0][c0]);
+      const VecT<ACC, VEC> cd = *reinterpret_cast<const VecT<ACC, VEC> *>(&s_coef[2][c0]);
+      VecT<ACC, VEC> cg;
+      if (HAS_DP) cg = *reinterpret_cast<const VecT<ACC, VEC> *>(&s_coef[1][c0]);
+      VecT<TB, VEC> out;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+         ACC t = ca.v[v] * fma(-cshift, (ACC)Elem<TB>::load(dc.v[v]), acc[v]);
+         if (HAS_DP) t = fma(cg.v[v], (ACC)Elem<TB>::load(dp.v[v]), t);
+         t = fma(cd.v[v], (ACC)Elem<TB>::load(rr.v[v]), t);
+         mx[v] = fmaxf(mx[v], (float)fabs(t));
+         out.v[v] = store_acc<TB>(t);
+      }
+      vstore_nt<TB, VEC>(Dn + row * k + c0, out);
+   }
+}
+
+template <typename TV, typename TB, int VEC, bool HAS_DP>
 __global__ __launch_bounds__(256) void cheb_step_kernel(
     int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
     const TV *__restrict__ va, int k, const TB *__restrict__ Dc, const TB *__restrict__ Dp,
-    const TB *__restrict__ R, ChebArgs ca_, TB *__restrict__ Dn, int lg, int64_t rows_per_block,
-    int64_t nblk) {
+    const TB *__restrict__ R, ChebArgs ca_, TB *__restrict__ Dn, int lg, int64_t nblk) {
    using ACC = typename std::conditional<std::is_same<TB, double>::value, double, float>::type;
+   __shared__ int32_t s_rp[kChebRows + 1];
+   __shared__ int32_t s_ci[kChebCap + 8];  // + 8: padding for the unconditional index reads
+   __shared__ TV s_va[kChebCap + 8];
    __shared__ float redm[256][VEC];
+   __shared__ __align__(16) ACC s_coef[3][kChebMaxK];
+   __shared__ int s_se[kChebMaxK];
    const int64_t blk = xcd_block(nblk);
    if (blk < 0) return;
    const int G = 1 << lg;
    const int GROUPS = 256 >> lg;
    const int lane_g = threadIdx.x & (G - 1), grp = threadIdx.x >> lg;
    const int c0 = lane_g * VEC;  // host guarantees k == G * VEC
-   const int64_t r0 = blk * rows_per_block;
-   const int64_t r1 = min(n, r0 + rows_per_block);
-   // per-column coefficients from the device-side scales:
-   // T = alpha (A Dc - c Dc) - beta Dp + w R over the stored blocks
-   ACC ca[VEC], cb[VEC], cg[VEC], cd[VEC], is[VEC];
-   float mx[VEC];
-#pragma unroll
-   for (int v = 0; v < VEC; ++v) {
-      const int j = c0 + v;
-      const double m = (double)ca_.maxc[j];
-      const double s = (m > 0.0) ? exp2(ceil(log2(2.0 * m)) - (double)ca_.target_exp) : 1.0;
-      const double sc = ca_.sc[j];
-      ca[v] = (ACC)(ca_.alpha * sc);
-      cb[v] = (ACC)(-ca_.alpha * ca_.c * sc);
-      cg[v] = (ACC)(ca_.sp ? -ca_.beta * ca_.sp[j] : 0.0);
-      cd[v] = (ACC)(ca_.w[j] * ca_.sR[j]);
-      is[v] = (ACC)(1.0 / s);
-      mx[v] = 0.f;
-      if (blk == 0 && grp == 0) {
-         ca_.sn_out[j] = s;
+   const int64_t r0 = blk * kChebRows;
+   const int nr = (int)min((int64_t)kChebRows, n - r0);
+   const int64_t base = rp[r0];
+   const int64_t nnzb = rp[r0 + nr] - base;
+   const bool in_lds = nnzb <= kChebCap;
+   for (int i = threadIdx.x; i <= nr; i += 256) s_rp[i] = (int32_t)(rp[r0 + i] - base);
+   if (in_lds)
+      for (int i = threadIdx.x; i < (int)nnzb + 8; i += 256) {
+         const bool ok = i < (int)nnzb;
+         s_ci[i] = ok ? ci[base + i] : 0;
+         s_va[i] = ok ? va[base + i] : TV(0);
+      }
+   // per-column coefficients from the device-side scales, with the output
+   // scale 2^-se folded in (exact: a power of two):
+   // T / s = (alpha sc / s)(A Dc - c Dc) + (-beta sp / s) Dp + (w sR / s) R
+   // They live in LDS and are re-read per row (in registers they cost occupancy).
+   for (int j = threadIdx.x; j < k; j += 256) {
+      const float m = ca_.maxc[j];
+      // stored scale s = 2^(ceil(log2(2 max|T|)) - target): the stored maximum lands in [2^(t-1), 2^t]
+      const int se = (m > 0.f) ? ceil_log2_2m(m) - ca_.target_exp : 0;
+      s_coef[0][j] = (ACC)ldexp(ca_.alpha * ca_.sc[j], -se);
+      s_coef[1][j] = (ACC)(HAS_DP ? ldexp(-ca_.beta * ca_.sp[j], -se) : 0.0);
+      s_coef[2][j] = (ACC)ldexp(ca_.w[j] * ca_.sR[j], -se);
+      s_se[j] = se;
+      if (blk == 0) {
+         ca_.sn_out[j] = ldexp(1.0, se);
          if (ca_.zero_next) ca_.zero_next[j] = 0.f;
       }
    }
-   for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
-      ACC acc[VEC];
+   float mx[VEC];
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) acc[v] = 0;
-      row_gather<TV, TB, VEC, ACC>(rp, ci, va, Dc, k, row, c0, acc);
-      VecT<TB, VEC> dc = vload<TB, VEC>(Dc + row * k + c0);
-      VecT<TB, VEC> rr = vload<TB, VEC>(R + row * k + c0);
-      VecT<TB, VEC> dp;
-      if (Dp) dp = vload<TB, VEC>(Dp + row * k + c0);
-      VecT<TB, VEC> out;
+   for (int v = 0; v < VEC; ++v) mx[v] = 0.f;
+   const ACC cshift = (ACC)ca_.c;
+   __syncthreads();
+   if (in_lds)
+      cheb_rows<TV, TB, VEC, ACC, HAS_DP, true>(r0, nr, grp, GROUPS, c0, k, s_ci, s_va, s_rp,
+                                                s_coef, cshift, Dc, Dp, R, Dn, mx);
+   else  // more than kChebCap entries in this block's rows: indices from global memory
+      cheb_rows<TV, TB, VEC, ACC, HAS_DP, false>(r0, nr, grp, GROUPS, c0, k, ci + base, va + base,
+                                                 s_rp, s_coef, cshift, Dc, Dp, R, Dn, mx);
+   // publish the unscaled column maxima (max |T / s| * s, exact)
 #pragma unroll
-      for (int v = 0; v < VEC; ++v) {
-         ACC t = ca[v] * acc[v];
-         t = fma(cb[v], (ACC)Elem<TB>::load(dc.v[v]), t);
-         if (Dp) t = fma(cg[v], (ACC)Elem<TB>::load(dp.v[v]), t);
-         t = fma(cd[v], (ACC)Elem<TB>::load(rr.v[v]), t);
-         mx[v] = fmaxf(mx[v], (float)fabs((double)t));
-         out.v[v] = Elem<TB>::store((double)(t * is[v]));
-      }
-      vstore<TB, VEC>(Dn + row * k + c0, out);
-   }
-#pragma unroll
-   for (int v = 0; v < VEC; ++v) redm[threadIdx.x][v] = mx[v];
+   for (int v = 0; v < VEC; ++v) redm[threadIdx.x][v] = ldexpf(mx[v], s_se[c0 + v]);
    __syncthreads();
    for (int t = threadIdx.x; t < G * VEC; t += blockDim.x) {
       const int lane = t / VEC, v = t % VEC;
@@ -288,7 +408,7 @@ int spmm_csr(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, con
       return set_error(ctx, kErrorBLASUnsupportedType, "spmm: operator values must be FP64, FP32 or FP16");
    if (ydt == OFRR_FP8)
       return set_error(ctx, kErrorBLASUnsupportedType, "spmm: FP8 output is produced by the filter step only");
-   return dispatch_dtype(vdt, [&](auto tv) -> int {
+   return dispatch_value_dtype(vdt, [&](auto tv) -> int {
       using TV = decltype(tv);
       return dispatch_dtype(xdt, [&](auto tx) -> int {
          using TX = decltype(tx);
@@ -330,23 +450,31 @@ int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, c
 int cheb_step(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const void *va,
               int vdt, int k, const void *Dc, const void *Dp, const void *R, int bdt,
               const ChebArgs &args, void *Dn) {
-   return dispatch_dtype(vdt, [&](auto tv) -> int {
+   if (vdt != OFRR_F64 && vdt != OFRR_F32 && vdt != OFRR_F16)
+      return set_error(ctx, kErrorBLASUnsupportedType, "cheb_step: operator values must be FP64, FP32 or FP16");
+   return dispatch_value_dtype(vdt, [&](auto tv) -> int {
       using TV = decltype(tv);
       return dispatch_dtype(bdt, [&](auto tb) -> int {
          using TB = decltype(tb);
          constexpr int VEC = 16 / sizeof(TB);
-         if (k % VEC != 0) return set_error(ctx, kErrorInvalidArgument, "cheb_step: k not a multiple of the vector width");
+         if (k % VEC != 0 || k > kChebMaxK)
+            return set_error(ctx, kErrorInvalidArgument, "cheb_step: k must be a multiple of the vector width and <= 256");
          int G = k / VEC;
          if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16 && G != 32 && G != 64)
             return set_error(ctx, kErrorInvalidArgument, "cheb_step: k / vector width must be a power of two <= 64");
-         int64_t rows_per_block = std::is_same<TB, double>::value ? 256 : 512;
-         int64_t nblk = (n + rows_per_block - 1) / rows_per_block;
+         if (n >= ((int64_t)1 << 31)) return set_error(ctx, kErrorInvalidArgument, "cheb_step: n >= 2^31");
+         int64_t nblk = (n + kChebRows - 1) / kChebRows;
          dim3 grid((unsigned)xcd_grid(nblk));
          int lg = 0;
          while ((1 << lg) < G) ++lg;
-         cheb_step_kernel<TV, TB, VEC><<<grid, 256, 0, ctx->stream>>>(
-             n, rp, ci, (const TV *)va, k, (const TB *)Dc, (const TB *)Dp, (const TB *)R, args,
-             (TB *)Dn, lg, rows_per_block, nblk);
+         if (Dp)
+            cheb_step_kernel<TV, TB, VEC, true><<<grid, 256, 0, ctx->stream>>>(
+                n, rp, ci, (const TV *)va, k, (const TB *)Dc, (const TB *)Dp, (const TB *)R, args,
+                (TB *)Dn, lg, nblk);
+         else
+            cheb_step_kernel<TV, TB, VEC, false><<<grid, 256, 0, ctx->stream>>>(
+                n, rp, ci, (const TV *)va, k, (const TB *)Dc, nullptr, (const TB *)R, args,
+                (TB *)Dn, lg, nblk);
          OFRR_LAUNCH(ctx);
          return kSuccess;
       });
