@@ -8,9 +8,12 @@
 // FP16 (128 B per row) and 2 rows per wave in FP64 (512 B per row).
 // Accumulation is FP64 for FP64 operands and FP32 otherwise; entries of a row
 // are accumulated with fused multiply-adds in storage order.  Each workgroup
-// owns a contiguous range of rows and the block index is remapped so that each
-// XCD walks a contiguous part of the matrix (its L2 then serves the +-N and
-// +-N^2 stencil neighbours).
+// owns 256 consecutive rows and first copies their row pointers, column
+// indices and values to LDS with coalesced loads; a row then costs one global
+// latency, its (up to 8) gathers in flight together.  The block index is
+// remapped so that each XCD walks a contiguous part of the matrix (its L2 then
+// serves the +-1 and +-N stencil neighbours).  Read-once and write-once blocks
+// use non-temporal accesses.
 //
 // Kernels:
 //   spmm_rm      Y = alpha A X + beta Y
@@ -104,56 +107,55 @@ __device__ inline void row_gather8(CI cs, VA vs, int64_t b, int64_t e, const TX 
    }
 }
 
-// acc[v] += sum_p a_p * X(col_p, c0 + v), entries in storage order, 4 in flight
-template <typename TV, typename TX, int VEC, typename ACC>
-__device__ inline void row_gather(const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                  const TV *__restrict__ va, const TX *__restrict__ X,
-                                  int64_t ldx, int64_t row, int c0, ACC acc[VEC]) {
-   const int64_t b = rp[row], e = rp[row + 1];
-   for (int64_t p = b; p < e; p += 4) {
-      int32_t c[4];
-      ACC a[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-         bool ok = p + u < e;
-         c[u] = ok ? ci[p + u] : (int32_t)row;
-         a[u] = ok ? (ACC)Elem<TV>::load(va[p + u]) : (ACC)0;
+// ---------------------------------------------------------------------------
+// LDS copy of the CSR rows of one 256-row block: row offsets relative to the
+// block's first entry, and the indices / values padded by 8 entries so that the
+// gathers may read past a row's end (row_gather8<..., true>).
+constexpr int kTileRows = 256;
+constexpr int kTileCap = 8 * kTileRows;
+template <typename TV> struct CsrTile {
+   int32_t rp[kTileRows + 1];
+   int32_t ci[kTileCap + 8];
+   TV va[kTileCap + 8];
+};
+// stage rows [r0, r0 + nr); false when they hold more than kTileCap entries
+// (the offsets are staged anyway; indices then come from ci + base, va + base)
+template <typename TV>
+__device__ inline bool stage_tile(CsrTile<TV> &t, const int64_t *__restrict__ rp,
+                                  const int32_t *__restrict__ ci, const TV *__restrict__ va,
+                                  int64_t r0, int nr, int64_t &base) {
+   base = rp[r0];
+   const int64_t nnzb = rp[r0 + nr] - base;
+   const bool in_lds = nnzb <= kTileCap;
+   for (int i = threadIdx.x; i <= nr; i += blockDim.x) t.rp[i] = (int32_t)(rp[r0 + i] - base);
+   if (in_lds)
+      for (int i = threadIdx.x; i < (int)nnzb + 8; i += blockDim.x) {
+         const bool ok = i < (int)nnzb;
+         t.ci[i] = ok ? ci[base + i] : 0;
+         t.va[i] = ok ? va[base + i] : TV(0);
       }
-      VecT<TX, VEC> xv[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) xv[u] = vload<TX, VEC>(X + (int64_t)c[u] * ldx + c0);
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-         for (int v = 0; v < VEC; ++v)
-            acc[v] = fma(a[u], (ACC)Elem<TX>::load(xv[u].v[v]), acc[v]);
-   }
+   return in_lds;
 }
 
 // ---------------------------------------------------------------------------
-template <typename TV, typename TX, typename TY, int VEC>
-__global__ __launch_bounds__(256) void spmm_rm_kernel(
-    int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
-    const TV *__restrict__ va, int k, const TX *__restrict__ X, int64_t ldx, double alpha,
-    double beta, TY *__restrict__ Y, int64_t ldy, int lg, int64_t rows_per_block, int64_t nblk) {
-   using ACC = typename std::conditional<std::is_same<TX, double>::value, double, float>::type;
-   const int64_t blk = xcd_block(nblk);
-   if (blk < 0) return;
-   const int G = 1 << lg;
-   const int lane_g = threadIdx.x & (G - 1), grp = threadIdx.x >> lg;
-   const int GROUPS = 256 >> lg;
-   const int64_t r0 = blk * rows_per_block;
-   const int64_t r1 = min(n, r0 + rows_per_block);
+template <typename TV, typename TX, typename TY, int VEC, typename ACC, bool OVERREAD, typename CI,
+          typename VA>
+__device__ inline void spmm_rows(int64_t r0, int nr, int grp, int groups, int lane_g, int G, int k,
+                                 CI cs, VA vs, const int32_t *s_rp, const TX *__restrict__ X,
+                                 int64_t ldx, double alpha, double beta, TY *__restrict__ Y,
+                                 int64_t ldy) {
    for (int c0 = lane_g * VEC; c0 < k; c0 += G * VEC) {
-      for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
+      for (int lr = grp; lr < nr; lr += groups) {
+         const int64_t row = r0 + lr;
          ACC acc[VEC];
 #pragma unroll
          for (int v = 0; v < VEC; ++v) acc[v] = 0;
-         row_gather<TV, TX, VEC, ACC>(rp, ci, va, X, ldx, row, c0, acc);
+         row_gather8<TV, TX, VEC, ACC, OVERREAD>(cs, vs, s_rp[lr], s_rp[lr + 1], X, ldx, row, c0,
+                                                 acc);
          TY *y = Y + row * ldy + c0;
 #pragma unroll
          for (int v = 0; v < VEC; ++v) {
-            double r = (double)alpha * (double)acc[v];
+            double r = alpha * (double)acc[v];
             if (beta != 0.0) r += beta * (double)Elem<TY>::load(y[v]);
             y[v] = Elem<TY>::store(r);
          }
@@ -161,7 +163,63 @@ __global__ __launch_bounds__(256) void spmm_rm_kernel(
    }
 }
 
+template <typename TV, typename TX, typename TY, int VEC>
+__global__ __launch_bounds__(256) void spmm_rm_kernel(
+    int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+    const TV *__restrict__ va, int k, const TX *__restrict__ X, int64_t ldx, double alpha,
+    double beta, TY *__restrict__ Y, int64_t ldy, int lg, int64_t nblk) {
+   using ACC = typename std::conditional<std::is_same<TX, double>::value, double, float>::type;
+   __shared__ CsrTile<TV> tile;
+   const int64_t blk = xcd_block(nblk);
+   if (blk < 0) return;
+   const int G = 1 << lg;
+   const int lane_g = threadIdx.x & (G - 1), grp = threadIdx.x >> lg;
+   const int GROUPS = 256 >> lg;
+   const int64_t r0 = blk * kTileRows;
+   const int nr = (int)min((int64_t)kTileRows, n - r0);
+   int64_t base;
+   const bool in_lds = stage_tile(tile, rp, ci, va, r0, nr, base);
+   __syncthreads();
+   if (in_lds)
+      spmm_rows<TV, TX, TY, VEC, ACC, true>(r0, nr, grp, GROUPS, lane_g, G, k, tile.ci, tile.va,
+                                            tile.rp, X, ldx, alpha, beta, Y, ldy);
+   else
+      spmm_rows<TV, TX, TY, VEC, ACC, false>(r0, nr, grp, GROUPS, lane_g, G, k, ci + base,
+                                             va + base, tile.rp, X, ldx, alpha, beta, Y, ldy);
+}
+
 // ---------------------------------------------------------------------------
+// W = A X (FP64), R = W - X theta, Rq = quantise(R * inv_sR), colmax |R|,
+// partial sums of r^2 and x^2 per block (fixed order -> deterministic)
+template <typename TB, bool OVERREAD, typename CI, typename VA>
+__device__ inline void resid_rows(int64_t r0, int nr, int grp, int groups, int c0, int k, CI cs,
+                                  VA vs, const int32_t *s_rp, const double *__restrict__ X,
+                                  const double th[2], const double isr[2],
+                                  double *__restrict__ W, TB *__restrict__ Rq, double s_r[2],
+                                  double s_x[2], float mx[2]) {
+   constexpr int VEC = 2;
+   for (int lr = grp; lr < nr; lr += groups) {
+      const int64_t row = r0 + lr;
+      const VecT<double, VEC> xv = vload<double, VEC>(X + row * k + c0);
+      double acc[VEC] = {0.0, 0.0};
+      row_gather8<double, double, VEC, double, OVERREAD>(cs, vs, s_rp[lr], s_rp[lr + 1], X, k, row,
+                                                         c0, acc);
+      VecT<double, VEC> wv;
+      VecT<TB, VEC> rq;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+         wv.v[v] = acc[v];
+         const double r = fma(-th[v], xv.v[v], acc[v]);
+         s_r[v] = fma(r, r, s_r[v]);
+         s_x[v] = fma(xv.v[v], xv.v[v], s_x[v]);
+         mx[v] = fmaxf(mx[v], (float)fabs(r));
+         rq.v[v] = Elem<TB>::store(r * isr[v]);
+      }
+      vstore_nt<double, VEC>(W + row * k + c0, wv);
+      vstore<TB, VEC>(Rq + row * k + c0, rq);
+   }
+}
+
 // W = A X (FP64), R = W - X theta, Rq = quantise(R * inv_sR), colmax |R|,
 // partial sums of r^2 and x^2 per block (fixed order -> deterministic)
 template <typename TB>
@@ -170,8 +228,9 @@ __global__ __launch_bounds__(256) void spmm_resid_kernel(
     const double *__restrict__ va, int k, const double *__restrict__ X,
     const double *__restrict__ theta, double *__restrict__ W, TB *__restrict__ Rq,
     const double *__restrict__ inv_sR, float *__restrict__ colmax, double *__restrict__ part,
-    int lg, int64_t rows_per_block, int64_t nblk) {
+    int lg, int64_t nblk) {
    constexpr int VEC = 2;
+   __shared__ CsrTile<double> tile;
    __shared__ double red[256][VEC * 2];
    __shared__ float redm[256][VEC];
    const int64_t blk = xcd_block(nblk);
@@ -180,30 +239,21 @@ __global__ __launch_bounds__(256) void spmm_resid_kernel(
    const int GROUPS = 256 >> lg;
    const int lane_g = threadIdx.x & (G - 1), grp = threadIdx.x >> lg;
    const int c0 = lane_g * VEC;  // host guarantees k == G * VEC
-   const int64_t r0 = blk * rows_per_block;
-   const int64_t r1 = min(n, r0 + rows_per_block);
+   const int64_t r0 = blk * kTileRows;
+   const int nr = (int)min((int64_t)kTileRows, n - r0);
+   int64_t base;
+   const bool in_lds = stage_tile(tile, rp, ci, va, r0, nr, base);
    double th[VEC], isr[VEC], s_r[VEC] = {0, 0}, s_x[VEC] = {0, 0};
    float mx[VEC] = {0.f, 0.f};
 #pragma unroll
    for (int v = 0; v < VEC; ++v) { th[v] = theta[c0 + v]; isr[v] = inv_sR[c0 + v]; }
-   for (int64_t row = r0 + grp; row < r1; row += GROUPS) {
-      double acc[VEC] = {0.0, 0.0};
-      row_gather<double, double, VEC, double>(rp, ci, va, X, k, row, c0, acc);
-      VecT<double, VEC> xv = vload<double, VEC>(X + row * k + c0);
-      VecT<double, VEC> wv;
-      VecT<TB, VEC> rq;
-#pragma unroll
-      for (int v = 0; v < VEC; ++v) {
-         wv.v[v] = acc[v];
-         double r = fma(-th[v], xv.v[v], acc[v]);
-         s_r[v] = fma(r, r, s_r[v]);
-         s_x[v] = fma(xv.v[v], xv.v[v], s_x[v]);
-         mx[v] = fmaxf(mx[v], (float)fabs(r));
-         rq.v[v] = Elem<TB>::store(r * isr[v]);
-      }
-      vstore<double, VEC>(W + row * k + c0, wv);
-      vstore<TB, VEC>(Rq + row * k + c0, rq);
-   }
+   __syncthreads();
+   if (in_lds)
+      resid_rows<TB, true>(r0, nr, grp, GROUPS, c0, k, tile.ci, tile.va, tile.rp, X, th, isr, W,
+                           Rq, s_r, s_x, mx);
+   else
+      resid_rows<TB, false>(r0, nr, grp, GROUPS, c0, k, ci + base, va + base, tile.rp, X, th, isr,
+                            W, Rq, s_r, s_x, mx);
 #pragma unroll
    for (int v = 0; v < VEC; ++v) {
       red[threadIdx.x][2 * v] = s_r[v];
@@ -233,10 +283,9 @@ __global__ __launch_bounds__(256) void spmm_resid_kernel(
 // and values are first copied to LDS with coalesced loads, so a row costs one
 // global latency (its gathers, all in flight together) instead of the
 // row-pointer -> index -> block chain; the streamed blocks (Dp, R, Dn) use
-// non-temporal accesses.  A block whose rows hold more than kChebCap entries
+// non-temporal accesses.  A block whose rows hold more than kTileCap entries
 // reads them from global memory instead (same arithmetic order).
-constexpr int kChebRows = 256;
-constexpr int kChebCap = 8 * kChebRows;
+constexpr int kChebRows = kTileRows;
 constexpr int kChebMaxK = 256;
 
 // the row loop of the Chebyshev step; HAS_DP and the index source are compile-
@@ -284,9 +333,7 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
     const TV *__restrict__ va, int k, const TB *__restrict__ Dc, const TB *__restrict__ Dp,
     const TB *__restrict__ R, ChebArgs ca_, TB *__restrict__ Dn, int lg, int64_t nblk) {
    using ACC = typename std::conditional<std::is_same<TB, double>::value, double, float>::type;
-   __shared__ int32_t s_rp[kChebRows + 1];
-   __shared__ int32_t s_ci[kChebCap + 8];  // + 8: padding for the unconditional index reads
-   __shared__ TV s_va[kChebCap + 8];
+   __shared__ CsrTile<TV> tile;
    __shared__ float redm[256][VEC];
    __shared__ __align__(16) ACC s_coef[3][kChebMaxK];
    __shared__ int s_se[kChebMaxK];
@@ -298,16 +345,8 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
    const int c0 = lane_g * VEC;  // host guarantees k == G * VEC
    const int64_t r0 = blk * kChebRows;
    const int nr = (int)min((int64_t)kChebRows, n - r0);
-   const int64_t base = rp[r0];
-   const int64_t nnzb = rp[r0 + nr] - base;
-   const bool in_lds = nnzb <= kChebCap;
-   for (int i = threadIdx.x; i <= nr; i += 256) s_rp[i] = (int32_t)(rp[r0 + i] - base);
-   if (in_lds)
-      for (int i = threadIdx.x; i < (int)nnzb + 8; i += 256) {
-         const bool ok = i < (int)nnzb;
-         s_ci[i] = ok ? ci[base + i] : 0;
-         s_va[i] = ok ? va[base + i] : TV(0);
-      }
+   int64_t base;
+   const bool in_lds = stage_tile(tile, rp, ci, va, r0, nr, base);
    // per-column coefficients from the device-side scales, with the output
    // scale 2^-se folded in (exact: a power of two):
    // T / s = (alpha sc / s)(A Dc - c Dc) + (-beta sp / s) Dp + (w sR / s) R
@@ -331,11 +370,11 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
    const ACC cshift = (ACC)ca_.c;
    __syncthreads();
    if (in_lds)
-      cheb_rows<TV, TB, VEC, ACC, HAS_DP, true>(r0, nr, grp, GROUPS, c0, k, s_ci, s_va, s_rp,
-                                                s_coef, cshift, Dc, Dp, R, Dn, mx);
-   else  // more than kChebCap entries in this block's rows: indices from global memory
+      cheb_rows<TV, TB, VEC, ACC, HAS_DP, true>(r0, nr, grp, GROUPS, c0, k, tile.ci, tile.va,
+                                                tile.rp, s_coef, cshift, Dc, Dp, R, Dn, mx);
+   else  // more than kTileCap entries in this block's rows: indices from global memory
       cheb_rows<TV, TB, VEC, ACC, HAS_DP, false>(r0, nr, grp, GROUPS, c0, k, ci + base, va + base,
-                                                 s_rp, s_coef, cshift, Dc, Dp, R, Dn, mx);
+                                                 tile.rp, s_coef, cshift, Dc, Dp, R, Dn, mx);
    // publish the unscaled column maxima (max |T / s| * s, exact)
 #pragma unroll
    for (int v = 0; v < VEC; ++v) redm[threadIdx.x][v] = ldexpf(mx[v], s_se[c0 + v]);
@@ -385,13 +424,12 @@ static int launch_spmm(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_
       int groups = k / VEC;
       int G = 1;
       while (G < groups && G < 64) G <<= 1;
-      int64_t rows_per_block = 256;
-      int64_t nblk = (n + rows_per_block - 1) / rows_per_block;
+      int64_t nblk = (n + kTileRows - 1) / kTileRows;
       dim3 grid((unsigned)xcd_grid(nblk));
       int lg = 0;
       while ((1 << lg) < G) ++lg;
       spmm_rm_kernel<TV, TX, TY, VEC><<<grid, 256, 0, ctx->stream>>>(
-          n, rp, ci, va, k, X, xrs, alpha, beta, Y, yrs, lg, rows_per_block, nblk);
+          n, rp, ci, va, k, X, xrs, alpha, beta, Y, yrs, lg, nblk);
    } else {
       int64_t total = n * (int64_t)k;
       spmm_strided_kernel<TV, TX, TY><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(
@@ -428,8 +466,7 @@ int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, c
    int G = k / 2;
    if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16 && G != 32 && G != 64)
       return set_error(ctx, kErrorInvalidArgument, "spmm_resid: k/2 must be a power of two");
-   int64_t rows_per_block = 512;
-   int64_t nblk = (n + rows_per_block - 1) / rows_per_block;
+   int64_t nblk = (n + kTileRows - 1) / kTileRows;
    double *part = (double *)workspace(ctx, (size_t)nblk * 2 * k * sizeof(double));
    if (!part) return set_error(ctx, kErrorCUDAMemory, "spmm_resid: workspace");
    dim3 grid((unsigned)xcd_grid(nblk));
@@ -438,8 +475,7 @@ int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, c
       int lg = 0;
       while ((1 << lg) < G) ++lg;
       spmm_resid_kernel<TB><<<grid, 256, 0, ctx->stream>>>(n, rp, ci, va, k, X, theta, W, (TB *)Rq,
-                                                           inv_sR, colmax, part, lg,
-                                                           rows_per_block, nblk);
+                                                           inv_sR, colmax, part, lg, nblk);
       OFRR_LAUNCH(ctx);
       return kSuccess;
    });
