@@ -26,6 +26,7 @@ constexpr int kJacobiThreads = 1024;
 constexpr int kLdsFullN = 128;    // full storage, row stride n + 1: <= 129 KiB of LDS
 constexpr int kLdsMaxN = 192;     // packed upper triangle: n(n+1)/2 doubles <= 144.8 KiB
 constexpr int kMaxSweeps = 30;
+constexpr int kMaxBlk = 9;  // 2x2 blocks per thread: ceil(128 * 129 / 2 / 1024) for n <= 256
 
 __device__ inline void pair_of(int nn, int r, int s, int &p, int &q) {
    if (s == 0) { p = nn - 1; q = r; }
@@ -58,8 +59,8 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
     int64_t strideLog, int32_t *__restrict__ rank, int32_t *__restrict__ nsweeps,
     int64_t strideInts, double *__restrict__ work, int64_t strideWork, int max_sweeps) {
    extern __shared__ double lds[];
-   __shared__ double rc[128], rsn[128];
-   __shared__ int sp_[128], sq_[128];
+   __shared__ double2 rcs[128];  // (c, s) of pair I in this round
+   __shared__ int2 rpq[128];     // (p, q) of pair I
    __shared__ int rotated, s_n;
    __shared__ double s_amax[16];
    const int b = blockIdx.x;
@@ -92,6 +93,22 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
    // rotations below eps * max|a_ij| cannot change any eigenvalue beyond the
    // accuracy the matrix carries: they end the sweep like the relative test
    const double tol_abs = 2.220446049250313e-16 * amax;
+   // this thread's 2x2 blocks (I <= J) of the upper block triangle, fixed for
+   // the whole solve; in full storage each block also writes its mirror
+   int blk_ij[kMaxBlk];
+   int nb_t = 0;
+   {
+      const int nblocks = np * (np + 1) / 2;
+      for (int t = threadIdx.x; t < nblocks && nb_t < kMaxBlk; t += blockDim.x) {
+         // I = largest row with I*np - I*(I-1)/2 <= t
+         const float b2 = 2.0f * np + 1.0f;
+         int I = (int)((b2 - sqrtf(b2 * b2 - 8.0f * (float)t)) * 0.5f);
+         while (I > 0 && I * np - I * (I - 1) / 2 > t) --I;
+         while ((I + 1) * np - (I + 1) * I / 2 <= t) ++I;
+         const int J = I + (t - (I * np - I * (I - 1) / 2));
+         blk_ij[nb_t++] = (I << 16) | J;
+      }
+   }
    int done = 0;
    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
       if (threadIdx.x == 0) rotated = 0;
@@ -111,27 +128,23 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
                   rotated = 1;
                }
             }
-            rc[s] = c;
-            rsn[s] = sn;
-            sp_[s] = p;
-            sq_[s] = q;
+            rcs[s] = make_double2(c, sn);
+            rpq[s] = make_int2(p, q);
             log[((int64_t)sweep * (nn - 1) + r) * np + s] = make_double2(c, sn);
          }
          __syncthreads();
-         // A' = J^T A J over the 2x2 blocks (I <= J) of the packed triangle,
-         // enumerated row by row of the upper block triangle
-         const int nblocks = np * (np + 1) / 2;
-         for (int t = threadIdx.x; t < nblocks; t += blockDim.x) {
-            // I = largest row with I*np - I*(I-1)/2 <= t
-            const float b2 = 2.0f * np + 1.0f;
-            int I = (int)((b2 - sqrtf(b2 * b2 - 8.0f * (float)t)) * 0.5f);
-            while (I > 0 && I * np - I * (I - 1) / 2 > t) --I;
-            while ((I + 1) * np - (I + 1) * I / 2 <= t) ++I;
-            const int J = I + (t - (I * np - I * (I - 1) / 2));
-            const double s1 = rsn[I], s2 = rsn[J];
+         // A' = J^T A J block by block over the upper block triangle:
+         // A'_IJ = J_I^T A_IJ J_J (I <= J); full storage mirrors each block
+#pragma unroll
+         for (int u = 0; u < kMaxBlk; ++u) {
+            if (u >= nb_t) break;
+            const int I = blk_ij[u] >> 16, J = blk_ij[u] & 0xffff;  // I <= J
+            const double2 csI = rcs[I], csJ = rcs[J];
+            const double s1 = csI.y, s2 = csJ.y;
             if (s1 == 0.0 && s2 == 0.0) continue;  // both rotations are the identity
-            const int p1 = sp_[I], q1 = sq_[I], p2 = sp_[J], q2 = sq_[J];
-            const double c1 = rc[I], c2 = rc[J];
+            const double c1 = csI.x, c2 = csJ.x;
+            const int2 pqI = rpq[I], pqJ = rpq[J];
+            const int p1 = pqI.x, q1 = pqI.y, p2 = pqJ.x, q2 = pqJ.y;
             const bool q1ok = q1 < n, q2ok = q2 < n;
             // J^T A_IJ J with J = [[c s][-s c]], explicit fused multiply-adds
             if (I == J) {
@@ -143,22 +156,22 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
                a[pk(q1, q1)] = fma(s1, m12, c1 * m22);
                a[pk(p1, q1)] = 0.0;
                if (FULL) a[pk(q1, p1)] = 0.0;
-            } else {
-               const int i11 = pk(p1, p2), i12 = pk(p1, q2), i21 = pk(q1, p2), i22 = pk(q1, q2);
-               const double a11 = a[i11];
-               const double a12 = q2ok ? a[i12] : 0.0;
-               const double a21 = q1ok ? a[i21] : 0.0;
-               const double a22 = (q1ok && q2ok) ? a[i22] : 0.0;
-               const double m11 = fma(c2, a11, -(s2 * a12)), m12 = fma(s2, a11, c2 * a12);
-               const double m21 = fma(c2, a21, -(s2 * a22)), m22 = fma(s2, a21, c2 * a22);
-               const double b11 = fma(c1, m11, -(s1 * m21)), b12 = fma(c1, m12, -(s1 * m22));
-               const double b21 = fma(s1, m11, c1 * m21), b22 = fma(s1, m12, c1 * m22);
-               a[i11] = b11;
-               if (FULL) a[pk(p2, p1)] = b11;
-               if (q2ok) { a[i12] = b12; if (FULL) a[pk(q2, p1)] = b12; }
-               if (q1ok) { a[i21] = b21; if (FULL) a[pk(p2, q1)] = b21; }
-               if (q1ok && q2ok) { a[i22] = b22; if (FULL) a[pk(q2, q1)] = b22; }
+               continue;
             }
+            const int i11 = pk(p1, p2), i12 = pk(p1, q2), i21 = pk(q1, p2), i22 = pk(q1, q2);
+            const double a11 = a[i11];
+            const double a12 = q2ok ? a[i12] : 0.0;
+            const double a21 = q1ok ? a[i21] : 0.0;
+            const double a22 = (q1ok && q2ok) ? a[i22] : 0.0;
+            const double m11 = fma(c2, a11, -(s2 * a12)), m12 = fma(s2, a11, c2 * a12);
+            const double m21 = fma(c2, a21, -(s2 * a22)), m22 = fma(s2, a21, c2 * a22);
+            const double b11 = fma(c1, m11, -(s1 * m21)), b12 = fma(c1, m12, -(s1 * m22));
+            const double b21 = fma(s1, m11, c1 * m21), b22 = fma(s1, m12, c1 * m22);
+            a[i11] = b11;
+            if (FULL) a[pk(p2, p1)] = b11;
+            if (q2ok) { a[i12] = b12; if (FULL) a[pk(q2, p1)] = b12; }
+            if (q1ok) { a[i21] = b21; if (FULL) a[pk(p2, q1)] = b21; }
+            if (q1ok && q2ok) { a[i22] = b22; if (FULL) a[pk(q2, q1)] = b22; }
          }
          __syncthreads();
       }

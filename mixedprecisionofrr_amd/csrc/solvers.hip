@@ -49,6 +49,7 @@ __global__ void arnoldi_elim_kernel(int i, int64_t n, const double *__restrict__
 
 // per-column sums of (Ax - theta x)^2 and x^2 for strided X, partial per block
 __global__ __launch_bounds__(256) void resid_cols_kernel(int64_t n, const double *__restrict__ AX,
+                                                         int64_t ars, int64_t acs,
                                                          const double *__restrict__ X,
                                                          int64_t xrs, int64_t xcs,
                                                          const double *__restrict__ theta,
@@ -60,7 +61,7 @@ __global__ __launch_bounds__(256) void resid_cols_kernel(int64_t n, const double
    const double th = theta[col];
    for (int64_t r = r0 + threadIdx.x; r < r1; r += 256) {
       double x = X[r * xrs + (int64_t)col * xcs];
-      double d = AX[r + (int64_t)col * n] - th * x;
+      double d = AX[r * ars + (int64_t)col * acs] - th * x;
       a = fma(d, d, a);
       b = fma(x, x, b);
    }
@@ -171,9 +172,11 @@ int ofrr_residuals(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, con
    double *part = (double *)((char *)th + al((size_t)k * 8));
    double *sums = (double *)((char *)part + al((size_t)nch * 2 * k * 8));
    OFRR_HIP(ctx, hipMemcpyAsync(th, theta_host, (size_t)k * 8, hipMemcpyHostToDevice, ctx->stream));
-   int st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, k, X, OFRR_F64, xrs, xcs, 1.0, 0.0, AX, OFRR_F64, 1, n);
+   // A X in the layout of X (row-major X takes the row-gather SpMM)
+   const int64_t ars = (xcs == 1) ? k : 1, acs = (xcs == 1) ? 1 : n;
+   int st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, k, X, OFRR_F64, xrs, xcs, 1.0, 0.0, AX, OFRR_F64, ars, acs);
    if (st) return st;
-   resid_cols_kernel<<<dim3(nch, k), 256, 0, ctx->stream>>>(n, AX, X, xrs, xcs, th, chunk, part);
+   resid_cols_kernel<<<dim3(nch, k), 256, 0, ctx->stream>>>(n, AX, ars, acs, X, xrs, xcs, th, chunk, part);
    OFRR_LAUNCH(ctx);
    if ((st = reduce_partials(ctx, nch, 2 * k, part, sums))) return st;
    OFRR_HIP(ctx, hipMemcpyAsync(hs, sums, (size_t)2 * k * 8, hipMemcpyDeviceToHost, ctx->stream));
