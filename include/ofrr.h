@@ -222,6 +222,48 @@ OFRR_API int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int6
                                double *X_dev, double *theta_host, double *res_host,
                                ofrr_filter_stats *stats);
 
+/* ---- dense operators -------------------------------------------------------
+ * C (R x nc, row-major, row stride ldc) = alpha P Q + beta C, with P R x K
+ * row-major (row stride ldp) and Q K x nc row-major (row stride ldq); nc a
+ * multiple of 16, <= 128.  Types: (BF16, BF16) or (FP16, FP16) on MFMA with FP32
+ * accumulation (P rows 16-byte aligned); (FP32 | FP64, FP64) on FP64 MFMA.  Any
+ * output type but FP8.  A X for column-major A is P = A^T row-major: pass the
+ * row-major copy; A^T Y is P = A column-major with ldp = lda.
+ * Replaces gemm(transA = false / true) over dense Matrix<T> operands
+ * (CPP/linalg/blas/mvops.hpp:79-82, mvops.cu:423-605; gemm.m:27-59). */
+OFRR_API int ofrr_dense_gemm(ofrr_ctx *ctx, int64_t R, int64_t K, int nc, const void *P_dev,
+                             int p_dtype, int64_t ldp, const void *Q_dev, int q_dtype, int64_t ldq,
+                             double alpha, double beta, void *C_dev, int c_dtype, int64_t ldc);
+
+/* Y = X converted element-wise to another format / layout: element (i, j) is
+ * X[i * xrs + j * xcs] and Y[i * yrs + j * ycs] (Matrix::cast, matrix.cu:178-187,
+ * without the host round trip). */
+OFRR_API int ofrr_convert(ofrr_ctx *ctx, int64_t rows, int64_t cols, const void *X_dev, int x_dtype,
+                          int64_t xrs, int64_t xcs, void *Y_dev, int y_dtype, int64_t yrs,
+                          int64_t ycs);
+
+/* Filtered OFRR on a dense symmetric FP64 operator A (n x n, column-major, lda):
+ * the si_eig(const Matrix<T_I>& A, ...) boundary over a dense A
+ * (CPP/linalg/solvers/eig/si_eig.hpp:24-27).  Same options, outputs and status as
+ * ofrr_filtered_eig; k a power of two in [16, 128]; basis FP64, FP16 or BF16 (the
+ * growth products then use a 16-bit copy of A on MFMA). */
+OFRR_API int ofrr_filtered_eig_dense(ofrr_ctx *ctx, int64_t n, const double *A_dev, int64_t lda,
+                                     const ofrr_filter_opts *opt, double *X_dev,
+                                     double *theta_host, double *res_host,
+                                     ofrr_filter_stats *stats);
+
+/* Truncated SVD of a dense M x N FP32 A (column-major, lda) by filtered OFRR on
+ * the implicit A^T A: the si_svd boundary (CPP/linalg/solvers/svd/si_svd.hpp,
+ * subspace_iter_svd.m).  V_dev (N x k row-major FP64): start block in, right
+ * singular vectors out; U_dev (M x k row-major FP64, nullable): left singular
+ * vectors A v / sigma; sigma_host descending; res_host: ||A^T A v - sigma^2 v|| /
+ * (sigma_1^2 ||v||).  Basis BF16 (growth products A X, A^T Y as BF16 MFMA GEMMs
+ * with FP32 accumulation) or FP64; the residual products use the FP32 A with
+ * FP64 accumulation. */
+OFRR_API int ofrr_svd_filtered(ofrr_ctx *ctx, int64_t M, int64_t N, const float *A_dev, int64_t lda,
+                               const ofrr_filter_opts *opt, double *V_dev, double *U_dev,
+                               double *sigma_host, double *res_host, ofrr_filter_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

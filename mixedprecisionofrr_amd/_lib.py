@@ -20,7 +20,8 @@ EXPORTS = [
     "ofrr_spmm_csr", "ofrr_iamax", "ofrr_matrix_iamax", "ofrr_scalecols", "ofrr_check_special",
     "ofrr_hessenberg", "ofrr_gram", "ofrr_basis_update", "ofrr_syev_batched", "ofrr_sygv",
     "ofrr_rr_truncated", "ofrr_rr_generalized", "ofrr_residuals", "ofrr_arnoldi_hess_eig",
-    "ofrr_subspace_iter_eig", "ofrr_filtered_eig",
+    "ofrr_subspace_iter_eig", "ofrr_filtered_eig", "ofrr_dense_gemm", "ofrr_convert",
+    "ofrr_filtered_eig_dense", "ofrr_svd_filtered",
 ]
 
 
@@ -84,6 +85,13 @@ def _decl(L):
     L.ofrr_subspace_iter_eig.argtypes = [vp, i64, i64, vp, vp, vp, i32, i32, i32, dbl, dbl, vp,
                                          i64, C.POINTER(dbl)]
     L.ofrr_filtered_eig.argtypes = [vp, i64, i64, vp, vp, vp, vp, i32, C.POINTER(FilterOpts), vp,
+                                    C.POINTER(dbl), C.POINTER(dbl), C.POINTER(FilterStats)]
+    L.ofrr_dense_gemm.argtypes = [vp, i64, i64, i32, vp, i32, i64, vp, i32, i64, dbl, dbl, vp, i32,
+                                  i64]
+    L.ofrr_convert.argtypes = [vp, i64, i64, vp, i32, i64, i64, vp, i32, i64, i64]
+    L.ofrr_filtered_eig_dense.argtypes = [vp, i64, vp, i64, C.POINTER(FilterOpts), vp,
+                                          C.POINTER(dbl), C.POINTER(dbl), C.POINTER(FilterStats)]
+    L.ofrr_svd_filtered.argtypes = [vp, i64, i64, vp, i64, C.POINTER(FilterOpts), vp, vp,
                                     C.POINTER(dbl), C.POINTER(dbl), C.POINTER(FilterStats)]
 
 

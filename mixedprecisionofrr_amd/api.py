@@ -345,3 +345,68 @@ def filtered_eig(ctx, A, X0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=to
         ctx.check(st, "filtered_eig")
     sd = {f: getattr(stats, f) for f, _ in FilterStats._fields_}
     return st, np.array(th[:]), X, np.array(res[:]), sd
+
+
+# ---- dense operators ---------------------------------------------------------
+def dense_gemm(ctx, P, Q, out_dtype=torch.float32, C=None, alpha=1.0, beta=0.0):
+    """C = alpha P Q + beta C for row-major P (R x K, unit column stride) and
+    Q (K x nc); gemm.m:27-59 over dense operands."""
+    R, K = P.shape
+    K2, nc = Q.shape
+    if K2 != K or P.stride(1) != 1 or Q.stride(1) != 1:
+        raise ValueError("dense_gemm: P (R x K) and Q (K x nc) must be row-major")
+    if C is None:
+        C = torch.zeros(R, nc, dtype=out_dtype, device=P.device)
+    ctx.check(ctx.lib.ofrr_dense_gemm(ctx.h, R, K, nc, _p(P), dtype_code(P), P.stride(0), _p(Q),
+                                      dtype_code(Q), Q.stride(0), alpha, beta, _p(C),
+                                      dtype_code(C), C.stride(0)), "dense_gemm")
+    return C
+
+
+def convert(ctx, X, Y):
+    """Y[...] = X element-wise (any formats and strides of two equally shaped 2-D tensors)."""
+    r, c = X.shape
+    ctx.check(ctx.lib.ofrr_convert(ctx.h, r, c, _p(X), dtype_code(X), X.stride(0), X.stride(1),
+                                   _p(Y), dtype_code(Y), Y.stride(0), Y.stride(1)), "convert")
+    return Y
+
+
+def filtered_eig_dense(ctx, A, X0, nwant, nd=1, degree=20, maxiter=100,
+                       basis_dtype=torch.float16, tol=1e-8, tau=1e-12, lower=0.0, normA=0.0):
+    """Filtered OFRR on a dense symmetric FP64 A (n x n, column-major storage =
+    the same tensor, symmetric).  Returns (status, theta, X, res, stats dict)."""
+    n, k = X0.shape
+    X = X0.to(torch.float64).contiguous().clone()
+    o = FilterOpts(k, nwant, nd, degree, maxiter, dtype_code(basis_dtype), tol, tau, lower, normA)
+    th = (C.c_double * k)()
+    res = (C.c_double * k)()
+    stats = FilterStats()
+    st = ctx.lib.ofrr_filtered_eig_dense(ctx.h, n, _p(A), A.stride(0), C.byref(o), _p(X), th, res,
+                                         C.byref(stats))
+    if st not in (STATUS_SUCCESS, STATUS_MAX_ITER):
+        ctx.check(st, "filtered_eig_dense")
+    sd = {f: getattr(stats, f) for f, _ in FilterStats._fields_}
+    return st, np.array(th[:]), X, np.array(res[:]), sd
+
+
+def svd_filtered(ctx, A, V0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=torch.bfloat16,
+                 tol=1e-8, tau=1e-12, want_u=True):
+    """Truncated SVD of a dense FP32 A (M x N) by filtered OFRR on A^T A.
+    A is given in column-major storage: pass A.t() of a row-major (N x M) tensor,
+    or any tensor with A.stride(0) == 1.  Returns (status, sigma, U, V, res, stats)."""
+    M, N = A.shape
+    if A.stride(0) != 1 or A.dtype != torch.float32:
+        raise ValueError("svd_filtered: A must be FP32 column-major (stride(0) == 1)")
+    k = V0.shape[1]
+    V = V0.to(torch.float64).contiguous().clone()
+    U = torch.zeros(M, k, dtype=torch.float64, device=A.device) if want_u else None
+    o = FilterOpts(k, nwant, nd, degree, maxiter, dtype_code(basis_dtype), tol, tau, 0.0, 0.0)
+    sig = (C.c_double * k)()
+    res = (C.c_double * k)()
+    stats = FilterStats()
+    st = ctx.lib.ofrr_svd_filtered(ctx.h, M, N, _p(A), A.stride(1), C.byref(o), _p(V),
+                                   _p(U) if U is not None else None, sig, res, C.byref(stats))
+    if st not in (STATUS_SUCCESS, STATUS_MAX_ITER):
+        ctx.check(st, "svd_filtered")
+    sd = {f: getattr(stats, f) for f, _ in FilterStats._fields_}
+    return st, np.array(sig[:]), U, V, np.array(res[:]), sd

@@ -52,6 +52,23 @@ int iamax(ofrr_ctx *ctx, int dtype, int64_t n, const void *x, int64_t inc, int32
 // per-column argmax |A(:,j)| with index base `base` (0 or 1) and signed value
 int argmax_columns(ofrr_ctx *ctx, int dtype, int64_t m, int n, const void *A, int64_t rs,
                    int64_t cs, int base, int32_t *idx, double *val);
+// C (R x nc, row-major) = alpha P Q + beta C with P R x K row-major and Q K x nc
+// row-major: (BF16|FP16) x same -> FP32 accumulation, (FP32|FP64) x FP64 -> FP64
+int dense_gemm(ofrr_ctx *ctx, int64_t R, int64_t K, int nc, const void *P, int pdt, int64_t ldp,
+               const void *Q, int qdt, int64_t ldq, double alpha, double beta, void *C, int cdt,
+               int64_t ldc);
+// Y = X element-wise in another format / layout
+int convert(ofrr_ctx *ctx, int64_t rows, int64_t cols, const void *X, int xdt, int64_t xrs,
+            int64_t xcs, void *Y, int ydt, int64_t yrs, int64_t ycs);
+// dense-operator epilogues: residual (R = W - X theta, quantised, norms) and the
+// Chebyshev combination from T = A Dc
+int resid_epilogue(ofrr_ctx *ctx, int64_t n, int k, const double *X, const double *th,
+                   const double *W, void *Rq, int rdt, const double *inv_sR, float *colmax,
+                   double *sums);
+int cheb_combine(ofrr_ctx *ctx, int64_t n, int k, const void *T, int tdt, const void *Dc,
+                 const void *Dp, const void *R, int bdt, const ChebArgs &args, void *Dn);
+// max_i (a_ii + sum_{j != i} |a_ij|) of a dense symmetric FP64 matrix (synchronous)
+int dense_gershgorin(ofrr_ctx *ctx, int64_t n, const double *A, int64_t lda, double *ub_host);
 // out[j] = sum over b of part[b * width + j], fixed summation order
 int reduce_partials(ofrr_ctx *ctx, int64_t nblk, int width, const double *part, double *out);
 

@@ -338,15 +338,176 @@ int ofrr_subspace_iter_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t 
    return kSuccess;
 }
 
-// ---------------------------------------------------------------------------
-int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, const int32_t *ci,
-                      const double *va, const void *va_lo, int lo_dt, const ofrr_filter_opts *o,
-                      double *Xio, double *theta_host, double *res_host, ofrr_filter_stats *stats) {
-   OFRR_CHECK_CTX(ctx);
-   (void)nnz;
-   if (!o || n <= 0 || !rp || !ci || !va || !Xio || !theta_host || !res_host)
-      return set_error(ctx, kErrorInvalidArgument, "filtered_eig: bad arguments");
-   const int k = o->k, nd = o->nd, d = o->degree, bdt = o->basis_dtype;
+}  // extern "C"
+
+namespace ofrr {
+
+constexpr double kFilterGain = 1e5;  // largest top-over-edge gain of one sweep
+
+// The operator of the filtered solver.  The CSR operator fuses each step into
+// one SpMM kernel; the dense operators (symmetric A, and A^T A for the SVD) run
+// MFMA products followed by the shared epilogue kernels.
+struct FilterOp {
+   int64_t n = 0;
+   virtual ~FilterOp() = default;
+   // an upper bound of the spectrum, or <= 0 when none is cheap to get
+   virtual int upper(ofrr_ctx *ctx, double *ub) = 0;
+   // W = A X, FP64, row-major n x k
+   virtual int apply64(ofrr_ctx *ctx, int k, const double *X, double *W) = 0;
+   // W = A X, R = W - X theta quantised into Rq (scale 1 / sR), column max |R|,
+   // FP64 sums [r^2, x^2] per column
+   virtual int resid(ofrr_ctx *ctx, int k, const double *X, const double *th, double *W, void *Rq,
+                     int bdt, const double *inv_sR, float *colmax, double *sums) = 0;
+   // one Chebyshev step
+   virtual int cheb(ofrr_ctx *ctx, int k, const void *Dc, const void *Dp, const void *R, int bdt,
+                    const ChebArgs &args, void *Dn) = 0;
+   // Z (row stride zrs) = A D in the growth arithmetic
+   virtual int apply_lo(ofrr_ctx *ctx, int k, const void *D, int bdt, void *Z, int zdt,
+                        int64_t zrs) = 0;
+};
+
+struct CsrOp final : FilterOp {
+   const int64_t *rp;
+   const int32_t *ci;
+   const double *va;
+   const void *vlo;
+   int vlo_dt;
+   int upper(ofrr_ctx *ctx, double *ub) override {
+      unsigned long long *gub = (unsigned long long *)workspace(ctx, 8);
+      if (!gub) return set_error(ctx, kErrorCUDAMemory, "filtered_eig: workspace");
+      OFRR_HIP(ctx, hipMemsetAsync(gub, 0, 8, ctx->stream));
+      gershgorin_upper_kernel<<<1024, 256, 0, ctx->stream>>>(n, rp, ci, va, gub);
+      OFRR_LAUNCH(ctx);
+      unsigned long long u = 0;
+      OFRR_HIP(ctx, hipMemcpyAsync(&u, gub, 8, hipMemcpyDeviceToHost, ctx->stream));
+      OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      *ub = ord_value(u);
+      return kSuccess;
+   }
+   int apply64(ofrr_ctx *ctx, int k, const double *X, double *W) override {
+      return spmm_csr(ctx, n, rp, ci, va, OFRR_F64, k, X, OFRR_F64, k, 1, 1.0, 0.0, W, OFRR_F64, k, 1);
+   }
+   int resid(ofrr_ctx *ctx, int k, const double *X, const double *th, double *W, void *Rq, int bdt,
+             const double *inv_sR, float *colmax, double *sums) override {
+      return spmm_resid(ctx, n, rp, ci, va, k, X, th, W, Rq, bdt, inv_sR, colmax, sums);
+   }
+   int cheb(ofrr_ctx *ctx, int k, const void *Dc, const void *Dp, const void *R, int bdt,
+            const ChebArgs &args, void *Dn) override {
+      return cheb_step(ctx, n, rp, ci, vlo, vlo_dt, k, Dc, Dp, R, bdt, args, Dn);
+   }
+   int apply_lo(ofrr_ctx *ctx, int k, const void *D, int bdt, void *Z, int zdt, int64_t zrs) override {
+      return spmm_csr(ctx, n, rp, ci, vlo, vlo_dt, k, D, bdt, k, 1, 1.0, 0.0, Z, zdt, zrs, 1);
+   }
+};
+
+// Y(i, j) *= s[j] for a row-major m x k block
+__global__ void scale_cols_rm_kernel(int64_t m, int k, double *__restrict__ Y,
+                                     const double *__restrict__ s) {
+   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (t < m * k) Y[t] *= s[t % k];
+}
+
+// device buffers owned by a dense operator for the length of one solve
+struct DevBuf {
+   void *p = nullptr;
+   ~DevBuf() { if (p) (void)hipFree(p); }
+   bool alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 16) == hipSuccess; }
+};
+
+static int64_t round8(int64_t x) { return (x + 7) / 8 * 8; }
+
+// Dense symmetric A (FP64, column-major = row-major).  The growth products use
+// a 16-bit copy of A on MFMA with FP32 accumulation (or A itself for an FP64
+// basis); the residual uses A in FP64.
+struct DenseSymOp final : FilterOp {
+   const double *A = nullptr;
+   int64_t lda = 0;
+   const void *Alo = nullptr;  // row-major n x n, row stride ldlo, type lo_dt (16-bit)
+   int64_t ldlo = 0;
+   int lo_dt = OFRR_F64;
+   void *T = nullptr;          // n x k product of a growth step
+   int upper(ofrr_ctx *ctx, double *ub) override { return dense_gershgorin(ctx, n, A, lda, ub); }
+   int apply64(ofrr_ctx *ctx, int k, const double *X, double *W) override {
+      return dense_gemm(ctx, n, n, k, A, OFRR_F64, lda, X, OFRR_F64, k, 1.0, 0.0, W, OFRR_F64, k);
+   }
+   int resid(ofrr_ctx *ctx, int k, const double *X, const double *th, double *W, void *Rq, int bdt,
+             const double *inv_sR, float *colmax, double *sums) override {
+      int st = apply64(ctx, k, X, W);
+      if (st) return st;
+      return resid_epilogue(ctx, n, k, X, th, W, Rq, bdt, inv_sR, colmax, sums);
+   }
+   int product(ofrr_ctx *ctx, int k, const void *D, int bdt, void *Z, int zdt, int64_t zrs) {
+      if (bdt == OFRR_F64)
+         return dense_gemm(ctx, n, n, k, A, OFRR_F64, lda, D, OFRR_F64, k, 1.0, 0.0, Z, zdt, zrs);
+      return dense_gemm(ctx, n, n, k, Alo, lo_dt, ldlo, D, bdt, k, 1.0, 0.0, Z, zdt, zrs);
+   }
+   int cheb(ofrr_ctx *ctx, int k, const void *Dc, const void *Dp, const void *R, int bdt,
+            const ChebArgs &args, void *Dn) override {
+      const int tdt = (bdt == OFRR_F64) ? OFRR_F64 : OFRR_F32;
+      int st = product(ctx, k, Dc, bdt, T, tdt, k);
+      if (st) return st;
+      return cheb_combine(ctx, n, k, T, tdt, Dc, Dp, R, bdt, args, Dn);
+   }
+   int apply_lo(ofrr_ctx *ctx, int k, const void *D, int bdt, void *Z, int zdt, int64_t zrs) override {
+      return product(ctx, k, D, bdt, Z, zdt, zrs);
+   }
+};
+
+// A^T A for a dense M x N A (FP32, column-major): the operator of the singular
+// value solver (north-star cfg3: OFRR on the implicit A^T A).  A X and A^T Y of
+// the growth steps are BF16 MFMA products with FP32 accumulation; the residual
+// and the final vectors use the FP32 A with FP64 accumulation.
+struct AtaOp final : FilterOp {
+   int64_t M = 0;
+   const float *Acm = nullptr;  // column-major, lda: as P of A^T Y
+   int64_t lda = 0;
+   const float *Arm = nullptr;  // row-major copy, row stride N: as P of A X
+   const void *Blo_rm = nullptr, *Blo_cm = nullptr;  // BF16 copies, row strides ldr / ldc
+   int64_t ldr = 0, ldc = 0;
+   double *Y64 = nullptr;       // M x k
+   void *Ylo = nullptr;         // M x k BF16
+   void *T = nullptr;           // N x k FP32
+   int upper(ofrr_ctx *, double *ub) override {
+      *ub = 0.0;  // no cheap bound of sigma_1^2: the window top follows the Ritz values
+      return kSuccess;
+   }
+   int apply64(ofrr_ctx *ctx, int k, const double *X, double *W) override {
+      int st = dense_gemm(ctx, M, n, k, Arm, OFRR_F32, n, X, OFRR_F64, k, 1.0, 0.0, Y64, OFRR_F64, k);
+      if (st) return st;
+      return dense_gemm(ctx, n, M, k, Acm, OFRR_F32, lda, Y64, OFRR_F64, k, 1.0, 0.0, W, OFRR_F64, k);
+   }
+   int resid(ofrr_ctx *ctx, int k, const double *X, const double *th, double *W, void *Rq, int bdt,
+             const double *inv_sR, float *colmax, double *sums) override {
+      int st = apply64(ctx, k, X, W);
+      if (st) return st;
+      return resid_epilogue(ctx, n, k, X, th, W, Rq, bdt, inv_sR, colmax, sums);
+   }
+   int product(ofrr_ctx *ctx, int k, const void *D, int bdt, void *Z, int zdt, int64_t zrs) {
+      int st;
+      if (bdt == OFRR_F64) {
+         if ((st = dense_gemm(ctx, M, n, k, Arm, OFRR_F32, n, D, OFRR_F64, k, 1.0, 0.0, Y64, OFRR_F64, k))) return st;
+         return dense_gemm(ctx, n, M, k, Acm, OFRR_F32, lda, Y64, OFRR_F64, k, 1.0, 0.0, Z, zdt, zrs);
+      }
+      if ((st = dense_gemm(ctx, M, n, k, Blo_rm, OFRR_BF16, ldr, D, OFRR_BF16, k, 1.0, 0.0, Ylo, OFRR_BF16, k))) return st;
+      return dense_gemm(ctx, n, M, k, Blo_cm, OFRR_BF16, ldc, Ylo, OFRR_BF16, k, 1.0, 0.0, Z, zdt, zrs);
+   }
+   int cheb(ofrr_ctx *ctx, int k, const void *Dc, const void *Dp, const void *R, int bdt,
+            const ChebArgs &args, void *Dn) override {
+      const int tdt = (bdt == OFRR_F64) ? OFRR_F64 : OFRR_F32;
+      int st = product(ctx, k, Dc, bdt, T, tdt, k);
+      if (st) return st;
+      return cheb_combine(ctx, n, k, T, tdt, Dc, Dp, R, bdt, args, Dn);
+   }
+   int apply_lo(ofrr_ctx *ctx, int k, const void *D, int bdt, void *Z, int zdt, int64_t zrs) override {
+      return product(ctx, k, D, bdt, Z, zdt, zrs);
+   }
+};
+
+static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o, double *Xio,
+                         double *theta_host, double *res_host, ofrr_filter_stats *stats) {
+   const int64_t n = op.n;
+   const int k = o->k, nd = o->nd, dmax = o->degree, bdt = o->basis_dtype;
+   const int d = dmax;  // buffer sizing: the largest degree of any sweep
    if (k < 2 || k > 128 || (k & (k - 1)) || nd < 1 || nd > 2 || d < 2 || o->nwant < 1 ||
        o->nwant > k || !dtype_valid(bdt) || (int64_t)k > n)
       return set_error(ctx, kErrorInvalidArgument,
@@ -354,10 +515,7 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
    const int m = k * (1 + nd);
    if (m > 256) return set_error(ctx, kErrorInvalidArgument, "filtered_eig: basis dimension > 256");
    const int zdt = (bdt == OFRR_F64) ? OFRR_F64 : OFRR_F32;
-   const void *vlo = va_lo ? va_lo : (const void *)va;
-   const int vlo_dt = va_lo ? lo_dt : OFRR_F64;
    const size_t bsz = dtype_size(bdt), zsz = dtype_size(zdt);
-   const int mid = d / 2;
    // ---- buffers
    size_t need = al((size_t)n * k * 8) * 3 + al((size_t)n * k * bsz) * 5 + al((size_t)n * k * nd * zsz) +
                  al((size_t)m * m * 8) * 2 + al((size_t)m * k * 8) + al((size_t)k * 8) * 14 +
@@ -406,20 +564,13 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
    OFRR_HIP(ctx, rec(0));
 
    int spmm_calls = 0, rank_min = m;
-   // ---- top of the filter window: the upper Gershgorin bound of A
-   double upper;
-   {
-      OFRR_HIP(ctx, hipMemsetAsync(gub, 0, 8, ctx->stream));
-      gershgorin_upper_kernel<<<1024, 256, 0, ctx->stream>>>(n, rp, ci, va, gub);
-      OFRR_LAUNCH(ctx);
-      unsigned long long u = 0;
-      OFRR_HIP(ctx, hipMemcpyAsync(&u, gub, 8, hipMemcpyDeviceToHost, ctx->stream));
-      OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      upper = ord_value(u);
-   }
+   // ---- top of the filter window: an upper bound of the spectrum (the
+   // Gershgorin bound for a CSR operator); without one, the top Ritz value
+   double upper = 0.0;
+   if ((st = op.upper(ctx, &upper))) return st;
    // ---- initial Rayleigh-Ritz on the start block
    OFRR_HIP(ctx, hipMemcpyAsync(X, Xio, (size_t)n * k * 8, hipMemcpyDeviceToDevice, ctx->stream));
-   if ((st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, k, X, OFRR_F64, k, 1, 1.0, 0.0, W, OFRR_F64, k, 1))) return st;
+   if ((st = op.apply64(ctx, k, X, W))) return st;
    ++spmm_calls;
    if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, X, OFRR_F64, k, 1, 1.0, 0.0, G, k))) return st;
    if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, W, OFRR_F64, k, 1, 1.0, 0.0, H, k))) return st;
@@ -456,7 +607,7 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
          OFRR_HIP(ctx, hipMemcpyAsync(invsR, inv.data(), (size_t)k * 8, hipMemcpyHostToDevice, ctx->stream));
          OFRR_HIP(ctx, hipMemcpyAsync(sRd, h_sR, (size_t)k * 8, hipMemcpyHostToDevice, ctx->stream));
          OFRR_HIP(ctx, hipMemsetAsync(cmaxR, 0, (size_t)k * 4, ctx->stream));
-         if ((st = spmm_resid(ctx, n, rp, ci, va, k, X, th, W, Rq, bdt, invsR, cmaxR, sums))) return st;
+         if ((st = op.resid(ctx, k, X, th, W, Rq, bdt, invsR, cmaxR, sums))) return st;
          ++spmm_calls;
          ++resid_passes;
          OFRR_HIP(ctx, hipMemcpyAsync(h_sums, sums, (size_t)2 * k * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -503,7 +654,24 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
       const double cut = theta[k - 1], lo = o->lower;
       const double e = 0.5 * (cut - lo), c0 = 0.5 * (cut + lo);
       for (int j = 0; j < k; ++j) Lth[j] = (theta[j] - c0) / e;
-      const double Lb = std::max(1.0, (std::max(theta[0], upper) - c0) / e);
+      const double top = (upper > 0.0) ? std::max(theta[0], upper) : theta[0] * (1.0 + 1e-3);
+      const double Lb = std::max(1.0, (top - c0) / e);
+      // Degree of this sweep.  The filter raises the top of the spectrum over
+      // the wanted edge by (rho(L(theta_1)) / rho(L(theta_nwant)))^degree,
+      // rho(x) = x + sqrt(x^2 - 1).  When that exceeds kFilterGain the new
+      // directions drop below the rounding of the converged ones and the
+      // residual stalls (a steeply decaying spectrum: A^T A of the SVD); the
+      // degree is capped there.  A flat top (the Laplacian) is not affected.
+      int d = dmax;
+      {
+         auto rho = [](double x) { return x > 1.0 ? x + std::sqrt(x * x - 1.0) : 1.0; };
+         const double ratio = rho(Lth[0]) / rho(Lth[o->nwant - 1]);
+         if (ratio > 1.0) {
+            const int cap = (int)std::floor(std::log(kFilterGain) / std::log(ratio));
+            d = std::max(2, std::min(dmax, cap));
+         }
+      }
+      const int mid = d / 2;
       std::vector<double> q(d + 1);
       q[0] = 1.0 / Lb;
       for (int s = 1; s <= d; ++s) q[s] = 1.0 / (2.0 * Lb - q[s - 1]);
@@ -550,7 +718,7 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
          ca.zero_next = (s + 1 < d) ? cmax[(s + 1) % 3] : nullptr;
          ca.sn_out = scale[out];
          ca.target_exp = tgt_step;
-         if ((st = cheb_step(ctx, n, rp, ci, vlo, vlo_dt, k, Dc, Dp, Rq, bdt, ca, Db[out]))) return st;
+         if ((st = op.cheb(ctx, k, Dc, Dp, Rq, bdt, ca, Db[out]))) return st;
          ++spmm_calls;
          ++filter_steps;
          prevIdx = curIdx;
@@ -564,11 +732,10 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
       OFRR_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
       // ---- projection: V = [X | Dd (| Dmid)], AV = [W | Z]
       const void *Dd = Dc;
-      if ((st = spmm_csr(ctx, n, rp, ci, vlo, vlo_dt, k, Dd, bdt, k, 1, 1.0, 0.0, Z, zdt, (int64_t)k * nd, 1))) return st;
+      if ((st = op.apply_lo(ctx, k, Dd, bdt, Z, zdt, (int64_t)k * nd))) return st;
       ++spmm_calls;
       if (nd == 2) {
-         if ((st = spmm_csr(ctx, n, rp, ci, vlo, vlo_dt, k, Dmid, bdt, k, 1, 1.0, 0.0,
-                            (char *)Z + (size_t)k * zsz, zdt, (int64_t)k * nd, 1))) return st;
+         if ((st = op.apply_lo(ctx, k, Dmid, bdt, (char *)Z + (size_t)k * zsz, zdt, (int64_t)k * nd))) return st;
          ++spmm_calls;
       }
       OFRR_HIP(ctx, hipEventRecord(ev[5], ctx->stream));
@@ -655,6 +822,119 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
    }
    if (st) return st;
    return converged ? kSuccess : kErrorSolverMaxIterations;
+}
+
+}  // namespace ofrr
+
+extern "C" {
+
+int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, const int32_t *ci,
+                      const double *va, const void *va_lo, int lo_dt, const ofrr_filter_opts *o,
+                      double *Xio, double *theta_host, double *res_host, ofrr_filter_stats *stats) {
+   OFRR_CHECK_CTX(ctx);
+   (void)nnz;
+   if (!o || n <= 0 || !rp || !ci || !va || !Xio || !theta_host || !res_host)
+      return set_error(ctx, kErrorInvalidArgument, "filtered_eig: bad arguments");
+   CsrOp op;
+   op.n = n;
+   op.rp = rp;
+   op.ci = ci;
+   op.va = va;
+   op.vlo = va_lo ? va_lo : (const void *)va;
+   op.vlo_dt = va_lo ? lo_dt : OFRR_F64;
+   return filtered_core(ctx, op, o, Xio, theta_host, res_host, stats);
+}
+
+
+int ofrr_filtered_eig_dense(ofrr_ctx *ctx, int64_t n, const double *A, int64_t lda,
+                            const ofrr_filter_opts *o, double *Xio, double *theta_host,
+                            double *res_host, ofrr_filter_stats *stats) {
+   OFRR_CHECK_CTX(ctx);
+   if (!o || n <= 0 || !A || lda < n || !Xio || !theta_host || !res_host)
+      return set_error(ctx, kErrorInvalidArgument, "filtered_eig_dense: bad arguments");
+   const int k = o->k, bdt = o->basis_dtype;
+   if (k < 16 || k > 128 || (k & (k - 1)))
+      return set_error(ctx, kErrorInvalidArgument, "filtered_eig_dense: k must be a power of two in [16, 128]");
+   if (bdt != OFRR_F64 && bdt != OFRR_F16 && bdt != OFRR_BF16)
+      return set_error(ctx, kErrorBLASUnsupportedType, "filtered_eig_dense: basis must be FP64, FP16 or BF16");
+   DenseSymOp op;
+   op.n = n;
+   op.A = A;
+   op.lda = lda;
+   DevBuf alo, tb;
+   if (bdt != OFRR_F64) {
+      op.lo_dt = bdt;
+      op.ldlo = round8(n);
+      if (!alo.alloc((size_t)n * op.ldlo * 2)) return set_error(ctx, kErrorCUDAMemory, "filtered_eig_dense: A copy");
+      // symmetric: the row-major copy of A is its column-major storage
+      int st = convert(ctx, n, n, A, OFRR_F64, 1, lda, alo.p, bdt, op.ldlo, 1);
+      if (st) return st;
+      op.Alo = alo.p;
+   }
+   if (!tb.alloc((size_t)n * k * 8)) return set_error(ctx, kErrorCUDAMemory, "filtered_eig_dense: product");
+   op.T = tb.p;
+   int st = filtered_core(ctx, op, o, Xio, theta_host, res_host, stats);
+   (void)hipStreamSynchronize(ctx->stream);
+   return st;
+}
+
+int ofrr_svd_filtered(ofrr_ctx *ctx, int64_t M, int64_t N, const float *A, int64_t lda,
+                      const ofrr_filter_opts *o, double *Vio, double *U, double *sigma_host,
+                      double *res_host, ofrr_filter_stats *stats) {
+   OFRR_CHECK_CTX(ctx);
+   if (!o || M <= 0 || N <= 0 || !A || lda < M || !Vio || !sigma_host || !res_host)
+      return set_error(ctx, kErrorInvalidArgument, "svd_filtered: bad arguments");
+   const int k = o->k, bdt = o->basis_dtype;
+   if (k < 16 || k > 128 || (k & (k - 1)) || k > N)
+      return set_error(ctx, kErrorInvalidArgument, "svd_filtered: k must be a power of two in [16, 128], <= N");
+   if (bdt != OFRR_F64 && bdt != OFRR_BF16)
+      return set_error(ctx, kErrorBLASUnsupportedType, "svd_filtered: basis must be BF16 or FP64");
+   AtaOp op;
+   op.n = N;
+   op.M = M;
+   op.Acm = A;
+   op.lda = lda;
+   DevBuf arm, brm, bcm, y64, ylo, tb;
+   int st;
+   if (!arm.alloc((size_t)M * N * 4) || !y64.alloc((size_t)M * k * 8) || !tb.alloc((size_t)N * k * 8))
+      return set_error(ctx, kErrorCUDAMemory, "svd_filtered: operator copies");
+   if ((st = convert(ctx, M, N, A, OFRR_F32, 1, lda, arm.p, OFRR_F32, N, 1))) return st;
+   op.Arm = (const float *)arm.p;
+   op.Y64 = (double *)y64.p;
+   op.T = tb.p;
+   if (bdt == OFRR_BF16) {
+      op.ldr = round8(N);
+      op.ldc = round8(M);
+      if (!brm.alloc((size_t)M * op.ldr * 2) || !bcm.alloc((size_t)N * op.ldc * 2) ||
+          !ylo.alloc((size_t)M * k * 2))
+         return set_error(ctx, kErrorCUDAMemory, "svd_filtered: BF16 copies");
+      if ((st = convert(ctx, M, N, A, OFRR_F32, 1, lda, brm.p, OFRR_BF16, op.ldr, 1))) return st;
+      if ((st = convert(ctx, M, N, A, OFRR_F32, 1, lda, bcm.p, OFRR_BF16, 1, op.ldc))) return st;
+      op.Blo_rm = brm.p;
+      op.Blo_cm = bcm.p;
+      op.Ylo = ylo.p;
+   }
+   std::vector<double> th(k);
+   st = filtered_core(ctx, op, o, Vio, th.data(), res_host, stats);
+   if (st != kSuccess && st != kErrorSolverMaxIterations) {
+      (void)hipStreamSynchronize(ctx->stream);
+      return st;
+   }
+   for (int j = 0; j < k; ++j) sigma_host[j] = std::sqrt(std::max(0.0, th[j]));
+   if (U) {
+      // U = A V diag(1 / sigma), FP64 accumulation
+      int st2 = dense_gemm(ctx, M, N, k, op.Arm, OFRR_F32, N, Vio, OFRR_F64, k, 1.0, 0.0, U, OFRR_F64, k);
+      if (st2) return st2;
+      std::vector<double> inv(k);
+      for (int j = 0; j < k; ++j) inv[j] = sigma_host[j] > 0.0 ? 1.0 / sigma_host[j] : 0.0;
+      double *dinv = (double *)workspace(ctx, (size_t)k * 8);
+      if (!dinv) return set_error(ctx, kErrorCUDAMemory, "svd_filtered: workspace");
+      OFRR_HIP(ctx, hipMemcpyAsync(dinv, inv.data(), (size_t)k * 8, hipMemcpyHostToDevice, ctx->stream));
+      scale_cols_rm_kernel<<<(unsigned)((M * k + 255) / 256), 256, 0, ctx->stream>>>(M, k, U, dinv);
+      OFRR_LAUNCH(ctx);
+   }
+   OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+   return st;
 }
 
 }  // extern "C"

@@ -667,9 +667,8 @@ static double colmaxabs(int64_t n, const double *a) {
 int orc_filtered_ofrr(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                       const double *vals, const orc_ofrr_opts *o, double *X, int64_t ldx,
                       double *theta, double *res) {
-   const int k = o->k, nd = o->nd, d = o->degree;
+   const int k = o->k, nd = o->nd, dmax = o->degree;
    const int m = k * (1 + nd);
-   const int mid = d / 2;
    double *V = (double *)malloc(sizeof(double) * (size_t)n * m);  /* [X | Dd | Dmid] */
    double *W = (double *)malloc(sizeof(double) * (size_t)n * m);  /* A V */
    double *Rr = (double *)malloc(sizeof(double) * (size_t)n * k);
@@ -709,6 +708,20 @@ int orc_filtered_ofrr(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
       /* Chebyshev filter on the correction: damp [lower, theta_k] */
       double cut = theta[k - 1], lo = o->lower;
       double e = 0.5 * (cut - lo), c0 = 0.5 * (cut + lo);
+      /* degree of this sweep: capped where the top-over-edge gain
+       * (rho(L(theta_1)) / rho(L(theta_nwant)))^d would exceed 1e5 */
+      int d = dmax;
+      {
+         const double l1 = (theta[0] - c0) / e, lw = (theta[o->nwant - 1] - c0) / e;
+         const double r1 = l1 > 1.0 ? l1 + sqrt(l1 * l1 - 1.0) : 1.0;
+         const double rw = lw > 1.0 ? lw + sqrt(lw * lw - 1.0) : 1.0;
+         const double ratio = r1 / rw;
+         if (ratio > 1.0) {
+            const int cap = (int)floor(log(1e5) / log(ratio));
+            d = cap < 2 ? 2 : (cap < dmax ? cap : dmax);
+         }
+      }
+      const int mid = d / 2;
       for (int j = 0; j < k; ++j) {
          double s = colmaxabs(n, Rr + (int64_t)j * n) / e;
          double f = s > 0 ? 1.0 / s : 1.0;
