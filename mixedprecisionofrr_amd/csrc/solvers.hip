@@ -448,8 +448,17 @@ struct DenseSymOp final : FilterOp {
       if (st) return st;
       return cheb_combine(ctx, n, k, T, tdt, Dc, Dp, R, bdt, args, Dn);
    }
+   // Z = A D for the projected matrix: the stored D (exact in its format) times
+   // the FP64 A with FP64 accumulation, so V^T A V carries no growth-product
+   // rounding (the error the truncated Rayleigh-Ritz would amplify along the
+   // small-eigenvalue directions of the Gram matrix)
+   double *D64 = nullptr;  // n x k
    int apply_lo(ofrr_ctx *ctx, int k, const void *D, int bdt, void *Z, int zdt, int64_t zrs) override {
-      return product(ctx, k, D, bdt, Z, zdt, zrs);
+      if (bdt == OFRR_F64)
+         return dense_gemm(ctx, n, n, k, A, OFRR_F64, lda, D, OFRR_F64, k, 1.0, 0.0, Z, zdt, zrs);
+      int st = convert(ctx, n, k, D, bdt, k, 1, D64, OFRR_F64, k, 1);
+      if (st) return st;
+      return dense_gemm(ctx, n, n, k, A, OFRR_F64, lda, D64, OFRR_F64, k, 1.0, 0.0, Z, zdt, zrs);
    }
 };
 
@@ -498,8 +507,20 @@ struct AtaOp final : FilterOp {
       if (st) return st;
       return cheb_combine(ctx, n, k, T, tdt, Dc, Dp, R, bdt, args, Dn);
    }
+   // Z = A^T (A D) for the projected matrix with the FP32 A and FP64
+   // accumulation (see DenseSymOp::apply_lo): BF16 products there pollute the
+   // Ritz values when the Gram matrix is nearly singular
+   double *D64 = nullptr;  // N x k
    int apply_lo(ofrr_ctx *ctx, int k, const void *D, int bdt, void *Z, int zdt, int64_t zrs) override {
-      return product(ctx, k, D, bdt, Z, zdt, zrs);
+      const void *Dq = D;
+      if (bdt != OFRR_F64) {
+         int st = convert(ctx, n, k, D, bdt, k, 1, D64, OFRR_F64, k, 1);
+         if (st) return st;
+         Dq = D64;
+      }
+      int st = dense_gemm(ctx, M, n, k, Arm, OFRR_F32, n, Dq, OFRR_F64, k, 1.0, 0.0, Y64, OFRR_F64, k);
+      if (st) return st;
+      return dense_gemm(ctx, n, M, k, Acm, OFRR_F32, lda, Y64, OFRR_F64, k, 1.0, 0.0, Z, zdt, zrs);
    }
 };
 
@@ -871,8 +892,11 @@ int ofrr_filtered_eig_dense(ofrr_ctx *ctx, int64_t n, const double *A, int64_t l
       if (st) return st;
       op.Alo = alo.p;
    }
-   if (!tb.alloc((size_t)n * k * 8)) return set_error(ctx, kErrorCUDAMemory, "filtered_eig_dense: product");
+   DevBuf d64;
+   if (!tb.alloc((size_t)n * k * 8) || !d64.alloc((size_t)n * k * 8))
+      return set_error(ctx, kErrorCUDAMemory, "filtered_eig_dense: product");
    op.T = tb.p;
+   op.D64 = (double *)d64.p;
    int st = filtered_core(ctx, op, o, Xio, theta_host, res_host, stats);
    (void)hipStreamSynchronize(ctx->stream);
    return st;
@@ -894,10 +918,12 @@ int ofrr_svd_filtered(ofrr_ctx *ctx, int64_t M, int64_t N, const float *A, int64
    op.M = M;
    op.Acm = A;
    op.lda = lda;
-   DevBuf arm, brm, bcm, y64, ylo, tb;
+   DevBuf arm, brm, bcm, y64, ylo, tb, d64;
    int st;
-   if (!arm.alloc((size_t)M * N * 4) || !y64.alloc((size_t)M * k * 8) || !tb.alloc((size_t)N * k * 8))
+   if (!arm.alloc((size_t)M * N * 4) || !y64.alloc((size_t)M * k * 8) || !tb.alloc((size_t)N * k * 8) ||
+       !d64.alloc((size_t)N * k * 8))
       return set_error(ctx, kErrorCUDAMemory, "svd_filtered: operator copies");
+   op.D64 = (double *)d64.p;
    if ((st = convert(ctx, M, N, A, OFRR_F32, 1, lda, arm.p, OFRR_F32, N, 1))) return st;
    op.Arm = (const float *)arm.p;
    op.Y64 = (double *)y64.p;
