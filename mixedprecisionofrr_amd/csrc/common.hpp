@@ -189,6 +189,37 @@ __device__ inline int64_t xcd_block(int64_t nblocks_logical) {
 }
 inline int64_t xcd_grid(int64_t nblocks) { return ((nblocks + 7) / 8) * 8; }
 
+// Strip schedule for row tiles of a banded operator whose long-range entries
+// sit about one "plane" of `plane` tiles away (a 3D stencil: the +-N^2
+// neighbours).  The tiles form chunks of `plane` tiles (a multiple of 8); each
+// chunk is cut into 8 strips of plane / 8 tiles and XCD x takes strip x of
+// every chunk, chunk after chunk.  Rows one plane apart are then processed
+// plane / 8 blocks apart on the same XCD, so the second and third reads of a
+// gathered row hit that XCD's L2 instead of returning to HBM (in row order they
+// are `plane` blocks apart and miss).  Narrower strips (several per XCD) were
+// measured slower on the 128^3 stencil: they lose more in-plane neighbours
+// across strip edges than they gain in reuse distance.  plane <= 0 selects
+// xcd_block.
+__device__ inline int64_t sched_block(int64_t nblocks_logical, int64_t plane) {
+   if (plane <= 0) return xcd_block(nblocks_logical);
+   const int64_t b = blockIdx.x, j = b >> 3, S = plane >> 3;
+   const int64_t t = (j / S) * plane + (b & 7) * S + j % S;
+   return t < nblocks_logical ? t : -1;
+}
+inline int64_t sched_grid(int64_t nblocks, int64_t plane) {
+   if (plane <= 0) return xcd_grid(nblocks);
+   return (nblocks + plane - 1) / plane * plane;
+}
+// plane size in tiles for an operator of bandwidth bw (max |col - row|): the
+// strip schedule pays when a plane spans at least 16 tiles and the matrix
+// holds at least two planes; 0 (row order) otherwise
+inline int64_t plane_tiles(int64_t n, int64_t bw, int tile_rows) {
+   const int64_t nblk = (n + tile_rows - 1) / tile_rows;
+   int64_t p = (bw + tile_rows - 1) / tile_rows;
+   p = (p + 7) / 8 * 8;
+   return (p >= 16 && nblk >= 2 * p) ? p : 0;
+}
+
 // wave64 reductions
 __device__ inline double wave_sum(double v) {
 #pragma unroll

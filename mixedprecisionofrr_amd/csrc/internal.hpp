@@ -4,16 +4,18 @@
 
 namespace ofrr {
 
-// Y = alpha A X + beta Y (any layout; row-major contiguous blocks take the fast path)
+// Y = alpha A X + beta Y (any layout; row-major contiguous blocks take the fast path).
+// bw: the operator's bandwidth max |col - row| when known (selects the strip
+// schedule of the row tiles, sched_block), 0 for row order
 int spmm_csr(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const void *va,
              int vdt, int k, const void *X, int xdt, int64_t xrs, int64_t xcs, double alpha,
-             double beta, void *Y, int ydt, int64_t yrs, int64_t ycs);
+             double beta, void *Y, int ydt, int64_t yrs, int64_t ycs, int64_t bw = 0);
 // W = A X, R = W - X diag(theta) quantised by inv_sR into Rq (rdt), column max
 // |R| folded into colmax (atomic max), sums[2j] = ||r_j||^2, sums[2j+1] = ||x_j||^2.
 // X, W, Rq row-major with row stride k.
 int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const double *va,
                int k, const double *X, const double *theta, double *W, void *Rq, int rdt,
-               const double *inv_sR, float *colmax, double *sums);
+               const double *inv_sR, float *colmax, double *sums, int64_t bw = 0);
 // one step of the normalised Chebyshev recurrence on row-major blocks of width k
 // (see spmm.hip and solvers.hip):
 // Dn = alpha (A Dc - c Dc) - beta Dp + w . R with stored blocks Dc = Dch sc,
@@ -31,7 +33,7 @@ struct ChebArgs {
 };
 int cheb_step(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const void *va,
               int vdt, int k, const void *Dc, const void *Dp, const void *R, int bdt,
-              const ChebArgs &args, void *Dn);
+              const ChebArgs &args, void *Dn, int64_t bw = 0);
 // G = alpha V^T W + beta G (FP64), strided operands
 int gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64_t vrs,
          int64_t vcs, const void *W, int wdt, int64_t wrs, int64_t wcs, double alpha, double beta,

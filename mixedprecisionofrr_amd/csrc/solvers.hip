@@ -99,21 +99,33 @@ static double ord_value(unsigned long long u) {
 }
 
 // upper Gershgorin bound max_i (a_ii + sum_{j != i} |a_ij|) of a symmetric CSR
-// matrix: the top of the window the Chebyshev filter is normalised on
+// matrix (the top of the window the Chebyshev filter is normalised on) into
+// out[0], and the bandwidth max |j - i| over the stored entries into out[1]
 __global__ void gershgorin_upper_kernel(int64_t n, const int64_t *__restrict__ rp,
                                         const int32_t *__restrict__ ci,
                                         const double *__restrict__ va,
                                         unsigned long long *__restrict__ out) {
    double m = -INFINITY;
+   unsigned long long bw = 0;
    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
         i += (int64_t)gridDim.x * blockDim.x) {
       double s = 0.0;
-      for (int64_t p = rp[i]; p < rp[i + 1]; ++p) s += (ci[p] == i) ? va[p] : fabs(va[p]);
+      for (int64_t p = rp[i]; p < rp[i + 1]; ++p) {
+         const int64_t j = ci[p];
+         s += (j == i) ? va[p] : fabs(va[p]);
+         bw = max(bw, (unsigned long long)(j > i ? j - i : i - j));
+      }
       m = fmax(m, s);
    }
 #pragma unroll
-   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-   if ((threadIdx.x & 63) == 0) atomicMax(out, ord_bits(m));
+   for (int o = 32; o > 0; o >>= 1) {
+      m = fmax(m, __shfl_xor(m, o, 64));
+      bw = max(bw, (unsigned long long)__shfl_xor((long long)bw, o, 64));
+   }
+   if ((threadIdx.x & 63) == 0) {
+      atomicMax(out, ord_bits(m));
+      atomicMax(out + 1, bw);
+   }
 }
 
 // ---- device arena for the drivers (grow-only, reused between solves) -------
@@ -372,31 +384,33 @@ struct CsrOp final : FilterOp {
    const double *va;
    const void *vlo;
    int vlo_dt;
+   int64_t bw = 0;  // bandwidth max |j - i|, found with the Gershgorin bound
    int upper(ofrr_ctx *ctx, double *ub) override {
-      unsigned long long *gub = (unsigned long long *)workspace(ctx, 8);
+      unsigned long long *gub = (unsigned long long *)workspace(ctx, 16);
       if (!gub) return set_error(ctx, kErrorCUDAMemory, "filtered_eig: workspace");
-      OFRR_HIP(ctx, hipMemsetAsync(gub, 0, 8, ctx->stream));
+      OFRR_HIP(ctx, hipMemsetAsync(gub, 0, 16, ctx->stream));
       gershgorin_upper_kernel<<<1024, 256, 0, ctx->stream>>>(n, rp, ci, va, gub);
       OFRR_LAUNCH(ctx);
-      unsigned long long u = 0;
-      OFRR_HIP(ctx, hipMemcpyAsync(&u, gub, 8, hipMemcpyDeviceToHost, ctx->stream));
+      unsigned long long u[2] = {0, 0};
+      OFRR_HIP(ctx, hipMemcpyAsync(u, gub, 16, hipMemcpyDeviceToHost, ctx->stream));
       OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      *ub = ord_value(u);
+      *ub = ord_value(u[0]);
+      bw = (int64_t)u[1];
       return kSuccess;
    }
    int apply64(ofrr_ctx *ctx, int k, const double *X, double *W) override {
-      return spmm_csr(ctx, n, rp, ci, va, OFRR_F64, k, X, OFRR_F64, k, 1, 1.0, 0.0, W, OFRR_F64, k, 1);
+      return spmm_csr(ctx, n, rp, ci, va, OFRR_F64, k, X, OFRR_F64, k, 1, 1.0, 0.0, W, OFRR_F64, k, 1, bw);
    }
    int resid(ofrr_ctx *ctx, int k, const double *X, const double *th, double *W, void *Rq, int bdt,
              const double *inv_sR, float *colmax, double *sums) override {
-      return spmm_resid(ctx, n, rp, ci, va, k, X, th, W, Rq, bdt, inv_sR, colmax, sums);
+      return spmm_resid(ctx, n, rp, ci, va, k, X, th, W, Rq, bdt, inv_sR, colmax, sums, bw);
    }
    int cheb(ofrr_ctx *ctx, int k, const void *Dc, const void *Dp, const void *R, int bdt,
             const ChebArgs &args, void *Dn) override {
-      return cheb_step(ctx, n, rp, ci, vlo, vlo_dt, k, Dc, Dp, R, bdt, args, Dn);
+      return cheb_step(ctx, n, rp, ci, vlo, vlo_dt, k, Dc, Dp, R, bdt, args, Dn, bw);
    }
    int apply_lo(ofrr_ctx *ctx, int k, const void *D, int bdt, void *Z, int zdt, int64_t zrs) override {
-      return spmm_csr(ctx, n, rp, ci, vlo, vlo_dt, k, D, bdt, k, 1, 1.0, 0.0, Z, zdt, zrs, 1);
+      return spmm_csr(ctx, n, rp, ci, vlo, vlo_dt, k, D, bdt, k, 1, 1.0, 0.0, Z, zdt, zrs, 1, bw);
    }
 };
 

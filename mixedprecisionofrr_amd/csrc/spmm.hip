@@ -11,9 +11,10 @@
 // owns 256 consecutive rows and first copies their row pointers, column
 // indices and values to LDS with coalesced loads; a row then costs one global
 // latency, its (up to 8) gathers in flight together.  The block index is
-// remapped so that each XCD walks a contiguous part of the matrix (its L2 then
-// serves the +-1 and +-N stencil neighbours).  Read-once and write-once blocks
-// use non-temporal accesses.
+// remapped so that each XCD walks its own part of the matrix: a contiguous row
+// range in general, and for operators with a wide band (3D stencils) one strip
+// of every plane (sched_block), so the XCD's L2 serves the +-1, +-N and +-N^2
+// neighbours.  Read-once and write-once blocks use non-temporal accesses.
 //
 // Kernels:
 //   spmm_rm      Y = alpha A X + beta Y
@@ -167,10 +168,10 @@ template <typename TV, typename TX, typename TY, int VEC>
 __global__ __launch_bounds__(256) void spmm_rm_kernel(
     int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
     const TV *__restrict__ va, int k, const TX *__restrict__ X, int64_t ldx, double alpha,
-    double beta, TY *__restrict__ Y, int64_t ldy, int lg, int64_t nblk) {
+    double beta, TY *__restrict__ Y, int64_t ldy, int lg, int64_t nblk, int64_t plane) {
    using ACC = typename std::conditional<std::is_same<TX, double>::value, double, float>::type;
    __shared__ CsrTile<TV> tile;
-   const int64_t blk = xcd_block(nblk);
+   const int64_t blk = sched_block(nblk, plane);
    if (blk < 0) return;
    const int G = 1 << lg;
    const int lane_g = threadIdx.x & (G - 1), grp = threadIdx.x >> lg;
@@ -228,12 +229,12 @@ __global__ __launch_bounds__(256) void spmm_resid_kernel(
     const double *__restrict__ va, int k, const double *__restrict__ X,
     const double *__restrict__ theta, double *__restrict__ W, TB *__restrict__ Rq,
     const double *__restrict__ inv_sR, float *__restrict__ colmax, double *__restrict__ part,
-    int lg, int64_t nblk) {
+    int lg, int64_t nblk, int64_t plane) {
    constexpr int VEC = 2;
    __shared__ CsrTile<double> tile;
    __shared__ double red[256][VEC * 2];
    __shared__ float redm[256][VEC];
-   const int64_t blk = xcd_block(nblk);
+   const int64_t blk = sched_block(nblk, plane);
    if (blk < 0) return;
    const int G = 1 << lg;
    const int GROUPS = 256 >> lg;
@@ -331,13 +332,13 @@ template <typename TV, typename TB, int VEC, bool HAS_DP>
 __global__ __launch_bounds__(256) void cheb_step_kernel(
     int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
     const TV *__restrict__ va, int k, const TB *__restrict__ Dc, const TB *__restrict__ Dp,
-    const TB *__restrict__ R, ChebArgs ca_, TB *__restrict__ Dn, int lg, int64_t nblk) {
+    const TB *__restrict__ R, ChebArgs ca_, TB *__restrict__ Dn, int lg, int64_t nblk, int64_t plane) {
    using ACC = typename std::conditional<std::is_same<TB, double>::value, double, float>::type;
    __shared__ CsrTile<TV> tile;
    __shared__ float redm[256][VEC];
    __shared__ __align__(16) ACC s_coef[3][kChebMaxK];
    __shared__ int s_se[kChebMaxK];
-   const int64_t blk = xcd_block(nblk);
+   const int64_t blk = sched_block(nblk, plane);
    if (blk < 0) return;
    const int G = 1 << lg;
    const int GROUPS = 256 >> lg;
@@ -416,7 +417,7 @@ static inline bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; 
 template <typename TV, typename TX, typename TY>
 static int launch_spmm(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci,
                        const TV *va, int k, const TX *X, int64_t xrs, int64_t xcs, double alpha,
-                       double beta, TY *Y, int64_t yrs, int64_t ycs) {
+                       double beta, TY *Y, int64_t yrs, int64_t ycs, int64_t bw) {
    constexpr int VEC = 16 / sizeof(TX) > 0 ? 16 / sizeof(TX) : 1;
    bool fast = xcs == 1 && ycs == 1 && k % VEC == 0 && aligned16(X) &&
                (xrs * (int64_t)sizeof(TX)) % 16 == 0;
@@ -425,11 +426,12 @@ static int launch_spmm(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_
       int G = 1;
       while (G < groups && G < 64) G <<= 1;
       int64_t nblk = (n + kTileRows - 1) / kTileRows;
-      dim3 grid((unsigned)xcd_grid(nblk));
+      const int64_t pt = plane_tiles(n, bw, kTileRows);
+      dim3 grid((unsigned)sched_grid(nblk, pt));
       int lg = 0;
       while ((1 << lg) < G) ++lg;
       spmm_rm_kernel<TV, TX, TY, VEC><<<grid, 256, 0, ctx->stream>>>(
-          n, rp, ci, va, k, X, xrs, alpha, beta, Y, yrs, lg, nblk);
+          n, rp, ci, va, k, X, xrs, alpha, beta, Y, yrs, lg, nblk, pt);
    } else {
       int64_t total = n * (int64_t)k;
       spmm_strided_kernel<TV, TX, TY><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(
@@ -441,7 +443,7 @@ static int launch_spmm(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_
 
 int spmm_csr(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const void *va,
              int vdt, int k, const void *X, int xdt, int64_t xrs, int64_t xcs, double alpha,
-             double beta, void *Y, int ydt, int64_t yrs, int64_t ycs) {
+             double beta, void *Y, int ydt, int64_t yrs, int64_t ycs, int64_t bw) {
    if (vdt != OFRR_F64 && vdt != OFRR_F32 && vdt != OFRR_F16)
       return set_error(ctx, kErrorBLASUnsupportedType, "spmm: operator values must be FP64, FP32 or FP16");
    if (ydt == OFRR_FP8)
@@ -453,7 +455,7 @@ int spmm_csr(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, con
          return dispatch_dtype(ydt, [&](auto ty) -> int {
             using TY = decltype(ty);
             return launch_spmm<TV, TX, TY>(ctx, n, rp, ci, (const TV *)va, k, (const TX *)X, xrs,
-                                           xcs, alpha, beta, (TY *)Y, yrs, ycs);
+                                           xcs, alpha, beta, (TY *)Y, yrs, ycs, bw);
          });
       });
    });
@@ -461,7 +463,7 @@ int spmm_csr(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, con
 
 int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const double *va,
                int k, const double *X, const double *theta, double *W, void *Rq, int rdt,
-               const double *inv_sR, float *colmax, double *sums /* 2k */) {
+               const double *inv_sR, float *colmax, double *sums /* 2k */, int64_t bw) {
    if (k % 2 != 0 || k > 128) return set_error(ctx, kErrorInvalidArgument, "spmm_resid: k must be even and <= 128");
    int G = k / 2;
    if (G != 1 && G != 2 && G != 4 && G != 8 && G != 16 && G != 32 && G != 64)
@@ -469,13 +471,14 @@ int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, c
    int64_t nblk = (n + kTileRows - 1) / kTileRows;
    double *part = (double *)workspace(ctx, (size_t)nblk * 2 * k * sizeof(double));
    if (!part) return set_error(ctx, kErrorCUDAMemory, "spmm_resid: workspace");
-   dim3 grid((unsigned)xcd_grid(nblk));
    int st = dispatch_dtype(rdt, [&](auto tb) -> int {
       using TB = decltype(tb);
+      const int64_t pt = plane_tiles(n, bw, kTileRows);
+      dim3 grid((unsigned)sched_grid(nblk, pt));
       int lg = 0;
       while ((1 << lg) < G) ++lg;
       spmm_resid_kernel<TB><<<grid, 256, 0, ctx->stream>>>(n, rp, ci, va, k, X, theta, W, (TB *)Rq,
-                                                           inv_sR, colmax, part, lg, nblk);
+                                                           inv_sR, colmax, part, lg, nblk, pt);
       OFRR_LAUNCH(ctx);
       return kSuccess;
    });
@@ -485,7 +488,7 @@ int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, c
 
 int cheb_step(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const void *va,
               int vdt, int k, const void *Dc, const void *Dp, const void *R, int bdt,
-              const ChebArgs &args, void *Dn) {
+              const ChebArgs &args, void *Dn, int64_t bw) {
    if (vdt != OFRR_F64 && vdt != OFRR_F32 && vdt != OFRR_F16)
       return set_error(ctx, kErrorBLASUnsupportedType, "cheb_step: operator values must be FP64, FP32 or FP16");
    return dispatch_value_dtype(vdt, [&](auto tv) -> int {
@@ -500,17 +503,18 @@ int cheb_step(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, co
             return set_error(ctx, kErrorInvalidArgument, "cheb_step: k / vector width must be a power of two <= 64");
          if (n >= ((int64_t)1 << 31)) return set_error(ctx, kErrorInvalidArgument, "cheb_step: n >= 2^31");
          int64_t nblk = (n + kChebRows - 1) / kChebRows;
-         dim3 grid((unsigned)xcd_grid(nblk));
+         const int64_t pt = plane_tiles(n, bw, kChebRows);
+         dim3 grid((unsigned)sched_grid(nblk, pt));
          int lg = 0;
          while ((1 << lg) < G) ++lg;
          if (Dp)
             cheb_step_kernel<TV, TB, VEC, true><<<grid, 256, 0, ctx->stream>>>(
                 n, rp, ci, (const TV *)va, k, (const TB *)Dc, (const TB *)Dp, (const TB *)R, args,
-                (TB *)Dn, lg, nblk);
+                (TB *)Dn, lg, nblk, pt);
          else
             cheb_step_kernel<TV, TB, VEC, false><<<grid, 256, 0, ctx->stream>>>(
                 n, rp, ci, (const TV *)va, k, (const TB *)Dc, nullptr, (const TB *)R, args,
-                (TB *)Dn, lg, nblk);
+                (TB *)Dn, lg, nblk, pt);
          OFRR_LAUNCH(ctx);
          return kSuccess;
       });
@@ -530,5 +534,5 @@ extern "C" int ofrr_spmm_csr(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_
    if (n == 0 || k == 0) return kSuccess;
    if (!rp || !ci || !va || !X || !Y) return set_error(ctx, kErrorInvalidArgument, "spmm_csr: null pointer");
    if (xrs < 1 && xcs < 1) return set_error(ctx, kErrorInvalidArgument, "spmm_csr: bad X strides");
-   return spmm_csr(ctx, n, rp, ci, va, vdt, k, X, xdt, xrs, xcs, alpha, beta, Y, ydt, yrs, ycs);
+   return spmm_csr(ctx, n, rp, ci, va, vdt, k, X, xdt, xrs, xcs, alpha, beta, Y, ydt, yrs, ycs, 0);
 }
