@@ -264,6 +264,94 @@ OFRR_API int ofrr_svd_filtered(ofrr_ctx *ctx, int64_t M, int64_t N, const float 
                                const ofrr_filter_opts *opt, double *V_dev, double *U_dev,
                                double *sigma_host, double *res_host, ofrr_filter_stats *stats);
 
+/* ---- matrix input (host only): the binary CSR files read by
+ * MatrixReader::read_csr_matrix (CPP/io/matrix_reader.cpp:8-45):
+ * <prefix>_row_ptr.bin (int32, rows + 1), <prefix>_col_ind.bin (int32, 0-based),
+ * <prefix>_values.bin (float64); cols = max column index + 1.  The structure is
+ * validated; a missing or empty file gives 1000 and the reference's message
+ * ("Failed to read row pointers from <file>", ...) in ofrr_io_last_error(). */
+OFRR_API int ofrr_csr_file_info(const char *prefix, int64_t *rows, int64_t *cols, int64_t *nnz);
+/* row_ptr (rows + 1, widened to int64), col_idx (nnz), vals (nnz): host arrays */
+OFRR_API int ofrr_csr_file_read(const char *prefix, int64_t *row_ptr, int32_t *col_idx,
+                                double *vals);
+/* <prefix>_singular_values.bin (float64; MatrixReader::read_singular_values,
+ * matrix_reader.cpp:47-50): *count values, the first min(cap, count) into out */
+OFRR_API int ofrr_singular_values_file(const char *prefix, double *out, int64_t cap,
+                                       int64_t *count);
+/* message of the last failed file call on this thread */
+OFRR_API const char *ofrr_io_last_error(void);
+
+/* Device bytes ofrr_filtered_eig holds for its solver state (Ritz, residual,
+ * filter and projected blocks) at n rows and the given options; the solve
+ * takes them from a grow-only per-thread arena. */
+OFRR_API int ofrr_filtered_eig_state_bytes(int64_t n, const ofrr_filter_opts *opt, int64_t *bytes);
+
+/* ---- serving: continuous batching of eigen requests on one GPU ---------------
+ * No counterpart in the reference (the caller of the path: a request loop that
+ * feeds ofrr_filtered_eig).  A server runs `slots` solves at once, each on its
+ * own host worker thread, context and non-blocking HIP stream, so the kernels of
+ * different solves overlap on the device.  Requests queue in arrival order; the
+ * head of the queue is admitted when a slot is free and its device footprint
+ * (operator + start block + solver state, ofrr_request_footprint) fits in the
+ * byte budget beside the solves already running.  A request's memory comes
+ * from the device's stream-ordered pool at admission and returns to it on
+ * completion.  Every reduction of a solve has a fixed order and no solve
+ * shares device memory with another, so a request's result does not depend on
+ * what runs beside it. */
+typedef struct ofrr_server ofrr_server;
+
+typedef struct {
+   int slots;          /* concurrent solves (0: 4) */
+   int64_t budget;     /* device bytes the running solves may hold together
+                          (0: 90% of the free device memory at creation) */
+} ofrr_server_opts;
+
+#define OFRR_SERVE_MAX_WANT 64
+typedef struct {
+   int dim;            /* 2: 5-point, 3: 7-point diffusion stencil */
+   int64_t N;          /* grid side; n = N^dim */
+   uint64_t coef_seed; /* face coefficients 10^(2u-1) (ofrr_stencil_csr); 0: Laplacian */
+   uint64_t start_seed;/* Philox seed of the N(0,1) start block */
+   int nwant;          /* largest eigenpairs wanted, 1..OFRR_SERVE_MAX_WANT */
+   int basis_dtype;    /* growth-block format (OFRR_F16, OFRR_BF16, OFRR_FP8, OFRR_F32, OFRR_F64) */
+   double tol;         /* ||A x - theta x|| / (||A|| ||x||) */
+   int degree;         /* filter degree per outer iteration (0: by size) */
+   int maxiter;        /* outer iterations (0: 200) */
+} ofrr_eig_request;
+
+typedef struct {
+   int status;         /* ofrr_filtered_eig's status for the solve */
+   int k, nd;          /* block width and correction blocks the solve used */
+   int iterations, converged, filter_steps;
+   int64_t n, footprint;             /* rows; device bytes held while running */
+   double t_submit, t_admit, t_done; /* seconds since the server was created */
+   double theta[OFRR_SERVE_MAX_WANT];/* descending Ritz values (nwant) */
+   double res[OFRR_SERVE_MAX_WANT];  /* their residuals */
+} ofrr_eig_result;
+
+typedef struct {
+   int64_t submitted, completed, running, queued;
+   int64_t bytes_running, bytes_peak; /* footprint of the running solves, and its maximum */
+   int running_peak;                  /* most solves running at once */
+   int64_t budget;
+} ofrr_server_stats;
+
+/* the plan of a request (block width k, correction blocks nd) and its device
+ * footprint in bytes; host only */
+OFRR_API int ofrr_request_footprint(const ofrr_eig_request *req, int *k, int *nd, int64_t *bytes);
+OFRR_API int ofrr_server_create(int device, const ofrr_server_opts *opt, ofrr_server **srv);
+/* enqueue; *id identifies the request.  1002 when its footprint exceeds the budget */
+OFRR_API int ofrr_server_submit(ofrr_server *srv, const ofrr_eig_request *req, int64_t *id);
+/* 0 and *out filled when request id is done (the server then forgets it),
+ * OFRR_PENDING while it is queued or running */
+#define OFRR_PENDING 1
+OFRR_API int ofrr_server_poll(ofrr_server *srv, int64_t id, ofrr_eig_result *out);
+/* blocks until request id is done; fills *out and forgets it */
+OFRR_API int ofrr_server_wait(ofrr_server *srv, int64_t id, ofrr_eig_result *out);
+OFRR_API int ofrr_server_stats_get(ofrr_server *srv, ofrr_server_stats *out);
+/* finishes the queued and running requests, then stops the workers */
+OFRR_API int ofrr_server_destroy(ofrr_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -711,6 +711,46 @@ MSVDStatus rr_svd(const Matrix<T_I> &A, Matrix<T_O> &U, Matrix<T_O> &S, Matrix<T
    return MSVDStatus::kSuccess;
 }
 
+// ---- matrix input: MatrixReader / CSRMatrix of CPP/io/matrix_reader.hpp:7-45,
+// reading the binary CSR files through ofrr_csr_file_read; a missing file
+// throws std::runtime_error with the reference's message
+// (matrix_reader.cpp:15-33).
+class CSRMatrix {
+  public:
+   std::vector<int> row_ptr;
+   std::vector<int> col_ind;
+   std::vector<double> values;
+   int rows;
+   int cols;
+   CSRMatrix() : rows(0), cols(0) {}
+};
+
+class MatrixReader {
+  public:
+   static CSRMatrix read_csr_matrix(const std::string &prefix) {
+      int64_t rows = 0, cols = 0, nnz = 0;
+      if (ofrr_csr_file_info(prefix.c_str(), &rows, &cols, &nnz) != 0)
+         throw std::runtime_error(ofrr_io_last_error());
+      std::vector<int64_t> rp((size_t)rows + 1);
+      CSRMatrix m;
+      m.col_ind.resize((size_t)nnz);
+      m.values.resize((size_t)nnz);
+      if (ofrr_csr_file_read(prefix.c_str(), rp.data(), m.col_ind.data(), m.values.data()) != 0)
+         throw std::runtime_error(ofrr_io_last_error());
+      m.row_ptr.assign(rp.begin(), rp.end());
+      m.rows = (int)rows;
+      m.cols = (int)cols;
+      return m;
+   }
+   static std::vector<double> read_singular_values(const std::string &prefix) {
+      int64_t count = 0;
+      if (ofrr_singular_values_file(prefix.c_str(), nullptr, 0, &count) != 0) return {};
+      std::vector<double> v((size_t)count);
+      (void)ofrr_singular_values_file(prefix.c_str(), v.data(), count, &count);
+      return v;
+   }
+};
+
 }  // namespace msvd
 
 #endif  // OFRR_MSVD_HPP

@@ -21,8 +21,14 @@ EXPORTS = [
     "ofrr_hessenberg", "ofrr_gram", "ofrr_basis_update", "ofrr_syev_batched", "ofrr_sygv",
     "ofrr_rr_truncated", "ofrr_rr_generalized", "ofrr_residuals", "ofrr_arnoldi_hess_eig",
     "ofrr_subspace_iter_eig", "ofrr_filtered_eig", "ofrr_dense_gemm", "ofrr_convert",
-    "ofrr_filtered_eig_dense", "ofrr_svd_filtered",
+    "ofrr_filtered_eig_dense", "ofrr_svd_filtered", "ofrr_filtered_eig_state_bytes",
+    "ofrr_request_footprint", "ofrr_server_create", "ofrr_server_submit", "ofrr_server_poll",
+    "ofrr_server_wait", "ofrr_server_stats_get", "ofrr_server_destroy",
+    "ofrr_csr_file_info", "ofrr_csr_file_read", "ofrr_singular_values_file", "ofrr_io_last_error",
 ]
+
+SERVE_MAX_WANT = 64
+PENDING = 1
 
 
 class FilterOpts(C.Structure):
@@ -37,6 +43,30 @@ class FilterStats(C.Structure):
                 ("t_total_ms", C.c_double), ("t_filter_ms", C.c_double),
                 ("t_zspmm_ms", C.c_double), ("t_gram_ms", C.c_double), ("t_rr_ms", C.c_double),
                 ("t_update_ms", C.c_double), ("t_resid_ms", C.c_double)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("slots", C.c_int), ("budget", C.c_int64)]
+
+
+class EigRequest(C.Structure):
+    _fields_ = [("dim", C.c_int), ("N", C.c_int64), ("coef_seed", C.c_uint64),
+                ("start_seed", C.c_uint64), ("nwant", C.c_int), ("basis_dtype", C.c_int),
+                ("tol", C.c_double), ("degree", C.c_int), ("maxiter", C.c_int)]
+
+
+class EigResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("k", C.c_int), ("nd", C.c_int), ("iterations", C.c_int),
+                ("converged", C.c_int), ("filter_steps", C.c_int), ("n", C.c_int64),
+                ("footprint", C.c_int64), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double), ("theta", C.c_double * SERVE_MAX_WANT),
+                ("res", C.c_double * SERVE_MAX_WANT)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("running", C.c_int64),
+                ("queued", C.c_int64), ("bytes_running", C.c_int64), ("bytes_peak", C.c_int64),
+                ("running_peak", C.c_int), ("budget", C.c_int64)]
 
 
 _lib = None
@@ -93,6 +123,19 @@ def _decl(L):
                                           C.POINTER(dbl), C.POINTER(dbl), C.POINTER(FilterStats)]
     L.ofrr_svd_filtered.argtypes = [vp, i64, i64, vp, i64, C.POINTER(FilterOpts), vp, vp,
                                     C.POINTER(dbl), C.POINTER(dbl), C.POINTER(FilterStats)]
+    L.ofrr_filtered_eig_state_bytes.argtypes = [i64, C.POINTER(FilterOpts), C.POINTER(i64)]
+    L.ofrr_request_footprint.argtypes = [C.POINTER(EigRequest), C.POINTER(i32), C.POINTER(i32),
+                                         C.POINTER(i64)]
+    L.ofrr_server_create.argtypes = [i32, C.POINTER(ServerOpts), C.POINTER(vp)]
+    L.ofrr_server_submit.argtypes = [vp, C.POINTER(EigRequest), C.POINTER(i64)]
+    L.ofrr_server_poll.argtypes = [vp, i64, C.POINTER(EigResult)]
+    L.ofrr_server_wait.argtypes = [vp, i64, C.POINTER(EigResult)]
+    L.ofrr_server_stats_get.argtypes = [vp, C.POINTER(ServerStats)]
+    L.ofrr_server_destroy.argtypes = [vp]
+    L.ofrr_csr_file_info.argtypes = [C.c_char_p, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
+    L.ofrr_csr_file_read.argtypes = [C.c_char_p, vp, vp, vp]
+    L.ofrr_singular_values_file.argtypes = [C.c_char_p, vp, i64, C.POINTER(i64)]
+    L.ofrr_io_last_error.restype = C.c_char_p
 
 
 def load():

@@ -117,6 +117,21 @@ class CSRMatrix:
         return CSRMatrix(n, rp, ci, va)
 
     @staticmethod
+    def from_files(ctx, prefix, scale=1.0, symmetrize=False, shift=0.0):
+        """The binary CSR files <prefix>_{row_ptr,col_ind,values}.bin
+        (MatrixReader::read_csr_matrix, CPP/io/matrix_reader.cpp:8-45) on the
+        device, with the experiment's preprocessing applied on the host
+        (MATLAB/+tests/+test4/test.m:31-49): values / scale, (K + K^T) / 2 when
+        symmetrize, + shift I."""
+        rp, ci, va, rows, cols = read_csr(prefix)
+        if rows != cols and (symmetrize or shift):
+            raise ValueError("from_files: symmetrize / shift need a square matrix")
+        va = va / scale
+        if symmetrize or shift:
+            rp, ci, va = _sym_shift(rp, ci, va, rows, symmetrize, shift)
+        return CSRMatrix.from_host(ctx, rp, ci, va)
+
+    @staticmethod
     def from_host(ctx, row_ptr, col_idx, vals):
         dev = f"cuda:{ctx.device}"
         rp = torch.as_tensor(np.asarray(row_ptr, np.int64), device=dev)
@@ -410,3 +425,147 @@ def svd_filtered(ctx, A, V0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=to
         ctx.check(st, "svd_filtered")
     sd = {f: getattr(stats, f) for f, _ in FilterStats._fields_}
     return st, np.array(sig[:]), U, V, np.array(res[:]), sd
+
+
+# ---- matrix input -----------------------------------------------------------
+def read_csr(prefix):
+    """Host arrays (row_ptr int64, col_idx int32, vals float64, rows, cols) of the
+    binary CSR files at prefix; errors carry the reference's messages."""
+    L = _lib.load()
+    pb = str(prefix).encode()
+    rows, cols, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+    st = L.ofrr_csr_file_info(pb, C.byref(rows), C.byref(cols), C.byref(nnz))
+    if st != 0:
+        raise OfrrError(st, L.ofrr_io_last_error().decode())
+    rp = np.zeros(rows.value + 1, np.int64)
+    ci = np.zeros(nnz.value, np.int32)
+    va = np.zeros(nnz.value, np.float64)
+    st = L.ofrr_csr_file_read(pb, rp.ctypes.data, ci.ctypes.data, va.ctypes.data)
+    if st != 0:
+        raise OfrrError(st, L.ofrr_io_last_error().decode())
+    return rp, ci, va, rows.value, cols.value
+
+
+def read_singular_values(prefix):
+    L = _lib.load()
+    pb = str(prefix).encode()
+    cnt = C.c_int64()
+    st = L.ofrr_singular_values_file(pb, None, 0, C.byref(cnt))
+    if st != 0:
+        raise OfrrError(st, L.ofrr_io_last_error().decode())
+    out = np.zeros(cnt.value, np.float64)
+    L.ofrr_singular_values_file(pb, out.ctypes.data, cnt.value, C.byref(cnt))
+    return out
+
+
+def _sym_shift(rp, ci, va, n, symmetrize, shift):
+    """CSR of (K + K^T) / 2 (symmetrize) + shift I, rows sorted by column."""
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))
+    r, c, v = [rows], [ci.astype(np.int64)], [va]
+    if symmetrize:
+        r, c, v = [rows, ci.astype(np.int64)], [ci.astype(np.int64), rows], [va / 2, va / 2]
+    if shift:
+        d = np.arange(n, dtype=np.int64)
+        r.append(d)
+        c.append(d)
+        v.append(np.full(n, float(shift)))
+    r, c, v = np.concatenate(r), np.concatenate(c), np.concatenate(v)
+    key = r * n + c
+    uk, inv = np.unique(key, return_inverse=True)
+    vals = np.zeros(len(uk))
+    np.add.at(vals, inv, v)
+    ur, uc = uk // n, uk % n
+    nrp = np.zeros(n + 1, np.int64)
+    np.add.at(nrp, ur + 1, 1)
+    return np.cumsum(nrp), uc.astype(np.int32), vals
+
+
+# ---- serving -----------------------------------------------------------------
+def request(dim, N, nwant, coef_seed=0, start_seed=0, basis_dtype=torch.float16, tol=1e-8,
+            degree=0, maxiter=0):
+    """An eigen request: the nwant largest eigenpairs of the dim-D diffusion
+    stencil on an N^dim grid (coef_seed 0: the Laplacian)."""
+    return _lib.EigRequest(dim, N, coef_seed, start_seed, nwant, dtype_code(basis_dtype), tol,
+                           degree, maxiter)
+
+
+def request_footprint(req):
+    """(block width k, correction blocks nd, device bytes) the server plans for
+    a request; host only."""
+    L = _lib.load()
+    k, nd, b = C.c_int(), C.c_int(), C.c_int64()
+    st = L.ofrr_request_footprint(C.byref(req), C.byref(k), C.byref(nd), C.byref(b))
+    if st != 0:
+        raise OfrrError(st, f"request_footprint: {L.ofrr_status_string(st).decode()}")
+    return k.value, nd.value, b.value
+
+
+def _result_dict(req, r):
+    w = req.nwant
+    return {"status": r.status, "k": r.k, "nd": r.nd, "iterations": r.iterations,
+            "converged": r.converged, "filter_steps": r.filter_steps, "n": r.n,
+            "footprint": r.footprint, "t_submit": r.t_submit, "t_admit": r.t_admit,
+            "t_done": r.t_done, "theta": np.array(r.theta[:w]), "res": np.array(r.res[:w])}
+
+
+class Server:
+    """Continuous-batching eigen server on one GPU (ofrr_server_*): `slots`
+    solves run at once on their own streams, admitted in arrival order against
+    a device-byte budget (0: 90% of free memory)."""
+
+    def __init__(self, device=0, slots=4, budget=0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        st = self.lib.ofrr_server_create(device, C.byref(_lib.ServerOpts(slots, budget)),
+                                         C.byref(h))
+        if st != 0:
+            raise OfrrError(st, f"ofrr_server_create: {self.lib.ofrr_status_string(st).decode()}")
+        self.h = h
+        self._reqs = {}
+
+    def submit(self, req):
+        i = C.c_int64()
+        st = self.lib.ofrr_server_submit(self.h, C.byref(req), C.byref(i))
+        if st != 0:
+            raise OfrrError(st, f"ofrr_server_submit: {self.lib.ofrr_status_string(st).decode()}")
+        self._reqs[i.value] = req
+        return i.value
+
+    def poll(self, rid):
+        """The result dict when request rid is done, else None."""
+        r = _lib.EigResult()
+        st = self.lib.ofrr_server_poll(self.h, rid, C.byref(r))
+        if st == _lib.PENDING:
+            return None
+        if st != 0:
+            raise OfrrError(st, "ofrr_server_poll")
+        return _result_dict(self._reqs.pop(rid), r)
+
+    def wait(self, rid):
+        r = _lib.EigResult()
+        st = self.lib.ofrr_server_wait(self.h, rid, C.byref(r))
+        if st != 0:
+            raise OfrrError(st, "ofrr_server_wait")
+        return _result_dict(self._reqs.pop(rid), r)
+
+    def stats(self):
+        s = _lib.ServerStats()
+        self.lib.ofrr_server_stats_get(self.h, C.byref(s))
+        return {f: getattr(s, f) for f, _ in _lib.ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            self.lib.ofrr_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -48,6 +48,10 @@ struct ofrr_ctx {
    // pinned host staging for small synchronous results
    void *pinned = nullptr;
    size_t pinned_bytes = 0;
+   // caller-owned device memory for the solver state of the next filtered
+   // solve (the serving path); null: the grow-only per-thread arena
+   void *state_mem = nullptr;
+   size_t state_bytes = 0;
    std::string last_error;
 };
 

@@ -18,6 +18,8 @@
 // truncation): S = diag(G)^{-1/2}; eig(S G S) = U L; keep L_i > tau max L;
 // B = S U_r L_r^{-1/2}; eig(B^T H B) = Z Theta; Y = B Z, descending.
 // The kept rank stays on the device: the kernels after the truncation read it.
+#include <mutex>
+
 #include "internal.hpp"
 
 namespace ofrr {
@@ -290,16 +292,20 @@ int syev_batched(ofrr_ctx *ctx, int batch, int n, const int32_t *n_dev, const do
    int32_t *rank = reinterpret_cast<int32_t *>(work + L.packed + L.log);
    int32_t *nsw = rank + n;
    const int64_t strideLog = strideWork / 2, strideInts = strideWork * 2;
-   static bool attr_set = false;
-   if (!attr_set) {
+   // the LDS limits of the two LDS-resident variants, set once per process
+   // (several host threads may run solves at once)
+   static std::once_flag attr_once;
+   static hipError_t attr_err = hipSuccess;
+   std::call_once(attr_once, [] {
       const int full = (int)((size_t)kLdsFullN * (kLdsFullN + 1) * sizeof(double));
       const int packed = (int)((size_t)kLdsMaxN * (kLdsMaxN + 1) / 2 * sizeof(double));
-      OFRR_HIP(ctx, hipFuncSetAttribute((const void *)jacobi_rot_kernel<true, true>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, full));
-      OFRR_HIP(ctx, hipFuncSetAttribute((const void *)jacobi_rot_kernel<true, false>,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, packed));
-      attr_set = true;
-   }
+      attr_err = hipFuncSetAttribute((const void *)jacobi_rot_kernel<true, true>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, full);
+      if (attr_err == hipSuccess)
+         attr_err = hipFuncSetAttribute((const void *)jacobi_rot_kernel<true, false>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, packed);
+   });
+   OFRR_HIP(ctx, attr_err);
    if (n <= kLdsFullN) {
       const size_t lds = (size_t)n * (n + 1) * sizeof(double);
       jacobi_rot_kernel<true, true><<<batch, kJacobiThreads, lds, ctx->stream>>>(

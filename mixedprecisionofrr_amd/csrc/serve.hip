@@ -1,0 +1,395 @@
+// Serving loop: continuous batching of eigen requests on one GPU.
+//
+// The reference has no request loop; this is the caller of the filtered OFRR
+// path (ofrr_filtered_eig), built for a stream of solves of differing n and k.
+//
+// A server owns `slots` workers.  Each worker is a host thread with its own
+// context and non-blocking HIP stream, so the kernels of the solves running in
+// different slots overlap on the device (the single-workgroup Rayleigh-Ritz
+// kernels of one solve leave most CUs to the filter sweeps of the others).
+// Requests wait in arrival order.  An idle worker admits the head of the queue
+// when its device footprint fits in the byte budget beside the solves already
+// running; a finished solve returns its memory and wakes the idle workers, so
+// requests that arrived while others ran start as soon as a slot and memory
+// are free.  Each request's memory (operator, start block, solver state) comes
+// from a stream-ordered pool owned by the server: allocation and release are
+// ordered on the worker's stream and never synchronise the other slots' work.
+//
+// Every reduction of a solve has a fixed order and the solves share no device
+// memory, so a request's result is the same alone and beside other requests.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace ofrr {
+namespace {
+
+constexpr int kDefaultSlots = 4;
+constexpr int kDefaultDegree = 120;
+constexpr int kDefaultMaxIter = 200;
+// device bytes a slot's context holds beside the request memory: the
+// Rayleigh-Ritz workspace and Jacobi rotation log at basis dimension 256
+constexpr int64_t kSlotOverhead = 64ll << 20;
+
+struct Plan {
+   int k = 0, nd = 0, degree = 0, maxiter = 0, lo_dt = OFRR_F64;
+   int64_t n = 0, nnz = 0;
+   size_t off_ci = 0, off_va = 0, off_lo = 0, off_x = 0, off_state = 0, total = 0;
+   size_t state_bytes = 0;
+};
+
+inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Block width: the smallest power of two >= 1.25 nwant (at least 16), so the
+// block carries a margin of unwanted directions beyond the wanted edge; two
+// correction blocks while the basis dimension stays <= 256, else one.
+int plan_request(const ofrr_eig_request &r, Plan &p) {
+   if ((r.dim != 2 && r.dim != 3) || r.N < 4 || r.nwant < 1 || r.nwant > OFRR_SERVE_MAX_WANT ||
+       !dtype_valid(r.basis_dtype) || !(r.tol > 0.0) || r.degree < 0 || r.maxiter < 0)
+      return kErrorInvalidArgument;
+   p.n = r.dim == 2 ? r.N * r.N : r.N * r.N * r.N;
+   if (p.n >= (int64_t)1 << 31) return kErrorBLASInvalidSize;
+   int64_t nnz = 0;
+   if (ofrr_stencil_nnz(r.dim, r.N, &nnz) != kSuccess) return kErrorInvalidArgument;
+   p.nnz = nnz;
+   const int want = (5 * r.nwant + 3) / 4;
+   p.k = 16;
+   while (p.k < want) p.k *= 2;
+   if (p.k > p.n) return kErrorInvalidArgument;
+   p.nd = (3 * p.k <= 256) ? 2 : 1;
+   p.degree = r.degree ? r.degree : kDefaultDegree;
+   p.maxiter = r.maxiter ? r.maxiter : kDefaultMaxIter;
+   // operator values of the growth products: FP64 for an FP64 basis, else a
+   // copy whose rounding does not exceed FP32's (FP16 is exact for the
+   // constant-coefficient Laplacian, FP32 keeps variable coefficients to 2^-24)
+   if (r.basis_dtype == OFRR_F64) p.lo_dt = OFRR_F64;
+   else p.lo_dt = (r.coef_seed == 0) ? OFRR_F16 : OFRR_F32;
+   ofrr_filter_opts o{};
+   o.k = p.k;
+   o.nd = p.nd;
+   o.degree = p.degree;
+   o.basis_dtype = r.basis_dtype;
+   int64_t sb = 0;
+   if (ofrr_filtered_eig_state_bytes(p.n, &o, &sb) != kSuccess) return kErrorInvalidArgument;
+   p.state_bytes = (size_t)sb;
+   size_t off = al256((size_t)(p.n + 1) * 8);
+   p.off_ci = off;
+   off += al256((size_t)p.nnz * 4);
+   p.off_va = off;
+   off += al256((size_t)p.nnz * 8);
+   p.off_lo = off;
+   if (p.lo_dt != OFRR_F64) off += al256((size_t)p.nnz * dtype_size(p.lo_dt));
+   p.off_x = off;
+   off += al256((size_t)p.n * p.k * 8);
+   p.off_state = off;
+   off += al256(p.state_bytes);
+   p.total = off;
+   return kSuccess;
+}
+
+enum RecState { kQueued = 0, kRunning = 1, kDone = 2 };
+
+struct Rec {
+   ofrr_eig_request req{};
+   Plan plan;
+   ofrr_eig_result res{};
+   int state = kQueued;
+   int64_t footprint = 0;
+};
+
+}  // namespace
+}  // namespace ofrr
+
+using namespace ofrr;
+
+struct ofrr_server {
+   int device = 0;
+   int64_t budget = 0;
+   hipMemPool_t pool = nullptr;
+   std::mutex mu;
+   std::condition_variable cv_admit, cv_done;
+   std::deque<int64_t> queue;
+   std::unordered_map<int64_t, Rec> recs;  // element references stay valid on rehash
+   int64_t next_id = 1;
+   bool stopping = false;
+   int64_t submitted = 0, completed = 0, bytes_running = 0, bytes_peak = 0;
+   int running = 0, running_peak = 0;
+   std::chrono::steady_clock::time_point t0;
+   std::vector<std::thread> workers;
+   std::vector<int> worker_status;
+
+   double now() const {
+      return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+   }
+   // the head of the queue may start: its footprint fits beside the running solves
+   bool head_fits() const {
+      if (queue.empty()) return false;
+      const Rec &r = recs.at(queue.front());
+      return bytes_running + r.footprint <= budget;
+   }
+   void run_one(ofrr_ctx *ctx, Rec &r);
+   void worker(int slot);
+};
+
+void ofrr_server::run_one(ofrr_ctx *ctx, Rec &r) {
+   const Plan &p = r.plan;
+   const ofrr_eig_request &q = r.req;
+   ofrr_eig_result &out = r.res;
+   out.k = p.k;
+   out.nd = p.nd;
+   out.n = p.n;
+   out.footprint = r.footprint;
+   for (int j = 0; j < OFRR_SERVE_MAX_WANT; ++j) out.theta[j] = out.res[j] = __builtin_nan("");
+   char *mem = nullptr;
+   hipError_t e = hipMallocFromPoolAsync((void **)&mem, p.total, pool, ctx->stream);
+   if (e != hipSuccess) {
+      out.status = hip_status(ctx, e, "serve: hipMallocFromPoolAsync");
+      return;
+   }
+   int64_t *rp = (int64_t *)mem;
+   int32_t *ci = (int32_t *)(mem + p.off_ci);
+   double *va = (double *)(mem + p.off_va);
+   void *vlo = (p.lo_dt == OFRR_F64) ? nullptr : (void *)(mem + p.off_lo);
+   double *X = (double *)(mem + p.off_x);
+   int st = ofrr_stencil_csr(ctx, q.dim, q.N, q.coef_seed, rp, ci, OFRR_F64, va);
+   if (!st && vlo) st = ofrr_convert(ctx, p.nnz, 1, va, OFRR_F64, 1, 1, vlo, p.lo_dt, 1, 1);
+   if (!st) st = ofrr_fill_randn(ctx, OFRR_F64, p.n * p.k, X, 1, q.start_seed, 0, 0);
+   if (!st) {
+      ofrr_filter_opts o{};
+      o.k = p.k;
+      o.nwant = q.nwant;
+      o.nd = p.nd;
+      o.degree = p.degree;
+      o.maxiter = p.maxiter;
+      o.basis_dtype = q.basis_dtype;
+      o.tol = q.tol;
+      o.tau = 1e-12;
+      o.lower = 0.0;  // the diffusion operators are positive definite
+      o.normA = 0.0;  // ||A||_2 = the largest Ritz value
+      std::vector<double> th(p.k), rs(p.k);
+      ofrr_filter_stats stats{};
+      ctx->state_mem = mem + p.off_state;
+      ctx->state_bytes = p.state_bytes;
+      st = ofrr_filtered_eig(ctx, p.n, p.nnz, rp, ci, va, vlo, p.lo_dt, &o, X, th.data(), rs.data(),
+                             &stats);
+      ctx->state_mem = nullptr;
+      ctx->state_bytes = 0;
+      for (int j = 0; j < q.nwant; ++j) {
+         out.theta[j] = th[j];
+         out.res[j] = rs[j];
+      }
+      out.iterations = stats.iterations;
+      out.converged = stats.converged;
+      out.filter_steps = stats.filter_steps;
+   }
+   out.status = st;
+   e = hipFreeAsync(mem, ctx->stream);
+   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+   if (e != hipSuccess && !out.status) out.status = hip_status(ctx, e, "serve: release");
+}
+
+void ofrr_server::worker(int slot) {
+   ofrr_ctx *ctx = nullptr;
+   hipStream_t s = nullptr;
+   int st = ofrr_ctx_create(device, &ctx);
+   if (!st && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) st = kErrorCUDAGeneral;
+   if (!st) st = ofrr_ctx_set_stream(ctx, s);
+   {
+      std::lock_guard<std::mutex> lk(mu);
+      worker_status[slot] = st;
+   }
+   cv_done.notify_all();
+   for (;;) {
+      std::unique_lock<std::mutex> lk(mu);
+      cv_admit.wait(lk, [&] { return (stopping && queue.empty()) || (!st && head_fits()); });
+      if (queue.empty() || st) break;
+      const int64_t id = queue.front();
+      queue.pop_front();
+      Rec &r = recs.at(id);
+      r.state = kRunning;
+      r.res.t_admit = now();
+      bytes_running += r.footprint;
+      bytes_peak = std::max(bytes_peak, bytes_running);
+      ++running;
+      running_peak = std::max(running_peak, running);
+      lk.unlock();
+      cv_admit.notify_all();  // the next head may fit as well
+      run_one(ctx, r);
+      lk.lock();
+      r.res.t_done = now();
+      r.state = kDone;
+      bytes_running -= r.footprint;
+      --running;
+      ++completed;
+      lk.unlock();
+      cv_done.notify_all();
+      cv_admit.notify_all();
+   }
+   if (ctx) ofrr_ctx_destroy(ctx);
+   if (s) (void)hipStreamDestroy(s);
+}
+
+extern "C" {
+
+int ofrr_request_footprint(const ofrr_eig_request *req, int *k, int *nd, int64_t *bytes) {
+   if (!req) return kErrorInvalidArgument;
+   Plan p;
+   const int st = plan_request(*req, p);
+   if (st) return st;
+   if (k) *k = p.k;
+   if (nd) *nd = p.nd;
+   if (bytes) *bytes = (int64_t)p.total + kSlotOverhead;
+   return kSuccess;
+}
+
+int ofrr_server_create(int device, const ofrr_server_opts *opt, ofrr_server **out) {
+   if (!out) return kErrorInvalidArgument;
+   *out = nullptr;
+   const int slots = (opt && opt->slots > 0) ? opt->slots : kDefaultSlots;
+   if (slots > 16) return kErrorInvalidArgument;
+   int ndev = 0;
+   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return kErrorCUDAGeneral;
+   if (device < 0 || device >= ndev) return kErrorInvalidArgument;
+   if (hipSetDevice(device) != hipSuccess) return kErrorCUDAGeneral;
+   ofrr_server *s = new ofrr_server();
+   s->device = device;
+   s->t0 = std::chrono::steady_clock::now();
+   if (opt && opt->budget > 0) {
+      s->budget = opt->budget;
+   } else {
+      size_t fr = 0, tot = 0;
+      if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+         delete s;
+         return kErrorCUDAGeneral;
+      }
+      s->budget = (int64_t)(0.9 * (double)fr);
+   }
+   hipMemPoolProps props{};
+   props.allocType = hipMemAllocationTypePinned;
+   props.handleTypes = hipMemHandleTypeNone;
+   props.location.type = hipMemLocationTypeDevice;
+   props.location.id = device;
+   if (hipMemPoolCreate(&s->pool, &props) != hipSuccess) {
+      delete s;
+      return kErrorCUDAMemory;
+   }
+   // freed blocks stay in the pool for the next request instead of returning
+   // to the driver at every synchronisation
+   uint64_t keep = UINT64_MAX;
+   (void)hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold, &keep);
+   s->worker_status.assign(slots, -1);
+   try {
+      for (int i = 0; i < slots; ++i) s->workers.emplace_back([s, i] { s->worker(i); });
+   } catch (...) {
+      {
+         std::lock_guard<std::mutex> lk(s->mu);
+         s->stopping = true;
+      }
+      s->cv_admit.notify_all();
+      for (auto &t : s->workers) t.join();
+      (void)hipMemPoolDestroy(s->pool);
+      delete s;
+      return kErrorInternalError;
+   }
+   // every worker has its context and stream before the server is handed out
+   int st = kSuccess;
+   {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_done.wait(lk, [&] {
+         for (int w : s->worker_status)
+            if (w < 0) return false;
+         return true;
+      });
+      for (int w : s->worker_status)
+         if (w) st = w;
+   }
+   if (st) {
+      ofrr_server_destroy(s);
+      return st;
+   }
+   *out = s;
+   return kSuccess;
+}
+
+int ofrr_server_submit(ofrr_server *s, const ofrr_eig_request *req, int64_t *id) {
+   if (!s || !req || !id) return kErrorInvalidArgument;
+   Rec r;
+   r.req = *req;
+   int st = plan_request(*req, r.plan);
+   if (st) return st;
+   r.footprint = (int64_t)r.plan.total + kSlotOverhead;
+   {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->stopping) return kErrorInvalidOperation;
+      if (r.footprint > s->budget) return kErrorOutOfMemory;
+      r.res.t_submit = s->now();
+      *id = s->next_id++;
+      s->recs.emplace(*id, r);
+      s->queue.push_back(*id);
+      ++s->submitted;
+   }
+   s->cv_admit.notify_all();
+   return kSuccess;
+}
+
+int ofrr_server_poll(ofrr_server *s, int64_t id, ofrr_eig_result *out) {
+   if (!s || !out) return kErrorInvalidArgument;
+   std::lock_guard<std::mutex> lk(s->mu);
+   auto it = s->recs.find(id);
+   if (it == s->recs.end()) return kErrorInvalidArgument;
+   if (it->second.state != kDone) return OFRR_PENDING;
+   *out = it->second.res;
+   s->recs.erase(it);
+   return kSuccess;
+}
+
+int ofrr_server_wait(ofrr_server *s, int64_t id, ofrr_eig_result *out) {
+   if (!s || !out) return kErrorInvalidArgument;
+   std::unique_lock<std::mutex> lk(s->mu);
+   auto it = s->recs.find(id);
+   if (it == s->recs.end()) return kErrorInvalidArgument;
+   // submissions may rehash the map while this thread waits: iterators are
+   // invalidated then, references to the element are not
+   const Rec &r = it->second;
+   s->cv_done.wait(lk, [&] { return r.state == kDone; });
+   *out = r.res;
+   s->recs.erase(id);
+   return kSuccess;
+}
+
+int ofrr_server_stats_get(ofrr_server *s, ofrr_server_stats *out) {
+   if (!s || !out) return kErrorInvalidArgument;
+   std::lock_guard<std::mutex> lk(s->mu);
+   out->submitted = s->submitted;
+   out->completed = s->completed;
+   out->running = s->running;
+   out->queued = (int64_t)s->queue.size();
+   out->bytes_running = s->bytes_running;
+   out->bytes_peak = s->bytes_peak;
+   out->running_peak = s->running_peak;
+   out->budget = s->budget;
+   return kSuccess;
+}
+
+int ofrr_server_destroy(ofrr_server *s) {
+   if (!s) return kSuccess;
+   {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stopping = true;
+   }
+   s->cv_admit.notify_all();
+   for (auto &t : s->workers)
+      if (t.joinable()) t.join();
+   (void)hipSetDevice(s->device);
+   if (s->pool) (void)hipMemPoolDestroy(s->pool);
+   delete s;
+   return kSuccess;
+}
+
+}  // extern "C"

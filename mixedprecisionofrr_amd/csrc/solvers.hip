@@ -135,8 +135,20 @@ struct Arena {
 };
 static thread_local Arena g_arena;
 static thread_local int g_arena_dev = -1;
+// caller-owned memory of the current solve on this thread (null: g_arena)
+static thread_local char *g_ext_base = nullptr;
+static thread_local size_t g_ext_off = 0;
 
 static int arena_reserve(ofrr_ctx *ctx, size_t bytes) {
+   if (ctx->state_mem) {
+      // memory the caller handed over for this solve (the serving path)
+      if (ctx->state_bytes < bytes)
+         return set_error(ctx, kErrorOutOfMemory, "filtered_eig: caller state memory too small");
+      g_ext_base = (char *)ctx->state_mem;
+      g_ext_off = 0;
+      return kSuccess;
+   }
+   g_ext_base = nullptr;
    if (g_arena.base && g_arena.bytes >= bytes && g_arena_dev == ctx->device) {
       g_arena.off = 0;
       return kSuccess;
@@ -153,6 +165,11 @@ static int arena_reserve(ofrr_ctx *ctx, size_t bytes) {
    return kSuccess;
 }
 static void *arena_take(size_t bytes) {
+   if (g_ext_base) {
+      const size_t a = (g_ext_off + 255) & ~(size_t)255;
+      g_ext_off = a + bytes;
+      return g_ext_base + a;
+   }
    size_t a = (g_arena.off + 255) & ~(size_t)255;
    g_arena.off = a + bytes;
    return g_arena.base + a;
@@ -538,6 +555,18 @@ struct AtaOp final : FilterOp {
    }
 };
 
+// device bytes of the filtered solver's state: X, X_new, W (FP64 n x k), the
+// quantised residual and four filter blocks (basis format), the projected-matrix
+// products Z (n x k nd), and the small m x m / per-column arrays
+static size_t filtered_state_bytes(int64_t n, const ofrr_filter_opts &o) {
+   const int k = o.k, nd = o.nd, d = o.degree, m = k * (1 + nd);
+   const int bdt = o.basis_dtype;
+   const size_t bsz = dtype_size(bdt), zsz = dtype_size(bdt == OFRR_F64 ? OFRR_F64 : OFRR_F32);
+   return al((size_t)n * k * 8) * 3 + al((size_t)n * k * bsz) * 5 + al((size_t)n * k * nd * zsz) +
+          al((size_t)m * m * 8) * 2 + al((size_t)m * k * 8) + al((size_t)k * 8) * 14 +
+          al((size_t)d * k * 8) + al((size_t)k * 4) * 4 + 4096;
+}
+
 static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o, double *Xio,
                          double *theta_host, double *res_host, ofrr_filter_stats *stats) {
    const int64_t n = op.n;
@@ -552,10 +581,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    const int zdt = (bdt == OFRR_F64) ? OFRR_F64 : OFRR_F32;
    const size_t bsz = dtype_size(bdt), zsz = dtype_size(zdt);
    // ---- buffers
-   size_t need = al((size_t)n * k * 8) * 3 + al((size_t)n * k * bsz) * 5 + al((size_t)n * k * nd * zsz) +
-                 al((size_t)m * m * 8) * 2 + al((size_t)m * k * 8) + al((size_t)k * 8) * 14 +
-                 al((size_t)d * k * 8) + al((size_t)k * 4) * 4 + 4096;
-   int st = arena_reserve(ctx, need);
+   int st = arena_reserve(ctx, filtered_state_bytes(n, *o));
    if (st) return st;
    double *X = (double *)arena_take((size_t)n * k * 8);
    double *Xn = (double *)arena_take((size_t)n * k * 8);
@@ -878,6 +904,13 @@ int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, 
    op.vlo = va_lo ? va_lo : (const void *)va;
    op.vlo_dt = va_lo ? lo_dt : OFRR_F64;
    return filtered_core(ctx, op, o, Xio, theta_host, res_host, stats);
+}
+
+int ofrr_filtered_eig_state_bytes(int64_t n, const ofrr_filter_opts *o, int64_t *bytes) {
+   if (!o || !bytes || n <= 0 || o->k < 1 || o->nd < 1 || o->degree < 2 || !dtype_valid(o->basis_dtype))
+      return kErrorInvalidArgument;
+   *bytes = (int64_t)filtered_state_bytes(n, *o);
+   return kSuccess;
 }
 
 

@@ -27,6 +27,22 @@ template <typename T> static std::vector<T> host_of(const Matrix<T> &A) {
 }
 
 int main() {
+   // ---- matrix input: the bcsstk01 fixture in the binary CSR format
+   {
+      const char *root = std::getenv("OFRR_FIXTURES");
+      const std::string pre = std::string(root ? root : "tests/golden/suitesparse") + "/bcsstk01";
+      CSRMatrix m = MatrixReader::read_csr_matrix(pre);
+      check(m.rows == 48 && m.cols == 48 && m.row_ptr.size() == 49 && m.values.size() == 400 &&
+                m.row_ptr.back() == 400,
+            "MatrixReader::read_csr_matrix bcsstk01");
+      bool thrown = false;
+      try {
+         MatrixReader::read_csr_matrix(pre + "_missing");
+      } catch (const std::runtime_error &e) {
+         thrown = std::string(e.what()).find("Failed to read row pointers from") == 0;
+      }
+      check(thrown, "MatrixReader missing file message");
+   }
    // ---- containers: copy is deep, move empties the source, round trips
    {
       Vector<double> v(1000);

@@ -24,7 +24,8 @@ def test_header_compiles_as_host_cpp():
 @pytest.mark.gpu
 def test_msvd_selftest():
     assert os.path.exists(BIN), "build() makes the self-test binary"
-    r = subprocess.run([BIN], capture_output=True, text=True, timeout=240)
+    env = dict(os.environ, OFRR_FIXTURES=os.path.join(ROOT, "tests", "golden", "suitesparse"))
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=240, env=env)
     print(r.stdout)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
     assert "FAIL" not in r.stdout

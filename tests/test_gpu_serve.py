@@ -1,0 +1,104 @@
+"""Serving path (ofrr_server_*): continuous batching of eigen requests on one GPU.
+
+* each result is the same alone and in the mix: the requests are solved
+  together on 4 slots, then one at a time on a 1-slot server, and the Ritz
+  values and residuals must agree exactly (tolerance 0: every reduction of a
+  solve has a fixed order and solves share no device memory);
+* the solves overlap (more than one ran at once) and every pair converged;
+* admission keeps the running footprint within the byte budget;
+* eigenvalues match an FP64 dense eigensolver on the oracle's operator
+  (variable coefficients) and the closed form (Laplacian).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# (dim, N, nwant, coef_seed): 2-D 5-point and 3-D 7-point diffusion operators,
+# variable face coefficients (seed > 0) and the Laplacian (seed 0)
+MIX = [
+    (2, 100, 8, 11), (3, 24, 16, 12), (2, 160, 16, 13), (3, 20, 8, 0),
+    (2, 120, 32, 14), (3, 30, 8, 15), (2, 64, 16, 0), (3, 16, 32, 16),
+]
+
+
+def _requests(api):
+    return [api.request(d, N, w, coef_seed=s, start_seed=100 + i, basis_dtype=torch.float16,
+                        tol=1e-8) for i, (d, N, w, s) in enumerate(MIX)]
+
+
+def _run(api, reqs, slots, budget=0):
+    with api.Server(0, slots=slots, budget=budget) as srv:
+        if slots == 1:
+            out = []
+            for r in reqs:  # strictly one after the other
+                out.append(srv.wait(srv.submit(r)))
+        else:
+            ids = [srv.submit(r) for r in reqs]
+            out = [srv.wait(i) for i in ids]
+        stats = srv.stats()
+    return out, stats
+
+
+@pytest.fixture(scope="module")
+def api():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from mixedprecisionofrr_amd import api as A
+    return A
+
+
+def test_serve_same_alone_and_in_mix(api):
+    reqs = _requests(api)
+    print("serve: mix of", len(reqs), "requests on 4 slots", flush=True)
+    mix, st_mix = _run(api, reqs, 4)
+    print("serve: the same requests one at a time", flush=True)
+    alone, _ = _run(api, _requests(api), 1)
+    for (d, N, w, s), a, b in zip(MIX, mix, alone):
+        assert a["status"] == 0 and b["status"] == 0, (d, N, w, s, a["status"], b["status"])
+        assert a["converged"] == w
+        assert np.max(a["res"]) <= 1e-8
+        assert a["iterations"] == b["iterations"]
+        assert np.array_equal(a["theta"], b["theta"]), (d, N, w, s)
+        assert np.array_equal(a["res"], b["res"]), (d, N, w, s)
+    assert st_mix["completed"] == len(reqs)
+    assert st_mix["running_peak"] > 1  # the solves overlapped
+    assert st_mix["bytes_running"] == 0 and st_mix["running"] == 0
+
+
+def test_serve_admission_respects_budget(api):
+    reqs = _requests(api)
+    fps = [api.request_footprint(r)[2] for r in reqs]
+    budget = int(max(fps) * 1.2)  # no two of the largest fit together
+    out, st = _run(api, reqs, 4, budget=budget)
+    assert st["budget"] == budget
+    assert st["bytes_peak"] <= budget
+    assert all(o["status"] == 0 for o in out)
+    # requests are admitted in arrival order
+    admits = [o["t_admit"] for o in out]
+    assert admits == sorted(admits)
+    # a request larger than the budget is refused at submission
+    with api.Server(0, slots=2, budget=min(fps) - 1) as srv:
+        with pytest.raises(api.OfrrError) as e:
+            srv.submit(reqs[0])
+        assert e.value.status == 1002
+
+
+def test_serve_eigenvalues_match_dense_fp64(api):
+    reqs = [api.request(2, 40, 16, coef_seed=7, start_seed=3, tol=1e-8),
+            api.request(3, 12, 8, coef_seed=0, start_seed=4, tol=1e-8)]
+    out, _ = _run(api, reqs, 2)
+    A = O.stencil(2, 40, seed=7)
+    dense = np.zeros((A.n, A.n))
+    for i in range(A.n):
+        for p in range(A.row_ptr[i], A.row_ptr[i + 1]):
+            dense[i, A.col_idx[p]] = A.vals[p]
+    ref = np.linalg.eigvalsh(dense)[::-1][:16]
+    assert out[0]["status"] == 0
+    assert np.max(np.abs(out[0]["theta"] - ref) / ref) < 1e-10
+    lap = O.laplacian_eigs(3, 12)[:8]
+    assert out[1]["status"] == 0
+    assert np.max(np.abs(out[1]["theta"] - lap) / lap) < 1e-10
