@@ -306,7 +306,7 @@ typedef struct {
                           (0: 90% of the free device memory at creation) */
 } ofrr_server_opts;
 
-#define OFRR_SERVE_MAX_WANT 64
+#define OFRR_SERVE_MAX_WANT 128
 typedef struct {
    int dim;            /* 2: 5-point, 3: 7-point diffusion stencil */
    int64_t N;          /* grid side; n = N^dim */

@@ -27,7 +27,7 @@ EXPORTS = [
     "ofrr_csr_file_info", "ofrr_csr_file_read", "ofrr_singular_values_file", "ofrr_io_last_error",
 ]
 
-SERVE_MAX_WANT = 64
+SERVE_MAX_WANT = 128
 PENDING = 1
 
 

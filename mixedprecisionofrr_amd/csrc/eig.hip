@@ -28,41 +28,65 @@ constexpr int kJacobiThreads = 1024;
 constexpr int kLdsFullN = 128;    // full storage, row stride n + 1: <= 129 KiB of LDS
 constexpr int kLdsMaxN = 192;     // packed upper triangle: n(n+1)/2 doubles <= 144.8 KiB
 constexpr int kMaxSweeps = 30;
-constexpr int kMaxBlk = 9;  // 2x2 blocks per thread: ceil(128 * 129 / 2 / 1024) for n <= 256
+// 2x2 blocks per thread: ceil(np (np + 1) / 2 / 1024) with np = nn / 2 pairs
+constexpr int kBlkFull = 3;    // nn <= 128
+constexpr int kBlkPacked = 5;  // nn <= 192
+constexpr int kBlkGlobal = 9;  // nn <= 256
+constexpr int kGroup = 3;      // blocks whose loads are in flight together
+static_assert(kBlkFull * kJacobiThreads >= (64 * 65) / 2, "blocks per thread, nn <= 128");
+static_assert(kBlkPacked * kJacobiThreads >= (96 * 97) / 2, "blocks per thread, nn <= 192");
+static_assert(kBlkGlobal * kJacobiThreads >= (128 * 129) / 2, "blocks per thread, nn <= 256");
 
-__device__ inline void pair_of(int nn, int r, int s, int &p, int &q) {
-   if (s == 0) { p = nn - 1; q = r; }
-   else { p = (r + s) % (nn - 1); q = (r - s + nn - 1) % (nn - 1); }
-   if (p > q) { int t = p; p = q; q = t; }
+// Pair s of round r in the circle schedule on nn indices (M = nn - 1): s = 0
+// pairs M with r, s > 0 pairs r + s with r - s (mod M); sorted so p < q.
+__device__ inline void pair_of(int M, int r, int s, int &p, int &q) {
+   if (s == 0) {
+      p = M;
+      q = r;
+   } else {
+      p = r + s;
+      if (p >= M) p -= M;
+      q = r - s;
+      if (q < 0) q += M;
+   }
+   if (p > q) { const int t = p; p = q; q = t; }
 }
 
-// storage of the symmetric working matrix: FULL keeps both triangles with row
-// stride ld (conflict-free LDS rows), otherwise the packed upper triangle
+// Storage of the symmetric working matrix on nn = n + (n odd) indices; an odd
+// n gets a virtual index n whose row and column hold zeros (its pairs never
+// rotate and its entries are never written, so the arithmetic equals skipping
+// it).  FULL keeps both triangles with row stride nn + 1 (conflict-free LDS
+// rows): entry (i, j) at off(i) + j with off(i) = i (nn + 1).  Otherwise the
+// packed upper triangle: (i, j), i <= j, at off(j) + i with off(j) = j (j + 1) / 2.
 template <bool FULL> struct SymIdx;
 template <> struct SymIdx<true> {
    int ld;
-   __device__ int operator()(int i, int j) const { return i * ld + j; }
+   __device__ int off(int i) const { return i * ld; }
+   __device__ int at(int i, int j, int oi, int oj) const { (void)oj; return oi + j; }
 };
 template <> struct SymIdx<false> {
    int ld;
-   __device__ int operator()(int i, int j) const {
-      return (i <= j) ? (j * (j + 1)) / 2 + i : (i * (i + 1)) / 2 + j;
-   }
+   __device__ int off(int i) const { return (i * (i + 1)) >> 1; }
+   __device__ int at(int i, int j, int oi, int oj) const { return (i <= j) ? oj + i : oi + j; }
 };
 
-// Rotations: A is diagonalised in place (packed storage, LDS or global) and the
-// (c, s) of every pair of every round is logged; the eigenvectors are formed
-// afterwards by jacobi_apply_kernel.  Outputs per matrix: eigenvalues ascending
-// (w), the rank of each diagonal index (rank), and the number of logged sweeps.
-template <bool LDS, bool FULL>
+// Rotations: A is diagonalised in place (LDS or global) and the (c, s) of every
+// pair of every round is logged; the eigenvectors are formed afterwards by
+// jacobi_apply_kernel.  Outputs per matrix: eigenvalues ascending (w), the rank
+// of each diagonal index (rank), and the number of logged sweeps.
+// Each thread owns up to NB 2x2 blocks (pair I, pair J >= I) of the upper block
+// triangle for the whole solve; a round loads every owned block before it
+// writes any (the blocks of a round are disjoint), so the loads of all blocks
+// are in flight together.
+template <bool LDS, bool FULL, int NB>
 __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
     int n_max, const int32_t *__restrict__ n_dev, const double *__restrict__ A, int lda,
     int64_t strideA, double *__restrict__ w, int64_t stridew, double2 *__restrict__ rotlog,
     int64_t strideLog, int32_t *__restrict__ rank, int32_t *__restrict__ nsweeps,
     int64_t strideInts, double *__restrict__ work, int64_t strideWork, int max_sweeps) {
    extern __shared__ double lds[];
-   __shared__ double2 rcs[128];  // (c, s) of pair I in this round
-   __shared__ int2 rpq[128];     // (p, q) of pair I
+   __shared__ double2 rcs[128];  // (c, s) of pair slot s in this round
+   __shared__ int4 rpq[128];     // (p, q, off(p), off(q)) of pair slot s, p < q
    __shared__ int rotated, s_n;
    __shared__ double s_amax[16];
    const int b = blockIdx.x;
@@ -75,17 +99,19 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
       if (threadIdx.x == 0) nsweeps[b * strideInts] = 0;
       return;
    }
-   const int nn = n + (n & 1), np = nn / 2;
+   const int nn = n + (n & 1), np = nn / 2, M = nn - 1;
    double *a = LDS ? lds : work + b * strideWork;
-   const SymIdx<FULL> pk{n + 1};
+   const SymIdx<FULL> pk{nn + 1};
    double lmax = 0.0;
-   for (int j = threadIdx.x; j < n; j += blockDim.x)
+   for (int j = threadIdx.x; j < nn; j += blockDim.x) {
+      const int oj = pk.off(j);
       for (int i = 0; i <= j; ++i) {
-         const double v = Ab[i + (int64_t)j * lda];
-         a[pk(i, j)] = v;
-         if (FULL) a[pk(j, i)] = v;
+         const double v = (j < n) ? Ab[i + (int64_t)j * lda] : 0.0;
+         a[pk.at(i, j, pk.off(i), oj)] = v;
+         if (FULL) a[pk.at(j, i, oj, pk.off(i))] = v;
          lmax = fmax(lmax, fabs(v));
       }
+   }
 #pragma unroll
    for (int o = 32; o > 0; o >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o, 64));
    if ((threadIdx.x & 63) == 0) s_amax[threadIdx.x >> 6] = lmax;
@@ -95,33 +121,38 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
    // rotations below eps * max|a_ij| cannot change any eigenvalue beyond the
    // accuracy the matrix carries: they end the sweep like the relative test
    const double tol_abs = 2.220446049250313e-16 * amax;
-   // this thread's 2x2 blocks (I <= J) of the upper block triangle, fixed for
-   // the whole solve; in full storage each block also writes its mirror
-   int blk_ij[kMaxBlk];
-   int nb_t = 0;
+   // this thread's blocks: t = threadIdx.x + u * kJacobiThreads in I-major order
+   int blk[NB];
    {
       const int nblocks = np * (np + 1) / 2;
-      for (int t = threadIdx.x; t < nblocks && nb_t < kMaxBlk; t += blockDim.x) {
-         // I = largest row with I*np - I*(I-1)/2 <= t
-         const float b2 = 2.0f * np + 1.0f;
-         int I = (int)((b2 - sqrtf(b2 * b2 - 8.0f * (float)t)) * 0.5f);
-         while (I > 0 && I * np - I * (I - 1) / 2 > t) --I;
-         while ((I + 1) * np - (I + 1) * I / 2 <= t) ++I;
-         const int J = I + (t - (I * np - I * (I - 1) / 2));
-         blk_ij[nb_t++] = (I << 16) | J;
+      const float b2 = 2.0f * np + 1.0f;
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+         const int t = threadIdx.x + u * kJacobiThreads;
+         blk[u] = -1;
+         if (t < nblocks) {
+            // I = largest row with I*np - I*(I-1)/2 <= t
+            int I = (int)((b2 - sqrtf(b2 * b2 - 8.0f * (float)t)) * 0.5f);
+            while (I > 0 && I * np - I * (I - 1) / 2 > t) --I;
+            while ((I + 1) * np - (I + 1) * I / 2 <= t) ++I;
+            const int J = I + (t - (I * np - I * (I - 1) / 2));
+            blk[u] = (I << 16) | J;
+         }
       }
    }
    int done = 0;
    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
       if (threadIdx.x == 0) rotated = 0;
       __syncthreads();
-      for (int r = 0; r < nn - 1; ++r) {
+      for (int r = 0; r < M; ++r) {
          for (int s = threadIdx.x; s < np; s += blockDim.x) {
             int p, q;
-            pair_of(nn, r, s, p, q);
+            pair_of(M, r, s, p, q);
+            const int op = pk.off(p), oq = pk.off(q);
             double c = 1.0, sn = 0.0;
             if (q < n) {
-               const double apq = a[pk(p, q)], app = a[pk(p, p)], aqq = a[pk(q, q)];
+               const double apq = a[pk.at(p, q, op, oq)], app = a[pk.at(p, p, op, op)],
+                            aqq = a[pk.at(q, q, oq, oq)];
                if (fabs(apq) > fmax(1.1102230246251565e-16 * sqrt(fabs(app) * fabs(aqq)), tol_abs)) {
                   const double th = (aqq - app) / (2.0 * apq);
                   const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(fma(th, th, 1.0)));
@@ -131,49 +162,69 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
                }
             }
             rcs[s] = make_double2(c, sn);
-            rpq[s] = make_int2(p, q);
-            log[((int64_t)sweep * (nn - 1) + r) * np + s] = make_double2(c, sn);
+            rpq[s] = make_int4(p, q, op, oq);
+            log[((int64_t)sweep * M + r) * np + s] = make_double2(c, sn);
          }
          __syncthreads();
          // A' = J^T A J block by block over the upper block triangle:
-         // A'_IJ = J_I^T A_IJ J_J (I <= J); full storage mirrors each block
+         // A'_IJ = J_I^T A_IJ J_J (I <= J), explicit fused multiply-adds.  A
+         // diagonal block (I = J) runs the same arithmetic; its a_pq is then 0.
+         // groups of kGroup blocks: all loads of a group, then its stores
 #pragma unroll
-         for (int u = 0; u < kMaxBlk; ++u) {
-            if (u >= nb_t) break;
-            const int I = blk_ij[u] >> 16, J = blk_ij[u] & 0xffff;  // I <= J
-            const double2 csI = rcs[I], csJ = rcs[J];
-            const double s1 = csI.y, s2 = csJ.y;
-            if (s1 == 0.0 && s2 == 0.0) continue;  // both rotations are the identity
-            const double c1 = csI.x, c2 = csJ.x;
-            const int2 pqI = rpq[I], pqJ = rpq[J];
-            const int p1 = pqI.x, q1 = pqI.y, p2 = pqJ.x, q2 = pqJ.y;
-            const bool q1ok = q1 < n, q2ok = q2 < n;
-            // J^T A_IJ J with J = [[c s][-s c]], explicit fused multiply-adds
-            if (I == J) {
-               if (!q1ok) continue;
-               const double app = a[pk(p1, p1)], apq = a[pk(p1, q1)], aqq = a[pk(q1, q1)];
-               const double m11 = fma(c1, app, -(s1 * apq)), m12 = fma(s1, app, c1 * apq);
-               const double m21 = fma(c1, apq, -(s1 * aqq)), m22 = fma(s1, apq, c1 * aqq);
-               a[pk(p1, p1)] = fma(c1, m11, -(s1 * m21));
-               a[pk(q1, q1)] = fma(s1, m12, c1 * m22);
-               a[pk(p1, q1)] = 0.0;
-               if (FULL) a[pk(q1, p1)] = 0.0;
-               continue;
+         for (int g = 0; g < NB; g += kGroup) {
+            double v11[kGroup], v12[kGroup], v21[kGroup], v22[kGroup];
+            double2 cI[kGroup], cJ[kGroup];
+            int4 qI[kGroup], qJ[kGroup];
+            bool act[kGroup];
+#pragma unroll
+            for (int e = 0; e < kGroup; ++e) {
+               const int u = g + e;
+               act[e] = (u < NB) && blk[u < NB ? u : 0] >= 0;
+               const int bu = act[e] ? blk[u < NB ? u : 0] : 0;
+               const int I = bu >> 16, J = bu & 0xffff;
+               cI[e] = rcs[I];
+               cJ[e] = rcs[J];
+               act[e] = act[e] && (cI[e].y != 0.0 || cJ[e].y != 0.0);
+               qI[e] = rpq[I];
+               qJ[e] = rpq[J];
+               if (act[e]) {
+                  // rows p1 < q1 (pair I), columns p2 < q2 (pair J); a virtual
+                  // index holds zeros, so every read is of stored memory
+                  v11[e] = a[pk.at(qI[e].x, qJ[e].x, qI[e].z, qJ[e].z)];
+                  v12[e] = a[pk.at(qI[e].x, qJ[e].y, qI[e].z, qJ[e].w)];
+                  v21[e] = a[pk.at(qI[e].y, qJ[e].x, qI[e].w, qJ[e].z)];
+                  v22[e] = a[pk.at(qI[e].y, qJ[e].y, qI[e].w, qJ[e].w)];
+               }
             }
-            const int i11 = pk(p1, p2), i12 = pk(p1, q2), i21 = pk(q1, p2), i22 = pk(q1, q2);
-            const double a11 = a[i11];
-            const double a12 = q2ok ? a[i12] : 0.0;
-            const double a21 = q1ok ? a[i21] : 0.0;
-            const double a22 = (q1ok && q2ok) ? a[i22] : 0.0;
-            const double m11 = fma(c2, a11, -(s2 * a12)), m12 = fma(s2, a11, c2 * a12);
-            const double m21 = fma(c2, a21, -(s2 * a22)), m22 = fma(s2, a21, c2 * a22);
-            const double b11 = fma(c1, m11, -(s1 * m21)), b12 = fma(c1, m12, -(s1 * m22));
-            const double b21 = fma(s1, m11, c1 * m21), b22 = fma(s1, m12, c1 * m22);
-            a[i11] = b11;
-            if (FULL) a[pk(p2, p1)] = b11;
-            if (q2ok) { a[i12] = b12; if (FULL) a[pk(q2, p1)] = b12; }
-            if (q1ok) { a[i21] = b21; if (FULL) a[pk(p2, q1)] = b21; }
-            if (q1ok && q2ok) { a[i22] = b22; if (FULL) a[pk(q2, q1)] = b22; }
+#pragma unroll
+            for (int e = 0; e < kGroup; ++e) {
+               if (!act[e]) continue;
+               const double c1 = cI[e].x, s1 = cI[e].y, c2 = cJ[e].x, s2 = cJ[e].y;
+               const int p1 = qI[e].x, q1 = qI[e].y, p2 = qJ[e].x, q2 = qJ[e].y;
+               const int op1 = qI[e].z, oq1 = qI[e].w, op2 = qJ[e].z, oq2 = qJ[e].w;
+               const double m11 = fma(c2, v11[e], -(s2 * v12[e])), m12 = fma(s2, v11[e], c2 * v12[e]);
+               const double m21 = fma(c2, v21[e], -(s2 * v22[e])), m22 = fma(s2, v21[e], c2 * v22[e]);
+               const double b11 = fma(c1, m11, -(s1 * m21));
+               double b12 = fma(c1, m12, -(s1 * m22));
+               double b21 = fma(s1, m11, c1 * m21);
+               const double b22 = fma(s1, m12, c1 * m22);
+               if (p1 == p2) b12 = b21 = 0.0;  // diagonal block
+               const bool q1ok = q1 < n, q2ok = q2 < n;
+               a[pk.at(p1, p2, op1, op2)] = b11;
+               if (FULL) a[pk.at(p2, p1, op2, op1)] = b11;
+               if (q2ok) {
+                  a[pk.at(p1, q2, op1, oq2)] = b12;
+                  if (FULL) a[pk.at(q2, p1, oq2, op1)] = b12;
+               }
+               if (q1ok) {
+                  a[pk.at(q1, p2, oq1, op2)] = b21;
+                  if (FULL) a[pk.at(p2, q1, op2, oq1)] = b21;
+               }
+               if (q1ok && q2ok) {
+                  a[pk.at(q1, q2, oq1, oq2)] = b22;
+                  if (FULL) a[pk.at(q2, q1, oq2, oq1)] = b22;
+               }
+            }
          }
          __syncthreads();
       }
@@ -185,17 +236,16 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
    double *wb = w + b * stridew;
    int32_t *rk = rank + b * strideInts;
    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const double di = a[pk(i, i)];
+      const double di = a[pk.at(i, i, pk.off(i), pk.off(i))];
       int rr = 0;
       for (int j = 0; j < n; ++j) {
-         const double dj = a[pk(j, j)];
+         const double dj = a[pk.at(j, j, pk.off(j), pk.off(j))];
          rr += (dj < di || (dj == di && j < i)) ? 1 : 0;
       }
       wb[rr] = di;
       rk[i] = rr;
    }
 }
-
 // V = J_1 J_2 ... (all logged rotations) applied to the identity.  Rows of V
 // are independent: one wave owns one row in LDS and applies every round in
 // order (the pairs of a round are disjoint).  The log is staged through LDS
@@ -269,7 +319,8 @@ struct JacobiLayout {
 static JacobiLayout jacobi_layout(int n) {
    const int nn = n + (n & 1);
    JacobiLayout L;
-   L.packed = ((size_t)n * (n + 1) / 2 + 1) & ~(size_t)1;  // even: the log is double2
+   // packed triangle on nn indices (an odd n gets a zero virtual index)
+   L.packed = ((size_t)nn * (nn + 1) / 2 + 1) & ~(size_t)1;  // even: the log is double2
    L.log = (size_t)kMaxSweeps * (nn - 1) * (nn / 2) * 2;
    L.ints = ((size_t)n + 2 + 1) / 2;
    L.total = (L.packed + L.log + L.ints + 1) & ~(size_t)1;
@@ -299,25 +350,27 @@ int syev_batched(ofrr_ctx *ctx, int batch, int n, const int32_t *n_dev, const do
    std::call_once(attr_once, [] {
       const int full = (int)((size_t)kLdsFullN * (kLdsFullN + 1) * sizeof(double));
       const int packed = (int)((size_t)kLdsMaxN * (kLdsMaxN + 1) / 2 * sizeof(double));
-      attr_err = hipFuncSetAttribute((const void *)jacobi_rot_kernel<true, true>,
+      attr_err = hipFuncSetAttribute((const void *)jacobi_rot_kernel<true, true, kBlkFull>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, full);
       if (attr_err == hipSuccess)
-         attr_err = hipFuncSetAttribute((const void *)jacobi_rot_kernel<true, false>,
+         attr_err = hipFuncSetAttribute((const void *)jacobi_rot_kernel<true, false, kBlkPacked>,
                                         hipFuncAttributeMaxDynamicSharedMemorySize, packed);
    });
    OFRR_HIP(ctx, attr_err);
-   if (n <= kLdsFullN) {
-      const size_t lds = (size_t)n * (n + 1) * sizeof(double);
-      jacobi_rot_kernel<true, true><<<batch, kJacobiThreads, lds, ctx->stream>>>(
+   // storage sized for nn = n + (n odd) indices (the largest of a batch)
+   const int nn = n + (n & 1);
+   if (nn <= kLdsFullN) {
+      const size_t lds = (size_t)nn * (nn + 1) * sizeof(double);
+      jacobi_rot_kernel<true, true, kBlkFull><<<batch, kJacobiThreads, lds, ctx->stream>>>(
           n, n_dev, A, lda, strideA, w, stridew, rotlog, strideLog, rank, nsw, strideInts, work,
           strideWork, kMaxSweeps);
-   } else if (n <= kLdsMaxN) {
-      const size_t lds = (size_t)n * (n + 1) / 2 * sizeof(double);
-      jacobi_rot_kernel<true, false><<<batch, kJacobiThreads, lds, ctx->stream>>>(
+   } else if (nn <= kLdsMaxN) {
+      const size_t lds = (size_t)nn * (nn + 1) / 2 * sizeof(double);
+      jacobi_rot_kernel<true, false, kBlkPacked><<<batch, kJacobiThreads, lds, ctx->stream>>>(
           n, n_dev, A, lda, strideA, w, stridew, rotlog, strideLog, rank, nsw, strideInts, work,
           strideWork, kMaxSweeps);
    } else {
-      jacobi_rot_kernel<false, false><<<batch, kJacobiThreads, 0, ctx->stream>>>(
+      jacobi_rot_kernel<false, false, kBlkGlobal><<<batch, kJacobiThreads, 0, ctx->stream>>>(
           n, n_dev, A, lda, strideA, w, stridew, rotlog, strideLog, rank, nsw, strideInts, work,
           strideWork, kMaxSweeps);
    }

@@ -13,6 +13,8 @@
 //                 for a basis whose column blocks are stored in different
 //                 precisions (FP64 Ritz block, low-precision growth blocks).
 //   small_gemm    C = alpha op(A) B + beta C for the k x k Rayleigh-Ritz algebra.
+#include <type_traits>
+
 #include "internal.hpp"
 
 namespace ofrr {
@@ -142,6 +144,145 @@ int gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64
 }
 
 // ---------------------------------------------------------------------------
+// Fused projection products of one solver iteration: every block V_a^T V_b of
+// the Gram matrix and of the projected matrix in one launch.  An item is one
+// 64 x 64 tile of one block; the operands are row-major n x k blocks whose
+// element kinds are FP64 (Ritz block, A X), the growth-block format TB or the
+// product format TZ.  The grid is laid out so that all items of a row chunk run
+// on one XCD at about the same time (workgroup w goes to XCD w mod 8), so each
+// chunk of rows comes from HBM once and from that XCD's L2 for the other items.
+// Per-chunk partial tiles are summed in a fixed order (deterministic).
+template <typename TA, typename TB>
+__device__ inline void gram_tile_rows(int64_t r0, int64_t r1, int k, const TA *__restrict__ va,
+                                      int64_t vrs, const TB *__restrict__ wb, int64_t wrs,
+                                      int i0, int j0, double *__restrict__ p) {
+   const int lane = threadIdx.x;
+   const int li = lane & 15, lk = lane >> 4;
+   dbl4 acc[4][4];
+#pragma unroll
+   for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+   bool cv[4], cw[4];
+#pragma unroll
+   for (int t = 0; t < 4; ++t) {
+      cv[t] = i0 + 16 * t + li < k;
+      cw[t] = j0 + 16 * t + li < k;
+   }
+   const TA *vb = va + i0 + li;
+   const TB *wp = wb + j0 + li;
+#pragma unroll 2
+   for (int64_t r = r0; r < r1; r += 4) {
+      const int64_t row = r + lk;
+      const bool rok = row < r1;
+      double a[4], b[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+         a[t] = (rok && cv[t]) ? (double)Elem<TA>::load(vb[row * vrs + 16 * t]) : 0.0;
+         b[t] = (rok && cw[t]) ? (double)Elem<TB>::load(wp[row * wrs + 16 * t]) : 0.0;
+      }
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+         for (int tj = 0; tj < 4; ++tj)
+            acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+   }
+#pragma unroll
+   for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+         for (int rg = 0; rg < 4; ++rg)
+            p[(16 * ti + lk + 4 * rg) * GT + 16 * tj + li] = acc[ti][tj][rg];
+}
+
+template <typename TB, typename TZ, typename TA>
+__device__ inline void gram_item_second(int kb, int64_t r0, int64_t r1, int k, const TA *va,
+                                        int64_t vrs, const void *w, int64_t wrs, int i0, int j0,
+                                        double *p) {
+   if (kb == 0) gram_tile_rows<TA, double>(r0, r1, k, va, vrs, (const double *)w, wrs, i0, j0, p);
+   else if (kb == 1) gram_tile_rows<TA, TB>(r0, r1, k, va, vrs, (const TB *)w, wrs, i0, j0, p);
+   else gram_tile_rows<TA, TZ>(r0, r1, k, va, vrs, (const TZ *)w, wrs, i0, j0, p);
+}
+
+template <typename TB, typename TZ>
+__global__ __launch_bounds__(64) void gram_multi_kernel(int64_t n, int k, GramMultiArgs args,
+                                                        int64_t rows_per_chunk, int nchunks,
+                                                        double *__restrict__ part) {
+   const int L = blockIdx.x;
+   const int xcd = L & 7, local = L >> 3;
+   const int item = local % args.nitems, chunk = (local / args.nitems) * 8 + xcd;
+   if (chunk >= nchunks) return;
+   const int64_t r0 = (int64_t)chunk * rows_per_chunk;
+   const int64_t r1 = min(n, r0 + rows_per_chunk);
+   const GramOperand A = args.op[args.ia[item]], B = args.op[args.ib[item]];
+   const int i0 = args.ti[item] * GT, j0 = args.tj[item] * GT;
+   double *p = part + ((int64_t)chunk * args.nitems + item) * (GT * GT);
+   // wave-uniform dispatch on the two operand kinds
+   if (A.kind == 0)
+      gram_item_second<TB, TZ, double>(B.kind, r0, r1, k, (const double *)A.p, A.rs, B.p, B.rs, i0, j0, p);
+   else if (A.kind == 1)
+      gram_item_second<TB, TZ, TB>(B.kind, r0, r1, k, (const TB *)A.p, A.rs, B.p, B.rs, i0, j0, p);
+   else
+      gram_item_second<TB, TZ, TZ>(B.kind, r0, r1, k, (const TZ *)A.p, A.rs, B.p, B.rs, i0, j0, p);
+}
+
+// item's k x k block tile (i0, j0) += sum over chunks, fixed order (as gram_reduce_kernel)
+__global__ __launch_bounds__(256) void gram_multi_reduce_kernel(int nchunks, int k,
+                                                                GramMultiArgs args,
+                                                                const double *__restrict__ part) {
+   __shared__ double gs[256];
+   const int item = blockIdx.y;
+   const int el = threadIdx.x % 32, grp = threadIdx.x / 32;
+   const int e = blockIdx.x * 32 + el;  // entry of the 64 x 64 tile
+   const int64_t stride = (int64_t)args.nitems * GT * GT;
+   const int gsz = (nchunks + kRedGroups - 1) / kRedGroups;
+   double s = 0.0;
+   {
+      const int c0 = grp * gsz, c1 = min(nchunks, c0 + gsz);
+      const double *pp = part + (int64_t)item * GT * GT + e;
+      for (int c = c0; c < c1; ++c) s += pp[c * stride];
+   }
+   gs[threadIdx.x] = s;
+   __syncthreads();
+   if (grp != 0) return;
+   double t = 0.0;
+   for (int g = 0; g < kRedGroups; ++g) t += gs[g * 32 + el];
+   const int i = args.ti[item] * GT + e / GT, j = args.tj[item] * GT + e % GT;
+   if (i >= k || j >= k) return;
+   args.dst[item][(args.ro[item] + i) + (int64_t)(args.co[item] + j) * args.ld[item]] = t;
+}
+
+int gram_multi(ofrr_ctx *ctx, int64_t n, int k, int tb, int tz, GramMultiArgs &args) {
+   if (args.nitems <= 0) return kSuccess;
+   if (args.nitems > kMaxGramItems || k <= 0 || k > 128)
+      return set_error(ctx, kErrorInvalidArgument, "gram_multi: items / k");
+   // about 6 waves per SIMD over the whole device, a multiple of 8 chunks
+   int64_t want = std::max<int64_t>(8, (6 * 4 * (int64_t)ctx->num_cus) / args.nitems);
+   int64_t rows = (n + want - 1) / want;
+   rows = std::max<int64_t>(64, ((rows + 3) / 4) * 4);
+   const int nchunks = (int)((n + rows - 1) / rows);
+   const int ngrp = (nchunks + 7) / 8;
+   double *part = (double *)workspace(ctx, (size_t)nchunks * args.nitems * GT * GT * sizeof(double));
+   if (!part) return set_error(ctx, kErrorCUDAMemory, "gram_multi: workspace");
+   const unsigned grid = (unsigned)(ngrp * 8 * args.nitems);
+   int st = dispatch_dtype(tb, [&](auto b) -> int {
+      using TB = decltype(b);
+      return dispatch_dtype(tz, [&](auto z) -> int {
+         using TZ = decltype(z);
+         gram_multi_kernel<TB, TZ><<<grid, 64, 0, ctx->stream>>>(n, k, args, rows, nchunks, part);
+         OFRR_LAUNCH(ctx);
+         return kSuccess;
+      });
+   });
+   if (st) return st;
+   gram_multi_reduce_kernel<<<dim3(GT * GT / 32, args.nitems), 256, 0, ctx->stream>>>(nchunks, k, args,
+                                                                                       part);
+   OFRR_LAUNCH(ctx);
+   return kSuccess;
+}
+
+// ---------------------------------------------------------------------------
 // Y(rows, 0:k) = sum_b V_b(rows, :) C_b, 64 rows x 64 columns per workgroup
 struct UpdBlock {
    const void *V;
@@ -224,6 +365,85 @@ __global__ __launch_bounds__(256) void basis_update_kernel(int64_t n, int k, Upd
    }
 }
 
+// Y (n x k, row-major) = sum_b V_b C_b on FP64 MFMA for row-major blocks whose
+// widths are multiples of 16.  One wave owns 16 rows of Y and all k columns.
+// The inner index of MFMA step s (4 per group of 16 inner columns) and lane
+// group g = lane / 16 is 16 G + 4 g + s, so each lane reads 4 consecutive
+// entries of its row of V_b; the coefficients are repacked beforehand
+// (repack_coef_kernel) so that its 4 matching entries of C are contiguous too.
+// Entry (i, j) of a 16 x 16 result sits in lane j + 16 (i % 4), register i / 4.
+struct UpdMfmaArgs {
+   const void *V[4];
+   int64_t rs[4];
+   int kb[4], dtype[4];
+   int nb;
+};
+
+// Cp[((G k + col) 4 + g) 4 + s] = C_b(16 G' + 4 g + s, col) with G running over
+// the 16-column groups of all blocks in order
+__global__ void repack_coef_kernel(int k, int nb, UpdArgs a, double *__restrict__ Cp) {
+   int G = blockIdx.x, b = 0, Gb = G;
+   while (b < nb && Gb >= a.b[b].kb / 16) { Gb -= a.b[b].kb / 16; ++b; }
+   if (b >= nb) return;
+   for (int t = threadIdx.x; t < k * 16; t += blockDim.x) {
+      const int col = t / 16, g = (t / 4) % 4, s = t % 4;
+      Cp[(((int64_t)G * k + col) * 4 + g) * 4 + s] =
+          a.b[b].C[(16 * Gb + 4 * g + s) + (int64_t)col * a.b[b].ldc];
+   }
+}
+
+template <typename TV>
+__device__ inline void upd_load4(const void *V, int64_t off, bool ok, double out[4]) {
+   const TV *p = (const TV *)V + off;
+#pragma unroll
+   for (int s = 0; s < 4; ++s) out[s] = ok ? (double)Elem<TV>::load(p[s]) : 0.0;
+}
+
+template <typename TY, int NCT>
+__global__ __launch_bounds__(64) void basis_update_mfma_kernel(int64_t n, int k, UpdMfmaArgs a,
+                                                               const double *__restrict__ Cp,
+                                                               TY *__restrict__ Y, int64_t yrs) {
+   const int lane = threadIdx.x, li = lane & 15, g = lane >> 4;
+   const int64_t row0 = (int64_t)blockIdx.x * 16;
+   const int64_t row = row0 + li;
+   const bool rok = row < n;
+   dbl4 acc[NCT];
+#pragma unroll
+   for (int c = 0; c < NCT; ++c) acc[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+   int G = 0;
+   for (int b = 0; b < a.nb; ++b) {
+      const int ng = a.kb[b] / 16, dt = a.dtype[b];
+      for (int gb = 0; gb < ng; ++gb, ++G) {
+         double av[4];
+         const int64_t off = row * a.rs[b] + 16 * gb + 4 * g;
+         switch (dt) {  // wave-uniform
+            case OFRR_F64: upd_load4<double>(a.V[b], off, rok, av); break;
+            case OFRR_F32: upd_load4<float>(a.V[b], off, rok, av); break;
+            case OFRR_F16: upd_load4<__half>(a.V[b], off, rok, av); break;
+            case OFRR_BF16: upd_load4<__hip_bfloat16>(a.V[b], off, rok, av); break;
+            default: upd_load4<fp8_t>(a.V[b], off, rok, av); break;
+         }
+         const double *cg = Cp + ((int64_t)G * k * 4 + g) * 4;
+#pragma unroll
+         for (int c = 0; c < NCT; ++c) {
+            const double *cb = cg + (int64_t)(16 * c + li) * 16;
+            const double b0 = cb[0], b1 = cb[1], b2 = cb[2], b3 = cb[3];
+            acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[0], b0, acc[c], 0, 0, 0);
+            acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[1], b1, acc[c], 0, 0, 0);
+            acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[2], b2, acc[c], 0, 0, 0);
+            acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[3], b3, acc[c], 0, 0, 0);
+         }
+      }
+   }
+#pragma unroll
+   for (int c = 0; c < NCT; ++c)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+         const int64_t r = row0 + 4 * t + g;
+         if (r < n) Y[r * yrs + 16 * c + li] = Elem<TY>::store(acc[c][t]);
+      }
+}
+
 int basis_update(ofrr_ctx *ctx, int64_t n, int k, int nb, const void *const *V, const int *vdt,
                  const int *kb, const int64_t *vrs, const int64_t *vcs, const double *const *C,
                  const int *ldc, void *Y, int ydt, int64_t yrs, int64_t ycs) {
@@ -233,6 +453,47 @@ int basis_update(ofrr_ctx *ctx, int64_t n, int k, int nb, const void *const *V, 
    for (int i = 0; i < nb; ++i) {
       if (!dtype_valid(vdt[i])) return set_error(ctx, kErrorBLASUnsupportedType, "basis_update: dtype");
       a.b[i] = UpdBlock{V[i], C[i], vrs[i], vcs[i], kb[i], ldc[i], vdt[i]};
+   }
+   // FP64 MFMA path: row-major blocks and output, widths multiples of 16
+   bool mfma = (ycs == 1) && (k % 16 == 0) && (k <= 128) && (ydt == OFRR_F64 || ydt == OFRR_F32);
+   int groups = 0;
+   for (int i = 0; i < nb; ++i) {
+      mfma = mfma && (vcs[i] == 1) && (kb[i] % 16 == 0) && (kb[i] > 0);
+      groups += kb[i] / 16;
+   }
+   if (mfma) {
+      double *Cp = (double *)workspace(ctx, (size_t)groups * k * 16 * sizeof(double));
+      if (!Cp) return set_error(ctx, kErrorCUDAMemory, "basis_update: workspace");
+      repack_coef_kernel<<<groups, 256, 0, ctx->stream>>>(k, nb, a, Cp);
+      OFRR_LAUNCH(ctx);
+      UpdMfmaArgs m{};
+      m.nb = nb;
+      for (int i = 0; i < nb; ++i) {
+         m.V[i] = V[i];
+         m.rs[i] = vrs[i];
+         m.kb[i] = kb[i];
+         m.dtype[i] = vdt[i];
+      }
+      const unsigned grid = (unsigned)((n + 15) / 16);
+      auto launch = [&](auto nct_tag) {
+         constexpr int NCT = decltype(nct_tag)::value;
+         if (ydt == OFRR_F64)
+            basis_update_mfma_kernel<double, NCT><<<grid, 64, 0, ctx->stream>>>(n, k, m, Cp, (double *)Y, yrs);
+         else
+            basis_update_mfma_kernel<float, NCT><<<grid, 64, 0, ctx->stream>>>(n, k, m, Cp, (float *)Y, yrs);
+      };
+      switch (k / 16) {
+         case 1: launch(std::integral_constant<int, 1>{}); break;
+         case 2: launch(std::integral_constant<int, 2>{}); break;
+         case 3: launch(std::integral_constant<int, 3>{}); break;
+         case 4: launch(std::integral_constant<int, 4>{}); break;
+         case 5: launch(std::integral_constant<int, 5>{}); break;
+         case 6: launch(std::integral_constant<int, 6>{}); break;
+         case 7: launch(std::integral_constant<int, 7>{}); break;
+         default: launch(std::integral_constant<int, 8>{}); break;
+      }
+      OFRR_LAUNCH(ctx);
+      return kSuccess;
    }
    dim3 grid((unsigned)((n + GT - 1) / GT), (unsigned)((k + GT - 1) / GT));
    if (ydt == OFRR_F64) {

@@ -38,6 +38,25 @@ int cheb_step(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, co
 int gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64_t vrs,
          int64_t vcs, const void *W, int wdt, int64_t wrs, int64_t wcs, double alpha, double beta,
          double *G, int ldg);
+// Fused projection products: item t is tile (ti[t], tj[t]) (64 x 64) of the k x k
+// block op[ia[t]]^T op[ib[t]], written to dst[t] (column-major, ld[t]) at rows
+// ro[t] + ..., columns co[t] + ....  Operands are row-major n x k blocks of
+// kind 0 (FP64), 1 (format tb) or 2 (format tz).  FP64 accumulation, fixed
+// summation order.
+constexpr int kMaxGramOps = 8, kMaxGramItems = 48;
+struct GramOperand {
+   const void *p;
+   int64_t rs;
+   int kind;
+};
+struct GramMultiArgs {
+   GramOperand op[kMaxGramOps];
+   int nitems;
+   int ia[kMaxGramItems], ib[kMaxGramItems], ti[kMaxGramItems], tj[kMaxGramItems];
+   double *dst[kMaxGramItems];
+   int ld[kMaxGramItems], ro[kMaxGramItems], co[kMaxGramItems];
+};
+int gram_multi(ofrr_ctx *ctx, int64_t n, int k, int tb, int tz, GramMultiArgs &args);
 // Y = sum_b V_b C_b
 int basis_update(ofrr_ctx *ctx, int64_t n, int k, int nb, const void *const *V, const int *vdt,
                  const int *kb, const int64_t *vrs, const int64_t *vcs, const double *const *C,

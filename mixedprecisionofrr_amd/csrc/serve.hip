@@ -46,9 +46,9 @@ struct Plan {
 
 inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// Block width: the smallest power of two >= 1.25 nwant (at least 16), so the
-// block carries a margin of unwanted directions beyond the wanted edge; two
-// correction blocks while the basis dimension stays <= 256, else one.
+// Block width: the smallest power of two >= 1.25 nwant (at least 16, at most
+// 128), so the block carries a margin of unwanted directions beyond the wanted
+// edge; two correction blocks while the basis dimension stays <= 256, else one.
 int plan_request(const ofrr_eig_request &r, Plan &p) {
    if ((r.dim != 2 && r.dim != 3) || r.N < 4 || r.nwant < 1 || r.nwant > OFRR_SERVE_MAX_WANT ||
        !dtype_valid(r.basis_dtype) || !(r.tol > 0.0) || r.degree < 0 || r.maxiter < 0)
@@ -60,7 +60,7 @@ int plan_request(const ofrr_eig_request &r, Plan &p) {
    p.nnz = nnz;
    const int want = (5 * r.nwant + 3) / 4;
    p.k = 16;
-   while (p.k < want) p.k *= 2;
+   while (p.k < want && p.k < 128) p.k *= 2;
    if (p.k > p.n) return kErrorInvalidArgument;
    p.nd = (3 * p.k <= 256) ? 2 : 1;
    p.degree = r.degree ? r.degree : kDefaultDegree;

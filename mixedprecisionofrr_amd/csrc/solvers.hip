@@ -801,22 +801,39 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       }
       OFRR_HIP(ctx, hipEventRecord(ev[5], ctx->stream));
       const int64_t zs = (int64_t)k * nd;
-      // G blocks (upper), column-major ld m
-      if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, X, OFRR_F64, k, 1, 1.0, 0.0, G, m))) return st;
-      if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, Dd, bdt, k, 1, 1.0, 0.0, G + (size_t)k * m, m))) return st;
-      if ((st = gram(ctx, n, k, k, Dd, bdt, k, 1, Dd, bdt, k, 1, 1.0, 0.0, G + k + (size_t)k * m, m))) return st;
-      if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, W, OFRR_F64, k, 1, 1.0, 0.0, H, m))) return st;
-      if ((st = gram(ctx, n, k, k, W, OFRR_F64, k, 1, Dd, bdt, k, 1, 1.0, 0.0, H + (size_t)k * m, m))) return st;
-      if ((st = gram(ctx, n, k, k, Dd, bdt, k, 1, Z, zdt, zs, 1, 1.0, 0.0, H + k + (size_t)k * m, m))) return st;
-      if (nd == 2) {
-         const size_t c2 = (size_t)2 * k * m;
-         const void *Zm = (const char *)Z + (size_t)k * zsz;
-         if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, Dmid, bdt, k, 1, 1.0, 0.0, G + c2, m))) return st;
-         if ((st = gram(ctx, n, k, k, Dd, bdt, k, 1, Dmid, bdt, k, 1, 1.0, 0.0, G + k + c2, m))) return st;
-         if ((st = gram(ctx, n, k, k, Dmid, bdt, k, 1, Dmid, bdt, k, 1, 1.0, 0.0, G + 2 * k + c2, m))) return st;
-         if ((st = gram(ctx, n, k, k, W, OFRR_F64, k, 1, Dmid, bdt, k, 1, 1.0, 0.0, H + c2, m))) return st;
-         if ((st = gram(ctx, n, k, k, Dd, bdt, k, 1, Zm, zdt, zs, 1, 1.0, 0.0, H + k + c2, m))) return st;
-         if ((st = gram(ctx, n, k, k, Dmid, bdt, k, 1, Zm, zdt, zs, 1, 1.0, 0.0, H + 2 * k + c2, m))) return st;
+      // G and H blocks (upper block triangles, column-major ld m) in one fused
+      // launch: G = [X Dd Dm]^T [X Dd Dm]; H = [X Dd Dm]^T [W Zd Zm] with the
+      // X-D blocks of H formed as W^T D (W = A X in FP64, D exact in its format)
+      {
+         GramMultiArgs ga{};
+         ga.op[0] = {X, (int64_t)k, 0};
+         ga.op[1] = {W, (int64_t)k, 0};
+         ga.op[2] = {Dd, (int64_t)k, 1};
+         ga.op[3] = {Dmid, (int64_t)k, 1};
+         ga.op[4] = {Z, zs, 2};
+         ga.op[5] = {(const char *)Z + (size_t)k * zsz, zs, 2};
+         // (rows operand, columns operand, destination, block row, block column)
+         struct Blk { int a, b; double *dst; int br, bc; };
+         Blk blks[12] = {{0, 0, G, 0, 0}, {0, 2, G, 0, 1}, {2, 2, G, 1, 1},
+                         {0, 1, H, 0, 0}, {1, 2, H, 0, 1}, {2, 4, H, 1, 1},
+                         {0, 3, G, 0, 2}, {2, 3, G, 1, 2}, {3, 3, G, 2, 2},
+                         {1, 3, H, 0, 2}, {2, 5, H, 1, 2}, {3, 5, H, 2, 2}};
+         const int nblk = (nd == 2) ? 12 : 6, nt = (k + 63) / 64;
+         int t = 0;
+         for (int q = 0; q < nblk; ++q)
+            for (int ti = 0; ti < nt; ++ti)
+               for (int tj = 0; tj < nt; ++tj, ++t) {
+                  ga.ia[t] = blks[q].a;
+                  ga.ib[t] = blks[q].b;
+                  ga.ti[t] = ti;
+                  ga.tj[t] = tj;
+                  ga.dst[t] = blks[q].dst;
+                  ga.ld[t] = m;
+                  ga.ro[t] = blks[q].br * k;
+                  ga.co[t] = blks[q].bc * k;
+               }
+         ga.nitems = t;
+         if ((st = gram_multi(ctx, n, k, bdt, zdt, ga))) return st;
       }
       mirror_upper_kernel<<<(m * m + 255) / 256, 256, 0, ctx->stream>>>(m, G, m);
       mirror_upper_kernel<<<(m * m + 255) / 256, 256, 0, ctx->stream>>>(m, H, m);

@@ -220,6 +220,32 @@ def test_basis_update_mixed_blocks(ctx):
     assert np.max(np.abs(Y - ref)) < 1e-12 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("n,k,kbs,dts,out", [
+    (1000, 16, (16, 32), (torch.float64, torch.bfloat16), torch.float64),  # ragged last tile
+    (4099, 128, (128, 128), (torch.float64, torch.float16), torch.float64),
+    (777, 48, (48, 48, 48), (torch.float64, torch.float8_e4m3fn, torch.float32), torch.float32),
+    (513, 64, (64, 40), (torch.float64, torch.float16), torch.float64),  # width 40: LDS kernel
+])
+def test_basis_update_shapes(ctx, n, k, kbs, dts, out):
+    """Back-transform Y = sum_b V_b C_b (FP64 MFMA path for row-major blocks of
+    widths that are multiples of 16, the LDS kernel otherwise) against FP64."""
+    from mixedprecisionofrr_amd import api
+    rng = np.random.default_rng(n + k)
+    blocks, ref = [], 0.0
+    for kb, dt in zip(kbs, dts):
+        V = _dev(rng.uniform(-1, 1, (n, kb)), dt)
+        Cb = _dev(rng.standard_normal((kb, k)).T).t()  # column-major kb x k
+        blocks.append((V, Cb))
+        ref = ref + V.double().cpu().numpy() @ Cb.cpu().numpy()
+    Y = api.basis_update(ctx, blocks, k, out_dtype=out).double().cpu().numpy()
+    tol = 1e-12 if out == torch.float64 else 1e-6
+    assert np.max(np.abs(Y - ref)) < tol * np.max(np.abs(ref))
+    # a column-major block takes the LDS kernel: same result within rounding
+    Vc = blocks[0][0].t().contiguous().t()
+    Y2 = api.basis_update(ctx, [(Vc, blocks[0][1])] + blocks[1:], k, out_dtype=out).double().cpu().numpy()
+    assert np.max(np.abs(Y2 - ref)) < tol * np.max(np.abs(ref))
+
+
 def test_syev_reference_kat_batched_and_random(ctx, kats):
     from mixedprecisionofrr_amd import api
     n = 10
@@ -245,7 +271,7 @@ def test_syev_bitwise_vs_oracle(ctx):
     (fixed reduction orders, explicit fused multiply-adds on both sides)."""
     from mixedprecisionofrr_amd import api
     rng = np.random.default_rng(21)
-    for m in (2, 3, 17, 64, 130, 200, 256):
+    for m in (2, 3, 17, 64, 127, 128, 130, 191, 192, 193, 200, 256):
         M = rng.standard_normal((m, m))
         S = M + M.T
         w, V = api.syev_batched(ctx, _dev(S[None]))
@@ -282,17 +308,19 @@ def test_sygv_reference_kat(ctx, kats):
     assert e.value.status == 3001
 
 
-@pytest.mark.parametrize("m", [64, 128, 192])
-def test_rr_truncated_vs_oracle(ctx, m):
+@pytest.mark.parametrize("m,drop", [(64, 4), (128, 4), (192, 4), (192, 3), (127, 3)])
+def test_rr_truncated_vs_oracle(ctx, m, drop):
+    """Truncated RR; an odd kept rank runs the eigensolver on an odd size read
+    from the device (zero-padded virtual index)."""
     from mixedprecisionofrr_amd import api
     Ah = O.stencil(3, 12)
     S = Ah.to_scipy()
     rng = np.random.default_rng(m)
     V = rng.standard_normal((Ah.n, m))
-    V[:, -4:] = V[:, :4] + 1e-13 * rng.standard_normal((Ah.n, 4))  # near rank drop
+    V[:, -drop:] = V[:, :drop] + 1e-13 * rng.standard_normal((Ah.n, drop))  # near rank drop
     G, H = V.T @ V, V.T @ (S @ V)
     r_o, th_o, Y_o = O.rr_truncated(G, H, 1e-10, 32)
-    print(f"rr_truncated m={m}", flush=True)
+    print(f"rr_truncated m={m} drop={drop}", flush=True)
     th, Y, r = api.rr_truncated(ctx, _dev(G.T).t(), _dev(H.T).t(), 1e-10, 32)
     assert r == r_o
     assert np.abs(th.cpu().numpy() - th_o).max() < 1e-12 * abs(th_o[0])

@@ -16,7 +16,8 @@ def _state_bytes(n, k, nd, degree, bdt):
 
 
 @pytest.mark.parametrize("nwant,k,nd", [(1, 16, 2), (8, 16, 2), (13, 32, 2), (16, 32, 2),
-                                        (32, 64, 2), (50, 64, 2), (52, 128, 1), (64, 128, 1)])
+                                        (32, 64, 2), (50, 64, 2), (52, 128, 1), (64, 128, 1),
+                                        (100, 128, 1), (128, 128, 1)])
 def test_block_width_and_corrections(nwant, k, nd):
     kk, ndd, _ = api.request_footprint(api.request(3, 64, nwant))
     assert (kk, ndd) == (k, nd)
@@ -46,7 +47,7 @@ def test_footprint_grows_with_size():
 
 
 @pytest.mark.parametrize("bad", [dict(dim=4, N=10, nwant=8), dict(dim=2, N=2, nwant=8),
-                                 dict(dim=2, N=100, nwant=0), dict(dim=2, N=100, nwant=65),
+                                 dict(dim=2, N=100, nwant=0), dict(dim=2, N=100, nwant=129),
                                  dict(dim=3, N=1300, nwant=8), dict(dim=2, N=4, nwant=64)])
 def test_invalid_requests_are_refused(bad):
     with pytest.raises(api.OfrrError):
