@@ -69,7 +69,7 @@ def measured_traffic(args):
     """HBM bytes per launch of the filter kernel from the committed rocprofv3
     counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated); it covers the
     128^3, k = 64, FP16 workload only."""
-    if args.grid != 128 or args.k != 64:
+    if args.grid != 128 or args.k != 64 or args.basis != "f16":
         return None
     try:
         import re
@@ -142,6 +142,8 @@ def main():
     p.add_argument("--tau", type=float, default=1e-12)
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--fp64-steps", type=int, default=1)
+    p.add_argument("--basis", choices=["f16", "bf16", "fp8"], default="f16",
+                   help="growth-block format of the mixed path (the workload names FP16)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--workload", choices=["eig", "svd", "serve"], default="eig",
@@ -199,13 +201,14 @@ def main():
     N = args.grid
     log(f"building 3D 7-point Laplacian {N}^3 on the GPU")
     A = api.CSRMatrix.stencil(ctx, 3, N).with_low_precision(torch.float16)
+    basis = {"f16": torch.float16, "bf16": torch.bfloat16, "fp8": torch.float8_e4m3fn}[args.basis]
     X0 = torch.empty(A.n, args.k, dtype=torch.float64, device=dev)
     api.fill_randn(ctx, X0, seed=args.seed)
     torch.cuda.synchronize()
 
     # ---- mixed path: warmup + timed steps
     for w in range(args.warmup):
-        st, th, X, res, stats = run_solver(api, ctx, A, X0, args, torch.float16, "mixed")
+        st, th, X, res, stats = run_solver(api, ctx, A, X0, args, basis, "mixed")
         log(f"warmup {w}: status {st} iters {stats['iterations']} {stats['t_total_ms']:.1f} ms")
     lat, pairs, last = [], 0, None
     filt_ms = filt_steps = 0.0
@@ -213,7 +216,7 @@ def main():
     t0 = time.perf_counter()
     for s in range(args.steps):
         ts = time.perf_counter()
-        st, th, X, res, stats = run_solver(api, ctx, A, X0, args, torch.float16, "mixed")
+        st, th, X, res, stats = run_solver(api, ctx, A, X0, args, basis, "mixed")
         lat.append((time.perf_counter() - ts) * 1e3)
         pairs += int(np.sum(res[:args.nwant] <= args.tol))
         filt_ms += stats["t_filter_ms"]
@@ -235,7 +238,8 @@ def main():
     sweeps = stats["iterations"]
     per_solve_steps = stats["filter_steps"]
     firsts = sweeps
-    bytes_solve = filter_bytes(n, nnz, k, 2, 2, per_solve_steps - firsts, firsts)
+    b_basis = 1 if args.basis == "fp8" else 2
+    bytes_solve = filter_bytes(n, nnz, k, 2, b_basis, per_solve_steps - firsts, firsts)
     avg_launch_ms = filt_ms / max(1, filt_steps)
     bytes_per_launch = bytes_solve / max(1, per_solve_steps)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
@@ -279,12 +283,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f16 storage / f32 accumulate (growth), f64 (Ritz block, Gram, RR, residual)",
+        "dtype": f"{args.basis} storage / f32 accumulate (growth), f64 (Ritz block, Gram, RR, residual)",
         "data": "synthetic",
         "config": {
             "workload": (f"mixed-precision filtered block subspace iteration + OFRR, 3D 7-point "
                          f"Dirichlet Laplacian {N}^3, block k={k}, basis dim {k * (1 + args.nd)}, "
-                         f"{args.nwant} largest eigenpairs to residual {args.tol:g}, FP16 basis"),
+                         f"{args.nwant} largest eigenpairs to residual {args.tol:g}, "
+                         f"{args.basis.upper()} basis"),
             "n": n, "nnz": nnz, "k": k, "basis_dim": k * (1 + args.nd), "nwant": args.nwant,
             "filter_degree": args.degree, "tol": args.tol, "seed": args.seed,
         },
@@ -299,7 +304,8 @@ def main():
                   "t_zspmm_ms": stats["t_zspmm_ms"], "t_gram_ms": stats["t_gram_ms"],
                   "t_rr_ms": stats["t_rr_ms"], "t_update_ms": stats["t_update_ms"],
                   "t_resid_ms": stats["t_resid_ms"], "rank_min": stats["rank_min"]},
-        "roofline": {"kernel": "cheb_step_kernel<__half,__half,8> (FP16 CSR SpMM + filter axpys)",
+        "roofline": {"kernel": f"cheb_step_kernel (FP16 operator, {args.basis.upper()} blocks: CSR SpMM "
+                               f"+ filter axpys)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
                      "traffic_source": "profiles/cheb_step_f16_laplacian128_k64.md",
@@ -366,7 +372,14 @@ def serve_stream(api, count, rate, slots, seed, big=False):
                 log(f"serve: collected {j + 1}")
         wall = time.perf_counter() - t0
         if big_id is not None:
-            b = srv.wait(big_id)
+            b, t_last = None, time.perf_counter()
+            while b is None:
+                b = srv.poll(big_id)
+                if b is None:
+                    time.sleep(0.5)
+                    if time.perf_counter() - t_last > 30:
+                        t_last = time.perf_counter()
+                        log("serve: large FP8 solve still running")
             log(f"serve: large FP8 solve status {b['status']} iterations {b['iterations']} "
                 f"{(b['t_done'] - b['t_submit']):.1f} s")
             big_out = {"workload": "3-D 7-point variable-coefficient diffusion, 256^3 grid "

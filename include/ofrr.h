@@ -264,6 +264,34 @@ OFRR_API int ofrr_svd_filtered(ofrr_ctx *ctx, int64_t M, int64_t N, const float 
                                const ofrr_filter_opts *opt, double *V_dev, double *U_dev,
                                double *sigma_host, double *res_host, ofrr_filter_stats *stats);
 
+/* Two-sided Rayleigh-Ritz for the SVD (svd_rr.m:54-91; the rr_svd boundary,
+ * CPP/linalg/solvers/svd/rr_svd.hpp) over a dense FP64 A (M x N, column-major):
+ * B = U^T A V; eig([0 B; B^T 0], blkdiag(U^T U, V^T V) + gtol I) when generalized,
+ * else eig([0 B; B^T 0]) (the SVD of B for orthonormal U, V); sigma_host (min(ku, kv),
+ * descending); U (M x ku) and V (N x kv), column-major, are replaced in their first
+ * min(ku, kv) columns by sqrt(2) U Z and sqrt(2) V W ((Z; W) the eigenvectors).
+ * ku + kv <= 256.  Synchronous. */
+OFRR_API int ofrr_svd_rr(ofrr_ctx *ctx, int64_t M, int64_t N, const double *A_dev, int64_t lda,
+                         int ku, int kv, double *U_dev, int64_t ldu, double *V_dev, int64_t ldv,
+                         int generalized, double gtol, double *sigma_host);
+/* Subspace iteration for the SVD (subspace_iter_svd.m:57-77; the si_svd boundary,
+ * CPP/linalg/solvers/svd/si_svd.hpp:25-28) in FP64 with function_orth = the
+ * Hessenberg process (select_tol, kept columns only) and function_rr = ofrr_svd_rr:
+ * V0 (N x k) start block; each iteration step_size x (U = A V, scalecols; V = A^T U,
+ * scalecols), then the RR of the two Hessenberg bases.  U (M x k), V (N x k) and
+ * sigma_host (k, NaN past the kept width *kept) out.  2 k <= 256.  Synchronous. */
+OFRR_API int ofrr_subspace_iter_svd(ofrr_ctx *ctx, int64_t M, int64_t N, const double *A_dev,
+                                    int64_t lda, int k, int maxiter, int step_size,
+                                    double select_tol, int generalized, double gtol,
+                                    const double *V0_dev, int64_t ldv0, double *U_dev, int64_t ldu,
+                                    double *V_dev, int64_t ldv, double *sigma_host, int *kept);
+
+/* Diagnostic (no counterpart in the reference): counts where gfx950's FP8
+ * conversion instructions disagree with the OCP FP8 E4M3 definition the library
+ * stores (round to nearest even, saturating at 448): counts_host[0] decodes of
+ * the 256 codes, [1] encodes of floats up to 448 in magnitude, [2] above 448. */
+OFRR_API int ofrr_fp8_check(ofrr_ctx *ctx, int64_t counts_host[3]);
+
 /* ---- matrix input (host only): the binary CSR files read by
  * MatrixReader::read_csr_matrix (CPP/io/matrix_reader.cpp:8-45):
  * <prefix>_row_ptr.bin (int32, rows + 1), <prefix>_col_ind.bin (int32, 0-based),

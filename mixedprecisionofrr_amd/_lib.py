@@ -25,6 +25,7 @@ EXPORTS = [
     "ofrr_request_footprint", "ofrr_server_create", "ofrr_server_submit", "ofrr_server_poll",
     "ofrr_server_wait", "ofrr_server_stats_get", "ofrr_server_destroy",
     "ofrr_csr_file_info", "ofrr_csr_file_read", "ofrr_singular_values_file", "ofrr_io_last_error",
+    "ofrr_svd_rr", "ofrr_subspace_iter_svd", "ofrr_fp8_check",
 ]
 
 SERVE_MAX_WANT = 128
@@ -136,6 +137,11 @@ def _decl(L):
     L.ofrr_csr_file_read.argtypes = [C.c_char_p, vp, vp, vp]
     L.ofrr_singular_values_file.argtypes = [C.c_char_p, vp, i64, C.POINTER(i64)]
     L.ofrr_io_last_error.restype = C.c_char_p
+    L.ofrr_fp8_check.argtypes = [vp, C.POINTER(i64)]
+    L.ofrr_svd_rr.argtypes = [vp, i64, i64, vp, i64, i32, i32, vp, i64, vp, i64, i32, dbl,
+                              C.POINTER(dbl)]
+    L.ofrr_subspace_iter_svd.argtypes = [vp, i64, i64, vp, i64, i32, i32, i32, dbl, i32, dbl, vp,
+                                         i64, vp, i64, vp, i64, C.POINTER(dbl), C.POINTER(i32)]
 
 
 def load():

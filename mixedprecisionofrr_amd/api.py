@@ -427,6 +427,45 @@ def svd_filtered(ctx, A, V0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=to
     return st, np.array(sig[:]), U, V, np.array(res[:]), sd
 
 
+# ---- two-sided SVD (reference algorithms, FP64) --------------------------------
+def svd_rr(ctx, A, U, V, generalized=True, gtol=0.0):
+    """svd_rr.m on the GPU over a dense FP64 A (M x N, any tensor with
+    A.stride(0) == 1, i.e. column-major).  U (M x ku), V (N x kv) are copied.
+    Returns (sigma (kk), U (M x kk), V (N x kk)), kk = min(ku, kv)."""
+    M, N = A.shape
+    if A.stride(0) != 1 or A.dtype != torch.float64:
+        raise ValueError("svd_rr: A must be FP64 column-major (stride(0) == 1)")
+    Uc = colmajor(M, U.shape[1], torch.float64, A.device)
+    Uc.copy_(U)
+    Vc = colmajor(N, V.shape[1], torch.float64, A.device)
+    Vc.copy_(V)
+    kk = min(U.shape[1], V.shape[1])
+    sig = (C.c_double * kk)()
+    ctx.check(ctx.lib.ofrr_svd_rr(ctx.h, M, N, _p(A), A.stride(1), U.shape[1], V.shape[1], _p(Uc),
+                                  M, _p(Vc), N, int(generalized), gtol, sig), "svd_rr")
+    return np.array(sig[:]), Uc[:, :kk], Vc[:, :kk]
+
+
+def subspace_iter_svd(ctx, A, V0, maxiter, step_size=1, select_tol=2.0 ** -52, generalized=True,
+                      gtol=0.0):
+    """subspace_iter_svd.m (Hessenberg orth + svd_rr) in FP64 on the GPU.
+    Returns (sigma (k, NaN past the kept width), U (M x k), V (N x k), kept)."""
+    M, N = A.shape
+    if A.stride(0) != 1 or A.dtype != torch.float64:
+        raise ValueError("subspace_iter_svd: A must be FP64 column-major (stride(0) == 1)")
+    k = V0.shape[1]
+    V0c = colmajor(N, k, torch.float64, A.device)
+    V0c.copy_(V0)
+    U = colmajor(M, k, torch.float64, A.device)
+    V = colmajor(N, k, torch.float64, A.device)
+    sig = (C.c_double * k)()
+    kept = C.c_int()
+    ctx.check(ctx.lib.ofrr_subspace_iter_svd(ctx.h, M, N, _p(A), A.stride(1), k, maxiter, step_size,
+                                             select_tol, int(generalized), gtol, _p(V0c), N, _p(U),
+                                             M, _p(V), N, sig, C.byref(kept)), "subspace_iter_svd")
+    return np.array(sig[:]), U, V, kept.value
+
+
 # ---- matrix input -----------------------------------------------------------
 def read_csr(prefix):
     """Host arrays (row_ptr int64, col_idx int32, vals float64, rows, cols) of the

@@ -160,11 +160,19 @@ template <> struct Elem<__hip_bfloat16> {
    __device__ static float load(__hip_bfloat16 v) { return __bfloat162float(v); }
    __device__ static __hip_bfloat16 store(double v) { return __float2bfloat16((float)v); }
 };
+// On the device the conversions are gfx950's v_cvt_f32_fp8 / v_cvt_pk_fp8_f32
+// (after a clamp to +-448, the saturation of float_to_fp8); they agree with
+// fp8_to_float / float_to_fp8 on every code and every finite float
+// (ofrr_fp8_check).
 template <> struct Elem<fp8_t> {
    using acc = float;
    static constexpr int id = OFRR_FP8;
-   __device__ static float load(fp8_t v) { return fp8_to_float(v); }
-   __device__ static fp8_t store(double v) { return float_to_fp8((float)v); }
+   __device__ static float load(fp8_t v) { return __builtin_amdgcn_cvt_f32_fp8((int)v, 0); }
+   __device__ static fp8_t store(double v) {
+      float f = (float)v;
+      f = (f == f) ? fminf(fmaxf(f, -448.0f), 448.0f) : f;  // saturate like float_to_fp8
+      return (fp8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(f, 0.0f, 0, false) & 0xff);
+   }
 };
 
 // largest finite magnitude of a storage type (quantisation range)

@@ -13,6 +13,7 @@
 //                 for a basis whose column blocks are stored in different
 //                 precisions (FP64 Ritz block, low-precision growth blocks).
 //   small_gemm    C = alpha op(A) B + beta C for the k x k Rayleigh-Ritz algebra.
+#include <mutex>
 #include <type_traits>
 
 #include "internal.hpp"
@@ -145,102 +146,154 @@ int gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64
 
 // ---------------------------------------------------------------------------
 // Fused projection products of one solver iteration: every block V_a^T V_b of
-// the Gram matrix and of the projected matrix in one launch.  An item is one
-// 64 x 64 tile of one block; the operands are row-major n x k blocks whose
-// element kinds are FP64 (Ritz block, A X), the growth-block format TB or the
-// product format TZ.  The grid is laid out so that all items of a row chunk run
-// on one XCD at about the same time (workgroup w goes to XCD w mod 8), so each
-// chunk of rows comes from HBM once and from that XCD's L2 for the other items.
-// Per-chunk partial tiles are summed in a fixed order (deterministic).
-template <typename TA, typename TB>
-__device__ inline void gram_tile_rows(int64_t r0, int64_t r1, int k, const TA *__restrict__ va,
-                                      int64_t vrs, const TB *__restrict__ wb, int64_t wrs,
-                                      int i0, int j0, double *__restrict__ p) {
-   const int lane = threadIdx.x;
-   const int li = lane & 15, lk = lane >> 4;
-   dbl4 acc[4][4];
-#pragma unroll
-   for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-   bool cv[4], cw[4];
-#pragma unroll
-   for (int t = 0; t < 4; ++t) {
-      cv[t] = i0 + 16 * t + li < k;
-      cw[t] = j0 + 16 * t + li < k;
-   }
-   const TA *vb = va + i0 + li;
-   const TB *wp = wb + j0 + li;
-#pragma unroll 2
-   for (int64_t r = r0; r < r1; r += 4) {
-      const int64_t row = r + lk;
-      const bool rok = row < r1;
-      double a[4], b[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-         a[t] = (rok && cv[t]) ? (double)Elem<TA>::load(vb[row * vrs + 16 * t]) : 0.0;
-         b[t] = (rok && cw[t]) ? (double)Elem<TB>::load(wp[row * wrs + 16 * t]) : 0.0;
-      }
-#pragma unroll
-      for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-         for (int tj = 0; tj < 4; ++tj)
-            acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
-   }
-#pragma unroll
-   for (int ti = 0; ti < 4; ++ti)
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj)
-#pragma unroll
-         for (int rg = 0; rg < 4; ++rg)
-            p[(16 * ti + lk + 4 * rg) * GT + 16 * tj + li] = acc[ti][tj][rg];
-}
+// the Gram matrix and of the projected matrix in one launch.  The output is cut
+// into 64 x 32 tiles, one wave each; a workgroup of 12 waves takes one chunk of
+// rows and up to 12 tiles.  It stages the rows of the operands its tiles read
+// into LDS as FP64, a slab of R rows at a time (the next slab is loaded into
+// registers while the MFMAs run on the current one), so every row
+// of every operand is read from HBM once per workgroup and converted once.  The
+// operands are row-major n x k blocks whose element kinds are FP64 (Ritz block,
+// A X), the growth-block format TB or the product format TZ.  Per-chunk partial
+// tiles are summed in a fixed order (deterministic, independent of other work).
+constexpr int kGramWaves = 12;                      // tiles (waves) per workgroup
+constexpr int kGramThreads = 64 * kGramWaves;
+constexpr int kGramStageOps = 6;                    // operands staged per workgroup
+constexpr int kGramPasses = 3;                      // staging loads per thread and operand
+constexpr int kMaxHalfItems = 2 * kMaxGramItems;
+constexpr int kMaxGramGroups = (kMaxHalfItems + kGramWaves - 1) / kGramWaves;
 
-template <typename TB, typename TZ, typename TA>
-__device__ inline void gram_item_second(int kb, int64_t r0, int64_t r1, int k, const TA *va,
-                                        int64_t vrs, const void *w, int64_t wrs, int i0, int j0,
-                                        double *p) {
-   if (kb == 0) gram_tile_rows<TA, double>(r0, r1, k, va, vrs, (const double *)w, wrs, i0, j0, p);
-   else if (kb == 1) gram_tile_rows<TA, TB>(r0, r1, k, va, vrs, (const TB *)w, wrs, i0, j0, p);
-   else gram_tile_rows<TA, TZ>(r0, r1, k, va, vrs, (const TZ *)w, wrs, i0, j0, p);
+struct GramLdsArgs {
+   GramOperand op[kMaxGramOps];
+   int nitems, R, stride, lk;
+   short slot_a[kMaxHalfItems], slot_b[kMaxHalfItems], i0[kMaxHalfItems], j0[kMaxHalfItems];
+   signed char gop[kMaxGramGroups][kGramStageOps];  // operand staged in each slot of a group
+   signed char gnop[kMaxGramGroups];
+};
+
+template <typename TB, typename TZ>
+__device__ inline double gram_load_kind(const GramOperand &o, int64_t off) {
+   if (o.kind == 0) return ((const double *)o.p)[off];
+   if (o.kind == 1) return (double)Elem<TB>::load(((const TB *)o.p)[off]);
+   return (double)Elem<TZ>::load(((const TZ *)o.p)[off]);
 }
 
 template <typename TB, typename TZ>
-__global__ __launch_bounds__(64) void gram_multi_kernel(int64_t n, int k, GramMultiArgs args,
-                                                        int64_t rows_per_chunk, int nchunks,
-                                                        double *__restrict__ part) {
-   const int L = blockIdx.x;
-   const int xcd = L & 7, local = L >> 3;
-   const int item = local % args.nitems, chunk = (local / args.nitems) * 8 + xcd;
-   if (chunk >= nchunks) return;
+__global__ __launch_bounds__(kGramThreads) void gram_lds_kernel(int64_t n, int k, GramLdsArgs args,
+                                                                int64_t rows_per_chunk, int nchunks,
+                                                                int ngroups, double *__restrict__ part) {
+   extern __shared__ double sbuf[];  // [nop][R][stride]
+   const int chunk = blockIdx.x / ngroups, grp = blockIdx.x % ngroups;
    const int64_t r0 = (int64_t)chunk * rows_per_chunk;
    const int64_t r1 = min(n, r0 + rows_per_chunk);
-   const GramOperand A = args.op[args.ia[item]], B = args.op[args.ib[item]];
-   const int i0 = args.ti[item] * GT, j0 = args.tj[item] * GT;
-   double *p = part + ((int64_t)chunk * args.nitems + item) * (GT * GT);
-   // wave-uniform dispatch on the two operand kinds
-   if (A.kind == 0)
-      gram_item_second<TB, TZ, double>(B.kind, r0, r1, k, (const double *)A.p, A.rs, B.p, B.rs, i0, j0, p);
-   else if (A.kind == 1)
-      gram_item_second<TB, TZ, TB>(B.kind, r0, r1, k, (const TB *)A.p, A.rs, B.p, B.rs, i0, j0, p);
-   else
-      gram_item_second<TB, TZ, TZ>(B.kind, r0, r1, k, (const TZ *)A.p, A.rs, B.p, B.rs, i0, j0, p);
+   const int R = args.R, stride = args.stride, nop = args.gnop[grp];
+   const int slab = R * stride;                       // doubles per operand slab
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+   const int li = lane & 15, lk = lane >> 4;
+   const int item = grp * kGramWaves + wave;
+   const bool has = item < args.nitems;
+   const int sa = has ? args.slot_a[item] : 0, sb = has ? args.slot_b[item] : 0;
+   const int i0 = has ? args.i0[item] : 0, j0 = has ? args.j0[item] : 0;
+   bool cv[4], cw[2];
+#pragma unroll
+   for (int t = 0; t < 4; ++t) cv[t] = i0 + 16 * t + li < k;
+#pragma unroll
+   for (int t = 0; t < 2; ++t) cw[t] = j0 + 16 * t + li < k;
+   dbl4 acc[4][2];
+#pragma unroll
+   for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
+   // staging: operand slot o (wave-uniform loop), element e = tid + 768 q of
+   // its R x k slab (R k <= 2048, k and R powers of two): row e >> lk, column
+   // e & (k - 1)
+   const int lk2 = args.lk, rk = R << lk2;
+   auto fetch = [&](int64_t rb, double pre[kGramPasses * kGramStageOps]) {
+#pragma unroll
+      for (int o = 0; o < kGramStageOps; ++o) {
+         if (o >= nop) break;
+         const GramOperand op = args.op[args.gop[grp][o]];
+#pragma unroll
+         for (int q = 0; q < kGramPasses; ++q) {
+            const int e = threadIdx.x + q * kGramThreads;
+            const int64_t row = rb + (e >> lk2);
+            double v = 0.0;
+            if (e < rk && row < r1) v = gram_load_kind<TB, TZ>(op, row * op.rs + (e & (k - 1)));
+            pre[kGramPasses * o + q] = v;
+         }
+      }
+   };
+   auto store = [&](const double pre[kGramPasses * kGramStageOps]) {
+#pragma unroll
+      for (int o = 0; o < kGramStageOps; ++o) {
+         if (o >= nop) break;
+#pragma unroll
+         for (int q = 0; q < kGramPasses; ++q) {
+            const int e = threadIdx.x + q * kGramThreads;
+            if (e < rk) sbuf[o * slab + (e >> lk2) * stride + (e & (k - 1))] = pre[kGramPasses * o + q];
+         }
+      }
+   };
+   double pre[kGramPasses * kGramStageOps];
+   fetch(r0, pre);
+   store(pre);
+   __syncthreads();
+   for (int64_t rb = r0; rb < r1; rb += R) {
+      const bool more = rb + R < r1;
+      if (more) fetch(rb + R, pre);  // in flight during the MFMAs below
+      if (has) {
+         const double *A = sbuf + sa * slab;
+         const double *B = sbuf + sb * slab;
+         for (int t = 0; t < R; t += 4) {
+            const int rr = t + lk;
+            double a[4], b[2];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a[q] = cv[q] ? A[rr * stride + i0 + 16 * q + li] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) b[q] = cw[q] ? B[rr * stride + j0 + 16 * q + li] : 0.0;
+#pragma unroll
+            for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+               for (int tj = 0; tj < 2; ++tj)
+                  acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+         }
+      }
+      __syncthreads();  // every wave is done with this slab
+      if (more) store(pre);
+      __syncthreads();
+   }
+   if (!has) return;
+   // entry (i, j) of a 16 x 16 result sits in lane j + 16 (i % 4), register i / 4
+   double *p = part + ((int64_t)chunk * args.nitems + item) * (GT * GT / 2);
+#pragma unroll
+   for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+         for (int rg = 0; rg < 4; ++rg)
+            p[(16 * ti + lk + 4 * rg) * (GT / 2) + 16 * tj + li] = acc[ti][tj][rg];
 }
 
-// item's k x k block tile (i0, j0) += sum over chunks, fixed order (as gram_reduce_kernel)
-__global__ __launch_bounds__(256) void gram_multi_reduce_kernel(int nchunks, int k,
-                                                                GramMultiArgs args,
-                                                                const double *__restrict__ part) {
+struct GramOutArgs {
+   double *dst[kMaxHalfItems];
+   int ld[kMaxHalfItems], ro[kMaxHalfItems], co[kMaxHalfItems];
+   short i0[kMaxHalfItems], j0[kMaxHalfItems];
+   int nitems;
+};
+
+// item's 64 x 32 tile = sum over chunks in a fixed order (kRedGroups groups of
+// consecutive chunks, each in chunk order, then the group sums in order)
+__global__ __launch_bounds__(256) void gram_lds_reduce_kernel(int nchunks, int k, GramOutArgs o,
+                                                              const double *__restrict__ part) {
    __shared__ double gs[256];
    const int item = blockIdx.y;
    const int el = threadIdx.x % 32, grp = threadIdx.x / 32;
-   const int e = blockIdx.x * 32 + el;  // entry of the 64 x 64 tile
-   const int64_t stride = (int64_t)args.nitems * GT * GT;
+   const int e = blockIdx.x * 32 + el;  // entry of the 64 x 32 tile
+   const int64_t stride = (int64_t)o.nitems * (GT * GT / 2);
    const int gsz = (nchunks + kRedGroups - 1) / kRedGroups;
    double s = 0.0;
    {
       const int c0 = grp * gsz, c1 = min(nchunks, c0 + gsz);
-      const double *pp = part + (int64_t)item * GT * GT + e;
+      const double *pp = part + (int64_t)item * (GT * GT / 2) + e;
       for (int c = c0; c < c1; ++c) s += pp[c * stride];
    }
    gs[threadIdx.x] = s;
@@ -248,36 +301,88 @@ __global__ __launch_bounds__(256) void gram_multi_reduce_kernel(int nchunks, int
    if (grp != 0) return;
    double t = 0.0;
    for (int g = 0; g < kRedGroups; ++g) t += gs[g * 32 + el];
-   const int i = args.ti[item] * GT + e / GT, j = args.tj[item] * GT + e % GT;
+   const int i = o.i0[item] + e / (GT / 2), j = o.j0[item] + e % (GT / 2);
    if (i >= k || j >= k) return;
-   args.dst[item][(args.ro[item] + i) + (int64_t)(args.co[item] + j) * args.ld[item]] = t;
+   o.dst[item][(o.ro[item] + i) + (int64_t)(o.co[item] + j) * o.ld[item]] = t;
 }
 
 int gram_multi(ofrr_ctx *ctx, int64_t n, int k, int tb, int tz, GramMultiArgs &args) {
    if (args.nitems <= 0) return kSuccess;
    if (args.nitems > kMaxGramItems || k <= 0 || k > 128)
       return set_error(ctx, kErrorInvalidArgument, "gram_multi: items / k");
-   // about 6 waves per SIMD over the whole device, a multiple of 8 chunks
-   int64_t want = std::max<int64_t>(8, (6 * 4 * (int64_t)ctx->num_cus) / args.nitems);
+   // 64 x 32 output tiles of the 64 x 64 items (halves past column k dropped)
+   GramLdsArgs la{};
+   GramOutArgs oa{};
+   for (int q = 0; q < kMaxGramOps; ++q) la.op[q] = args.op[q];
+   int nh = 0;
+   int ha[kMaxHalfItems], hb[kMaxHalfItems];
+   for (int t = 0; t < args.nitems; ++t)
+      for (int h = 0; h < 2; ++h) {
+         const int j0 = args.tj[t] * GT + h * (GT / 2);
+         if (j0 >= k) continue;
+         ha[nh] = args.ia[t];
+         hb[nh] = args.ib[t];
+         la.i0[nh] = oa.i0[nh] = (short)(args.ti[t] * GT);
+         la.j0[nh] = oa.j0[nh] = (short)j0;
+         oa.dst[nh] = args.dst[t];
+         oa.ld[nh] = args.ld[t];
+         oa.ro[nh] = args.ro[t];
+         oa.co[nh] = args.co[t];
+         ++nh;
+      }
+   la.nitems = oa.nitems = nh;
+   const int ngroups = (nh + kGramWaves - 1) / kGramWaves;
+   int nop_max = 0;
+   for (int g = 0; g < ngroups; ++g) {
+      int nop = 0;
+      auto slot = [&](int opi) {
+         for (int s = 0; s < nop; ++s)
+            if (la.gop[g][s] == opi) return s;
+         la.gop[g][nop] = (signed char)opi;
+         return nop++;
+      };
+      for (int t = g * kGramWaves; t < std::min(nh, (g + 1) * kGramWaves); ++t) {
+         la.slot_a[t] = (short)slot(ha[t]);
+         la.slot_b[t] = (short)slot(hb[t]);
+         if (nop > kGramStageOps) return set_error(ctx, kErrorInternalError, "gram_multi: operands per group");
+      }
+      la.gnop[g] = (signed char)nop;
+      nop_max = std::max(nop_max, nop);
+   }
+   // R rows per slab: R k <= 2048 elements per operand, three per thread
+   la.R = std::max(4, std::min(128, 2048 / k));  // a power of two: R k <= 2048 <= 3 per thread
+   la.lk = 0;
+   while ((1 << la.lk) < k) ++la.lk;
+   if ((1 << la.lk) != k) return set_error(ctx, kErrorInvalidArgument, "gram_multi: k a power of two");
+   la.stride = k + ((16 - k % 32) + 32) % 32;  // row stride = 16 mod 32 doubles: two-pass LDS reads
+   const size_t lds = (size_t)nop_max * la.R * la.stride * sizeof(double);
+   // about two workgroups per CU over the device
+   int64_t want = std::max<int64_t>(1, (2 * (int64_t)ctx->num_cus) / ngroups);
    int64_t rows = (n + want - 1) / want;
-   rows = std::max<int64_t>(64, ((rows + 3) / 4) * 4);
+   rows = std::max<int64_t>(la.R, (rows + la.R - 1) / la.R * la.R);
    const int nchunks = (int)((n + rows - 1) / rows);
-   const int ngrp = (nchunks + 7) / 8;
-   double *part = (double *)workspace(ctx, (size_t)nchunks * args.nitems * GT * GT * sizeof(double));
+   double *part = (double *)workspace(ctx, (size_t)nchunks * nh * (GT * GT / 2) * sizeof(double));
    if (!part) return set_error(ctx, kErrorCUDAMemory, "gram_multi: workspace");
-   const unsigned grid = (unsigned)(ngrp * 8 * args.nitems);
+   const unsigned grid = (unsigned)(nchunks * ngroups);
    int st = dispatch_dtype(tb, [&](auto b) -> int {
       using TB = decltype(b);
       return dispatch_dtype(tz, [&](auto z) -> int {
          using TZ = decltype(z);
-         gram_multi_kernel<TB, TZ><<<grid, 64, 0, ctx->stream>>>(n, k, args, rows, nchunks, part);
+         static std::once_flag once;
+         static hipError_t attr = hipSuccess;
+         std::call_once(once, [] {
+            attr = hipFuncSetAttribute((const void *)gram_lds_kernel<TB, TZ>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+         });
+         OFRR_HIP(ctx, attr);
+         gram_lds_kernel<TB, TZ><<<grid, kGramThreads, lds, ctx->stream>>>(n, k, la, rows, nchunks,
+                                                                          ngroups, part);
          OFRR_LAUNCH(ctx);
          return kSuccess;
       });
    });
    if (st) return st;
-   gram_multi_reduce_kernel<<<dim3(GT * GT / 32, args.nitems), 256, 0, ctx->stream>>>(nchunks, k, args,
-                                                                                       part);
+   gram_lds_reduce_kernel<<<dim3(GT * GT / 2 / 32, nh), 256, 0, ctx->stream>>>(nchunks, k, oa, part);
    OFRR_LAUNCH(ctx);
    return kSuccess;
 }

@@ -367,6 +367,227 @@ int ofrr_subspace_iter_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t 
    return kSuccess;
 }
 
+// ---- two-sided Rayleigh-Ritz for the SVD and its subspace-iteration driver
+// (svd_rr.m:54-91, subspace_iter_svd.m:57-77) over a dense FP64 A (M x N,
+// column-major).  A V and A^T U are tall products on the back-transform
+// kernel (A^T is A read with swapped strides), U^T A V, U^T U and V^T V are
+// Gram products, and the augmented pencil goes to the Cholesky-reduced FP64
+// eigensolver.
+
+}  // extern "C"
+
+namespace ofrr {
+
+// Ae = [0 B; B^T 0] and Me = blkdiag(Gu, Gv) (generalised) or I, K = ku + kv
+__global__ void svd_aug_kernel(int ku, int kv, const double *__restrict__ B,
+                               const double *__restrict__ Gu, const double *__restrict__ Gv,
+                               int generalized, double *__restrict__ Ae, double *__restrict__ Me) {
+   const int K = ku + kv;
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= K * K) return;
+   const int i = t % K, j = t / K;
+   double a = 0.0, m = 0.0;
+   if (i < ku && j >= ku) a = B[i + (int64_t)(j - ku) * ku];
+   else if (i >= ku && j < ku) a = B[j + (int64_t)(i - ku) * ku];
+   if (generalized) {
+      if (i < ku && j < ku) m = Gu[i + (int64_t)j * ku];
+      else if (i >= ku && j >= ku) m = Gv[(i - ku) + (int64_t)(j - ku) * kv];
+   } else {
+      m = (i == j) ? 1.0 : 0.0;
+   }
+   Ae[t] = a;
+   Me[t] = m;
+}
+
+// C(0:rows, 0:kk) = sqrt(2) Y(off:off+rows, 0:kk), Y column-major ld K
+__global__ void svd_coef_kernel(int rows, int kk, int off, int K, const double *__restrict__ Y,
+                                double *__restrict__ C) {
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= rows * kk) return;
+   const int i = t % rows, c = t / rows;
+   C[t] = 1.4142135623730951 * Y[(off + i) + (int64_t)c * K];
+}
+
+// the two-sided RR on device buffers: U (M x ku, ld M) and V (N x kv, ld N) are
+// replaced by their first kk = min(ku, kv) Ritz vectors; sigma (kk) on the device
+static int svd_rr_dev(ofrr_ctx *ctx, int64_t M, int64_t N, const double *A, int64_t lda, int ku,
+                      int kv, double *U, double *V, int generalized, double gtol, double *sigma,
+                      double *work_tall /* max(M, N) x max(ku, kv) */, double *small) {
+   const int K = ku + kv, kk = std::min(ku, kv);
+   int st;
+   double *B = small, *Gu = B + (size_t)ku * kv, *Gv = Gu + (size_t)ku * ku;
+   double *Ae = Gv + (size_t)kv * kv, *Me = Ae + (size_t)K * K, *Y = Me + (size_t)K * K;
+   double *Cu = Y + (size_t)K * K, *Cv = Cu + (size_t)ku * kk;
+   // AV = A V into work_tall (M x kv)
+   {
+      const void *vb[1] = {A};
+      int dt[1] = {OFRR_F64}, kb[1] = {(int)N}, ldc[1] = {(int)N};
+      int64_t rs[1] = {1}, cs[1] = {lda};
+      const double *cb[1] = {V};
+      if ((st = basis_update(ctx, M, kv, 1, vb, dt, kb, rs, cs, cb, ldc, work_tall, OFRR_F64, 1, M))) return st;
+   }
+   if ((st = gram(ctx, M, ku, kv, U, OFRR_F64, 1, M, work_tall, OFRR_F64, 1, M, 1.0, 0.0, B, ku))) return st;
+   if (generalized) {
+      if ((st = gram(ctx, M, ku, ku, U, OFRR_F64, 1, M, U, OFRR_F64, 1, M, 1.0, 0.0, Gu, ku))) return st;
+      if ((st = gram(ctx, N, kv, kv, V, OFRR_F64, 1, N, V, OFRR_F64, 1, N, 1.0, 0.0, Gv, kv))) return st;
+   }
+   svd_aug_kernel<<<(K * K + 255) / 256, 256, 0, ctx->stream>>>(ku, kv, B, Gu, Gv, generalized, Ae, Me);
+   OFRR_LAUNCH(ctx);
+   if ((st = rr_generalized(ctx, K, Me, K, Ae, K, generalized ? gtol : 0.0, sigma, Y, K))) return st;
+   svd_coef_kernel<<<(ku * kk + 255) / 256, 256, 0, ctx->stream>>>(ku, kk, 0, K, Y, Cu);
+   svd_coef_kernel<<<(kv * kk + 255) / 256, 256, 0, ctx->stream>>>(kv, kk, ku, K, Y, Cv);
+   OFRR_LAUNCH(ctx);
+   // U <- U Cu (through work_tall), V <- V Cv
+   {
+      const void *vb[1] = {U};
+      int dt[1] = {OFRR_F64}, kb[1] = {ku}, ldc[1] = {ku};
+      int64_t rs[1] = {1}, cs[1] = {M};
+      const double *cb[1] = {Cu};
+      if ((st = basis_update(ctx, M, kk, 1, vb, dt, kb, rs, cs, cb, ldc, work_tall, OFRR_F64, 1, M))) return st;
+      OFRR_HIP(ctx, hipMemcpyAsync(U, work_tall, (size_t)M * kk * 8, hipMemcpyDeviceToDevice, ctx->stream));
+   }
+   {
+      const void *vb[1] = {V};
+      int dt[1] = {OFRR_F64}, kb[1] = {kv}, ldc[1] = {kv};
+      int64_t rs[1] = {1}, cs[1] = {N};
+      const double *cb[1] = {Cv};
+      if ((st = basis_update(ctx, N, kk, 1, vb, dt, kb, rs, cs, cb, ldc, work_tall, OFRR_F64, 1, N))) return st;
+      OFRR_HIP(ctx, hipMemcpyAsync(V, work_tall, (size_t)N * kk * 8, hipMemcpyDeviceToDevice, ctx->stream));
+   }
+   return kSuccess;
+}
+
+static size_t svd_small_doubles(int ku, int kv) {
+   const int K = ku + kv, kk = std::min(ku, kv);
+   return (size_t)ku * kv + (size_t)ku * ku + (size_t)kv * kv + 3 * (size_t)K * K +
+          (size_t)(ku + kv) * kk + K;
+}
+
+// keep the columns of T (rows x kc, ld rows) whose skip flag is 0; returns the kept width
+static int compact_kept(ofrr_ctx *ctx, double *T, int64_t rows, int kc, const int32_t *skip_dev,
+                        int32_t *hskip, int *kept) {
+   OFRR_HIP(ctx, hipMemcpyAsync(hskip, skip_dev, (size_t)kc * 4, hipMemcpyDeviceToHost, ctx->stream));
+   OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+   int keep = 0;
+   for (int j = 0; j < kc; ++j)
+      if (!hskip[j]) {
+         if (keep != j)
+            OFRR_HIP(ctx, hipMemcpyAsync(T + (int64_t)keep * rows, T + (int64_t)j * rows, (size_t)rows * 8,
+                                         hipMemcpyDeviceToDevice, ctx->stream));
+         ++keep;
+      }
+   *kept = keep;
+   return kSuccess;
+}
+
+}  // namespace ofrr
+
+extern "C" {
+
+int ofrr_svd_rr(ofrr_ctx *ctx, int64_t M, int64_t N, const double *A, int64_t lda, int ku, int kv,
+                double *U, int64_t ldu, double *V, int64_t ldv, int generalized, double gtol,
+                double *sigma_host) {
+   OFRR_CHECK_CTX(ctx);
+   if (M <= 0 || N <= 0 || ku < 1 || kv < 1 || ku + kv > 256 || ku > M || kv > N || lda < M ||
+       ldu < M || ldv < N || !A || !U || !V || !sigma_host)
+      return set_error(ctx, kErrorInvalidArgument, "svd_rr: bad arguments");
+   const int kk = std::min(ku, kv);
+   const int64_t tall = std::max(M, N) * std::max(ku, kv);
+   size_t need = al((size_t)M * ku * 8) + al((size_t)N * kv * 8) + al((size_t)tall * 8) +
+                 al(svd_small_doubles(ku, kv) * 8) + al((size_t)(ku + kv) * 8);
+   int st = arena_reserve(ctx, need);
+   if (st) return st;
+   double *Ud = (double *)arena_take((size_t)M * ku * 8);
+   double *Vd = (double *)arena_take((size_t)N * kv * 8);
+   double *wt = (double *)arena_take((size_t)tall * 8);
+   double *small = (double *)arena_take(svd_small_doubles(ku, kv) * 8);
+   double *sig = (double *)arena_take((size_t)(ku + kv) * 8);
+   OFRR_HIP(ctx, hipMemcpy2DAsync(Ud, (size_t)M * 8, U, (size_t)ldu * 8, (size_t)M * 8, ku,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+   OFRR_HIP(ctx, hipMemcpy2DAsync(Vd, (size_t)N * 8, V, (size_t)ldv * 8, (size_t)N * 8, kv,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+   if ((st = svd_rr_dev(ctx, M, N, A, lda, ku, kv, Ud, Vd, generalized, gtol, sig, wt, small))) return st;
+   OFRR_HIP(ctx, hipMemcpy2DAsync(U, (size_t)ldu * 8, Ud, (size_t)M * 8, (size_t)M * 8, kk,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+   OFRR_HIP(ctx, hipMemcpy2DAsync(V, (size_t)ldv * 8, Vd, (size_t)N * 8, (size_t)N * 8, kk,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+   double *h = (double *)pinned(ctx, (size_t)kk * 8);
+   if (!h) return set_error(ctx, kErrorCUDAMemory, "svd_rr: pinned");
+   OFRR_HIP(ctx, hipMemcpyAsync(h, sig, (size_t)kk * 8, hipMemcpyDeviceToHost, ctx->stream));
+   OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+   for (int j = 0; j < kk; ++j) sigma_host[j] = h[j];
+   return kSuccess;
+}
+
+int ofrr_subspace_iter_svd(ofrr_ctx *ctx, int64_t M, int64_t N, const double *A, int64_t lda, int k,
+                           int maxiter, int step_size, double select_tol, int generalized,
+                           double gtol, const double *V0, int64_t ldv0, double *U, int64_t ldu,
+                           double *V, int64_t ldv, double *sigma_host, int *kept) {
+   OFRR_CHECK_CTX(ctx);
+   if (M <= 0 || N <= 0 || k < 1 || 2 * k > 256 || k > M || k > N || lda < M || ldv0 < N || ldu < M ||
+       ldv < N || step_size < 1 || maxiter < 0 || !A || !V0 || !U || !V || !sigma_host)
+      return set_error(ctx, kErrorInvalidArgument, "subspace_iter_svd: bad arguments");
+   const int64_t tall = std::max(M, N) * k;
+   size_t need = al((size_t)M * k * 8) + al((size_t)N * k * 8) + al((size_t)tall * 8) +
+                 al(svd_small_doubles(k, k) * 8) + al((size_t)2 * k * 8) + al((size_t)k * k * 8) +
+                 al((size_t)k * 8) + al((size_t)k * 4) * 2;
+   int st = arena_reserve(ctx, need);
+   if (st) return st;
+   double *Tu = (double *)arena_take((size_t)M * k * 8);
+   double *Tv = (double *)arena_take((size_t)N * k * 8);
+   double *wt = (double *)arena_take((size_t)tall * 8);
+   double *small = (double *)arena_take(svd_small_doubles(k, k) * 8);
+   double *sig = (double *)arena_take((size_t)2 * k * 8);
+   double *R = (double *)arena_take((size_t)k * k * 8);
+   double *piv = (double *)arena_take((size_t)k * 8);
+   int32_t *perm = (int32_t *)arena_take((size_t)k * 4);
+   int32_t *skip = (int32_t *)arena_take((size_t)k * 4);
+   int32_t *hskip = (int32_t *)pinned(ctx, (size_t)k * 4 + (size_t)k * 8);
+   if (!hskip) return set_error(ctx, kErrorCUDAMemory, "subspace_iter_svd: pinned");
+   double *hsig = (double *)(hskip + k);
+   OFRR_HIP(ctx, hipMemsetAsync(Tu, 0, (size_t)M * k * 8, ctx->stream));
+   OFRR_HIP(ctx, hipMemcpy2DAsync(Tv, (size_t)N * 8, V0, (size_t)ldv0 * 8, (size_t)N * 8, k,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+   if ((st = ofrr_scalecols(ctx, OFRR_F64, N, k, Tv, 1, N, piv))) return st;
+   int kc = k;
+   for (int it = 0; it < maxiter && kc > 0; ++it) {
+      for (int s = 0; s < step_size; ++s) {
+         // Tu = A Tv, Tv = A^T Tu (A^T: element (i, j) = A[j + i lda])
+         const void *va[1] = {A};
+         int dt[1] = {OFRR_F64}, kbn[1] = {(int)N}, kbm[1] = {(int)M}, ldn[1] = {(int)N}, ldm[1] = {(int)M};
+         int64_t rs1[1] = {1}, cs1[1] = {lda}, rs2[1] = {lda}, cs2[1] = {1};
+         const double *cv[1] = {Tv}, *cu[1] = {Tu};
+         if ((st = basis_update(ctx, M, kc, 1, va, dt, kbn, rs1, cs1, cv, ldn, Tu, OFRR_F64, 1, M))) return st;
+         if ((st = ofrr_scalecols(ctx, OFRR_F64, M, kc, Tu, 1, M, piv))) return st;
+         if ((st = basis_update(ctx, N, kc, 1, va, dt, kbm, rs2, cs2, cu, ldm, Tv, OFRR_F64, 1, N))) return st;
+         if ((st = ofrr_scalecols(ctx, OFRR_F64, N, kc, Tv, 1, N, piv))) return st;
+      }
+      int ku = 0, kv = 0;
+      if ((st = ofrr_hessenberg(ctx, OFRR_F64, M, kc, Tu, 1, M, select_tol, R, k, perm, skip))) return st;
+      if ((st = compact_kept(ctx, Tu, M, kc, skip, hskip, &ku))) return st;
+      if ((st = ofrr_hessenberg(ctx, OFRR_F64, N, kc, Tv, 1, N, select_tol, R, k, perm, skip))) return st;
+      if ((st = compact_kept(ctx, Tv, N, kc, skip, hskip, &kv))) return st;
+      if (ku == 0 || kv == 0) { kc = 0; break; }
+      if ((st = svd_rr_dev(ctx, M, N, A, lda, ku, kv, Tu, Tv, generalized, gtol, sig, wt, small))) return st;
+      kc = std::min(ku, kv);
+   }
+   if (kc > 0) {
+      OFRR_HIP(ctx, hipMemcpy2DAsync(U, (size_t)ldu * 8, Tu, (size_t)M * 8, (size_t)M * 8, kc,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+      OFRR_HIP(ctx, hipMemcpy2DAsync(V, (size_t)ldv * 8, Tv, (size_t)N * 8, (size_t)N * 8, kc,
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+      OFRR_HIP(ctx, hipMemcpyAsync(hsig, sig, (size_t)kc * 8, hipMemcpyDeviceToHost, ctx->stream));
+   }
+   for (int j = kc; j < k; ++j) {
+      OFRR_HIP(ctx, hipMemsetAsync(U + (int64_t)j * ldu, 0, (size_t)M * 8, ctx->stream));
+      OFRR_HIP(ctx, hipMemsetAsync(V + (int64_t)j * ldv, 0, (size_t)N * 8, ctx->stream));
+   }
+   OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+   for (int j = 0; j < k; ++j) sigma_host[j] = (j < kc) ? hsig[j] : NAN;
+   if (kept) *kept = kc;
+   return kc > 0 ? kSuccess : set_error(ctx, kErrorFactorizationFailed, "subspace_iter_svd: every column dropped");
+}
+
 }  // extern "C"
 
 namespace ofrr {

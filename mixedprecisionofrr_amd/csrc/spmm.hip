@@ -139,8 +139,8 @@ __device__ inline bool stage_tile(CsrTile<TV> &t, const int64_t *__restrict__ rp
 }
 
 // ---------------------------------------------------------------------------
-template <typename TV, typename TX, typename TY, int VEC, typename ACC, bool OVERREAD, typename CI,
-          typename VA>
+template <typename TV, typename TX, typename TY, int VEC, typename ACC, bool OVERREAD, bool YVEC,
+          typename CI, typename VA>
 __device__ inline void spmm_rows(int64_t r0, int nr, int grp, int groups, int lane_g, int G, int k,
                                  CI cs, VA vs, const int32_t *s_rp, const TX *__restrict__ X,
                                  int64_t ldx, double alpha, double beta, TY *__restrict__ Y,
@@ -154,17 +154,32 @@ __device__ inline void spmm_rows(int64_t r0, int nr, int grp, int groups, int la
          row_gather8<TV, TX, VEC, ACC, OVERREAD>(cs, vs, s_rp[lr], s_rp[lr + 1], X, ldx, row, c0,
                                                  acc);
          TY *y = Y + row * ldy + c0;
+         if (YVEC) {
+            // the lane's VEC outputs are contiguous and aligned: one vector store
+            VecT<TY, VEC> out;
+            if (beta != 0.0) {
+               const VecT<TY, VEC> yo = vload<TY, VEC>(y);
 #pragma unroll
-         for (int v = 0; v < VEC; ++v) {
-            double r = alpha * (double)acc[v];
-            if (beta != 0.0) r += beta * (double)Elem<TY>::load(y[v]);
-            y[v] = Elem<TY>::store(r);
+               for (int v = 0; v < VEC; ++v)
+                  out.v[v] = Elem<TY>::store(alpha * (double)acc[v] + beta * (double)Elem<TY>::load(yo.v[v]));
+            } else {
+#pragma unroll
+               for (int v = 0; v < VEC; ++v) out.v[v] = Elem<TY>::store(alpha * (double)acc[v]);
+            }
+            vstore<TY, VEC>(y, out);
+         } else {
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) {
+               double r = alpha * (double)acc[v];
+               if (beta != 0.0) r += beta * (double)Elem<TY>::load(y[v]);
+               y[v] = Elem<TY>::store(r);
+            }
          }
       }
    }
 }
 
-template <typename TV, typename TX, typename TY, int VEC>
+template <typename TV, typename TX, typename TY, int VEC, bool YVEC>
 __global__ __launch_bounds__(256) void spmm_rm_kernel(
     int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
     const TV *__restrict__ va, int k, const TX *__restrict__ X, int64_t ldx, double alpha,
@@ -182,11 +197,11 @@ __global__ __launch_bounds__(256) void spmm_rm_kernel(
    const bool in_lds = stage_tile(tile, rp, ci, va, r0, nr, base);
    __syncthreads();
    if (in_lds)
-      spmm_rows<TV, TX, TY, VEC, ACC, true>(r0, nr, grp, GROUPS, lane_g, G, k, tile.ci, tile.va,
-                                            tile.rp, X, ldx, alpha, beta, Y, ldy);
+      spmm_rows<TV, TX, TY, VEC, ACC, true, YVEC>(r0, nr, grp, GROUPS, lane_g, G, k, tile.ci,
+                                                  tile.va, tile.rp, X, ldx, alpha, beta, Y, ldy);
    else
-      spmm_rows<TV, TX, TY, VEC, ACC, false>(r0, nr, grp, GROUPS, lane_g, G, k, ci + base,
-                                             va + base, tile.rp, X, ldx, alpha, beta, Y, ldy);
+      spmm_rows<TV, TX, TY, VEC, ACC, false, YVEC>(r0, nr, grp, GROUPS, lane_g, G, k, ci + base,
+                                                   va + base, tile.rp, X, ldx, alpha, beta, Y, ldy);
 }
 
 // ---------------------------------------------------------------------------
@@ -430,8 +445,16 @@ static int launch_spmm(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_
       dim3 grid((unsigned)sched_grid(nblk, pt));
       int lg = 0;
       while ((1 << lg) < G) ++lg;
-      spmm_rm_kernel<TV, TX, TY, VEC><<<grid, 256, 0, ctx->stream>>>(
-          n, rp, ci, va, k, X, xrs, alpha, beta, Y, yrs, lg, nblk, pt);
+      // a lane's VEC outputs go out as one vector store when Y's rows keep
+      // that alignment
+      const size_t yb = sizeof(TY) * VEC;
+      const bool yvec = ((uintptr_t)Y % yb) == 0 && ((size_t)yrs * sizeof(TY)) % yb == 0;
+      if (yvec)
+         spmm_rm_kernel<TV, TX, TY, VEC, true><<<grid, 256, 0, ctx->stream>>>(
+             n, rp, ci, va, k, X, xrs, alpha, beta, Y, yrs, lg, nblk, pt);
+      else
+         spmm_rm_kernel<TV, TX, TY, VEC, false><<<grid, 256, 0, ctx->stream>>>(
+             n, rp, ci, va, k, X, xrs, alpha, beta, Y, yrs, lg, nblk, pt);
    } else {
       int64_t total = n * (int64_t)k;
       spmm_strided_kernel<TV, TX, TY><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(

@@ -640,6 +640,118 @@ void orc_subspace_iter_eig(int64_t n, const int64_t *row_ptr, const int32_t *col
    free(T); free(AV); free(R); free(G); free(H); free(Y); free(sel); free(perm);
 }
 
+/* Two-sided Rayleigh-Ritz for the SVD (MATLAB/+src/+rr/svd_rr.m:54-91) over a
+ * dense A (m x n, column-major): AV = A V, B = U^T AV (ku x kv); the augmented
+ * pencil eig([0 B; B^T 0], blkdiag(U^T U, V^T V) + gtol I) (the generalised form,
+ * svd_rr.m:57-76) or eig([0 B; B^T 0]) (the SVD of B, svd_rr.m:78); sorted
+ * descending; sigma = the top kk = min(ku, kv) eigenvalues and, with (Z; W)
+ * their eigenvectors, U <- sqrt(2) U Z, V <- sqrt(2) V W (svd_rr.m:85-91).
+ * U (m x ku) and V (n x kv) in; their first kk columns out.  Returns kk. */
+int orc_svd_rr(int64_t m, int64_t n, const double *A, int64_t lda, int ku, int kv, double *U,
+               int64_t ldu, double *V, int64_t ldv, int generalized, double gtol, double *sigma) {
+   const int K = ku + kv, kk = ku < kv ? ku : kv;
+   double *AV = (double *)malloc(sizeof(double) * (size_t)m * kv);
+   double *B = (double *)malloc(sizeof(double) * (size_t)ku * kv);
+   double *Ae = (double *)calloc((size_t)K * K, sizeof(double));
+   double *Me = (double *)calloc((size_t)K * K, sizeof(double));
+   double *th = (double *)malloc(sizeof(double) * (size_t)K);
+   double *Y = (double *)malloc(sizeof(double) * (size_t)K * K);
+   double *Un = (double *)malloc(sizeof(double) * (size_t)m * kk);
+   double *Vn = (double *)malloc(sizeof(double) * (size_t)n * kk);
+   orc_gemm_nn(m, kv, n, A, lda, V, ldv, AV, m);
+   orc_gemm_tn(m, ku, kv, U, ldu, AV, m, B, ku);
+   for (int j = 0; j < kv; ++j)
+      for (int i = 0; i < ku; ++i) {
+         Ae[i + (int64_t)(ku + j) * K] = B[i + (int64_t)j * ku];
+         Ae[(ku + j) + (int64_t)i * K] = B[i + (int64_t)j * ku];
+      }
+   if (generalized) {
+      double *G = (double *)malloc(sizeof(double) * (size_t)(ku > kv ? ku : kv) * (ku > kv ? ku : kv));
+      orc_gemm_tn(m, ku, ku, U, ldu, U, ldu, G, ku);
+      for (int j = 0; j < ku; ++j)
+         for (int i = 0; i < ku; ++i) Me[i + (int64_t)j * K] = G[i + (int64_t)j * ku];
+      orc_gemm_tn(n, kv, kv, V, ldv, V, ldv, G, kv);
+      for (int j = 0; j < kv; ++j)
+         for (int i = 0; i < kv; ++i) Me[(ku + i) + (int64_t)(ku + j) * K] = G[i + (int64_t)j * kv];
+      free(G);
+   } else {
+      for (int i = 0; i < K; ++i) Me[i + (int64_t)i * K] = 1.0;
+      gtol = 0.0;
+   }
+   const int info = orc_rr_generalized(K, Me, K, Ae, K, gtol, th, Y, K);
+   if (!info) {
+      const double r2 = sqrt(2.0);
+      /* Z = Y(0:ku, 0:kk), W = Y(ku:K, 0:kk) */
+      orc_gemm_nn(m, kk, ku, U, ldu, Y, K, Un, m);
+      orc_gemm_nn(n, kk, kv, V, ldv, Y + ku, K, Vn, n);
+      for (int c = 0; c < kk; ++c) {
+         sigma[c] = th[c];
+         for (int64_t i = 0; i < m; ++i) U[i + (int64_t)c * ldu] = r2 * Un[i + (int64_t)c * m];
+         for (int64_t i = 0; i < n; ++i) V[i + (int64_t)c * ldv] = r2 * Vn[i + (int64_t)c * n];
+      }
+   }
+   free(AV); free(B); free(Ae); free(Me); free(th); free(Y); free(Un); free(Vn);
+   return info ? -info : kk;
+}
+
+/* subspace_iter_svd.m:57-77 with function_orth = hessenberg (select_tol; the
+ * kept columns only) and function_rr = svd_rr (generalised with gtol, or not):
+ * V (n x k) start block, scaled by scalecols; each iteration step_size x (U = A V,
+ * scalecols; V = A^T U, scalecols), then P = hessenberg(U), Q = hessenberg(V),
+ * [U, S, V] = svd_rr(A, P, Q).  U (m x k), V (n x k) and sigma (k) out (sigma NaN
+ * and zero columns past the kept width).  Returns the kept width. */
+int orc_subspace_iter_svd(int64_t m, int64_t n, const double *A, int64_t lda, int k, int maxiter,
+                          int step_size, double select_tol, int generalized, double gtol,
+                          const double *V0, double *U, int64_t ldu, double *V, int64_t ldv,
+                          double *sigma) {
+   double *Tu = (double *)calloc((size_t)m * k, sizeof(double));
+   double *Tv = (double *)malloc(sizeof(double) * (size_t)n * k);
+   double *R = (double *)malloc(sizeof(double) * (size_t)k * k);
+   double *At = (double *)malloc(sizeof(double) * (size_t)n * m);
+   int *sel = (int *)malloc(sizeof(int) * (size_t)k), *perm = (int *)malloc(sizeof(int) * (size_t)k);
+   for (int64_t j = 0; j < m; ++j)
+      for (int64_t i = 0; i < n; ++i) At[i + j * n] = A[j + i * lda];
+   for (int j = 0; j < k; ++j) memcpy(Tv + (int64_t)j * n, V0 + (int64_t)j * n, sizeof(double) * (size_t)n);
+   orc_scalecols(n, k, Tv, n);
+   int kc = k;
+   for (int it = 0; it < maxiter && kc > 0; ++it) {
+      for (int s = 0; s < step_size; ++s) {
+         orc_gemm_nn(m, kc, n, A, lda, Tv, n, Tu, m);
+         orc_scalecols(m, kc, Tu, m);
+         orc_gemm_nn(n, kc, m, At, n, Tu, m, Tv, n);
+         orc_scalecols(n, kc, Tv, n);
+      }
+      int ku = 0, kv = 0;
+      orc_hessenberg(m, kc, Tu, m, R, k, select_tol, 0, sel, perm);
+      for (int j = 0; j < kc; ++j)
+         if (sel[j]) {
+            if (ku != j) memcpy(Tu + (int64_t)ku * m, Tu + (int64_t)j * m, sizeof(double) * (size_t)m);
+            ++ku;
+         }
+      orc_hessenberg(n, kc, Tv, n, R, k, select_tol, 0, sel, perm);
+      for (int j = 0; j < kc; ++j)
+         if (sel[j]) {
+            if (kv != j) memcpy(Tv + (int64_t)kv * n, Tv + (int64_t)j * n, sizeof(double) * (size_t)n);
+            ++kv;
+         }
+      if (ku == 0 || kv == 0) { kc = 0; break; }
+      kc = orc_svd_rr(m, n, A, lda, ku, kv, Tu, m, Tv, n, generalized, gtol, sigma);
+      if (kc < 0) { kc = 0; break; }
+   }
+   for (int j = 0; j < k; ++j) {
+      if (j < kc) {
+         memcpy(U + (int64_t)j * ldu, Tu + (int64_t)j * m, sizeof(double) * (size_t)m);
+         memcpy(V + (int64_t)j * ldv, Tv + (int64_t)j * n, sizeof(double) * (size_t)n);
+      } else {
+         sigma[j] = NAN;
+         memset(U + (int64_t)j * ldu, 0, sizeof(double) * (size_t)m);
+         memset(V + (int64_t)j * ldv, 0, sizeof(double) * (size_t)n);
+      }
+   }
+   free(Tu); free(Tv); free(R); free(At); free(sel); free(perm);
+   return kc;
+}
+
 void orc_residuals(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *vals,
                    int k, const double *X, int64_t ldx, const double *theta, double normA,
                    double *res) {

@@ -94,6 +94,17 @@ void orc_subspace_iter_eig(int64_t n, const int64_t *row_ptr, const int32_t *col
                            const double *vals, int m, int maxiter, int step_size,
                            double select_tol, double gtol, double *V, int64_t ldv, double *theta);
 
+/* svd_rr.m:54-91 over a dense column-major A (m x n): the augmented pencil
+ * (generalised: blkdiag(U^T U, V^T V) + gtol I, else I); sigma (kk), U, V in/out
+ * (first kk = min(ku, kv) columns written, scaled by sqrt 2).  Returns kk. */
+int orc_svd_rr(int64_t m, int64_t n, const double *A, int64_t lda, int ku, int kv, double *U,
+               int64_t ldu, double *V, int64_t ldv, int generalized, double gtol, double *sigma);
+/* subspace_iter_svd.m:57-77 with hessenberg orth and svd_rr; returns the kept width */
+int orc_subspace_iter_svd(int64_t m, int64_t n, const double *A, int64_t lda, int k, int maxiter,
+                          int step_size, double select_tol, int generalized, double gtol,
+                          const double *V0, double *U, int64_t ldu, double *V, int64_t ldv,
+                          double *sigma);
+
 /* Filtered OFRR (block subspace iteration with a Chebyshev filter on the
  * residual correction, FP64 throughout) -- the algorithm the FP64 GPU path runs.
  * X (n x k) start block in / Ritz block out; theta (k) out descending;

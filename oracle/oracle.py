@@ -67,6 +67,11 @@ def lib():
                                         C.c_int64, _dp, _dp]
         L.orc_residuals.argtypes = [C.c_int64, _ip64, _ip32, _dp, C.c_int, _dp, C.c_int64, _dp,
                                     C.c_double, _dp]
+        L.orc_svd_rr.argtypes = [C.c_int64, C.c_int64, _dp, C.c_int64, C.c_int, C.c_int, _dp,
+                                 C.c_int64, _dp, C.c_int64, C.c_int, C.c_double, _dp]
+        L.orc_subspace_iter_svd.argtypes = [C.c_int64, C.c_int64, _dp, C.c_int64, C.c_int, C.c_int,
+                                            C.c_int, C.c_double, C.c_int, C.c_double, _dp, _dp,
+                                            C.c_int64, _dp, C.c_int64, _dp]
         _lib = L
     return _lib
 
@@ -225,6 +230,32 @@ def subspace_iter_eig(A, V0, maxiter, step_size, select_tol=2.0 ** -52, gtol=0.0
     lib().orc_subspace_iter_eig(A.n, A.row_ptr, A.col_idx, A.vals, m, maxiter, step_size,
                                 select_tol, gtol, V, A.n, th)
     return th, V
+
+
+def svd_rr(A, U, V, generalized=True, gtol=0.0):
+    """svd_rr.m over a dense A: (sigma (kk), U (m x kk), V (n x kk)), kk = min(ku, kv)."""
+    A = _f(A)
+    U = _f(U).copy(order="F")
+    V = _f(V).copy(order="F")
+    m, n = A.shape
+    ku, kv = U.shape[1], V.shape[1]
+    sig = np.zeros(min(ku, kv))
+    kk = lib().orc_svd_rr(m, n, A, m, ku, kv, U, m, V, n, int(generalized), gtol, sig)
+    return sig[:kk], U[:, :kk], V[:, :kk]
+
+
+def subspace_iter_svd(A, V0, maxiter, step_size=1, select_tol=2.0 ** -52, generalized=True,
+                      gtol=0.0):
+    """subspace_iter_svd.m with hessenberg + svd_rr: (sigma (k), U, V, kept width)."""
+    A = _f(A)
+    m, n = A.shape
+    k = V0.shape[1]
+    U = np.zeros((m, k), order="F")
+    V = np.zeros((n, k), order="F")
+    sig = np.zeros(k)
+    kc = lib().orc_subspace_iter_svd(m, n, A, m, k, maxiter, step_size, select_tol,
+                                     int(generalized), gtol, _f(V0), U, m, V, n, sig)
+    return sig, U, V, kc
 
 
 def filtered_ofrr(A, X0, nwant, nd=1, degree=20, maxiter=50, tol=1e-8, tau=1e-12, lower=0.0,

@@ -352,3 +352,15 @@ def test_residuals_vs_oracle(ctx):
     ref = O.residuals(Ah, X, th, lam[-1])
     got = api.residuals(ctx, _csr(ctx, Ah), _dev(X.T).t(), th, lam[-1])
     assert np.allclose(got, ref, rtol=1e-12, atol=0)
+
+
+def test_fp8_hardware_conversion_agrees(ctx):
+    """The device FP8 conversions (gfx950's v_cvt_f32_fp8 / v_cvt_pk_fp8_f32,
+    clamped to +-448) against the OCP E4M3 software definition: every code
+    decodes identically and every float encodes identically (round to nearest
+    even, saturating)."""
+    import ctypes
+    counts = (ctypes.c_int64 * 3)()
+    ctx.check(ctx.lib.ofrr_fp8_check(ctx.h, counts), "fp8_check")
+    print("fp8 mismatches (decode, encode <= 448, encode > 448):", list(counts), flush=True)
+    assert counts[0] == 0 and counts[1] == 0 and counts[2] == 0
