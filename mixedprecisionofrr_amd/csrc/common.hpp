@@ -244,7 +244,10 @@ __device__ inline float wave_max(float v) {
    return v;
 }
 
-// non-negative float max through integer atomics (order independent)
+// non-negative float max through integer atomics (order independent).  Issued
+// without reading the value first: a coherent read makes the workgroup wait for
+// an L2 round trip before it retires (measured 245 -> 291 us per FP16 filter
+// step), while the atomic itself is posted.
 __device__ inline void atomic_max_nonneg(float *addr, float v) {
    atomicMax(reinterpret_cast<unsigned int *>(addr), __float_as_uint(v));
 }

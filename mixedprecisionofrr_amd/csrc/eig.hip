@@ -246,21 +246,25 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
       rk[i] = rr;
    }
 }
-// V = J_1 J_2 ... (all logged rotations) applied to the identity.  Rows of V
-// are independent: one wave owns one row in LDS and applies every round in
-// order (the pairs of a round are disjoint).  The log is staged through LDS
-// kApplyChunk rounds at a time, so one global-memory latency covers many rounds,
-// and each lane steps its pair (p, q) from round to round instead of recomputing
-// it: in the circle schedule p = r + s and q = r - s (mod nn - 1) both advance
-// by one per round (pair 0 keeps p = nn - 1 and has q = r).  The columns are written to their
-// ascending-eigenvalue positions.
+// V = J_1 J_2 ... (all logged rotations) applied to the identity, one row of V
+// per wave.  The row lives in registers, arranged like the pairs of the circle
+// schedule: lane l holds the row's entries at the two indices of pair slot l
+// (and slot l + 64), its a-side (r + s mod M; M for s = 0) and b-side
+// (r - s mod M).  A round rotates each slot's two entries, then the indices move
+// to the next round's slots: every a-side entry one slot down, every b-side
+// entry one slot up, with a[1] -> b[0] and b[np-1] -> a[np-1] at the ends and
+// a[0] (index M) fixed.  The arithmetic per entry is the one of the rotation
+// kernel's logged (c, s) on sorted (p, q), in round order, as before.  The log
+// is staged through LDS kApplyChunk rounds at a time, the next chunk loaded into
+// registers while the current one is applied.
 constexpr int kApplyChunk = 16;
+constexpr int kApplyPer = kApplyChunk * 128 / 64;  // log entries per lane and chunk
 __global__ __launch_bounds__(64) void jacobi_apply_kernel(
     int n_max, const int32_t *__restrict__ n_dev, const double2 *__restrict__ rotlog,
     int64_t strideLog, const int32_t *__restrict__ rank, const int32_t *__restrict__ nsweeps,
     int64_t strideInts, double *__restrict__ V, int ldv, int64_t strideV) {
-   __shared__ double row[256];
    __shared__ double2 lg[kApplyChunk * 128];
+   __shared__ double row[256];
    const int b = blockIdx.y;
    const int n = n_dev ? min((int)n_dev[b], n_max) : n_max;
    const int r0 = blockIdx.x;
@@ -269,44 +273,75 @@ __global__ __launch_bounds__(64) void jacobi_apply_kernel(
    const int rounds = nsweeps[b * strideInts] * M;
    const double2 *log = rotlog + b * strideLog;
    const int lane = threadIdx.x;
-   for (int c = lane; c < nn; c += 64) row[c] = (r0 == c) ? 1.0 : 0.0;
-   // pair counters of s = lane and s = lane + 64 at round 0
-   int pa = lane == 0 ? M : lane, qa = lane == 0 ? 0 : M - lane;
-   int pb = lane + 64, qb = M - (lane + 64);
-   const bool hasA = lane < np, hasB = lane + 64 < np;
-   for (int c0 = 0; c0 < rounds; c0 += kApplyChunk) {
+   const int sA = lane, sB = lane + 64;  // the lane's two slots
+   const bool hasA = sA < np, hasB = sB < np;
+   // round 0: slot 0 = (M, 0), slot s = (s, M - s)
+   int ia = (sA == 0) ? M : sA, ib = (sA == 0) ? 0 : M - sA;   // slot sA indices (a, b)
+   int ja = sB, jb = M - sB;                                     // slot sB indices (a, b)
+   double a0 = (hasA && ia == r0) ? 1.0 : 0.0, b0 = (hasA && ib == r0) ? 1.0 : 0.0;
+   double a1 = (hasB && ja == r0) ? 1.0 : 0.0, b1 = (hasB && jb == r0) ? 1.0 : 0.0;
+   double2 pre[kApplyPer];
+   auto fetch = [&](int c0) {
       const int nr = min(kApplyChunk, rounds - c0);
       const double2 *src = log + (int64_t)c0 * np;
-      __syncthreads();  // the previous chunk is consumed
-      for (int i = lane; i < nr * np; i += 64) lg[i] = src[i];
+#pragma unroll
+      for (int u = 0; u < kApplyPer; ++u) {
+         const int i = lane + 64 * u;
+         pre[u] = (i < nr * np) ? src[i] : make_double2(1.0, 0.0);
+      }
+   };
+   auto rotate = [](double2 cs, int ia_, int ib_, double &va, double &vb) {
+      if (cs.y == 0.0) return;
+      // (p, q) = the sorted indices; row[p] = c vp - s vq, row[q] = s vp + c vq
+      const bool aIsP = ia_ < ib_;
+      const double vp = aIsP ? va : vb, vq = aIsP ? vb : va;
+      const double np_ = fma(cs.x, vp, -(cs.y * vq));
+      const double nq_ = fma(cs.y, vp, cs.x * vq);
+      va = aIsP ? np_ : nq_;
+      vb = aIsP ? nq_ : np_;
+   };
+   if (rounds > 0) fetch(0);
+   for (int c0 = 0; c0 < rounds; c0 += kApplyChunk) {
+      const int nr = min(kApplyChunk, rounds - c0);
+#pragma unroll
+      for (int u = 0; u < kApplyPer; ++u) lg[lane + 64 * u] = pre[u];
       __syncthreads();
+      if (c0 + kApplyChunk < rounds) fetch(c0 + kApplyChunk);  // in flight during this chunk
       for (int k = 0; k < nr; ++k) {
          const double2 *lr = lg + k * np;
-         if (hasA) {
-            const double2 cs = lr[lane];
-            if (cs.y != 0.0) {
-               const int p = min(pa, qa), q = max(pa, qa);
-               const double vp = row[p], vq = row[q];
-               row[p] = fma(cs.x, vp, -(cs.y * vq));
-               row[q] = fma(cs.y, vp, cs.x * vq);
-            }
+         if (hasA) rotate(lr[sA], ia, ib, a0, b0);
+         if (hasB) rotate(lr[sB], ja, jb, a1, b1);
+         // move to the next round's slots
+         const double a0n = __shfl(a0, lane + 1, 64);       // a[s + 1], same set
+         const double a1n = __shfl(a1, lane + 1, 64);
+         const double a1f = __shfl(a1, 0, 64);              // a[64] for slot 63
+         const double b0p = __shfl(b0, lane - 1, 64);       // b[s - 1], same set
+         const double b1p = __shfl(b1, lane - 1, 64);
+         const double b0l = __shfl(b0, 63, 64);             // b[63] for slot 64
+         const double a01 = __shfl(a0, 1, 64);              // a[1] for slot 0
+         // slot sA
+         double na0, nb0;
+         if (sA == 0) { na0 = a0; nb0 = (np > 1) ? a01 : b0; }  // one slot: nothing moves
+         else {
+            na0 = (sA == np - 1) ? b0 : ((sA == 63) ? a1f : a0n);
+            nb0 = b0p;
          }
-         if (hasB) {
-            const double2 cs = lr[lane + 64];
-            if (cs.y != 0.0) {
-               const int p = min(pb, qb), q = max(pb, qb);
-               const double vp = row[p], vq = row[q];
-               row[p] = fma(cs.x, vp, -(cs.y * vq));
-               row[q] = fma(cs.y, vp, cs.x * vq);
-            }
-         }
-         if (lane != 0) pa = (pa + 1 == M) ? 0 : pa + 1;
-         qa = (qa + 1 == M) ? 0 : qa + 1;
-         pb = (pb + 1 == M) ? 0 : pb + 1;
-         qb = (qb + 1 == M) ? 0 : qb + 1;
-         __syncthreads();  // one wave: orders this round's LDS updates before the next
+         // slot sB (>= 64)
+         const double na1 = (sB == np - 1) ? b1 : a1n;
+         const double nb1 = (sB == 64) ? b0l : b1p;
+         a0 = na0; b0 = nb0; a1 = na1; b1 = nb1;
+         // indices of the next round
+         if (sA == 0) { ib = (ib + 1 >= M) ? 0 : ib + 1; }
+         else { ia = (ia + 1 == M) ? 0 : ia + 1; ib = (ib + 1 == M) ? 0 : ib + 1; }
+         ja = (ja + 1 == M) ? 0 : ja + 1;
+         jb = (jb + 1 == M) ? 0 : jb + 1;
       }
+      __syncthreads();  // the chunk is consumed before the next one is stored
    }
+   // scatter the row back to index order through LDS, then to its ranked columns
+   if (hasA) { row[ia] = a0; row[ib] = b0; }
+   if (hasB) { row[ja] = a1; row[jb] = b1; }
+   __syncthreads();
    double *Vb = V + b * strideV;
    const int32_t *rk = rank + b * strideInts;
    for (int c = lane; c < n; c += 64) Vb[r0 + (int64_t)rk[c] * ldv] = row[c];
