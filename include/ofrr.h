@@ -193,6 +193,12 @@ typedef struct {
    double tau;       /* Gram truncation tolerance */
    double lower;     /* lower bound of the spectrum (damped interval start) */
    double normA;     /* ||A||_2, 0: use the largest Ritz value */
+   int refine;       /* CSR operator only.  1: precision ladder -- once the wanted
+                        residuals fall below refine_tol or have not halved in three
+                        iterations, the growth blocks continue in refine_dtype (an
+                        FP64 refinement also uses the FP64 operator values); 0: off */
+   int refine_dtype; /* ofrr_dtype of the refinement format */
+   double refine_tol;
 } ofrr_filter_opts;
 
 typedef struct {

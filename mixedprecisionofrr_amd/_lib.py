@@ -35,7 +35,8 @@ PENDING = 1
 class FilterOpts(C.Structure):
     _fields_ = [("k", C.c_int), ("nwant", C.c_int), ("nd", C.c_int), ("degree", C.c_int),
                 ("maxiter", C.c_int), ("basis_dtype", C.c_int), ("tol", C.c_double),
-                ("tau", C.c_double), ("lower", C.c_double), ("normA", C.c_double)]
+                ("tau", C.c_double), ("lower", C.c_double), ("normA", C.c_double),
+                ("refine", C.c_int), ("refine_dtype", C.c_int), ("refine_tol", C.c_double)]
 
 
 class FilterStats(C.Structure):

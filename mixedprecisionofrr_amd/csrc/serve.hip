@@ -75,6 +75,10 @@ int plan_request(const ofrr_eig_request &r, Plan &p) {
    o.nd = p.nd;
    o.degree = p.degree;
    o.basis_dtype = r.basis_dtype;
+   if (r.basis_dtype == OFRR_FP8) {  // the FP16 refinement of run_one
+      o.refine = 1;
+      o.refine_dtype = OFRR_F16;
+   }
    int64_t sb = 0;
    if (ofrr_filtered_eig_state_bytes(p.n, &o, &sb) != kSuccess) return kErrorInvalidArgument;
    p.state_bytes = (size_t)sb;
@@ -172,6 +176,13 @@ void ofrr_server::run_one(ofrr_ctx *ctx, Rec &r) {
       o.tau = 1e-12;
       o.lower = 0.0;  // the diffusion operators are positive definite
       o.normA = 0.0;  // ||A||_2 = the largest Ritz value
+      if (q.basis_dtype == OFRR_FP8) {
+         // FP8 corrections gain at most ~2^4 per sweep: finish in FP16 once the
+         // wanted residuals are below 1e-4 or stall
+         o.refine = 1;
+         o.refine_dtype = OFRR_F16;
+         o.refine_tol = 1e-4;
+      }
       std::vector<double> th(p.k), rs(p.k);
       ofrr_filter_stats stats{};
       ctx->state_mem = mem + p.off_state;

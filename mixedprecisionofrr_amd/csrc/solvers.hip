@@ -614,6 +614,10 @@ struct FilterOp {
    // Z (row stride zrs) = A D in the growth arithmetic
    virtual int apply_lo(ofrr_ctx *ctx, int k, const void *D, int bdt, void *Z, int zdt,
                         int64_t zrs) = 0;
+   // whether the growth products may change format mid-solve (the precision
+   // ladder), and the switch to the FP64 operator values for an FP64 format
+   virtual bool can_refine() const { return false; }
+   virtual void set_precise(bool) {}
 };
 
 struct CsrOp final : FilterOp {
@@ -643,13 +647,18 @@ struct CsrOp final : FilterOp {
              const double *inv_sR, float *colmax, double *sums) override {
       return spmm_resid(ctx, n, rp, ci, va, k, X, th, W, Rq, bdt, inv_sR, colmax, sums, bw);
    }
+   bool precise = false;  // growth products on the FP64 values
    int cheb(ofrr_ctx *ctx, int k, const void *Dc, const void *Dp, const void *R, int bdt,
             const ChebArgs &args, void *Dn) override {
-      return cheb_step(ctx, n, rp, ci, vlo, vlo_dt, k, Dc, Dp, R, bdt, args, Dn, bw);
+      return cheb_step(ctx, n, rp, ci, precise ? (const void *)va : vlo, precise ? OFRR_F64 : vlo_dt,
+                       k, Dc, Dp, R, bdt, args, Dn, bw);
    }
    int apply_lo(ofrr_ctx *ctx, int k, const void *D, int bdt, void *Z, int zdt, int64_t zrs) override {
-      return spmm_csr(ctx, n, rp, ci, vlo, vlo_dt, k, D, bdt, k, 1, 1.0, 0.0, Z, zdt, zrs, 1, bw);
+      return spmm_csr(ctx, n, rp, ci, precise ? (const void *)va : vlo, precise ? OFRR_F64 : vlo_dt,
+                      k, D, bdt, k, 1, 1.0, 0.0, Z, zdt, zrs, 1, bw);
    }
+   bool can_refine() const override { return true; }
+   void set_precise(bool p) override { precise = p; }
 };
 
 // Y(i, j) *= s[j] for a row-major m x k block
@@ -781,8 +790,10 @@ struct AtaOp final : FilterOp {
 // products Z (n x k nd), and the small m x m / per-column arrays
 static size_t filtered_state_bytes(int64_t n, const ofrr_filter_opts &o) {
    const int k = o.k, nd = o.nd, d = o.degree, m = k * (1 + nd);
-   const int bdt = o.basis_dtype;
-   const size_t bsz = dtype_size(bdt), zsz = dtype_size(bdt == OFRR_F64 ? OFRR_F64 : OFRR_F32);
+   const int bdt = o.basis_dtype, rdt = o.refine ? o.refine_dtype : bdt;
+   // blocks sized for the larger of the growth format and the refinement format
+   const size_t bsz = std::max(dtype_size(bdt), dtype_size(rdt));
+   const size_t zsz = (bdt == OFRR_F64 || rdt == OFRR_F64) ? 8 : 4;
    return al((size_t)n * k * 8) * 3 + al((size_t)n * k * bsz) * 5 + al((size_t)n * k * nd * zsz) +
           al((size_t)m * m * 8) * 2 + al((size_t)m * k * 8) + al((size_t)k * 8) * 14 +
           al((size_t)d * k * 8) + al((size_t)k * 4) * 4 + 4096;
@@ -791,16 +802,24 @@ static size_t filtered_state_bytes(int64_t n, const ofrr_filter_opts &o) {
 static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o, double *Xio,
                          double *theta_host, double *res_host, ofrr_filter_stats *stats) {
    const int64_t n = op.n;
-   const int k = o->k, nd = o->nd, dmax = o->degree, bdt = o->basis_dtype;
-   const int d = dmax;  // buffer sizing: the largest degree of any sweep
+   const int k = o->k, nd = o->nd, dmax = o->degree;
+   int bdt = o->basis_dtype;  // the growth format of the current iteration
+   const int d = dmax;        // buffer sizing: the largest degree of any sweep
    if (k < 2 || k > 128 || (k & (k - 1)) || nd < 1 || nd > 2 || d < 2 || o->nwant < 1 ||
        o->nwant > k || !dtype_valid(bdt) || (int64_t)k > n)
       return set_error(ctx, kErrorInvalidArgument,
                        "filtered_eig: need k a power of two in [2,128], nd in {1,2}, degree >= 2, 1 <= nwant <= k");
+   if (o->refine && (!dtype_valid(o->refine_dtype) || !op.can_refine()))
+      return set_error(ctx, kErrorInvalidArgument,
+                       "filtered_eig: the refinement format is valid for a CSR operator only");
    const int m = k * (1 + nd);
    if (m > 256) return set_error(ctx, kErrorInvalidArgument, "filtered_eig: basis dimension > 256");
-   const int zdt = (bdt == OFRR_F64) ? OFRR_F64 : OFRR_F32;
-   const size_t bsz = dtype_size(bdt), zsz = dtype_size(zdt);
+   int zdt = (bdt == OFRR_F64) ? OFRR_F64 : OFRR_F32;
+   size_t zsz = dtype_size(zdt);
+   // buffers hold the larger of the growth and refinement formats
+   const int rdt0 = o->refine ? o->refine_dtype : bdt;
+   const size_t bsz = std::max(dtype_size(bdt), dtype_size(rdt0));
+   const size_t zsz_max = (bdt == OFRR_F64 || rdt0 == OFRR_F64) ? 8 : 4;
    // ---- buffers
    int st = arena_reserve(ctx, filtered_state_bytes(n, *o));
    if (st) return st;
@@ -810,7 +829,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    void *Rq = arena_take((size_t)n * k * bsz);
    void *Db[4];
    for (int i = 0; i < 4; ++i) Db[i] = arena_take((size_t)n * k * bsz);
-   void *Z = arena_take((size_t)n * k * nd * zsz);
+   void *Z = arena_take((size_t)n * k * nd * zsz_max);
    double *G = (double *)arena_take((size_t)m * m * 8);
    double *H = (double *)arena_take((size_t)m * m * 8);
    double *C = (double *)arena_take((size_t)m * k * 8);
@@ -875,15 +894,27 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    std::vector<float> m1(k);
    // quantisation targets: the stored maximum of R sits near 2^tgt_resid and
    // the filter blocks near 2^tgt_step (FP8 E4M3 uses the top of its range)
-   const bool scaled = (bdt == OFRR_F16 || bdt == OFRR_FP8);
-   const int tgt_resid = (bdt == OFRR_FP8) ? 8 : 0;
-   const int tgt_step = (bdt == OFRR_FP8) ? 6 : 0;
-   const double lo_ratio = (bdt == OFRR_FP8) ? 16.0 : 1.0 / 256.0;
+   bool scaled = false;
+   int tgt_resid = 0, tgt_step = 0;
+   double lo_ratio = 1.0;
+   auto set_format = [&](int dt) {
+      bdt = dt;
+      zdt = (bdt == OFRR_F64) ? OFRR_F64 : OFRR_F32;
+      zsz = dtype_size(zdt);
+      scaled = (bdt == OFRR_F16 || bdt == OFRR_FP8);
+      tgt_resid = (bdt == OFRR_FP8) ? 8 : 0;
+      tgt_step = (bdt == OFRR_FP8) ? 6 : 0;
+      lo_ratio = (bdt == OFRR_FP8) ? 16.0 : 1.0 / 256.0;
+   };
+   set_format(bdt);
+   // precision ladder: the largest wanted residual of each iteration
+   std::vector<double> worst_hist;
    for (int j = 0; j < k; ++j) h_sR[j] = 1.0;
    for (int j = 0; j < k; ++j) theta[j] = h_th[j];
    for (it = 0;; ++it) {
       // ---- FP64 operator application + residual, quantised R
       OFRR_HIP(ctx, rec(1));
+   resid_pass:
       for (int pass = 0; pass < 2; ++pass) {
          for (int j = 0; j < k; ++j) inv[j] = 1.0 / h_sR[j];
          OFRR_HIP(ctx, hipMemcpyAsync(invsR, inv.data(), (size_t)k * 8, hipMemcpyHostToDevice, ctx->stream));
@@ -913,9 +944,25 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       OFRR_HIP(ctx, hipEventSynchronize(ev[2]));
       const double normA = o->normA > 0 ? o->normA : theta[0];
       nconv = 0;
+      double worst = 0.0;
       for (int j = 0; j < k; ++j) {
          res[j] = std::sqrt(h_sums[2 * j]) / (normA * std::sqrt(h_sums[2 * j + 1]));
          if (j < o->nwant && res[j] <= o->tol) ++nconv;
+         if (j < o->nwant) worst = std::max(worst, res[j]);
+      }
+      // precision ladder: continue in the refinement format once the wanted
+      // residuals are below refine_tol or have not halved in three iterations;
+      // R is then formed again in the new format
+      if (o->refine && bdt != o->refine_dtype && nconv < o->nwant) {
+         worst_hist.push_back(worst);
+         const size_t h = worst_hist.size();
+         const bool stalled = h >= 4 && worst > 0.5 * worst_hist[h - 4];
+         if (worst <= o->refine_tol || stalled) {
+            set_format(o->refine_dtype);
+            op.set_precise(bdt == OFRR_F64);
+            for (int j = 0; j < k; ++j) h_sR[j] = 1.0;
+            goto resid_pass;
+         }
       }
       {
          float ms_;

@@ -163,3 +163,24 @@ def test_filtered_same_alone_and_repeated(ctx):
     r2 = api.filtered_eig(ctx, A, X0, 16, nd=1, degree=12, maxiter=40, basis_dtype=torch.float16)
     assert r1[0] == 0
     assert np.array_equal(r1[1], r2[1]) and torch.equal(r1[2], r2[2])
+
+
+@pytest.mark.parametrize("refine", [torch.float16, torch.float64])
+def test_filtered_fp8_precision_ladder_reaches_1e8(ctx, refine):
+    """FP8 E4M3 growth blocks with the precision ladder: the solve switches to
+    the refinement format (FP16, or FP64 with the FP64 operator values) and
+    reaches residual 1e-8 with eigenvalues equal to the closed form."""
+    from mixedprecisionofrr_amd import api
+    N, k, nwant = 40, 32, 20
+    A = api.CSRMatrix.stencil(ctx, 3, N).with_low_precision(torch.float16)
+    X0 = torch.empty(A.n, k, dtype=torch.float64, device="cuda:0")
+    api.fill_randn(ctx, X0, seed=9)
+    print(f"fp8 ladder -> {refine}", flush=True)
+    st, th, X, res, stats = api.filtered_eig(ctx, A, X0, nwant, nd=2, degree=60, maxiter=200,
+                                             basis_dtype=torch.float8_e4m3fn, tol=1e-8,
+                                             refine_dtype=refine, refine_tol=1e-4)
+    assert st == 0, (st, stats)
+    exact = O.laplacian_eigs(3, N)[:nwant]
+    assert np.max(np.abs(th[:nwant] - exact) / exact) < 1e-11
+    res_o = O.residuals(O.stencil(3, N), X.cpu().numpy(), th, exact[0])
+    assert np.max(res_o[:nwant]) < 2e-8
