@@ -107,6 +107,46 @@ OFRR_API int ofrr_hessenberg(ofrr_ctx *ctx, int q_dtype, int64_t m, int n, void 
                              int64_t rs, int64_t cs, double select_tol, double *R_dev, int ldr,
                              int32_t *perm_dev, int32_t *skip_dev);
 
+/* ---- Gram-Schmidt QR, the orthogonalizing baselines (CPP/linalg/
+ * factorization/qr.hpp:9-141, MATLAB/+src/+qr/mgs.m, cgs.m).  In place on Q
+ * (m x n, m >= n, dtype q_dtype: FP64, FP32, FP16, BF16); dot products and
+ * norms accumulate in FP64.  R (n x n FP64, column-major, device) receives the
+ * triangular factor, skip_dev 1 for dropped columns.  A dropped column keeps its
+ * unnormalized residual in Q and is left out of later projections.
+ *   OFRR_QR_MGS     mgs (qr.hpp:9-28, qr.cu:12-249): sequential projections,
+ *                   reorthogonalized while orth_tol <= ||v|| < reorth_tol *
+ *                   ||v before the pass|| (reorth_tol <= 0: off); dropped when
+ *                   the initial or final norm is below select_tol, R(i,i) = 0;
+ *   OFRR_QR_MGS_V2  mgs_v2 (qr.hpp:30-50, qr.cu:251-393): right-looking MGS,
+ *                   no reorthogonalization; dropped R(i,i) = 1;
+ *   OFRR_QR_CGS     cgs (qr.hpp:77-98, qr.cu:395-569): one projection product
+ *                   per pass; reorth_tol > 0 adds passes under the MGS
+ *                   criterion (cgs.m:73-89; the C++ cgs ignores it: pass 0);
+ *                   dropped R(i,i) = 1;
+ *   OFRR_QR_CGS2    cgs2 (qr.hpp:110-131, qr.cu:571-754): two passes, R(:,i) =
+ *                   r1 + r2, dropped R(i,i) = 1.  The second projection is
+ *                   subtracted from v (the reference adds r2 to R only,
+ *                   qr.cu:713-717, which leaves the CGS Q).
+ * Decisions stay on the device; with reorthogonalization on, one flag per column
+ * is read back (the call then synchronizes once per column). */
+#define OFRR_QR_MGS 0
+#define OFRR_QR_MGS_V2 1
+#define OFRR_QR_CGS 2
+#define OFRR_QR_CGS2 3
+OFRR_API int ofrr_gs_qr(ofrr_ctx *ctx, int variant, int q_dtype, int64_t m, int n, void *Q_dev,
+                        int64_t rs, int64_t cs, double select_tol, double orth_tol,
+                        double reorth_tol, double *R_dev, int ldr, int32_t *skip_dev);
+
+/* One column step: column i of Q against the final columns 0..i-1 whose drop
+ * flags are in skip_dev[0..i): mgs_step (qr.hpp:52-75, qr.cu:84-249),
+ * cgs_step (qr.hpp:100-107, qr.cu:489-569), cgs2_step (qr.hpp:133-141,
+ * qr.cu:666-754).  Writes R(0..i, i), skip_dev[i] and *skipped_host;
+ * synchronous.  OFRR_QR_MGS_V2 has no step (1000). */
+OFRR_API int ofrr_gs_qr_step(ofrr_ctx *ctx, int variant, int q_dtype, int64_t m, int n,
+                             void *Q_dev, int64_t rs, int64_t cs, int i, double select_tol,
+                             double orth_tol, double reorth_tol, double *R_dev, int ldr,
+                             int32_t *skip_dev, int32_t *skipped_host);
+
 /* ---- projection products: G = alpha * (V^T W) + beta * G in FP64
  * accumulation with a fixed-order (deterministic) split over rows.
  * gemm(transA = true, ...) of mvops.hpp:79-82 / eig_rr.m:49,51 used as

@@ -25,7 +25,7 @@ EXPORTS = [
     "ofrr_request_footprint", "ofrr_server_create", "ofrr_server_submit", "ofrr_server_poll",
     "ofrr_server_wait", "ofrr_server_stats_get", "ofrr_server_destroy",
     "ofrr_csr_file_info", "ofrr_csr_file_read", "ofrr_singular_values_file", "ofrr_io_last_error",
-    "ofrr_svd_rr", "ofrr_subspace_iter_svd", "ofrr_fp8_check",
+    "ofrr_svd_rr", "ofrr_subspace_iter_svd", "ofrr_fp8_check", "ofrr_gs_qr", "ofrr_gs_qr_step",
 ]
 
 SERVE_MAX_WANT = 128
@@ -101,6 +101,9 @@ def _decl(L):
     L.ofrr_scalecols.argtypes = [vp, i32, i64, i32, vp, i64, i64, vp]
     L.ofrr_check_special.argtypes = [vp, i32, i64, i32, vp, i64, i64]
     L.ofrr_hessenberg.argtypes = [vp, i32, i64, i32, vp, i64, i64, dbl, vp, i32, vp, vp]
+    L.ofrr_gs_qr.argtypes = [vp, i32, i32, i64, i32, vp, i64, i64, dbl, dbl, dbl, vp, i32, vp]
+    L.ofrr_gs_qr_step.argtypes = [vp, i32, i32, i64, i32, vp, i64, i64, i32, dbl, dbl, dbl, vp,
+                                  i32, vp, C.POINTER(C.c_int32)]
     L.ofrr_gram.argtypes = [vp, i64, i32, i32, vp, i32, i64, i64, vp, i32, i64, i64, dbl, dbl,
                             vp, i32]
     L.ofrr_basis_update.argtypes = [vp, i64, i32, i32, C.POINTER(vp), C.POINTER(i32),

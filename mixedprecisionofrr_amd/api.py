@@ -222,6 +222,50 @@ def hessenberg(ctx, A, select_tol=None):
     return Q, R, perm, skip
 
 
+QR_VARIANTS = {"mgs": 0, "mgs_v2": 1, "cgs": 2, "cgs2": 3}
+_EPS = {torch.float64: 2.0 ** -52, torch.float32: 2.0 ** -23, torch.float16: 2.0 ** -10,
+        torch.bfloat16: 2.0 ** -7}
+
+
+def gs_qr(ctx, A, variant="mgs", select_tol=None, orth_tol=None, reorth_tol=None):
+    """[Q, R, skip] of a Gram-Schmidt QR of A (qr.hpp:9-141): 'mgs' (mgs, reorthogonalized
+    with the default reorth_tol = 1/sqrt(2); <= 0 turns it off), 'mgs_v2', 'cgs'
+    (reorth_tol > 0: iterated, cgs.m:73-89; default off as in the C++ cgs) or 'cgs2'.
+    Tolerances default to the output format's epsilon (type_utils.hpp:125-140).  Q keeps
+    A's storage order (column- or row-major)."""
+    Q = A.clone()
+    m, n = Q.shape
+    eps = _EPS[Q.dtype]
+    select_tol = eps if select_tol is None else select_tol
+    orth_tol = eps if orth_tol is None else orth_tol
+    if reorth_tol is None:
+        reorth_tol = 2.0 ** -0.5 if variant == "mgs" else -1.0
+    R = colmajor(n, n, torch.float64, Q.device)
+    skip = torch.zeros(n, dtype=torch.int32, device=Q.device)
+    ctx.check(ctx.lib.ofrr_gs_qr(ctx.h, QR_VARIANTS[variant], dtype_code(Q), m, n, _p(Q),
+                                 Q.stride(0), Q.stride(1), select_tol, orth_tol, reorth_tol, _p(R),
+                                 n, _p(skip)), "gs_qr")
+    return Q, R, skip
+
+
+def gs_qr_step(ctx, Q, R, skip, i, variant="mgs", select_tol=None, orth_tol=None,
+               reorth_tol=None):
+    """Column step i in place on (Q, R, skip): mgs_step / cgs_step / cgs2_step
+    (qr.hpp:52-141).  Columns 0..i-1 must be final.  Returns the drop flag of column i."""
+    m, n = Q.shape
+    eps = _EPS[Q.dtype]
+    select_tol = eps if select_tol is None else select_tol
+    orth_tol = eps if orth_tol is None else orth_tol
+    if reorth_tol is None:
+        reorth_tol = 2.0 ** -0.5 if variant == "mgs" else -1.0
+    out = C.c_int32(0)
+    ctx.check(ctx.lib.ofrr_gs_qr_step(ctx.h, QR_VARIANTS[variant], dtype_code(Q), m, n, _p(Q),
+                                      Q.stride(0), Q.stride(1), i, select_tol, orth_tol,
+                                      reorth_tol, _p(R), R.stride(1), _p(skip),
+                                      C.byref(out)), "gs_qr_step")
+    return int(out.value)
+
+
 def gram(ctx, V, W, alpha=1.0, beta=0.0, G=None):
     """G = alpha V^T W + beta G, FP64 accumulation (eig_rr.m:49,51)."""
     n, k1 = V.shape

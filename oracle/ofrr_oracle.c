@@ -22,6 +22,10 @@
  *   orc_arnoldi_hess_eig exact termination: v0 = sum of j sine modes -> closed-form eigenvalues
  *   orc_subspace_iter_eig / orc_filtered_ofrr  closed-form Laplacian spectra, residual bound
  *   orc_residuals       hand-computed residual of exact eigenpairs (zero) and perturbed ones
+ *   orc_gs_qr           numpy Householder QR (sign-fixed) on full-rank input; the textbook
+ *                       loss of orthogonality of each variant on graded matrices (CGS ~ u k^2,
+ *                       MGS ~ u k, reorthogonalized and CGS2 ~ u); the duplicated-column skip
+ *                       pattern of qr_test.cpp:90-120; zero-column and drop conventions
  */
 #include "ofrr_oracle.h"
 
@@ -279,6 +283,102 @@ void orc_hessenberg(int64_t m, int n, double *Q, int64_t ldq, double *R, int ldr
          for (int64_t r = 0; r < m; ++r) v[r] = v[r] / piv;
       }
    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Gram-Schmidt QR.  MGS follows qr.cu:84-249 / mgs.m:55-92 (left-looking,     */
+/* r = v^T q_j then v -= r q_j per kept column, reorthogonalization while      */
+/* orth_tol <= t < reorth_tol * normv); MGS_V2 qr.cu:251-393 (right-looking);  */
+/* CGS qr.cu:489-569 with the iterated passes of cgs.m:73-89; CGS2 the two     */
+/* passes of qr.hpp:110-131 with the second projection subtracted.             */
+/* ------------------------------------------------------------------------- */
+static double o_dot(int64_t m, const double *x, const double *y) {
+   double s = 0.0;
+   for (int64_t r = 0; r < m; ++r) s = fma(x[r], y[r], s);
+   return s;
+}
+
+void orc_gs_qr(int64_t m, int n, double *Q, int64_t ldq, double *R, int ldr, int variant,
+               double select_tol, double orth_tol, double reorth_tol, int *skip) {
+   for (int j = 0; j < n; ++j) {
+      skip[j] = 0;
+      for (int i = 0; i < n; ++i) R[i + (int64_t)j * ldr] = 0.0;
+   }
+#define RR(i, j) R[(i) + (int64_t)(j) * ldr]
+   const double rd_skip = variant == 0 ? 0.0 : 1.0;
+   if (variant == 1) {
+      for (int i = 0; i < n; ++i) {
+         double *v = Q + (int64_t)i * ldq;
+         const double t = sqrt(o_dot(m, v, v));
+         if (t < select_tol) {
+            skip[i] = 1;
+            RR(i, i) = 1.0;
+            continue;
+         }
+         RR(i, i) = t;
+         const double s = 1.0 / t;
+         for (int64_t r = 0; r < m; ++r) v[r] = v[r] * s;
+         for (int c = i + 1; c < n; ++c) {
+            double *w = Q + (int64_t)c * ldq;
+            const double rc = o_dot(m, v, w);
+            RR(i, c) = rc;
+            for (int64_t r = 0; r < m; ++r) w[r] = fma(-rc, v[r], w[r]);
+         }
+      }
+      return;
+   }
+   double *rv = (double *)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+   const int reorth = reorth_tol > 0.0 && (variant == 0 || variant == 2);
+   for (int i = 0; i < n; ++i) {
+      double *v = Q + (int64_t)i * ldq;
+      double normv = sqrt(o_dot(m, v, v));
+      if (normv < select_tol) {
+         skip[i] = 1;
+         RR(i, i) = rd_skip;
+         continue;
+      }
+      int pass = 0, reorths = 0;
+      for (;;) {
+         if (variant == 0) {
+            for (int j = 0; j < i; ++j) {
+               if (skip[j]) continue;
+               const double *q = Q + (int64_t)j * ldq;
+               const double r = o_dot(m, v, q);
+               RR(j, i) += r;
+               for (int64_t k = 0; k < m; ++k) v[k] = fma(-r, q[k], v[k]);
+            }
+         } else {
+            for (int j = 0; j < i; ++j) rv[j] = skip[j] ? 0.0 : o_dot(m, Q + (int64_t)j * ldq, v);
+            for (int j = 0; j < i; ++j) RR(j, i) += rv[j];
+            for (int64_t k = 0; k < m; ++k) {
+               double s = 0.0;
+               for (int j = 0; j < i; ++j)
+                  if (rv[j] != 0.0) s = fma(rv[j], Q[k + (int64_t)j * ldq], s);
+               v[k] = v[k] - s;
+            }
+         }
+         ++pass;
+         if (variant == 3 && pass < 2) continue;
+         const double t = sqrt(o_dot(m, v, v));
+         /* at most 64 reorthogonalization passes; none for t = 0 */
+         if (reorth && reorths < 64 && t > 0.0 && t >= orth_tol && t < reorth_tol * normv) {
+            normv = t;
+            ++reorths;
+            continue;
+         }
+         if (t < select_tol) {
+            skip[i] = 1;
+            RR(i, i) = rd_skip;
+         } else {
+            RR(i, i) = t;
+            const double s = 1.0 / t;
+            for (int64_t k = 0; k < m; ++k) v[k] = v[k] * s;
+         }
+         break;
+      }
+   }
+#undef RR
+   free(rv);
 }
 
 /* ------------------------------------------------------------------------- */

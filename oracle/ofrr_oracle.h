@@ -63,6 +63,15 @@ void orc_gemm_nn(int64_t m, int n, int k, const double *A, int64_t lda, const do
 void orc_hessenberg(int64_t m, int n, double *Q, int64_t ldq, double *R, int ldr,
                     double select_tol, int variant, int *select, int *perm);
 
+/* Gram-Schmidt QR (CPP/linalg/factorization/qr.cu, MATLAB/+src/+qr/mgs.m,
+ * cgs.m).  variant 0 MGS (reorth_tol > 0: reorthogonalization), 1 MGS_V2,
+ * 2 CGS (reorth_tol > 0: iterated, cgs.m:73-89), 3 CGS2 (second projection
+ * subtracted).  Q holds A on entry.  skip[i] = 1 for dropped columns, which keep
+ * their residual in Q and are left out of later projections; R(i,i) of a dropped
+ * column is 0 (MGS) or 1 (the others). */
+void orc_gs_qr(int64_t m, int n, double *Q, int64_t ldq, double *R, int ldr, int variant,
+               double select_tol, double orth_tol, double reorth_tol, int *skip);
+
 /* ---- dense symmetric eigensolvers (FP64) --------------------------------- */
 /* eigenvalues ascending in w, eigenvectors in V (LAPACK dsyev 'V','U' order);
  * returns the number of Jacobi sweeps run */

@@ -53,6 +53,8 @@ def lib():
                                   _dp, C.c_int64]
         L.orc_hessenberg.argtypes = [C.c_int64, C.c_int, _dp, C.c_int64, _dp, C.c_int, C.c_double,
                                      C.c_int, _ip32, _ip32]
+        L.orc_gs_qr.argtypes = [C.c_int64, C.c_int, _dp, C.c_int64, _dp, C.c_int, C.c_int,
+                                C.c_double, C.c_double, C.c_double, _ip32]
         L.orc_syev.argtypes = [C.c_int, _dp, C.c_int, _dp, _dp, C.c_int]
         L.orc_sygv.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int]
         L.orc_rr_truncated.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_int,
@@ -177,6 +179,20 @@ def hessenberg(A, select_tol, variant=0):
     perm = np.zeros(n, np.int32)
     lib().orc_hessenberg(m, n, Q, m, R, n, select_tol, variant, sel, perm)
     return Q, R, sel, perm
+
+
+QR_VARIANTS = {"mgs": 0, "mgs_v2": 1, "cgs": 2, "cgs2": 3}
+
+
+def gs_qr(A, variant, select_tol=2.0 ** -52, orth_tol=2.0 ** -52, reorth_tol=-1.0):
+    """(Q, R, skip) of the Gram-Schmidt variant ('mgs', 'mgs_v2', 'cgs', 'cgs2')."""
+    Q = _f(A).copy(order="F")
+    m, n = Q.shape
+    R = np.zeros((n, n), order="F")
+    skip = np.zeros(n, np.int32)
+    lib().orc_gs_qr(m, n, Q, m, R, n, QR_VARIANTS[variant], select_tol, orth_tol, reorth_tol,
+                    skip)
+    return Q, R, skip
 
 
 def syev(A):

@@ -89,6 +89,20 @@ kats = {
         "dup_select_tol": {"f64": 1e-12, "f32": 1e-6, "f16": 1e-2},
         "dup_columns": {"copied_from": [0, 1, 2], "copied_to": [3, 4, 5]},
     },
+    "qr": {
+        "source": "CPP/tests/qr_test.cpp:30-651 (A uniform [-1, 1), Q^T Q = I and A = Q R "
+                  "within tol, then columns 3..5 copied from 0..2 and dropped)",
+        "shape": [2000, 200],
+        "tol": {"mgs": {"f64": 1e-10, "f32": 2e-6, "f16": 2e-2},
+                "cgs2": {"f64": 1e-10, "f32": 2e-6, "f16": 2e-2},
+                "mgs_v2": {"f64": 1e-10}},
+        "dup_select_tol": {"mgs": {"f64": 1e-12, "f32": 1e-6, "f16": 1e-2},
+                           "cgs2": {"f64": 1e-12, "f32": 5e-6, "f16": 5e-2},
+                           "mgs_v2": {"f64": 1e-12}},
+        "dup_orth_tol": {"mgs": {"f64": 1e-12, "f32": 1e-6, "f16": 1e-2}},
+        "reorth_tol": 0.7071067811865476,
+        "dup_columns": {"copied_from": [0, 1, 2], "copied_to": [3, 4, 5]},
+    },
     "laplacian": {
         "source": "closed form lambda = sum_d 4 sin^2(p_d pi / (2 (N+1)))",
         "cfg1_2d_N64": {"n": 4096, "nnz": 20224, "top": lap_eigs(2, 64, 24)},
