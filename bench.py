@@ -146,7 +146,7 @@ def main():
                    help="growth-block format of the mixed path (the workload names FP16)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
-    p.add_argument("--workload", choices=["eig", "svd", "serve"], default="eig",
+    p.add_argument("--workload", choices=["eig", "svd", "serve", "qr"], default="eig",
                    help="eig: the 128^3 Laplacian eigensolve (default); svd: truncated SVD of "
                         "the dense 262,144 x 4,096 low-rank-plus-noise matrix; serve: a stream "
                         "of eigen requests with Poisson arrivals through the serving loop")
@@ -159,6 +159,9 @@ def main():
                    help="also submit the large FP8 solve (3-D variable-coefficient diffusion on "
                         "a 256^3 grid, 100 largest pairs, FP8 E4M3 growth blocks) a quarter of "
                         "the way into the stream")
+    p.add_argument("--qr-sizes", default="25000x200,50000x200,50000x400",
+                   help="qr workload: m x n shapes (CPP/examples/ex00.cu:160-164)")
+    p.add_argument("--qr-dtypes", default="f64,f32,f16")
     p.add_argument("--svd-m", type=int, default=262144)
     p.add_argument("--svd-n", type=int, default=4096)
     p.add_argument("--copy-probe", action="store_true",
@@ -177,6 +180,8 @@ def main():
 
     if args.workload == "svd":
         return svd_main(args)
+    if args.workload == "qr":
+        return qr_main(args)
     if args.workload == "serve":
         r = serve_stream(api, args.serve_requests, args.serve_rate, args.serve_slots,
                          args.serve_seed, big=args.serve_big)
@@ -492,6 +497,71 @@ def svd_main(args):
         "fp64_gpu": {"value": int(np.sum(res64[:nwant] <= args.tol)) / t64, "ms_per_solve": t64 * 1e3,
                      "iterations": stats64["iterations"], "status": st64},
     }
+    print(json.dumps(out), flush=True)
+
+
+def qr_main(args):
+    """The orthogonalizing baselines against the Hessenberg process: the comparison of
+    CPP/examples/ex00.cu:55-150 (5 factorizations per method after a warm-up, time
+    per factorization; A uniform [-1, 1), column-major).  MGS without and with
+    reorthogonalization (reorth_tol = -1 and 1/sqrt(2)), MGS_V2, CGS, CGS2, Hessenberg.
+    Besides the time, ||Q^T Q - I||_max over the kept columns and ||A - QR||_max."""
+    import numpy as np
+    import torch
+
+    from mixedprecisionofrr_amd import api
+
+    dev = "cuda:0"
+    ctx = api.Context(0)
+    dts = {"f64": torch.float64, "f32": torch.float32, "f16": torch.float16}
+    methods = [("MGS", "mgs", -1.0), ("MGS R", "mgs", 2 ** -0.5), ("MGS V2", "mgs_v2", -1.0),
+               ("CGS", "cgs", -1.0), ("CGS2", "cgs2", -1.0), ("Hessenberg", None, None)]
+    ntests = max(1, args.steps)
+    rows = []
+    g = torch.Generator(device=dev)
+    for shape in args.qr_sizes.split(","):
+        m, n = (int(x) for x in shape.split("x"))
+        g.manual_seed(m + n)
+        A64 = (torch.rand(n, m, device=dev, dtype=torch.float64, generator=g) * 2 - 1).t()
+        for dname in args.qr_dtypes.split(","):
+            A = A64.to(dts[dname]).t().contiguous().t()
+            for label, var, rtol in methods:
+                def run():
+                    if var is None:
+                        Q, R, _, skip = api.hessenberg(ctx, A)
+                    else:
+                        Q, R, skip = api.gs_qr(ctx, A, var, reorth_tol=rtol)
+                    return Q, R, skip
+                run()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(ntests):
+                    Q, R, skip = run()
+                torch.cuda.synchronize()
+                ms = (time.perf_counter() - t0) * 1e3 / ntests
+                Qd = Q.double()
+                keep = skip == 0
+                Qk = Qd[:, keep]
+                eye = torch.eye(Qk.shape[1], dtype=torch.float64, device=dev)
+                loss = (Qk.t() @ Qk - eye).abs().max().item()
+                res = (A.double() - Qd @ R).abs().max().item()
+                rows.append({"m": m, "n": n, "dtype": dname, "method": label, "ms": ms,
+                             "orth_loss": loss if var is not None else None,
+                             "qr_residual": res, "dropped": int((~keep).sum().item())})
+                ltxt = f"{loss:.2e}" if var is not None else "n/a (not orthogonal)"
+                log(f"qr {m}x{n} {dname} {label}: {ms:.3f} ms, loss {ltxt}, |A-QR| {res:.2e}")
+            del A
+        del A64
+    pick = [r for r in rows if r["method"] == "CGS2" and r["dtype"] == "f64"]
+    ref = pick[-1] if pick else rows[-1]
+    out = {"metric": "Gram-Schmidt QR baselines vs the Hessenberg process: ms per factorization",
+           "value": ref["ms"], "unit": "ms", "n_gpus": 1, "steps": ntests, "warmup": 1,
+           "ms_per_step": ref["ms"], "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64/f32/f16 storage, f64 dot products and norms",
+           "data": "synthetic",
+           "config": {"workload": f"QR of uniform [-1,1) matrices {args.qr_sizes}; value = CGS2 "
+                                  f"FP64 at {ref['m']}x{ref['n']}"},
+           "table": rows}
     print(json.dumps(out), flush=True)
 
 

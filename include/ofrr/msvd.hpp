@@ -446,6 +446,151 @@ std::vector<int> hessenberg(const Matrix<T_I> &A, Matrix<T_O> &Q, Matrix<T_O> &R
 }
 
 // ---------------------------------------------------------------------------
+// Gram-Schmidt QR (qr.hpp:9-141): A and Q on the device, R on the host; returns
+// the skip flags (1 = dropped column).  orth_tol and reorth_tol act in mgs
+// only; the others take them as placeholders, as the reference does.
+namespace detail {
+template <typename T_I, typename T_O>
+std::vector<int> gs_qr(int variant, const char *name, const Matrix<T_I> &A, Matrix<T_O> &Q,
+                       Matrix<T_O> &R, double select_tol, double orth_tol, double reorth_tol) {
+   if (A.location() != Location::kDEVICE) throw std::runtime_error("Input matrix A must be on device");
+   if (Q.location() != Location::kDEVICE) throw std::runtime_error("Output matrix Q must be on device");
+   if (R.location() != Location::kHOST) throw std::runtime_error("Output matrix R must be on host");
+   const size_t m = A.rows(), n = A.cols();
+   if (Q.rows() != m || Q.cols() != n) throw std::runtime_error("Output matrix Q has incorrect dimensions");
+   if (R.rows() != n || R.cols() != n) throw std::runtime_error("Output matrix R has incorrect dimensions");
+   ofrr_ctx *cx = ctx();
+   int st = ofrr_convert(cx, (int64_t)m, (int64_t)n, A.data(), dtype<T_I>::value, 1, (int64_t)A.ld(), Q.data(),
+                         dtype<T_O>::value, 1, (int64_t)Q.ld());
+   Matrix<double> dR(n, n);
+   Vector<int32_t> skip(n);
+   if (!st)
+      st = ofrr_gs_qr(cx, variant, dtype<T_O>::value, (int64_t)m, (int)n, Q.data(), 1, (int64_t)Q.ld(),
+                      select_tol, orth_tol, reorth_tol, dR.data(), (int)n, skip.data());
+   if (done(st) != MSVDStatus::kSuccess)
+      throw std::runtime_error(std::string(name) + ": " + ofrr_ctx_last_error(cx));
+   store(R, dR.data(), n, n, n);
+   std::vector<int32_t> hs(n);
+   hip_check(hipMemcpy(hs.data(), skip.data(), n * 4, hipMemcpyDeviceToHost), name);
+   return std::vector<int>(hs.begin(), hs.end());
+}
+
+// column i of Q against its final columns 0..i-1 (the *_step functions); on
+// return R_dev holds R(0..i, i) in FP64 and skip[i] the drop flag
+template <typename T_O>
+void gs_qr_step(int variant, const char *name, Matrix<T_O> &Q, std::vector<int> &skip, size_t i,
+                double select_tol, double orth_tol, double reorth_tol, Matrix<double> &R_dev) {
+   const size_t m = Q.rows(), n = Q.cols();
+   if (Q.location() != Location::kDEVICE) throw std::runtime_error("Output matrix Q must be on device");
+   if (i >= n || skip.size() < n) throw std::runtime_error(std::string(name) + ": bad column");
+   std::vector<int32_t> hs(skip.begin(), skip.begin() + n);
+   Vector<int32_t> dskip(n);
+   hip_check(hipMemcpy(dskip.data(), hs.data(), n * 4, hipMemcpyHostToDevice), name);
+   int32_t dropped = 0;
+   int st = ofrr_gs_qr_step(ctx(), variant, dtype<T_O>::value, (int64_t)m, (int)n, Q.data(), 1, (int64_t)Q.ld(),
+                            (int)i, select_tol, orth_tol, reorth_tol, R_dev.data(), (int)n, dskip.data(),
+                            &dropped);
+   if (st != 0) throw std::runtime_error(std::string(name) + ": " + ofrr_ctx_last_error(ctx()));
+   skip[i] = dropped;
+}
+}  // namespace detail
+
+template <typename T_I, typename T_O, typename T_COMPUTE>
+std::vector<int> mgs(const Matrix<T_I> &A, Matrix<T_O> &Q, Matrix<T_O> &R,
+                     T_O select_tol = T_O(detail::eps_of(detail::dtype<T_O>::value)),
+                     T_O orth_tol = T_O(detail::eps_of(detail::dtype<T_O>::value)),
+                     T_O reorth_tol = T_O(0.70710678118654752)) {
+   return detail::gs_qr(OFRR_QR_MGS, "mgs", A, Q, R, (double)select_tol, (double)orth_tol, (double)reorth_tol);
+}
+
+template <typename T_I, typename T_O, typename T_COMPUTE>
+std::vector<int> mgs_v2(const Matrix<T_I> &A, Matrix<T_O> &Q, Matrix<T_O> &R,
+                        T_O select_tol = T_O(detail::eps_of(detail::dtype<T_O>::value)),
+                        T_O orth_tol = T_O(detail::eps_of(detail::dtype<T_O>::value)),
+                        T_O reorth_tol = T_O(0.70710678118654752)) {
+   (void)reorth_tol;
+   return detail::gs_qr(OFRR_QR_MGS_V2, "mgs_v2", A, Q, R, (double)select_tol, (double)orth_tol, -1.0);
+}
+
+template <typename T_I, typename T_O, typename T_COMPUTE>
+std::vector<int> cgs(const Matrix<T_I> &A, Matrix<T_O> &Q, Matrix<T_O> &R,
+                     T_O select_tol = T_O(detail::eps_of(detail::dtype<T_O>::value)),
+                     T_O orth_tol = T_O(detail::eps_of(detail::dtype<T_O>::value)),
+                     T_O reorth_tol = T_O(0.70710678118654752)) {
+   (void)reorth_tol;
+   return detail::gs_qr(OFRR_QR_CGS, "cgs", A, Q, R, (double)select_tol, (double)orth_tol, -1.0);
+}
+
+template <typename T_I, typename T_O, typename T_COMPUTE>
+std::vector<int> cgs2(const Matrix<T_I> &A, Matrix<T_O> &Q, Matrix<T_O> &R,
+                      T_O select_tol = T_O(detail::eps_of(detail::dtype<T_O>::value)),
+                      T_O orth_tol = T_O(detail::eps_of(detail::dtype<T_O>::value)),
+                      T_O reorth_tol = T_O(0.70710678118654752)) {
+   (void)reorth_tol;
+   return detail::gs_qr(OFRR_QR_CGS2, "cgs2", A, Q, R, (double)select_tol, (double)orth_tol, -1.0);
+}
+
+// mgs_step (qr.hpp:52-75): R on the host; R(j, i) for the kept j < i and R(i, i)
+// of an accepted column are written, other entries are left as they are
+template <typename T_O, typename T_COMPUTE>
+int mgs_step(Matrix<T_O> &Q, Matrix<T_O> &R, std::vector<int> &skip, size_t i,
+             T_O select_tol = T_O(detail::eps_of(detail::dtype<T_O>::value)),
+             T_O orth_tol = T_O(detail::eps_of(detail::dtype<T_O>::value)),
+             T_O reorth_tol = T_O(0.70710678118654752)) {
+   if (R.location() != Location::kHOST) throw std::runtime_error("Output matrix R must be on host");
+   const size_t n = Q.cols();
+   Matrix<double> dR(n, n);
+   detail::gs_qr_step(OFRR_QR_MGS, "mgs_step", Q, skip, i, (double)select_tol, (double)orth_tol,
+                      (double)reorth_tol, dR);
+   std::vector<double> col(i + 1);
+   detail::hip_check(hipMemcpy(col.data(), dR.data() + i * n, (i + 1) * 8, hipMemcpyDeviceToHost), "mgs_step");
+   for (size_t j = 0; j < i; ++j)
+      if (!skip[j]) R(j, i) = (T_O)col[j];
+   if (!skip[i]) R(i, i) = (T_O)col[i];
+   return skip[i];
+}
+
+// cgs_step / cgs2_step (qr.hpp:100-107, 133-141): R_device (device, n x n)
+// receives R(0..i-1, i), r_diag[i] (host) the norm of an accepted column
+template <typename T_O, typename T_COMPUTE>
+int cgs_step(Matrix<T_O> &Q, Matrix<T_O> &R_device, Vector<T_O> &r_diag, std::vector<int> &skip, size_t i,
+             T_O select_tol = T_O(detail::eps_of(detail::dtype<T_O>::value))) {
+   const size_t n = Q.cols();
+   Matrix<double> dR(n, n);
+   detail::gs_qr_step(OFRR_QR_CGS, "cgs_step", Q, skip, i, (double)select_tol,
+                      detail::eps_of(detail::dtype<T_O>::value), -1.0, dR);
+   if (i > 0) {
+      int st = ofrr_convert(detail::ctx(), (int64_t)i, 1, dR.data() + i * n, OFRR_F64, 1, (int64_t)n,
+                            &R_device(0, i), detail::dtype<T_O>::value, 1, (int64_t)R_device.ld());
+      if (detail::done(st) != MSVDStatus::kSuccess) throw std::runtime_error("cgs_step: R store");
+   }
+   double d = 0.0;
+   detail::hip_check(hipMemcpy(&d, dR.data() + i * n + i, 8, hipMemcpyDeviceToHost), "cgs_step");
+   if (!skip[i]) r_diag[i] = (T_O)d;
+   return skip[i];
+}
+
+template <typename T_O, typename T_COMPUTE>
+int cgs2_step(Matrix<T_O> &Q, Matrix<T_O> &R_device, Vector<T_O> &r_correction, Vector<T_O> &r_diag,
+              std::vector<int> &skip, size_t i,
+              T_O select_tol = T_O(detail::eps_of(detail::dtype<T_O>::value))) {
+   (void)r_correction;
+   const size_t n = Q.cols();
+   Matrix<double> dR(n, n);
+   detail::gs_qr_step(OFRR_QR_CGS2, "cgs2_step", Q, skip, i, (double)select_tol,
+                      detail::eps_of(detail::dtype<T_O>::value), -1.0, dR);
+   if (i > 0) {
+      int st = ofrr_convert(detail::ctx(), (int64_t)i, 1, dR.data() + i * n, OFRR_F64, 1, (int64_t)n,
+                            &R_device(0, i), detail::dtype<T_O>::value, 1, (int64_t)R_device.ld());
+      if (detail::done(st) != MSVDStatus::kSuccess) throw std::runtime_error("cgs2_step: R store");
+   }
+   double d = 0.0;
+   detail::hip_check(hipMemcpy(&d, dR.data() + i * n + i, 8, hipMemcpyDeviceToHost), "cgs2_step");
+   if (!skip[i]) r_diag[i] = (T_O)d;
+   return skip[i];
+}
+
+// ---------------------------------------------------------------------------
 // Rayleigh-Ritz (rr_eig.hpp:21-35)
 
 // ofrr_eig: A = V^T A V-like, B = V^T V-like (m x m); V (m x k) <- the Ritz

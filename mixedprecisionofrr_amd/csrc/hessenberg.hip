@@ -10,10 +10,12 @@
 // Right-looking formulation, all on the device (the reference copies the skip
 // flag to the host every column, hessenberg.cu:519): per column one argmax
 // (two small kernels), one kernel that records the pivot decision and gathers
-// the pivot row of the trailing columns into R, and one kernel in which each
-// thread owns a row: it scales the row's entry of column i and eliminates it
-// from every trailing column.
+// the pivot row of the trailing columns into R, and one kernel over (row block,
+// group of 16 trailing columns) in which each thread owns a row: it scales the
+// row's entry of column i and eliminates it from the group's columns.
 #include "internal.hpp"
+
+#include <algorithm>
 
 namespace ofrr {
 
@@ -35,22 +37,32 @@ __global__ void hess_pivot_kernel(int n, int i, const T *__restrict__ Q, int64_t
       R[i + (int64_t)c * ldr] = drop ? 0.0 : (double)Elem<T>::load(Q[(int64_t)pr * rs + (int64_t)c * cs]);
 }
 
+// Elimination of column i (not dropped) from the trailing columns of this
+// workgroup's column group (kHessCols columns, blockIdx.y), one row per thread,
+// with q = Q(r, i) / R(i,i) rounded to the storage format.  Every column group
+// reads column i unscaled, so the scaled column is stored by the next launch:
+// column group 0 stores the scaled column `prev` (the previous pivot column,
+// when kept).  i < 0: that store only.
+constexpr int kHessCols = 16;
 template <typename T, typename ACC>
-__global__ __launch_bounds__(256) void hess_update_kernel(int64_t m, int n, int i, T *__restrict__ Q,
-                                                          int64_t rs, int64_t cs,
+__global__ __launch_bounds__(256) void hess_update_kernel(int64_t m, int n, int i, int prev,
+                                                          T *__restrict__ Q, int64_t rs, int64_t cs,
                                                           const double *__restrict__ R, int ldr,
                                                           const int32_t *__restrict__ skip) {
-   if (skip[i]) return;
    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
    if (r >= m) return;
+   if (blockIdx.y == 0 && prev >= 0 && !skip[prev]) {
+      T *qp = Q + r * rs + (int64_t)prev * cs;
+      *qp = Elem<T>::store((double)((ACC)Elem<T>::load(*qp) / (ACC)R[prev + (int64_t)prev * ldr]));
+   }
+   if (i < 0 || skip[i]) return;
    const ACC piv = (ACC)R[i + (int64_t)i * ldr];
-   T *qi = Q + r * rs + (int64_t)i * cs;
-   const T qs = Elem<T>::store((double)((ACC)Elem<T>::load(*qi) / piv));
-   *qi = qs;
-   const ACC q = (ACC)Elem<T>::load(qs);
-   for (int c = i + 1; c < n; ++c) {
+   const ACC q = (ACC)Elem<T>::load(
+       Elem<T>::store((double)((ACC)Elem<T>::load(Q[r * rs + (int64_t)i * cs]) / piv)));
+   const int c0 = i + 1 + (int)blockIdx.y * kHessCols, c1 = min(n, c0 + kHessCols);
+   for (int c = c0; c < c1; ++c) {
       T *p = Q + r * rs + (int64_t)c * cs;
-      ACC coef = (ACC)R[i + (int64_t)c * ldr];
+      const ACC coef = (ACC)R[i + (int64_t)c * ldr];
       *p = Elem<T>::store((double)fma(-coef, q, (ACC)Elem<T>::load(*p)));
    }
 }
@@ -75,9 +87,12 @@ static int hessenberg_impl(ofrr_ctx *ctx, int64_t m, int n, T *Q, int64_t rs, in
       if (st) return st;
       hess_pivot_kernel<T><<<1, 256, 0, ctx->stream>>>(
           n, i, Q, rs, cs, tol, R, ldr, perm, skip);
-      hess_update_kernel<T, ACC><<<ublocks, 256, 0, ctx->stream>>>(m, n, i, Q, rs, cs, R, ldr, skip);
+      dim3 grid(ublocks, std::max(1, (n - i - 1 + kHessCols - 1) / kHessCols));
+      hess_update_kernel<T, ACC><<<grid, 256, 0, ctx->stream>>>(m, n, i, i - 1, Q, rs, cs, R, ldr, skip);
       OFRR_LAUNCH(ctx);
    }
+   hess_update_kernel<T, ACC><<<ublocks, 256, 0, ctx->stream>>>(m, n, -1, n - 1, Q, rs, cs, R, ldr, skip);
+   OFRR_LAUNCH(ctx);
    return kSuccess;
 }
 

@@ -321,15 +321,42 @@ __global__ __launch_bounds__(kQrThreads) void qr_rank1_kernel(int64_t m, int64_t
    }
 }
 
-// left-looking update v = Q(:, vcol) -= sum_{c < c1} coef[c] Q(:, c) (zero
-// coefficients skipped), vnorm_part[b]: partial ||v||^2 of the result
+// Left-looking update v = Q(:, vcol) -= sum_{c < c1} coef[c] Q(:, c) in two
+// launches, so that the columns spread over workgroups as in the dot launch:
+// qr_gemvn_part_kernel forms S[g][r] = sum of coef[c] Q(r, c) over the columns
+// of group g (kQrGemvCols columns, blockIdx.y; zero coefficients skipped), and
+// qr_gemvn_apply_kernel subtracts sum_g S[g][r] (fixed order) from v and forms
+// vnorm_part[b] = partial ||v||^2 of the result.
+constexpr int kQrGemvCols = 32;
 template <typename T>
-__global__ __launch_bounds__(kQrThreads) void qr_gemvn_kernel(int64_t m, int64_t chunk,
-                                                              T *__restrict__ Q, int64_t rs,
-                                                              int64_t cs, int vcol, int c1,
-                                                              const double *__restrict__ coef,
-                                                              const int32_t *gate,
-                                                              double *__restrict__ vnorm_part) {
+__global__ __launch_bounds__(kQrThreads) void qr_gemvn_part_kernel(int64_t m, int64_t chunk,
+                                                                   const T *__restrict__ Q,
+                                                                   int64_t rs, int64_t cs, int c1,
+                                                                   const double *__restrict__ coef,
+                                                                   const int32_t *gate,
+                                                                   double *__restrict__ S) {
+   if (*gate) return;
+   const int64_t r0 = (int64_t)blockIdx.x * chunk, r1 = min(m, r0 + chunk);
+   const int g = blockIdx.y;
+   const int cb = g * kQrGemvCols, ce = min(c1, cb + kQrGemvCols);
+   double *Sg = S + (int64_t)g * m;
+   for (int64_t r = r0 + threadIdx.x; r < r1; r += kQrThreads) {
+      double s = 0.0;
+      for (int c = cb; c < ce; ++c) {
+         const double cf = coef[c];
+         if (cf != 0.0) s = fma(cf, ld_d(qat(Q, rs, cs, r, c)), s);
+      }
+      Sg[r] = s;
+   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kQrThreads) void qr_gemvn_apply_kernel(int64_t m, int64_t chunk,
+                                                                    T *__restrict__ Q, int64_t rs,
+                                                                    int64_t cs, int vcol, int ngroups,
+                                                                    const double *__restrict__ S,
+                                                                    const int32_t *gate,
+                                                                    double *__restrict__ vnorm_part) {
    using ACC = typename Elem<T>::acc;
    if (*gate) return;
    __shared__ double red[kQrWaves];
@@ -338,10 +365,7 @@ __global__ __launch_bounds__(kQrThreads) void qr_gemvn_kernel(int64_t m, int64_t
    double a = 0.0;
    for (int64_t r = r0 + threadIdx.x; r < r1; r += kQrThreads) {
       double s = 0.0;
-      for (int c = 0; c < c1; ++c) {
-         const double cf = coef[c];
-         if (cf != 0.0) s = fma(cf, ld_d(qat(Q, rs, cs, r, c)), s);
-      }
+      for (int g = 0; g < ngroups; ++g) s += S[(int64_t)g * m + r];
       T *p = qat(Q, rs, cs, r, vcol);
       const T y = Elem<T>::store((double)((ACC)Elem<T>::load(*p) - (ACC)s));
       *p = y;
@@ -409,7 +433,7 @@ struct GsQr {
    int ldr;
    int32_t *skip;
    QrLayout L;
-   double *part, *coef, *normv, *scale;
+   double *part, *coef, *normv, *scale, *gsum;
    int32_t *pre, *flag, *hflag;
 
    double *slot(int s) const { return part + (int64_t)s * L.nblk; }
@@ -418,7 +442,10 @@ struct GsQr {
 
    int setup() {
       L = qr_layout(m);
-      const size_t bytes = ((size_t)(n + 3) * L.nblk + 3 * (size_t)n) * 8 + (size_t)n * 4 + 16;
+      const bool left = variant == OFRR_QR_CGS || variant == OFRR_QR_CGS2;
+      const size_t sgroups = left ? (size_t)(n + kQrGemvCols - 1) / kQrGemvCols : 0;
+      const size_t bytes = ((size_t)(n + 3) * L.nblk + 3 * (size_t)n + sgroups * (size_t)m) * 8 +
+                           (size_t)n * 4 + 16;
       char *w = (char *)workspace(ctx, bytes);
       hflag = (int32_t *)pinned(ctx, 16);
       if (!w || !hflag) return set_error(ctx, kErrorCUDAMemory, "gs_qr workspace");
@@ -426,7 +453,8 @@ struct GsQr {
       coef = part + (size_t)(n + 3) * L.nblk;
       normv = coef + n;
       scale = normv + n;
-      pre = (int32_t *)(scale + n);
+      gsum = scale + n;
+      pre = (int32_t *)(gsum + sgroups * (size_t)m);
       flag = pre + n;
       return kSuccess;
    }
@@ -492,8 +520,15 @@ struct GsQr {
    }
 
    int gemvn(int i) {
-      qr_gemvn_kernel<T><<<L.nblk, kQrThreads, 0, ctx->stream>>>(m, L.chunk, Q, rs, cs, i, i, coef,
-                                                                 skip + i, norm_slot());
+      const int ngroups = (i + kQrGemvCols - 1) / kQrGemvCols;
+      if (ngroups > 0) {
+         dim3 grid(L.nblk, ngroups);
+         qr_gemvn_part_kernel<T><<<grid, kQrThreads, 0, ctx->stream>>>(m, L.chunk, Q, rs, cs, i,
+                                                                        coef, skip + i, gsum);
+         OFRR_LAUNCH(ctx);
+      }
+      qr_gemvn_apply_kernel<T><<<L.nblk, kQrThreads, 0, ctx->stream>>>(
+          m, L.chunk, Q, rs, cs, i, ngroups, gsum, skip + i, norm_slot());
       OFRR_LAUNCH(ctx);
       return kSuccess;
    }

@@ -141,6 +141,111 @@ int main() {
       }
       check(threw, "hessenberg rejects a host A");
    }
+   // ---- Gram-Schmidt QR (qr_test.cpp:30-651): 2000 x 200 uniform, Q^T Q = I and
+   // A = QR within 1e-10 (FP64) / 2e-6 (FP32 cgs2); duplicated columns 3..5 dropped;
+   // the column steps reproduce the factorization
+   {
+      const int m = 2000, n = 200;
+      Matrix<double> A(m, n), Q(m, n), R(n, n, Location::kHOST);
+      A.fill_random(11);
+      auto qr_err = [&](const std::vector<double> &ha, const std::vector<double> &hq,
+                        const Matrix<double> &Rm, const std::vector<int> &skip) {
+         double e = 0;
+         for (int j = 0; j < n; ++j)
+            for (int i = 0; i < m; i += 7) {
+               double s = 0;
+               for (int p = 0; p <= j; ++p) s += hq[i + (size_t)p * m] * Rm(p, j);
+               e = std::fmax(e, std::fabs(s - ha[i + (size_t)j * m]));
+            }
+         for (int a = 0; a < n; a += 3)
+            for (int b = a; b < n; b += 5) {
+               if (skip[a] || skip[b]) continue;
+               double s = 0;
+               for (int i = 0; i < m; ++i) s += hq[i + (size_t)a * m] * hq[i + (size_t)b * m];
+               e = std::fmax(e, std::fabs(s - (a == b ? 1.0 : 0.0)));
+            }
+         return e;
+      };
+      std::vector<int> skip = mgs<double, double, double>(A, Q, R);
+      double err = qr_err(host_of(A), host_of(Q), R, skip);
+      int dropped = 0;
+      for (int s : skip) dropped += s;
+      check(err < 1e-10 && dropped == 0, "mgs Q^T Q = I and A = QR", err);
+      skip = cgs2<double, double, double>(A, Q, R);
+      err = qr_err(host_of(A), host_of(Q), R, skip);
+      check(err < 1e-10, "cgs2 Q^T Q = I and A = QR", err);
+      {
+         auto h = host_of(A);
+         for (int j = 3; j <= 5; ++j)
+            for (int i = 0; i < m; ++i) h[i + (size_t)j * m] = h[i + (size_t)(j - 3) * m];
+         (void)hipMemcpy(A.data(), h.data(), h.size() * 8, hipMemcpyHostToDevice);
+      }
+      for (int v = 0; v < 3; ++v) {
+         const char *name[3] = {"mgs", "mgs_v2", "cgs"};
+         skip = v == 0 ? mgs<double, double, double>(A, Q, R, 1e-12, 1e-12, 0.70710678118654752)
+                : v == 1 ? mgs_v2<double, double, double>(A, Q, R, 1e-12)
+                         : cgs<double, double, double>(A, Q, R, 1e-12);
+         err = qr_err(host_of(A), host_of(Q), R, skip);
+         bool pattern = skip[3] == 1 && skip[4] == 1 && skip[5] == 1 && skip[2] == 0 && skip[6] == 0;
+         check(err < 1e-10 && pattern, (std::string(name[v]) + " duplicated columns dropped").c_str(), err);
+      }
+      // column steps on the same matrix: mgs_step into a host R, cgs_step into R_device + r_diag
+      Matrix<double> Qs(m, n), Rs(n, n, Location::kHOST), Rd(n, n);
+      Vector<double> rdiag(n, Location::kHOST);
+      std::vector<int> ss(n, 0), sref = mgs<double, double, double>(A, Q, R, 1e-12);
+      (void)hipMemcpy(Qs.data(), A.data(), (size_t)m * n * 8, hipMemcpyDeviceToDevice);
+      for (int j = 0; j < n; ++j)
+         for (int i = 0; i < n; ++i) Rs(i, j) = 0.0;
+      for (int i = 0; i < n; ++i) mgs_step<double, double>(Qs, Rs, ss, i, 1e-12);
+      double dq = 0;
+      auto hq = host_of(Q), hqs = host_of(Qs);
+      for (size_t t = 0; t < hq.size(); ++t) dq = std::fmax(dq, std::fabs(hq[t] - hqs[t]));
+      for (int j = 0; j < n; ++j)
+         for (int i = 0; i < n; ++i) dq = std::fmax(dq, std::fabs(R(i, j) - Rs(i, j)));
+      check(dq < 1e-12 && ss == sref, "mgs_step sequence = mgs", dq);
+      sref = cgs<double, double, double>(A, Q, R, 1e-12);
+      (void)hipMemcpy(Qs.data(), A.data(), (size_t)m * n * 8, hipMemcpyDeviceToDevice);
+      std::fill(ss.begin(), ss.end(), 0);
+      for (int i = 0; i < n; ++i) rdiag[i] = 1.0;
+      for (int i = 0; i < n; ++i) cgs_step<double, double>(Qs, Rd, rdiag, ss, i, 1e-12);
+      auto hrd = host_of(Rd);
+      dq = 0;
+      hq = host_of(Q);
+      hqs = host_of(Qs);
+      for (size_t t = 0; t < hq.size(); ++t) dq = std::fmax(dq, std::fabs(hq[t] - hqs[t]));
+      for (int j = 0; j < n; ++j) {
+         for (int i = 0; i < j; ++i) dq = std::fmax(dq, std::fabs(R(i, j) - hrd[i + (size_t)j * n]));
+         dq = std::fmax(dq, std::fabs(R(j, j) - rdiag[j]));
+      }
+      check(dq == 0.0 && ss == sref, "cgs_step sequence = cgs (bitwise)", dq);
+      // FP32 cgs2 (qr_test.cpp:386-470): 2e-6
+      Matrix<float> Af(m, n), Qf(m, n), Rf(n, n, Location::kHOST);
+      Af.fill_random(12);
+      std::vector<int> sf = cgs2<float, float, float>(Af, Qf, Rf);
+      auto haf = host_of(Af), hqf = host_of(Qf);
+      double ef = 0;
+      for (int a = 0; a < n; a += 3)
+         for (int b = a; b < n; b += 5) {
+            double s = 0;
+            for (int i = 0; i < m; ++i) s += (double)hqf[i + (size_t)a * m] * hqf[i + (size_t)b * m];
+            ef = std::fmax(ef, std::fabs(s - (a == b ? 1.0 : 0.0)));
+         }
+      for (int j = 0; j < n; j += 3)
+         for (int i = 0; i < m; i += 11) {
+            double s = 0;
+            for (int p = 0; p <= j; ++p) s += (double)hqf[i + (size_t)p * m] * Rf(p, j);
+            ef = std::fmax(ef, std::fabs(s - haf[i + (size_t)j * m]));
+         }
+      check(ef < 2e-6 && sf[0] == 0, "cgs2 FP32 within 2e-6", ef);
+      bool threw = false;
+      try {
+         Matrix<double> Rdev(n, n);
+         mgs<double, double, double>(A, Q, Rdev);
+      } catch (const std::runtime_error &e) {
+         threw = std::string(e.what()) == "Output matrix R must be on host";
+      }
+      check(threw, "mgs rejects a device R with the reference's message");
+   }
    // ---- si_eig / arnoldi_eig on the dense 2D Laplacian, 24 x 24 grid
    {
       const int N = 24, n = N * N;
