@@ -724,7 +724,7 @@ struct DenseSymOp final : FilterOp {
 };
 
 // A^T A for a dense M x N A (FP32, column-major): the operator of the singular
-// value solver (north-star cfg3: OFRR on the implicit A^T A).  A X and A^T Y of
+// value solver (filtered OFRR on the implicit A^T A).  A X and A^T Y of
 // the growth steps are BF16 MFMA products with FP32 accumulation; the residual
 // and the final vectors use the FP32 A with FP64 accumulation.
 struct AtaOp final : FilterOp {

@@ -6,7 +6,7 @@
  * What pins each function (tests/test_oracle_*.py):
  *   orc_philox4x32_10   Random123 published known-answer vectors (Philox4x32-10)
  *   orc_randn/randu     moments + Kolmogorov-Smirnov against the normal/uniform laws
- *   orc_stencil_csr     closed-form nnz counts (cfg1 20,224 / cfg2 14,581,760), symmetry,
+ *   orc_stencil_csr     closed-form nnz counts (64^2 grid 20,224 / 128^3 grid 14,581,760), symmetry,
  *                       sine-mode eigenvectors A s = lambda s (closed form), scipy kron build
  *   orc_spmm_csr        brute-force dense A X, integer-valued exact results
  *   orc_scalecols       hand cases incl. ties and zero columns

@@ -105,9 +105,9 @@ kats = {
     },
     "laplacian": {
         "source": "closed form lambda = sum_d 4 sin^2(p_d pi / (2 (N+1)))",
-        "cfg1_2d_N64": {"n": 4096, "nnz": 20224, "top": lap_eigs(2, 64, 24)},
-        "cfg2_3d_N128": {"n": 2097152, "nnz": 14581760, "top": lap_eigs(3, 128, 70)},
-        "cfg5_3d_N256": {"n": 16777216, "nnz": 117047296},
+        "lap2d_N64": {"n": 4096, "nnz": 20224, "top": lap_eigs(2, 64, 24)},
+        "lap3d_N128": {"n": 2097152, "nnz": 14581760, "top": lap_eigs(3, 128, 70)},
+        "lap3d_N256": {"n": 16777216, "nnz": 117047296},
     },
 }
 

@@ -1,7 +1,8 @@
 """Dense operators on the GPU: MFMA products (gemm over dense operands,
 mvops.cu:423-605 / gemm.m:27-59), filtered OFRR on a dense symmetric A
 (si_eig over a dense Matrix, si_eig.hpp:24-27) and the truncated SVD by
-OFRR on A^T A (si_svd / subspace_iter_svd.m, north-star cfg3 at a reduced size).
+OFRR on A^T A (si_svd / subspace_iter_svd.m; the dense low-rank-plus-noise SVD workload at a
+reduced size).
 
 References: FP64 products of the same (rounded) operands, the closed-form
 Laplacian spectrum, and numpy's LAPACK SVD of the same FP32 matrix."""

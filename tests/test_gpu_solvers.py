@@ -14,8 +14,8 @@ def _csr(ctx, Ah):
     return api.CSRMatrix.from_host(ctx, Ah.row_ptr, Ah.col_idx, Ah.vals)
 
 
-def test_arnoldi_cfg1_matches_cpu(ctx):
-    """cfg1: 2D 64x64 Laplacian, kdim 60, 20 largest, N(0,1) start from seed 0:
+def test_arnoldi_laplacian64_matches_cpu(ctx):
+    """2D 64x64 Laplacian, kdim 60, 20 largest, N(0,1) start from seed 0:
     FP64 GPU Ritz values within 1e-12 (relative) of the CPU FP64 oracle."""
     from mixedprecisionofrr_amd import api
     Ah = O.stencil(2, 64)
@@ -24,7 +24,7 @@ def test_arnoldi_cfg1_matches_cpu(ctx):
     api.fill_randn(ctx, v0, seed=0)
     v0h = v0.cpu().numpy()
     for maxiter in (61, 1 + 60 * 3):
-        print(f"arnoldi cfg1 maxiter={maxiter}", flush=True)
+        print(f"arnoldi 64x64 maxiter={maxiter}", flush=True)
         th, X, built = api.arnoldi_hess_eig(ctx, A, 20, 60, maxiter, v0)
         th_o, X_o, built_o = O.arnoldi_hess_eig(Ah, 20, 60, maxiter, v0h)
         assert built == built_o == 61

@@ -49,7 +49,7 @@ def test_stencil_closed_form(dim, N, kats):
     lam = sum(4 * np.sin(pi * np.pi / (2 * (N + 1))) ** 2 for pi in p)
     assert np.abs(O.spmm(A, s)[:, 0] - lam * s).max() < 1e-12
     if (dim, N) == (2, 64):
-        assert np.allclose(O.laplacian_eigs(2, 64)[:24], kats["laplacian"]["cfg1_2d_N64"]["top"], rtol=0, atol=1e-15)
+        assert np.allclose(O.laplacian_eigs(2, 64)[:24], kats["laplacian"]["lap2d_N64"]["top"], rtol=0, atol=1e-15)
 
 
 def test_stencil_variable_coefficients():
@@ -225,7 +225,7 @@ def test_arnoldi_exact_termination():
     assert np.allclose(th, np.sort(lam[idx])[::-1], rtol=1e-12)
 
 
-def test_arnoldi_cfg1_converges_top():
+def test_arnoldi_laplacian64_converges_top():
     A = O.stencil(2, 64)
     v0 = O.randn(0, A.n)
     th, X, built = O.arnoldi_hess_eig(A, 20, 60, 1 + 60 * 4, v0)
