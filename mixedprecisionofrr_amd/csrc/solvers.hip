@@ -78,6 +78,43 @@ __global__ __launch_bounds__(256) void resid_cols_kernel(int64_t n, const double
    }
 }
 
+// the same sums for row-major X (row stride xrs) and AX (row stride k): thread
+// t serves column t % k of the rows t / k, t / k + G, ... (G = 256 / k rows
+// per pass, k <= 256), so a wave reads whole row segments; the G partial sums
+// of a column are added in a fixed order
+__global__ __launch_bounds__(256) void resid_rows_kernel(int64_t n, int k,
+                                                         const double *__restrict__ AX,
+                                                         const double *__restrict__ X, int64_t xrs,
+                                                         const double *__restrict__ theta,
+                                                         int64_t chunk, double *__restrict__ part) {
+   __shared__ double s1[256], s2[256];
+   const int G = 256 / k;
+   const int t = threadIdx.x, c = t % k, g = t / k;
+   const int64_t r0 = (int64_t)blockIdx.x * chunk, r1 = min(n, r0 + chunk);
+   double a = 0.0, b = 0.0;
+   if (g < G) {
+      const double th = theta[c];
+      for (int64_t r = r0 + g; r < r1; r += G) {
+         const double x = X[r * xrs + c];
+         const double d = AX[r * k + c] - th * x;
+         a = fma(d, d, a);
+         b = fma(x, x, b);
+      }
+   }
+   s1[t] = a;
+   s2[t] = b;
+   __syncthreads();
+   if (t < k) {
+      double sa = 0.0, sb = 0.0;
+      for (int q = 0; q < G; ++q) {
+         sa += s1[q * k + t];
+         sb += s2[q * k + t];
+      }
+      part[(int64_t)blockIdx.x * 2 * k + 2 * t] = sa;
+      part[(int64_t)blockIdx.x * 2 * k + 2 * t + 1] = sb;
+   }
+}
+
 // mirror the upper triangle of an m x m column-major matrix into its lower one
 __global__ void mirror_upper_kernel(int m, double *__restrict__ G, int ldg) {
    int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -190,7 +227,8 @@ int ofrr_residuals(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, con
    if (n <= 0 || k <= 0 || !rp || !ci || !va || !X || !theta_host || !res_host)
       return set_error(ctx, kErrorInvalidArgument, "residuals: bad arguments");
    (void)nnz;
-   int64_t chunk = 16384;
+   const bool rowmajor = xcs == 1 && k <= 256;
+   int64_t chunk = rowmajor ? 2048 : 16384;
    int nch = (int)((n + chunk - 1) / chunk);
    size_t need = al((size_t)n * k * 8) + al((size_t)k * 8) + al((size_t)nch * 2 * k * 8) + al((size_t)2 * k * 8);
    char *ws = (char *)workspace(ctx, need);
@@ -205,7 +243,10 @@ int ofrr_residuals(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, con
    const int64_t ars = (xcs == 1) ? k : 1, acs = (xcs == 1) ? 1 : n;
    int st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, k, X, OFRR_F64, xrs, xcs, 1.0, 0.0, AX, OFRR_F64, ars, acs);
    if (st) return st;
-   resid_cols_kernel<<<dim3(nch, k), 256, 0, ctx->stream>>>(n, AX, ars, acs, X, xrs, xcs, th, chunk, part);
+   if (rowmajor)
+      resid_rows_kernel<<<nch, 256, 0, ctx->stream>>>(n, k, AX, X, xrs, th, chunk, part);
+   else
+      resid_cols_kernel<<<dim3(nch, k), 256, 0, ctx->stream>>>(n, AX, ars, acs, X, xrs, xcs, th, chunk, part);
    OFRR_LAUNCH(ctx);
    if ((st = reduce_partials(ctx, nch, 2 * k, part, sums))) return st;
    OFRR_HIP(ctx, hipMemcpyAsync(hs, sums, (size_t)2 * k * 8, hipMemcpyDeviceToHost, ctx->stream));

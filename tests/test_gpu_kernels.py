@@ -352,6 +352,9 @@ def test_residuals_vs_oracle(ctx):
     ref = O.residuals(Ah, X, th, lam[-1])
     got = api.residuals(ctx, _csr(ctx, Ah), _dev(X.T).t(), th, lam[-1])
     assert np.allclose(got, ref, rtol=1e-12, atol=0)
+    # row-major block (the layout the solvers keep): the coalesced row kernel
+    got_r = api.residuals(ctx, _csr(ctx, Ah), _dev(X), th, lam[-1])
+    assert np.allclose(got_r, ref, rtol=1e-12, atol=0)
 
 
 def test_fp8_hardware_conversion_agrees(ctx):
