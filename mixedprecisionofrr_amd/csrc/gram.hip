@@ -158,34 +158,62 @@ int gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64
 constexpr int kGramWaves = 12;                      // tiles (waves) per workgroup
 constexpr int kGramThreads = 64 * kGramWaves;
 constexpr int kGramStageOps = 6;                    // operands staged per workgroup
-constexpr int kGramPasses = 3;                      // staging loads per thread and operand
+// staging loads per thread and operand: slabs of R = 1024 / k rows.  Three
+// (R = 2048 / k) need 36 prefetch registers per thread and spill 19-26 at
+// 3 waves per SIMD; two measured 30.1 against 31.4 ms of mixed and 27.1
+// against 41.5 ms of FP64 projection products per 128^3 solve
+constexpr int kGramPasses = 2;
 constexpr int kMaxHalfItems = 2 * kMaxGramItems;
-constexpr int kMaxGramGroups = (kMaxHalfItems + kGramWaves - 1) / kGramWaves;
 
+// the staged operands op[0 .. N0 + N1 + N2) in kind order: N0 FP64, N1 in the
+// basis format TB, N2 in the format TZ; every workgroup stages them all
 struct GramLdsArgs {
    GramOperand op[kMaxGramOps];
    int nitems, R, stride, lk;
    short slot_a[kMaxHalfItems], slot_b[kMaxHalfItems], i0[kMaxHalfItems], j0[kMaxHalfItems];
-   signed char gop[kMaxGramGroups][kGramStageOps];  // operand staged in each slot of a group
-   signed char gnop[kMaxGramGroups];
 };
 
-template <typename TB, typename TZ>
-__device__ inline double gram_load_kind(const GramOperand &o, int64_t off) {
-   if (o.kind == 0) return ((const double *)o.p)[off];
-   if (o.kind == 1) return (double)Elem<TB>::load(((const TB *)o.p)[off]);
-   return (double)Elem<TZ>::load(((const TZ *)o.p)[off]);
+// raw bits of one staged element as two 32-bit halves: a load of any width
+// writes its registers directly and nothing touches them until the store
+// (hi is read for 8-byte formats only)
+struct GramRaw {
+   uint32_t lo, hi;
+};
+template <typename T> __device__ inline void gram_raw_load(GramRaw &r, const void *p, int64_t off) {
+   if constexpr (sizeof(T) == 8) {
+      const uint2 v = reinterpret_cast<const uint2 *>(p)[off];
+      r.lo = v.x;
+      r.hi = v.y;
+   } else if constexpr (sizeof(T) == 4) {
+      r.lo = reinterpret_cast<const uint32_t *>(p)[off];
+   } else if constexpr (sizeof(T) == 2) {
+      r.lo = reinterpret_cast<const uint16_t *>(p)[off];
+   } else {
+      r.lo = reinterpret_cast<const uint8_t *>(p)[off];
+   }
+}
+template <typename T> __device__ inline double gram_raw_value(const GramRaw &r) {
+   if constexpr (sizeof(T) == 8)
+      return __builtin_bit_cast(double, ((uint64_t)r.hi << 32) | r.lo);
+   else if constexpr (sizeof(T) == 4)
+      return (double)Elem<T>::load(__builtin_bit_cast(T, r.lo));
+   else if constexpr (sizeof(T) == 2)
+      return (double)Elem<T>::load(__builtin_bit_cast(T, (uint16_t)r.lo));
+   else
+      return (double)Elem<T>::load((T)(uint8_t)r.lo);
 }
 
-template <typename TB, typename TZ>
+template <typename TB, typename TZ, int N0, int N1, int N2>
 __global__ __launch_bounds__(kGramThreads) void gram_lds_kernel(int64_t n, int k, GramLdsArgs args,
                                                                 int64_t rows_per_chunk, int nchunks,
                                                                 int ngroups, double *__restrict__ part) {
-   extern __shared__ double sbuf[];  // [nop][R][stride]
+   constexpr int NOP = N0 + N1 + N2;
+   static_assert(NOP <= kGramStageOps, "staged operands");
+   extern __shared__ double sbuf[];  // [NOP][R][stride]
    const int chunk = blockIdx.x / ngroups, grp = blockIdx.x % ngroups;
    const int64_t r0 = (int64_t)chunk * rows_per_chunk;
    const int64_t r1 = min(n, r0 + rows_per_chunk);
-   const int R = args.R, stride = args.stride, nop = args.gnop[grp];
+   const int R = args.R, stride = args.stride;
    const int slab = R * stride;                       // doubles per operand slab
    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
    const int li = lane & 15, lk = lane >> 4;
@@ -203,39 +231,49 @@ __global__ __launch_bounds__(kGramThreads) void gram_lds_kernel(int64_t n, int k
    for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) acc[a][b] = dbl4{0.0, 0.0, 0.0, 0.0};
-   // staging: operand slot o (wave-uniform loop), element e = tid + 768 q of
-   // its R x k slab (R k <= 2048, k and R powers of two): row e >> lk, column
-   // e & (k - 1)
+   // staging: operand slot o, element e = tid + 768 q of its R x k slab
+   // (R k <= 1024, k and R powers of two): row e >> lk, column e & (k - 1).
+   // The slots' formats are compile-time, so the prefetch is a run of
+   // unconditional loads from clamped (valid) addresses with nothing done to
+   // their registers until store() converts them after the MFMAs; store()
+   // zeroes the rows past the end of the chunk.
    const int lk2 = args.lk, rk = R << lk2;
-   auto fetch = [&](int64_t rb, double pre[kGramPasses * kGramStageOps]) {
+   auto fetch = [&](int64_t rb, GramRaw pre[kGramPasses * NOP]) {
 #pragma unroll
-      for (int o = 0; o < kGramStageOps; ++o) {
-         if (o >= nop) break;
-         const GramOperand op = args.op[args.gop[grp][o]];
+      for (int o = 0; o < NOP; ++o) {
+         const GramOperand &op = args.op[o];
 #pragma unroll
          for (int q = 0; q < kGramPasses; ++q) {
-            const int e = threadIdx.x + q * kGramThreads;
-            const int64_t row = rb + (e >> lk2);
-            double v = 0.0;
-            if (e < rk && row < r1) v = gram_load_kind<TB, TZ>(op, row * op.rs + (e & (k - 1)));
-            pre[kGramPasses * o + q] = v;
+            const int e = min((int)threadIdx.x + q * kGramThreads, rk - 1);
+            const int64_t row = min(rb + (e >> lk2), r1 - 1);
+            const int64_t off = row * op.rs + (e & (k - 1));
+            GramRaw &r = pre[kGramPasses * o + q];
+            if (o < N0)
+               gram_raw_load<double>(r, op.p, off);
+            else if (o < N0 + N1)
+               gram_raw_load<TB>(r, op.p, off);
+            else
+               gram_raw_load<TZ>(r, op.p, off);
          }
       }
    };
-   auto store = [&](const double pre[kGramPasses * kGramStageOps]) {
+   auto store = [&](int64_t rb, const GramRaw pre[kGramPasses * NOP]) {
 #pragma unroll
-      for (int o = 0; o < kGramStageOps; ++o) {
-         if (o >= nop) break;
+      for (int o = 0; o < NOP; ++o) {
 #pragma unroll
          for (int q = 0; q < kGramPasses; ++q) {
             const int e = threadIdx.x + q * kGramThreads;
-            if (e < rk) sbuf[o * slab + (e >> lk2) * stride + (e & (k - 1))] = pre[kGramPasses * o + q];
+            const GramRaw &r = pre[kGramPasses * o + q];
+            double v = (o < N0) ? gram_raw_value<double>(r)
+                       : (o < N0 + N1) ? gram_raw_value<TB>(r) : gram_raw_value<TZ>(r);
+            if (rb + (e >> lk2) >= r1) v = 0.0;
+            if (e < rk) sbuf[o * slab + (e >> lk2) * stride + (e & (k - 1))] = v;
          }
       }
    };
-   double pre[kGramPasses * kGramStageOps];
+   GramRaw pre[kGramPasses * NOP];
    fetch(r0, pre);
-   store(pre);
+   store(r0, pre);
    __syncthreads();
    for (int64_t rb = r0; rb < r1; rb += R) {
       const bool more = rb + R < r1;
@@ -258,7 +296,7 @@ __global__ __launch_bounds__(kGramThreads) void gram_lds_kernel(int64_t n, int k
          }
       }
       __syncthreads();  // every wave is done with this slab
-      if (more) store(pre);
+      if (more) store(rb + R, pre);
       __syncthreads();
    }
    if (!has) return;
@@ -313,7 +351,6 @@ int gram_multi(ofrr_ctx *ctx, int64_t n, int k, int tb, int tz, GramMultiArgs &a
    // 64 x 32 output tiles of the 64 x 64 items (halves past column k dropped)
    GramLdsArgs la{};
    GramOutArgs oa{};
-   for (int q = 0; q < kMaxGramOps; ++q) la.op[q] = args.op[q];
    int nh = 0;
    int ha[kMaxHalfItems], hb[kMaxHalfItems];
    for (int t = 0; t < args.nitems; ++t)
@@ -332,25 +369,28 @@ int gram_multi(ofrr_ctx *ctx, int64_t n, int k, int tb, int tz, GramMultiArgs &a
       }
    la.nitems = oa.nitems = nh;
    const int ngroups = (nh + kGramWaves - 1) / kGramWaves;
-   int nop_max = 0;
-   for (int g = 0; g < ngroups; ++g) {
-      int nop = 0;
-      auto slot = [&](int opi) {
-         for (int s = 0; s < nop; ++s)
-            if (la.gop[g][s] == opi) return s;
-         la.gop[g][nop] = (signed char)opi;
-         return nop++;
-      };
-      for (int t = g * kGramWaves; t < std::min(nh, (g + 1) * kGramWaves); ++t) {
-         la.slot_a[t] = (short)slot(ha[t]);
-         la.slot_b[t] = (short)slot(hb[t]);
-         if (nop > kGramStageOps) return set_error(ctx, kErrorInternalError, "gram_multi: operands per group");
-      }
-      la.gnop[g] = (signed char)nop;
-      nop_max = std::max(nop_max, nop);
+   // the staged operands: those the items use, in kind order
+   bool used[kMaxGramOps] = {};
+   for (int t = 0; t < nh; ++t) used[ha[t]] = used[hb[t]] = true;
+   int pos[kMaxGramOps], cnt[3] = {0, 0, 0}, nop = 0;
+   for (int kind = 0; kind < 3; ++kind)
+      for (int q = 0; q < kMaxGramOps; ++q)
+         if (used[q] && args.op[q].kind == kind) {
+            pos[q] = nop;
+            la.op[nop++] = args.op[q];
+            ++cnt[kind];
+         }
+   for (int q = 0; q < kMaxGramOps; ++q)
+      if (used[q] && (args.op[q].kind < 0 || args.op[q].kind > 2))
+         return set_error(ctx, kErrorInvalidArgument, "gram_multi: operand kind");
+   if (nop > kGramStageOps) return set_error(ctx, kErrorInternalError, "gram_multi: staged operands");
+   for (int t = 0; t < nh; ++t) {
+      la.slot_a[t] = (short)pos[ha[t]];
+      la.slot_b[t] = (short)pos[hb[t]];
    }
-   // R rows per slab: R k <= 2048 elements per operand, three per thread
-   la.R = std::max(4, std::min(128, 2048 / k));  // a power of two: R k <= 2048 <= 3 per thread
+   const int nop_max = nop;
+   // R rows per slab: R k <= 1024 elements per operand, two per thread
+   la.R = std::max(4, std::min(128, 1024 / k));  // a power of two: R k <= 1024 <= 2 x 768
    la.lk = 0;
    while ((1 << la.lk) < k) ++la.lk;
    if ((1 << la.lk) != k) return set_error(ctx, kErrorInvalidArgument, "gram_multi: k a power of two");
@@ -364,21 +404,32 @@ int gram_multi(ofrr_ctx *ctx, int64_t n, int k, int tb, int tz, GramMultiArgs &a
    double *part = (double *)workspace(ctx, (size_t)nchunks * nh * (GT * GT / 2) * sizeof(double));
    if (!part) return set_error(ctx, kErrorCUDAMemory, "gram_multi: workspace");
    const unsigned grid = (unsigned)(nchunks * ngroups);
+   // operand patterns (FP64, basis format, Z format) of the filtered solver:
+   // [X W | D_d D_mid | Z_d Z_mid] and [X W | D_d | Z_d]
+   const int pattern = (cnt[0] == 2 && cnt[1] == 2 && cnt[2] == 2) ? 222
+                       : (cnt[0] == 2 && cnt[1] == 1 && cnt[2] == 1) ? 211 : 0;
+   if (!pattern) return set_error(ctx, kErrorInvalidArgument, "gram_multi: operand pattern");
    int st = dispatch_dtype(tb, [&](auto b) -> int {
       using TB = decltype(b);
       return dispatch_dtype(tz, [&](auto z) -> int {
          using TZ = decltype(z);
-         static std::once_flag once;
-         static hipError_t attr = hipSuccess;
-         std::call_once(once, [] {
-            attr = hipFuncSetAttribute((const void *)gram_lds_kernel<TB, TZ>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-         });
-         OFRR_HIP(ctx, attr);
-         gram_lds_kernel<TB, TZ><<<grid, kGramThreads, lds, ctx->stream>>>(n, k, la, rows, nchunks,
-                                                                          ngroups, part);
-         OFRR_LAUNCH(ctx);
-         return kSuccess;
+         // one instantiation (and one attribute set-up) per pattern
+         auto launch = [&](auto tag) -> int {
+            constexpr int P = decltype(tag)::value;
+            constexpr auto kern = (P == 222) ? gram_lds_kernel<TB, TZ, 2, 2, 2> : gram_lds_kernel<TB, TZ, 2, 1, 1>;
+            static std::once_flag once;
+            static hipError_t attr = hipSuccess;
+            std::call_once(once, [] {
+               attr = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          160 * 1024);
+            });
+            OFRR_HIP(ctx, attr);
+            kern<<<grid, kGramThreads, lds, ctx->stream>>>(n, k, la, rows, nchunks, ngroups, part);
+            OFRR_LAUNCH(ctx);
+            return kSuccess;
+         };
+         return pattern == 222 ? launch(std::integral_constant<int, 222>{})
+                               : launch(std::integral_constant<int, 211>{});
       });
    });
    if (st) return st;
