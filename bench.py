@@ -51,15 +51,16 @@ def laplacian_top(dim, N, count):
     return np.sort(L)[::-1][:count]
 
 
-def filter_bytes(n, nnz, k, b_val, b_basis, steps_with_prev, steps_first):
-    """Algorithmic HBM bytes of the Chebyshev-step launches: operator (values +
-    int32 columns + int64 row pointers) once per launch, the gathered block once,
-    the previous block, the residual block and the output block.  The first step
-    of each sweep gathers the residual block itself and has no previous block."""
+def filter_bytes(n, nnz, b_val, b_basis, launches, col_steps, first_cols):
+    """Algorithmic HBM bytes of the Chebyshev-step launches of a solve: the
+    operator (values + int32 columns + int64 row pointers) once per launch; per
+    launch of width w (the columns filtered: k, or fewer once pairs are locked)
+    the gathered block, the previous block, the residual block and the output
+    block, n w elements each.  The first step of each sweep gathers the residual
+    block itself and has no previous block (2 blocks).  col_steps is the sum of
+    the widths over the launches, first_cols the sum over the first steps."""
     op = nnz * (b_val + 4) + 8 * (n + 1)
-    full = op + 4 * n * k * b_basis
-    first = op + 2 * n * k * b_basis
-    return steps_with_prev * full + steps_first * first
+    return launches * op + n * b_basis * (4 * col_steps - 2 * first_cols)
 
 
 PROFILE = os.path.join(ROOT, "profiles", "cheb_step_f16_laplacian128_k64.md")
@@ -88,9 +89,12 @@ def run_solver(api, ctx, A, X0, args, basis, label):
     return st, th, X, res, stats
 
 
-def cpu_baseline(args, gpu_iters, A_dims):
+def cpu_baseline(args, gpu_iters, gpu_col_steps, A_dims):
     """FP64 oracle on the same operator and start block: one outer iteration at
-    two filter degrees (bounded sample), extrapolated to the GPU iteration count."""
+    two filter degrees (bounded sample), extrapolated to the GPU solve: the
+    per-iteration remainder times its iterations plus the cost of a k-column
+    filter step times its filtered columns / k (soft locking narrows the later
+    sweeps)."""
     import numpy as np
 
     from oracle import oracle as O
@@ -111,8 +115,8 @@ def cpu_baseline(args, gpu_iters, A_dims):
         log(f"cpu baseline degree {deg}: {times[deg]:.1f} s")
     step = max(0.0, (times[4] - times[2]) / 2.0)
     fixed = max(0.0, times[2] - step)          # initial RR + 1 iteration at degree 2 - 1 step
-    per_iter = fixed + (args.degree - 1) * step
-    solve_s = per_iter * max(1, gpu_iters)
+    col_steps = gpu_col_steps / float(args.k)
+    solve_s = fixed * max(1, gpu_iters) + step * col_steps
     return {
         "value": args.nwant / solve_s,
         "unit": "eigenpairs/s",
@@ -121,8 +125,9 @@ def cpu_baseline(args, gpu_iters, A_dims):
         "sample": (f"FP64 oracle (oracle/ofrr_oracle.c, same algorithm and start block) on the "
                    f"full {N}^3 operator: one outer iteration at filter degree 2 and 4 "
                    f"({times[2]:.1f} s, {times[4]:.1f} s) -> {step:.2f} s per filter step, "
-                   f"{fixed:.2f} s per-iteration remainder; solve time = {args.degree - 1} steps + "
-                   f"remainder times the GPU's {gpu_iters} iterations = {solve_s:.0f} s; "
+                   f"{fixed:.2f} s per-iteration remainder; solve time = remainder x the GPU's "
+                   f"{gpu_iters} iterations + step x its {col_steps:.1f} k-column-equivalent filter "
+                   f"steps = {solve_s:.0f} s; "
                    f"operator build {t_build:.1f} s excluded"),
     }
 
@@ -242,9 +247,9 @@ def main():
     n, nnz, k = A.n, A.nnz, args.k
     sweeps = stats["iterations"]
     per_solve_steps = stats["filter_steps"]
-    firsts = sweeps
     b_basis = 1 if args.basis == "fp8" else 2
-    bytes_solve = filter_bytes(n, nnz, k, 2, b_basis, per_solve_steps - firsts, firsts)
+    bytes_solve = filter_bytes(n, nnz, 2, b_basis, per_solve_steps, stats["filter_col_steps"],
+                               stats["filter_first_cols"])
     avg_launch_ms = filt_ms / max(1, filt_steps)
     bytes_per_launch = bytes_solve / max(1, per_solve_steps)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
@@ -272,7 +277,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(args, stats["iterations"], (3, N))
+            cpu = cpu_baseline(args, stats["iterations"], stats["filter_col_steps"], (3, N))
         except Exception as e:  # reported, never silently replaced
             cpu = {"value": None, "unit": "eigenpairs/s", "cores": 0, "kind": "port",
                    "sample": f"failed: {e!r}"}

@@ -239,6 +239,11 @@ typedef struct {
                         FP64 refinement also uses the FP64 operator values); 0: off */
    int refine_dtype; /* ofrr_dtype of the refinement format */
    double refine_tol;
+   int lock;         /* CSR operator only.  1: soft locking -- when the leading wanted
+                        pairs 0..p-1 meet tol, the sweep filters only the trailing
+                        columns: width the smallest power of two >= k - p, halved from
+                        k, at least 16 (the Ritz block keeps all k columns, so locked
+                        pairs stay in every Rayleigh-Ritz); 0: every sweep filters k */
 } ofrr_filter_opts;
 
 typedef struct {
@@ -255,6 +260,8 @@ typedef struct {
    double t_rr_ms;       /* Rayleigh-Ritz */
    double t_update_ms;   /* back-transform */
    double t_resid_ms;    /* FP64 operator application + residual (incl. host checks) */
+   int filter_col_steps; /* sum over the Chebyshev steps of the columns filtered */
+   int filter_first_cols;/* sum over the sweeps of the columns of the first step */
 } ofrr_filter_stats;
 
 /* X_dev: n x k FP64 row-major (row stride k), start block in / Ritz block out;

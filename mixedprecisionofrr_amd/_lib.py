@@ -36,7 +36,8 @@ class FilterOpts(C.Structure):
     _fields_ = [("k", C.c_int), ("nwant", C.c_int), ("nd", C.c_int), ("degree", C.c_int),
                 ("maxiter", C.c_int), ("basis_dtype", C.c_int), ("tol", C.c_double),
                 ("tau", C.c_double), ("lower", C.c_double), ("normA", C.c_double),
-                ("refine", C.c_int), ("refine_dtype", C.c_int), ("refine_tol", C.c_double)]
+                ("refine", C.c_int), ("refine_dtype", C.c_int), ("refine_tol", C.c_double),
+                ("lock", C.c_int)]
 
 
 class FilterStats(C.Structure):
@@ -44,7 +45,8 @@ class FilterStats(C.Structure):
                 ("rank_min", C.c_int), ("filter_steps", C.c_int), ("resid_passes", C.c_int),
                 ("t_total_ms", C.c_double), ("t_filter_ms", C.c_double),
                 ("t_zspmm_ms", C.c_double), ("t_gram_ms", C.c_double), ("t_rr_ms", C.c_double),
-                ("t_update_ms", C.c_double), ("t_resid_ms", C.c_double)]
+                ("t_update_ms", C.c_double), ("t_resid_ms", C.c_double),
+                ("filter_col_steps", C.c_int), ("filter_first_cols", C.c_int)]
 
 
 class ServerOpts(C.Structure):

@@ -386,16 +386,18 @@ def subspace_iter_eig(ctx, A, V0, maxiter, step_size=1, select_tol=2.0 ** -52, g
 
 def filtered_eig(ctx, A, X0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=torch.float16,
                  tol=1e-8, tau=1e-12, lower=0.0, normA=0.0, use_lo_values=True,
-                 refine_dtype=None, refine_tol=0.0):
+                 refine_dtype=None, refine_tol=0.0, lock=True):
     """Filtered OFRR solve; X0 (n x k, FP64) is copied.  refine_dtype: the
-    precision ladder's refinement format (None: off).  Returns
-    (status, theta, X row-major, res, stats dict)."""
+    precision ladder's refinement format (None: off).  lock: soft locking (a
+    sweep filters only the columns after the leading converged wanted pairs).
+    Returns (status, theta, X row-major, res, stats dict)."""
     n, k = X0.shape
     X = X0.to(torch.float64).contiguous().clone()
     bdt = dtype_code(basis_dtype)
     o = FilterOpts(k, nwant, nd, degree, maxiter, bdt, tol, tau, lower, normA,
                    0 if refine_dtype is None else 1,
-                   0 if refine_dtype is None else dtype_code(refine_dtype), refine_tol)
+                   0 if refine_dtype is None else dtype_code(refine_dtype), refine_tol,
+                   1 if lock else 0)
     th = (C.c_double * k)()
     res = (C.c_double * k)()
     stats = FilterStats()

@@ -659,6 +659,8 @@ struct FilterOp {
    // ladder), and the switch to the FP64 operator values for an FP64 format
    virtual bool can_refine() const { return false; }
    virtual void set_precise(bool) {}
+   // whether a sweep may filter a trailing column range only (soft locking)
+   virtual bool can_lock() const { return false; }
 };
 
 struct CsrOp final : FilterOp {
@@ -700,7 +702,20 @@ struct CsrOp final : FilterOp {
    }
    bool can_refine() const override { return true; }
    void set_precise(bool p) override { precise = p; }
+   bool can_lock() const override { return true; }
 };
+
+// Y (n x wr 8-byte words, row-major) = words [c0, c0 + wr) of every row of X
+// (row stride ws words): the trailing columns of a row-major block, compacted
+__global__ void copy_cols_kernel(int64_t n, int wr, int64_t ws, int64_t c0,
+                                 const uint64_t *__restrict__ X, uint64_t *__restrict__ Y) {
+   const int64_t total = n * wr;
+   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+        t += (int64_t)gridDim.x * blockDim.x) {
+      const int64_t i = t / wr;
+      Y[t] = X[i * ws + c0 + (t - i * wr)];
+   }
+}
 
 // Y(i, j) *= s[j] for a row-major m x k block
 __global__ void scale_cols_rm_kernel(int64_t m, int k, double *__restrict__ Y,
@@ -901,7 +916,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    hipEvent_t ev[8];
    for (int i = 0; i < 8; ++i) OFRR_HIP(ctx, hipEventCreate(&ev[i]));
    float t_acc[7] = {0, 0, 0, 0, 0, 0, 0};
-   int filter_steps = 0, resid_passes = 0;
+   int filter_steps = 0, resid_passes = 0, filter_col_steps = 0, filter_first_cols = 0;
    auto rec = [&](int i) { return hipEventRecord(ev[i], ctx->stream); };
    OFRR_HIP(ctx, rec(0));
 
@@ -1042,31 +1057,57 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
          }
       }
       const int mid = d / 2;
+      // Soft locking: the leading wanted pairs that meet tol keep their Ritz
+      // vectors in X (and in every Rayleigh-Ritz) but get no correction; the
+      // sweep filters columns [c, k), ka = k - c a power of two (the step
+      // kernels' lane layout) and >= 16 (the back-transform's column groups)
+      int ka = k;
+      if (o->lock && op.can_lock()) {
+         int pre = 0;
+         while (pre < o->nwant && res[pre] <= o->tol) ++pre;
+         while (ka / 2 >= 16 && ka / 2 >= k - pre) ka /= 2;
+      }
+      const int c = k - ka, me = k + nd * ka;
       std::vector<double> q(d + 1);
       q[0] = 1.0 / Lb;
       for (int s = 1; s <= d; ++s) q[s] = 1.0 / (2.0 * Lb - q[s - 1]);
-      for (int j = 0; j < k; ++j) {
+      for (int jj = 0; jj < ka; ++jj) {
+         const int j = c + jj;
          double zp = 1.0, zc = Lth[j] / Lb;  // zeta_0, zeta_1
          for (int s = 1; s < d; ++s) {
-            h_z[(size_t)(s - 1) * k + j] = (2.0 / e) * zc * q[s];  // weight of R in step s
+            h_z[(size_t)(s - 1) * ka + jj] = (2.0 / e) * zc * q[s];  // weight of R in step s
             const double zn = 2.0 * Lth[j] * zc * q[s] - zp * q[s - 1] * q[s];
             zp = zc;
             zc = zn;
          }
       }
-      OFRR_HIP(ctx, hipMemcpyAsync(zall, h_z, (size_t)(d - 1) * k * 8, hipMemcpyHostToDevice, ctx->stream));
+      OFRR_HIP(ctx, hipMemcpyAsync(zall, h_z, (size_t)(d - 1) * ka * 8, hipMemcpyHostToDevice, ctx->stream));
       // Dh_1 = R / (e L(b)), stored in Rq: scale sR / (e Lb), range max|R| / (e Lb)
-      for (int j = 0; j < k; ++j) s1[j] = h_sR[j] / (e * Lb);
-      OFRR_HIP(ctx, hipMemcpyAsync(sD1, s1.data(), (size_t)k * 8, hipMemcpyHostToDevice, ctx->stream));
-      for (int j = 0; j < k; ++j) m1[j] = (float)((double)h_cmaxR[j] / (e * Lb));
-      OFRR_HIP(ctx, hipMemcpyAsync(cmax[0], m1.data(), (size_t)k * 4, hipMemcpyHostToDevice, ctx->stream));
+      for (int jj = 0; jj < ka; ++jj) s1[jj] = h_sR[c + jj] / (e * Lb);
+      OFRR_HIP(ctx, hipMemcpyAsync(sD1, s1.data(), (size_t)ka * 8, hipMemcpyHostToDevice, ctx->stream));
+      for (int jj = 0; jj < ka; ++jj) m1[jj] = (float)((double)h_cmaxR[c + jj] / (e * Lb));
+      OFRR_HIP(ctx, hipMemcpyAsync(cmax[0], m1.data(), (size_t)ka * 4, hipMemcpyHostToDevice, ctx->stream));
       OFRR_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+      // the filtered columns of the quantised residual, n x ka row-major: Rq
+      // itself, or (locked columns present) a compact copy in the second half
+      // of Db[0], whose first half is all a step writes (n ka <= n k / 2)
+      const void *Ra = Rq;
+      if (c > 0) {
+         const int es = (int)dtype_size(bdt);
+         const int wr = ka * es / 8;
+         void *dst = (char *)Db[0] + (size_t)n * ka * es;
+         const unsigned g = (unsigned)std::min<int64_t>((n * wr + 255) / 256, 65536);
+         copy_cols_kernel<<<g, 256, 0, ctx->stream>>>(n, wr, (int64_t)k * es / 8, (int64_t)c * es / 8,
+                                                      (const uint64_t *)Rq, (uint64_t *)dst);
+         OFRR_LAUNCH(ctx);
+         Ra = dst;
+      }
       // recurrence state: current / previous stored block and their scales;
       // Db[b] has scale scale[b]; D_1 lives in Rq with scale sD1
-      const void *Dc = Rq, *Dp = nullptr;
+      const void *Dc = Ra, *Dp = nullptr;
       double *sc = sD1, *sp = nullptr;
       int curIdx = -1, prevIdx = -1, snap = -1;
-      const void *Dmid = (nd == 2 && mid == 1) ? Rq : nullptr;
+      const void *Dmid = (nd == 2 && mid == 1) ? Ra : nullptr;
       // the column maxima rotate over three buffers: step s reads cmax[(s-1)%3],
       // accumulates into cmax[s%3] (cleared by step s-1, or here for s = 1) and
       // clears cmax[(s+1)%3] for step s+1
@@ -1079,8 +1120,8 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
          ca.alpha = (2.0 / e) * q[s];
          ca.beta = q[s - 1] * q[s];
          ca.c = c0;
-         ca.w = zall + (size_t)(s - 1) * k;
-         ca.sR = sRd;
+         ca.w = zall + (size_t)(s - 1) * ka;
+         ca.sR = sRd + c;
          ca.sc = sc;
          ca.sp = sp;
          ca.maxc = cmax[(s - 1) % 3];
@@ -1088,9 +1129,11 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
          ca.zero_next = (s + 1 < d) ? cmax[(s + 1) % 3] : nullptr;
          ca.sn_out = scale[out];
          ca.target_exp = tgt_step;
-         if ((st = op.cheb(ctx, k, Dc, Dp, Rq, bdt, ca, Db[out]))) return st;
+         if ((st = op.cheb(ctx, ka, Dc, Dp, Ra, bdt, ca, Db[out]))) return st;
          ++spmm_calls;
          ++filter_steps;
+         filter_col_steps += ka;
+         if (s == 1) filter_first_cols += ka;
          prevIdx = curIdx;
          Dp = Dc;
          sp = sc;
@@ -1102,10 +1145,10 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       OFRR_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
       // ---- projection: V = [X | Dd (| Dmid)], AV = [W | Z]
       const void *Dd = Dc;
-      if ((st = op.apply_lo(ctx, k, Dd, bdt, Z, zdt, (int64_t)k * nd))) return st;
+      if ((st = op.apply_lo(ctx, ka, Dd, bdt, Z, zdt, (int64_t)ka * nd))) return st;
       ++spmm_calls;
       if (nd == 2) {
-         if ((st = op.apply_lo(ctx, k, Dmid, bdt, (char *)Z + (size_t)k * zsz, zdt, (int64_t)k * nd))) return st;
+         if ((st = op.apply_lo(ctx, ka, Dmid, bdt, (char *)Z + (size_t)ka * zsz, zdt, (int64_t)ka * nd))) return st;
          ++spmm_calls;
       }
       OFRR_HIP(ctx, hipEventRecord(ev[5], ctx->stream));
@@ -1113,7 +1156,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       // G and H blocks (upper block triangles, column-major ld m) in one fused
       // launch: G = [X Dd Dm]^T [X Dd Dm]; H = [X Dd Dm]^T [W Zd Zm] with the
       // X-D blocks of H formed as W^T D (W = A X in FP64, D exact in its format)
-      {
+      if (c == 0) {
          GramMultiArgs ga{};
          ga.op[0] = {X, (int64_t)k, 0};
          ga.op[1] = {W, (int64_t)k, 0};
@@ -1143,18 +1186,44 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
                }
          ga.nitems = t;
          if ((st = gram_multi(ctx, n, k, bdt, zdt, ga))) return st;
+      } else {
+         // locked columns present: the blocks have widths k (X, W) and ka (D,
+         // Z), one tall-skinny product each (upper block triangle, ld me)
+         const int64_t zr = (int64_t)ka * nd;
+         const void *Zm = (const char *)Z + (size_t)ka * zsz;
+         struct Prod { const void *a; int adt, wa; int64_t ars; const void *b; int bdt_, wb; int64_t brs; double *dst; int ro, co; };
+         const Prod pr[12] = {
+             {X, OFRR_F64, k, k, X, OFRR_F64, k, k, G, 0, 0},
+             {X, OFRR_F64, k, k, Dd, bdt, ka, ka, G, 0, k},
+             {Dd, bdt, ka, ka, Dd, bdt, ka, ka, G, k, k},
+             {X, OFRR_F64, k, k, W, OFRR_F64, k, k, H, 0, 0},
+             {W, OFRR_F64, k, k, Dd, bdt, ka, ka, H, 0, k},
+             {Dd, bdt, ka, ka, Z, zdt, ka, zr, H, k, k},
+             {X, OFRR_F64, k, k, Dmid, bdt, ka, ka, G, 0, k + ka},
+             {Dd, bdt, ka, ka, Dmid, bdt, ka, ka, G, k, k + ka},
+             {Dmid, bdt, ka, ka, Dmid, bdt, ka, ka, G, k + ka, k + ka},
+             {W, OFRR_F64, k, k, Dmid, bdt, ka, ka, H, 0, k + ka},
+             {Dd, bdt, ka, ka, Zm, zdt, ka, zr, H, k, k + ka},
+             {Dmid, bdt, ka, ka, Zm, zdt, ka, zr, H, k + ka, k + ka}};
+         const int np_ = (nd == 2) ? 12 : 6;
+         for (int q = 0; q < np_; ++q) {
+            const Prod &P = pr[q];
+            if ((st = gram(ctx, n, P.wa, P.wb, P.a, P.adt, P.ars, 1, P.b, P.bdt_, P.brs, 1, 1.0, 0.0,
+                           P.dst + P.ro + (size_t)P.co * me, me)))
+               return st;
+         }
       }
-      mirror_upper_kernel<<<(m * m + 255) / 256, 256, 0, ctx->stream>>>(m, G, m);
-      mirror_upper_kernel<<<(m * m + 255) / 256, 256, 0, ctx->stream>>>(m, H, m);
+      mirror_upper_kernel<<<(me * me + 255) / 256, 256, 0, ctx->stream>>>(me, G, me);
+      mirror_upper_kernel<<<(me * me + 255) / 256, 256, 0, ctx->stream>>>(me, H, me);
       OFRR_LAUNCH(ctx);
       OFRR_HIP(ctx, hipEventRecord(ev[6], ctx->stream));
-      if ((st = rr_truncated(ctx, m, G, m, H, m, o->tau, k, th, C, m, rank))) return st;
+      if ((st = rr_truncated(ctx, me, G, me, H, me, o->tau, k, th, C, me, rank))) return st;
       OFRR_HIP(ctx, hipEventRecord(ev[7], ctx->stream));
       {
          const void *vb[3] = {X, Dd, Dmid};
-         int dt[3] = {OFRR_F64, bdt, bdt}, kb[3] = {k, k, k}, ldc[3] = {m, m, m};
-         int64_t rs[3] = {k, k, k}, cs[3] = {1, 1, 1};
-         const double *cb[3] = {C, C + k, C + 2 * k};
+         int dt[3] = {OFRR_F64, bdt, bdt}, kb[3] = {k, ka, ka}, ldc[3] = {me, me, me};
+         int64_t rs[3] = {k, ka, ka}, cs[3] = {1, 1, 1};
+         const double *cb[3] = {C, C + k, C + k + ka};
          if ((st = basis_update(ctx, n, k, 1 + nd, vb, dt, kb, rs, cs, cb, ldc, Xn, OFRR_F64, k, 1))) return st;
          std::swap(X, Xn);
       }
@@ -1198,6 +1267,8 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       stats->spmm_calls = spmm_calls;
       stats->rank_min = rank_min;
       stats->filter_steps = filter_steps;
+      stats->filter_col_steps = filter_col_steps;
+      stats->filter_first_cols = filter_first_cols;
       stats->resid_passes = resid_passes;
       stats->t_total_ms = total;
       stats->t_filter_ms = t_acc[0];

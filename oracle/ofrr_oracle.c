@@ -901,7 +901,7 @@ int orc_filtered_ofrr(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
    memcpy(V, T, sizeof(double) * (size_t)n * k);
    int it, converged = 0;
    for (it = 0; it < o->maxiter; ++it) {
-      double *Xc = V, *Dd = V + (int64_t)k * n, *Dm = V + (int64_t)2 * k * n;
+      double *Xc = V;
       orc_spmm_csr(n, row_ptr, col_idx, vals, k, Xc, n, W, n);
       double normA = o->normA > 0 ? o->normA : theta[0];
       converged = 1;
@@ -934,46 +934,59 @@ int orc_filtered_ofrr(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
          }
       }
       const int mid = d / 2;
-      for (int j = 0; j < k; ++j) {
+      /* soft locking: with the leading wanted pairs 0..pre-1 converged, the
+       * sweep filters columns [c, k) only, ka = k - c halved from k while it
+       * stays >= 16 and >= k - pre; the Ritz block keeps all k columns */
+      int ka = k;
+      if (o->lock) {
+         int pre = 0;
+         while (pre < o->nwant && res[pre] <= o->tol) ++pre;
+         while (ka / 2 >= 16 && ka / 2 >= k - pre) ka /= 2;
+      }
+      const int c = k - ka, me = k + nd * ka;
+      double *Dd = V + (int64_t)k * n, *Dm = V + (int64_t)(k + ka) * n;
+      for (int jj = 0; jj < ka; ++jj) {
+         const int j = c + jj;
          double s = colmaxabs(n, Rr + (int64_t)j * n) / e;
          double f = s > 0 ? 1.0 / s : 1.0;
          Lth[j] = (theta[j] - c0) / e;
          for (int64_t i = 0; i < n; ++i) {
-            Dd[i + j * n] = Rr[i + j * n] / e * f; /* D_1 */
-            Dp[i + j * n] = 0.0;                 /* D_0 */
+            Dd[i + jj * n] = Rr[i + j * n] / e * f; /* D_1 */
+            Dp[i + jj * n] = 0.0;                 /* D_0 */
          }
          zp[j] = f;            /* z_0 * f */
          zc[j] = Lth[j] * f;   /* z_1 * f */
       }
-      if (nd == 2 && mid == 1) memcpy(Dm, Dd, sizeof(double) * (size_t)n * k);
+      if (nd == 2 && mid == 1) memcpy(Dm, Dd, sizeof(double) * (size_t)n * ka);
       for (int step = 1; step < d; ++step) {
-         orc_spmm_csr(n, row_ptr, col_idx, vals, k, Dd, n, T, n);
-         for (int j = 0; j < k; ++j) {
+         orc_spmm_csr(n, row_ptr, col_idx, vals, ka, Dd, n, T, n);
+         for (int jj = 0; jj < ka; ++jj) {
+            const int j = c + jj;
             double mx = 0.0;
             for (int64_t i = 0; i < n; ++i) {
-               double t = (2.0 / e) * (T[i + j * n] - c0 * Dd[i + j * n]) - Dp[i + j * n] +
+               double t = (2.0 / e) * (T[i + jj * n] - c0 * Dd[i + jj * n]) - Dp[i + jj * n] +
                           (2.0 / e) * Rr[i + j * n] * zc[j];
-               T[i + j * n] = t;
+               T[i + jj * n] = t;
                mx = fmax(mx, fabs(t));
             }
             double f = mx > 0 ? 1.0 / mx : 1.0;
             double znext = 2.0 * Lth[j] * zc[j] - zp[j];
             for (int64_t i = 0; i < n; ++i) {
-               Dp[i + j * n] = Dd[i + j * n] * f;
-               Dd[i + j * n] = T[i + j * n] * f;
+               Dp[i + jj * n] = Dd[i + jj * n] * f;
+               Dd[i + jj * n] = T[i + jj * n] * f;
             }
             zp[j] = zc[j] * f;
             zc[j] = znext * f;
          }
-         if (nd == 2 && step + 1 == mid) memcpy(Dm, Dd, sizeof(double) * (size_t)n * k);
+         if (nd == 2 && step + 1 == mid) memcpy(Dm, Dd, sizeof(double) * (size_t)n * ka);
       }
       /* projection products over V = [X | Dd (| Dmid)] */
-      orc_spmm_csr(n, row_ptr, col_idx, vals, m - k, Dd, n, W + (int64_t)k * n, n);
-      orc_gemm_tn(n, m, m, V, n, V, n, G, m);
-      orc_gemm_tn(n, m, m, V, n, W, n, H, m);
-      int r = orc_rr_truncated(m, G, m, H, m, o->tau, k, theta, C, m);
+      orc_spmm_csr(n, row_ptr, col_idx, vals, me - k, Dd, n, W + (int64_t)k * n, n);
+      orc_gemm_tn(n, me, me, V, n, V, n, G, me);
+      orc_gemm_tn(n, me, me, V, n, W, n, H, me);
+      int r = orc_rr_truncated(me, G, me, H, me, o->tau, k, theta, C, me);
       if (r < k) { it = -(it + 1); break; }
-      orc_gemm_nn(n, k, m, V, n, C, m, T, n);
+      orc_gemm_nn(n, k, me, V, n, C, me, T, n);
       memcpy(V, T, sizeof(double) * (size_t)n * k);
    }
    if (!converged && it >= 0) {

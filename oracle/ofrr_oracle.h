@@ -128,6 +128,8 @@ typedef struct {
    double tau;       /* Gram truncation tolerance */
    double lower;     /* lower bound of the spectrum */
    double normA;     /* ||A||_2 (0: use the largest Ritz value) */
+   int lock;         /* 1: soft locking (a sweep filters the columns after the
+                        leading converged wanted pairs only); 0: all k columns */
 } orc_ofrr_opts;
 int orc_filtered_ofrr(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                       const double *vals, const orc_ofrr_opts *opt, double *X, int64_t ldx,

@@ -23,7 +23,7 @@ _u32 = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 class OfrrOpts(C.Structure):
     _fields_ = [("k", C.c_int), ("nwant", C.c_int), ("nd", C.c_int), ("degree", C.c_int),
                 ("maxiter", C.c_int), ("tol", C.c_double), ("tau", C.c_double),
-                ("lower", C.c_double), ("normA", C.c_double)]
+                ("lower", C.c_double), ("normA", C.c_double), ("lock", C.c_int)]
 
 
 def build():
@@ -275,10 +275,10 @@ def subspace_iter_svd(A, V0, maxiter, step_size=1, select_tol=2.0 ** -52, genera
 
 
 def filtered_ofrr(A, X0, nwant, nd=1, degree=20, maxiter=50, tol=1e-8, tau=1e-12, lower=0.0,
-                  normA=0.0):
+                  normA=0.0, lock=True):
     X = _f(X0).copy(order="F")
     k = X.shape[1]
-    o = OfrrOpts(k, nwant, nd, degree, maxiter, tol, tau, lower, normA)
+    o = OfrrOpts(k, nwant, nd, degree, maxiter, tol, tau, lower, normA, 1 if lock else 0)
     th = np.zeros(k)
     res = np.zeros(k)
     it = lib().orc_filtered_ofrr(A.n, A.row_ptr, A.col_idx, A.vals, C.byref(o), X, A.n, th, res)

@@ -100,6 +100,57 @@ def test_filtered_fp64_strip_schedule_matches_cpu(ctx):
     assert abs(fro - fro_o) <= 1e-6 * fro_o
 
 
+def test_filtered_fp64_soft_locking_matches_cpu(ctx):
+    """Soft locking engaged (the leading pairs converge first and the later
+    sweeps filter 16 of the 32 columns): the FP64 path converges in the
+    iterations of the CPU restatement, to the same Ritz values (1e-12) and the
+    closed-form spectrum."""
+    from mixedprecisionofrr_amd import api
+    N, k, nwant = 14, 32, 20
+    Ah = O.stencil(3, N)
+    X0 = O.randn(0, Ah.n * k).reshape(Ah.n, k)
+    it_o, th_o, X_o, res_o = O.filtered_ofrr(Ah, np.asfortranarray(X0), nwant, nd=2, degree=10,
+                                             maxiter=60, tol=1e-9)
+    print("filtered fp64 soft locking N=14 k=32", flush=True)
+    st, th, X, res, stats = api.filtered_eig(ctx, _csr(ctx, Ah), torch.as_tensor(X0, device="cuda:0"),
+                                             nwant, nd=2, degree=10, maxiter=60,
+                                             basis_dtype=torch.float64, tol=1e-9)
+    assert st == 0 and stats["iterations"] == it_o, (st, stats, it_o)
+    assert stats["filter_col_steps"] < stats["filter_steps"] * k, stats
+    assert stats["filter_col_steps"] % 16 == 0
+    assert np.max(np.abs(th - th_o)[:nwant] / np.abs(th_o[:nwant])) < 1e-12
+    ex = O.laplacian_eigs(3, N)[:nwant]
+    assert np.abs(th[:nwant] - ex).max() / ex[0] < 1e-12
+    assert res[:nwant].max() <= 1e-9
+
+
+@pytest.mark.parametrize("bdt", [torch.float16, torch.bfloat16, torch.float8_e4m3fn, torch.float64])
+def test_filtered_soft_locking_formats(ctx, bdt):
+    """k = 64, basis 192, degree 120 on the 32^3 Laplacian, every growth format:
+    with and without soft locking the solve reaches residual 1e-8 and the
+    closed-form eigenvalues; locking filters fewer columns (FP8 refines in FP16)."""
+    from mixedprecisionofrr_amd import api
+    N, k, nwant = 32, 64, 50
+    A = api.CSRMatrix.stencil(ctx, 3, N).with_low_precision(torch.float16)
+    X0 = torch.empty(A.n, k, dtype=torch.float64, device="cuda:0")
+    api.fill_randn(ctx, X0, seed=0)
+    ex = O.laplacian_eigs(3, N)[:nwant]
+    Ah = O.stencil(3, N)
+    fp8 = bdt == torch.float8_e4m3fn
+    out = {}
+    for lock in (True, False):
+        print(f"filtered soft locking basis={bdt} lock={lock}", flush=True)
+        st, th, X, res, stats = api.filtered_eig(
+            ctx, A, X0, nwant, nd=2, degree=120, maxiter=100, basis_dtype=bdt, tol=1e-8,
+            lock=lock, refine_dtype=torch.float16 if fp8 else None, refine_tol=1e-4)
+        assert st == 0, (lock, stats)
+        assert np.max(np.abs(th[:nwant] - ex) / ex) < 1e-10
+        assert O.residuals(Ah, X.cpu().numpy(), th, ex[0])[:nwant].max() <= 1.05e-8
+        out[lock] = stats
+    assert out[False]["filter_col_steps"] == out[False]["filter_steps"] * k
+    assert out[True]["filter_col_steps"] <= out[False]["filter_col_steps"]
+
+
 def _box_laplacian(nx, ny, nz):
     """7-point Dirichlet Laplacian on an nx x ny x nz box, x fastest (scipy kron sums)."""
     import scipy.sparse as sp

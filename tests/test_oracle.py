@@ -259,6 +259,30 @@ def test_filtered_ofrr_closed_form(nd):
     assert np.allclose(res2[:nwant], res[:nwant], rtol=1e-6, atol=1e-14)
 
 
+def test_filtered_ofrr_soft_locking():
+    """Soft locking: with no pair converged the solve is the unlocked one,
+    bitwise; once the leading pairs converge and later sweeps filter only the
+    trailing columns, the spectrum is still the closed form and the residuals
+    meet the tolerance, in the same number of iterations."""
+    N, k, nwant = 14, 32, 20
+    A = O.stencil(3, N)
+    X0 = O.randn(0, A.n * k).reshape(A.n, k, order="F")
+    a = O.filtered_ofrr(A, X0, nwant, nd=2, degree=10, maxiter=3, tol=1e-30, lock=True)
+    b = O.filtered_ofrr(A, X0, nwant, nd=2, degree=10, maxiter=3, tol=1e-30, lock=False)
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    ex = O.laplacian_eigs(3, N)[:nwant]
+    its = {}
+    for lock in (True, False):
+        it, th, X, res = O.filtered_ofrr(A, X0, nwant, nd=2, degree=10, maxiter=60, tol=1e-9,
+                                         lock=lock)
+        assert it >= 0
+        assert np.abs(th[:nwant] - ex).max() / ex[0] < 1e-12
+        assert res[:nwant].max() <= 1e-9
+        assert np.allclose(O.residuals(A, X, th, th[0])[:nwant], res[:nwant], rtol=1e-6, atol=1e-14)
+        its[lock] = it
+    assert its[True] == its[False]
+
+
 def test_residuals_hand_case():
     A = O.stencil(2, 10)
     lam, U = np.linalg.eigh(A.to_scipy().toarray())
