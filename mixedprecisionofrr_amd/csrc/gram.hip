@@ -56,6 +56,9 @@ __global__ __launch_bounds__(64) void gram_mfma_kernel(
    }
    const TV *vb = V + (int64_t)(i0 + li) * vcs;
    const TW *wb = W + (int64_t)(j0 + li) * wcs;
+   // 16-column groups of the tile inside k1 x k2: a narrow product (a 32- or
+   // 16-column block) issues only the products of its own groups
+   const int nti = min(4, (k1 - i0 + 15) / 16), ntj = min(4, (k2 - j0 + 15) / 16);
 #pragma unroll 2
    for (int64_t r = r0; r < r1; r += 4) {
       const int64_t row = r + lk;
@@ -70,7 +73,8 @@ __global__ __launch_bounds__(64) void gram_mfma_kernel(
       for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
          for (int tj = 0; tj < 4; ++tj)
-            acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+            if (ti < nti && tj < ntj)
+               acc[ti][tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
    }
    const int64_t tile = (int64_t)blockIdx.y * gridDim.z + blockIdx.z;
    double *p = part + ((int64_t)blockIdx.x * gridDim.y * gridDim.z + tile) * (GT * GT);
