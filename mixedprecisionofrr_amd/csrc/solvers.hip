@@ -717,6 +717,18 @@ __global__ void copy_cols_kernel(int64_t n, int wr, int64_t ws, int64_t c0,
    }
 }
 
+// Y (n x 2 wr words, row-major) = [A | B], A and B n x wr words row-major
+__global__ void pack2_rows_kernel(int64_t n, int wr, const uint64_t *__restrict__ A,
+                                  const uint64_t *__restrict__ B, uint64_t *__restrict__ Y) {
+   const int64_t total = n * 2 * wr;
+   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+        t += (int64_t)gridDim.x * blockDim.x) {
+      const int64_t i = t / (2 * wr);
+      const int w = (int)(t - i * 2 * wr);
+      Y[t] = (w < wr) ? A[i * wr + w] : B[i * wr + w - wr];
+   }
+}
+
 // Y(i, j) *= s[j] for a row-major m x k block
 __global__ void scale_cols_rm_kernel(int64_t m, int k, double *__restrict__ Y,
                                      const double *__restrict__ s) {
@@ -1187,28 +1199,36 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
          ga.nitems = t;
          if ((st = gram_multi(ctx, n, k, bdt, zdt, ga))) return st;
       } else {
-         // locked columns present: the blocks have widths k (X, W) and ka (D,
-         // Z), one tall-skinny product each (upper block triangle, ld me)
-         const int64_t zr = (int64_t)ka * nd;
-         const void *Zm = (const char *)Z + (size_t)ka * zsz;
-         struct Prod { const void *a; int adt, wa; int64_t ars; const void *b; int bdt_, wb; int64_t brs; double *dst; int ro, co; };
-         const Prod pr[12] = {
-             {X, OFRR_F64, k, k, X, OFRR_F64, k, k, G, 0, 0},
-             {X, OFRR_F64, k, k, Dd, bdt, ka, ka, G, 0, k},
-             {Dd, bdt, ka, ka, Dd, bdt, ka, ka, G, k, k},
-             {X, OFRR_F64, k, k, W, OFRR_F64, k, k, H, 0, 0},
-             {W, OFRR_F64, k, k, Dd, bdt, ka, ka, H, 0, k},
-             {Dd, bdt, ka, ka, Z, zdt, ka, zr, H, k, k},
-             {X, OFRR_F64, k, k, Dmid, bdt, ka, ka, G, 0, k + ka},
-             {Dd, bdt, ka, ka, Dmid, bdt, ka, ka, G, k, k + ka},
-             {Dmid, bdt, ka, ka, Dmid, bdt, ka, ka, G, k + ka, k + ka},
-             {W, OFRR_F64, k, k, Dmid, bdt, ka, ka, H, 0, k + ka},
-             {Dd, bdt, ka, ka, Zm, zdt, ka, zr, H, k, k + ka},
-             {Dmid, bdt, ka, ka, Zm, zdt, ka, zr, H, k + ka, k + ka}};
-         const int np_ = (nd == 2) ? 12 : 6;
-         for (int q = 0; q < np_; ++q) {
+         // locked columns present: X and W have width k, the correction blocks
+         // ka.  With two blocks, [D_d | D_mid] is packed row by row next to the
+         // Z layout [Z_d | Z_mid] (a free filter buffer: neither block, nor
+         // Db[0] whose second half may hold D_mid = D_1), so the six products
+         // X^T X, X^T D, D^T D, X^T W, W^T D, D^T Z each take one launch
+         // (their lower blocks are replaced by the mirror below)
+         const int kd = nd * ka;
+         const void *Dp2 = Dd;
+         if (nd == 2) {
+            int b = -1;
+            for (int i = 1; i < 4 && b < 0; ++i)
+               if (i != curIdx && i != snap) b = i;
+            const int es = (int)dtype_size(bdt);
+            const int wr = ka * es / 8;
+            const unsigned g = (unsigned)std::min<int64_t>((n * 2 * wr + 255) / 256, 65536);
+            pack2_rows_kernel<<<g, 256, 0, ctx->stream>>>(n, wr, (const uint64_t *)Dd,
+                                                          (const uint64_t *)Dmid, (uint64_t *)Db[b]);
+            OFRR_LAUNCH(ctx);
+            Dp2 = Db[b];
+         }
+         struct Prod { const void *a; int adt, wa; const void *b; int bdt_, wb; double *dst; int ro, co; };
+         const Prod pr[6] = {{X, OFRR_F64, k, X, OFRR_F64, k, G, 0, 0},
+                             {X, OFRR_F64, k, Dp2, bdt, kd, G, 0, k},
+                             {Dp2, bdt, kd, Dp2, bdt, kd, G, k, k},
+                             {X, OFRR_F64, k, W, OFRR_F64, k, H, 0, 0},
+                             {W, OFRR_F64, k, Dp2, bdt, kd, H, 0, k},
+                             {Dp2, bdt, kd, Z, zdt, kd, H, k, k}};
+         for (int q = 0; q < 6; ++q) {
             const Prod &P = pr[q];
-            if ((st = gram(ctx, n, P.wa, P.wb, P.a, P.adt, P.ars, 1, P.b, P.bdt_, P.brs, 1, 1.0, 0.0,
+            if ((st = gram(ctx, n, P.wa, P.wb, P.a, P.adt, P.wa, 1, P.b, P.bdt_, P.wb, 1, 1.0, 0.0,
                            P.dst + P.ro + (size_t)P.co * me, me)))
                return st;
          }
