@@ -176,6 +176,7 @@ void ofrr_server::run_one(ofrr_ctx *ctx, Rec &r) {
       o.tau = 1e-12;
       o.lower = 0.0;  // the diffusion operators are positive definite
       o.normA = 0.0;  // ||A||_2 = the largest Ritz value
+      o.lock = 1;     // later sweeps filter only the columns after the converged pairs
       if (q.basis_dtype == OFRR_FP8) {
          // FP8 corrections gain at most ~2^4 per sweep: finish in FP16 once the
          // wanted residuals are below 1e-4 or stall
