@@ -20,7 +20,9 @@
  *   orc_rr_truncated    exact Ritz values for sine-mode bases, planted rank drop
  *   orc_rr_generalized  scipy.linalg.eigh(H, G) on random SPD pencils
  *   orc_arnoldi_hess_eig exact termination: v0 = sum of j sine modes -> closed-form eigenvalues
- *   orc_subspace_iter_eig / orc_filtered_ofrr  closed-form Laplacian spectra, residual bound
+ *   orc_subspace_iter_eig / orc_filtered_ofrr  closed-form Laplacian spectra, residual bound;
+ *                       soft locking: bitwise equal to the unlocked solve while no pair has
+ *                       converged, closed-form spectrum in the unlocked iteration count
  *   orc_residuals       hand-computed residual of exact eigenpairs (zero) and perturbed ones
  *   orc_gs_qr           numpy Householder QR (sign-fixed) on full-rank input; the textbook
  *                       loss of orthogonality of each variant on graded matrices (CGS ~ u k^2,
