@@ -100,28 +100,33 @@ def test_filtered_fp64_strip_schedule_matches_cpu(ctx):
     assert abs(fro - fro_o) <= 1e-6 * fro_o
 
 
-def test_filtered_fp64_soft_locking_matches_cpu(ctx):
+@pytest.mark.parametrize("nd", [1, 2])
+def test_filtered_fp64_soft_locking_matches_cpu(ctx, nd):
     """Soft locking engaged (the leading pairs converge first and the later
-    sweeps filter 16 of the 32 columns): the FP64 path converges in the
-    iterations of the CPU restatement, to the same Ritz values (1e-12) and the
-    closed-form spectrum."""
+    sweeps filter 16 of the 32 columns; with two correction blocks they are
+    packed for the products): the FP64 path converges in the iterations of the
+    CPU restatement, to the same Ritz values (1e-12) and the closed-form
+    spectrum."""
     from mixedprecisionofrr_amd import api
     N, k, nwant = 14, 32, 20
     Ah = O.stencil(3, N)
     X0 = O.randn(0, Ah.n * k).reshape(Ah.n, k)
-    it_o, th_o, X_o, res_o = O.filtered_ofrr(Ah, np.asfortranarray(X0), nwant, nd=2, degree=10,
-                                             maxiter=60, tol=1e-9)
-    print("filtered fp64 soft locking N=14 k=32", flush=True)
+    # tol is chosen away from the residuals at the deciding iterations: at 1e-9 the
+    # one-block solve's last residual is 1.01e-9 on the CPU, within rounding of tol
+    tol = 1e-9 if nd == 2 else 2e-9
+    it_o, th_o, X_o, res_o = O.filtered_ofrr(Ah, np.asfortranarray(X0), nwant, nd=nd, degree=10,
+                                             maxiter=60, tol=tol)
+    print(f"filtered fp64 soft locking N=14 k=32 nd={nd}", flush=True)
     st, th, X, res, stats = api.filtered_eig(ctx, _csr(ctx, Ah), torch.as_tensor(X0, device="cuda:0"),
-                                             nwant, nd=2, degree=10, maxiter=60,
-                                             basis_dtype=torch.float64, tol=1e-9)
+                                             nwant, nd=nd, degree=10, maxiter=60,
+                                             basis_dtype=torch.float64, tol=tol)
     assert st == 0 and stats["iterations"] == it_o, (st, stats, it_o)
     assert stats["filter_col_steps"] < stats["filter_steps"] * k, stats
     assert stats["filter_col_steps"] % 16 == 0
     assert np.max(np.abs(th - th_o)[:nwant] / np.abs(th_o[:nwant])) < 1e-12
     ex = O.laplacian_eigs(3, N)[:nwant]
     assert np.abs(th[:nwant] - ex).max() / ex[0] < 1e-12
-    assert res[:nwant].max() <= 1e-9
+    assert res[:nwant].max() <= tol
 
 
 @pytest.mark.parametrize("bdt", [torch.float16, torch.bfloat16, torch.float8_e4m3fn, torch.float64])
