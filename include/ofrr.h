@@ -75,6 +75,30 @@ OFRR_API int ofrr_spmm_csr(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t 
                            double alpha, double beta, void *Y_dev, int y_dtype, int64_t y_rs,
                            int64_t y_cs);
 
+/* Rectangular A (nrows x ncols CSR, column indices in [0, ncols)) and the
+ * transposed product, csrmv(Acsr, trans, X) of MATLAB/+src/+csr/csrmv.m:27-76:
+ * trans 0: Y (nrows x k) = alpha A X + beta Y, X ncols x k ('N', :48-59);
+ * trans 1: Y (ncols x k) = alpha A^T X + beta Y, X nrows x k ('T', :61-71).
+ * The transposed product runs over A^T built on the device (ofrr_csr_transpose,
+ * in the context's workspace, synchronous), so FP64 results equal the 'T'
+ * branch's sequential scatter (rows in increasing order, fused multiply-adds)
+ * bit for bit.  Accumulation as ofrr_spmm_csr. */
+OFRR_API int ofrr_spmm_csr_rect(ofrr_ctx *ctx, int trans, int64_t nrows, int64_t ncols, int64_t nnz,
+                                const int64_t *row_ptr_dev, const int32_t *col_idx_dev,
+                                const void *vals_dev, int val_dtype, int k, const void *X_dev,
+                                int x_dtype, int64_t x_rs, int64_t x_cs, double alpha, double beta,
+                                void *Y_dev, int y_dtype, int64_t y_rs, int64_t y_cs);
+/* A^T of a CSR A (nrows x ncols) as a CSR of ncols x nrows (row_ptrT ncols + 1,
+ * col_idxT and valsT nnz, device), the entries of each row of A^T in increasing
+ * row of A (a stable sort by column).  Build it once for repeated transposed
+ * products (subspace_iter_svd.m:63-77 alternates A and A^T) and pass it to
+ * ofrr_spmm_csr_rect with trans 0.  1000 when a column index lies outside
+ * [0, ncols).  Synchronous. */
+OFRR_API int ofrr_csr_transpose(ofrr_ctx *ctx, int64_t nrows, int64_t ncols, int64_t nnz,
+                                const int64_t *row_ptr_dev, const int32_t *col_idx_dev,
+                                const void *vals_dev, int val_dtype, int64_t *row_ptrT_dev,
+                                int32_t *col_idxT_dev, void *valsT_dev);
+
 /* ---- pivots and scaling */
 /* 1-based index of the first max |x_i| and its signed value, both written to
  * device memory: iamax<T_O,T_COMPUTE>(x, result, result_val, kDEVICE)
@@ -398,6 +422,15 @@ typedef struct {
    double tol;         /* ||A x - theta x|| / (||A|| ||x||) */
    int degree;         /* filter degree per outer iteration (0: by size) */
    int maxiter;        /* outer iterations (0: 200) */
+   int k;              /* Ritz block width, nwant..128 (0: the smallest power of two
+                          >= 1.25 nwant, at least 16) */
+   int nd;             /* correction blocks, 1 or 2 (0: 2 while the basis dimension
+                          k (1 + nd) stays <= 256, else 1); basis dimension <= 384 */
+   int refine;         /* precision ladder: 0 default (FP8 growth finishes in FP16 below
+                          1e-4), 1 use refine_dtype / refine_tol, -1 off */
+   int refine_dtype;   /* refinement format when refine = 1 (OFRR_F64: the growth
+                          products then also use the FP64 operator values) */
+   double refine_tol;  /* switch once the wanted residuals are below it (or stall) */
 } ofrr_eig_request;
 
 typedef struct {
@@ -415,6 +448,9 @@ typedef struct {
    int64_t bytes_running, bytes_peak; /* footprint of the running solves, and its maximum */
    int running_peak;                  /* most solves running at once */
    int64_t budget;
+   int64_t slot_bytes;                /* workspace the slots hold for the server's life
+                                         (admission: bytes_running + footprint <=
+                                         budget - slot_bytes) */
 } ofrr_server_stats;
 
 /* the plan of a request (block width k, correction blocks nd) and its device

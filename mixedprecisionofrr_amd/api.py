@@ -90,14 +90,25 @@ class Context:
 
 class CSRMatrix:
     """Device CSR operator: int64 row_ptr, int32 col_idx, FP64 values, and an
-    optional low-precision copy of the values for the growth SpMMs."""
+    optional low-precision copy of the values for the growth SpMMs.  n rows and
+    ncols columns (default n); only spmm / spmm_rect / transpose take a
+    rectangular matrix, every other entry point needs a square one."""
 
-    def __init__(self, n, row_ptr, col_idx, vals):
+    def __init__(self, n, row_ptr, col_idx, vals, ncols=None):
         self.n = int(n)
+        self.ncols = self.n if ncols is None else int(ncols)
         self.row_ptr = row_ptr
         self.col_idx = col_idx
         self.vals = vals
         self.vals_lo = None
+
+    @property
+    def shape(self):
+        return (self.n, self.ncols)
+
+    def require_square(self, what):
+        if self.n != self.ncols:
+            raise ValueError(f"{what}: needs a square operator, got {self.n} x {self.ncols}")
 
     @property
     def nnz(self):
@@ -129,35 +140,94 @@ class CSRMatrix:
         va = va / scale
         if symmetrize or shift:
             rp, ci, va = _sym_shift(rp, ci, va, rows, symmetrize, shift)
-        return CSRMatrix.from_host(ctx, rp, ci, va)
+        return CSRMatrix.from_host(ctx, rp, ci, va, ncols=cols)
 
     @staticmethod
-    def from_host(ctx, row_ptr, col_idx, vals):
+    def from_host(ctx, row_ptr, col_idx, vals, ncols=None):
+        """Upload host CSR arrays; ncols defaults to the row count.  The
+        structure is checked on the host (monotone row pointers spanning the
+        entries, column indices in [0, ncols)), so a bad matrix is a ValueError
+        here rather than an out-of-bounds gather on the device."""
+        rp_h = np.asarray(row_ptr, np.int64)
+        ci_h = np.asarray(col_idx, np.int32)
+        va_h = np.asarray(vals, np.float64)
+        n = len(rp_h) - 1
+        nc = n if ncols is None else int(ncols)
+        _check_csr(rp_h, ci_h, va_h, n, nc)
         dev = f"cuda:{ctx.device}"
-        rp = torch.as_tensor(np.asarray(row_ptr, np.int64), device=dev)
-        ci = torch.as_tensor(np.asarray(col_idx, np.int32), device=dev)
-        va = torch.as_tensor(np.asarray(vals, np.float64), device=dev)
-        return CSRMatrix(len(row_ptr) - 1, rp, ci, va)
+        rp = torch.as_tensor(rp_h, device=dev)
+        ci = torch.as_tensor(ci_h, device=dev)
+        va = torch.as_tensor(va_h, device=dev)
+        return CSRMatrix(n, rp, ci, va, ncols=nc)
 
     def with_low_precision(self, dtype=torch.float16):
         self.vals_lo = self.vals.to(dtype)
         return self
 
 
+def _check_csr(rp, ci, va, n, ncols):
+    if n < 0 or ncols < 0:
+        raise ValueError("CSR: negative dimension")
+    if len(ci) != len(va):
+        raise ValueError("CSR: column indices and values differ in length")
+    if rp[0] != 0 or rp[-1] != len(ci) or np.any(np.diff(rp) < 0):
+        raise ValueError("CSR: row pointers must start at 0, never decrease and end at nnz")
+    if len(ci) and (ci.min() < 0 or ci.max() >= ncols):
+        raise ValueError(f"CSR: column indices must lie in [0, {ncols})")
+
+
+def _check_rows(X, rows, what, name="X"):
+    if X.shape[0] != rows:
+        raise ValueError(f"{what}: {name} has {X.shape[0]} rows, the operator needs {rows}")
+
+
 # ---------------------------------------------------------------------------
 def spmm(ctx, A, X, out_dtype=None, alpha=1.0, beta=0.0, Y=None, vals=None):
-    """Y = alpha A X + beta Y (csrmv semantics, csrmv.m:48-59)."""
+    """Y = alpha A X + beta Y (csrmv semantics, csrmv.m:48-59) for a square or
+    rectangular A: X has A.ncols rows, Y A.n rows."""
+    return spmm_rect(ctx, A, X, trans=False, out_dtype=out_dtype, alpha=alpha, beta=beta, Y=Y,
+                     vals=vals)
+
+
+def spmm_rect(ctx, A, X, trans=False, out_dtype=None, alpha=1.0, beta=0.0, Y=None, vals=None):
+    """csrmv(Acsr, trans, X) of csrmv.m:27-76: Y = alpha op(A) X + beta Y with
+    op(A) = A (X: A.ncols rows, Y: A.n rows) or A^T (trans; X: A.n rows, Y:
+    A.ncols rows; the 'T' branch, :61-71, bitwise in FP64)."""
     if X.dim() == 1:
         X = X.view(-1, 1)
-    n, k = X.shape
+    xr, yr = (A.n, A.ncols) if trans else (A.ncols, A.n)
+    _check_rows(X, xr, "spmm")
+    k = X.shape[1]
     if Y is None:
-        Y = torch.zeros(n, k, dtype=out_dtype or X.dtype, device=X.device)
+        Y = torch.zeros(yr, k, dtype=out_dtype or X.dtype, device=X.device)
+    if tuple(Y.shape) != (yr, k):
+        raise ValueError(f"spmm: Y must be {yr} x {k}, got {tuple(Y.shape)}")
     va = A.vals if vals is None else vals
-    ctx.check(ctx.lib.ofrr_spmm_csr(ctx.h, A.n, A.nnz, _p(A.row_ptr), _p(A.col_idx), _p(va),
-                                    dtype_code(va), k, _p(X), dtype_code(X), X.stride(0),
-                                    X.stride(1), alpha, beta, _p(Y), dtype_code(Y), Y.stride(0),
-                                    Y.stride(1)), "spmm_csr")
+    if not trans and A.n == A.ncols:
+        ctx.check(ctx.lib.ofrr_spmm_csr(ctx.h, A.n, A.nnz, _p(A.row_ptr), _p(A.col_idx), _p(va),
+                                        dtype_code(va), k, _p(X), dtype_code(X), X.stride(0),
+                                        X.stride(1), alpha, beta, _p(Y), dtype_code(Y),
+                                        Y.stride(0), Y.stride(1)), "spmm_csr")
+    else:
+        ctx.check(ctx.lib.ofrr_spmm_csr_rect(ctx.h, 1 if trans else 0, A.n, A.ncols, A.nnz,
+                                             _p(A.row_ptr), _p(A.col_idx), _p(va), dtype_code(va),
+                                             k, _p(X), dtype_code(X), X.stride(0), X.stride(1),
+                                             alpha, beta, _p(Y), dtype_code(Y), Y.stride(0),
+                                             Y.stride(1)), "spmm_csr_rect")
     return Y
+
+
+def transpose(ctx, A):
+    """A^T as a CSRMatrix (A.ncols x A.n), each row's entries in increasing
+    row of A (ofrr_csr_transpose)."""
+    dev = A.row_ptr.device
+    rp = torch.empty(A.ncols + 1, dtype=torch.int64, device=dev)
+    ci = torch.empty(max(1, A.nnz), dtype=torch.int32, device=dev)
+    va = torch.empty(max(1, A.nnz), dtype=A.vals.dtype, device=dev)
+    ctx.check(ctx.lib.ofrr_csr_transpose(ctx.h, A.n, A.ncols, A.nnz, _p(A.row_ptr),
+                                         _p(A.col_idx), _p(A.vals), dtype_code(A.vals), _p(rp),
+                                         _p(ci), _p(va)), "csr_transpose")
+    return CSRMatrix(A.ncols, rp, ci[:A.nnz], va[:A.nnz], ncols=A.n)
 
 
 def fill_randn(ctx, t, seed, stream=0, offset=0):
@@ -351,7 +421,11 @@ def rr_generalized(ctx, G, H, gtol=0.0):
 
 def residuals(ctx, A, X, theta, normA):
     """||A x_j - theta_j x_j|| / (normA ||x_j||) in FP64 (test4/test.m:340-350)."""
+    A.require_square("residuals")
+    _check_rows(X, A.n, "residuals")
     k = X.shape[1]
+    if len(theta) < k:
+        raise ValueError("residuals: one Ritz value per column of X")
     th = (C.c_double * k)(*[float(t) for t in theta])
     res = (C.c_double * k)()
     ctx.check(ctx.lib.ofrr_residuals(ctx.h, A.n, A.nnz, _p(A.row_ptr), _p(A.col_idx), _p(A.vals),
@@ -362,6 +436,9 @@ def residuals(ctx, A, X, theta, normA):
 
 def arnoldi_hess_eig(ctx, A, k, kdim, maxiter, v0, select_tol=2.0 ** -52, gtol=0.0):
     """arnoldi_hess_eig.m in FP64 on the GPU: (theta desc, X col-major, built)."""
+    A.require_square("arnoldi_hess_eig")
+    if v0.numel() != A.n:
+        raise ValueError(f"arnoldi_hess_eig: the start vector needs {A.n} entries")
     th = (C.c_double * k)()
     X = colmajor(A.n, k, torch.float64, v0.device)
     built = C.c_int()
@@ -374,6 +451,8 @@ def arnoldi_hess_eig(ctx, A, k, kdim, maxiter, v0, select_tol=2.0 ** -52, gtol=0
 
 def subspace_iter_eig(ctx, A, V0, maxiter, step_size=1, select_tol=2.0 ** -52, gtol=0.0):
     """subspace_iter_eig.m with hessenberg + eig_rr(generalised), FP64 on the GPU."""
+    A.require_square("subspace_iter_eig")
+    _check_rows(V0, A.n, "subspace_iter_eig", "V0")
     m = V0.shape[1]
     V = colmajor(A.n, m, torch.float64, V0.device)
     V.copy_(V0)
@@ -391,6 +470,8 @@ def filtered_eig(ctx, A, X0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=to
     precision ladder's refinement format (None: off).  lock: soft locking (a
     sweep filters only the columns after the leading converged wanted pairs).
     Returns (status, theta, X row-major, res, stats dict)."""
+    A.require_square("filtered_eig")
+    _check_rows(X0, A.n, "filtered_eig", "X0")
     n, k = X0.shape
     X = X0.to(torch.float64).contiguous().clone()
     bdt = dtype_code(basis_dtype)

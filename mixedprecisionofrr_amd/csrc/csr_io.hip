@@ -8,8 +8,10 @@
 // 23-39).  Host code only: the caller uploads the arrays (int64 row pointers on
 // the device, the layout every CSR kernel here reads).  Unlike the reference,
 // the structure is validated (row pointers start at 0, never decrease and end
-// at the number of stored entries; column indices are non-negative), so a
-// damaged file is an error rather than an out-of-bounds gather later.
+// at the number of stored entries; column indices are non-negative; no file
+// ends in a partial element), so a damaged file is an error rather than an
+// out-of-bounds gather later.  cols may differ from rows: a rectangular matrix
+// reaches only the entry points that take both dimensions.
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -26,7 +28,8 @@ int io_fail(int status, const std::string &msg) {
    return status;
 }
 
-// whole file as elements of T; false when it cannot be opened or is empty
+// whole file as elements of T; false when it cannot be opened, is empty or
+// ends in a partial element (a truncated file)
 template <typename T>
 bool read_all(const std::string &path, std::vector<T> &out) {
    out.clear();
@@ -35,7 +38,7 @@ bool read_all(const std::string &path, std::vector<T> &out) {
    std::fseek(f, 0, SEEK_END);
    const long bytes = std::ftell(f);
    std::fseek(f, 0, SEEK_SET);
-   if (bytes <= 0) {
+   if (bytes <= 0 || (size_t)bytes % sizeof(T) != 0) {
       std::fclose(f);
       return false;
    }
@@ -113,6 +116,7 @@ int ofrr_csr_file_read(const char *prefix, int64_t *row_ptr, int32_t *col_idx, d
 
 int ofrr_singular_values_file(const char *prefix, double *out, int64_t cap, int64_t *count) {
    if (!prefix || !count) return io_fail(kErrorInvalidArgument, "singular_values: null argument");
+   if (cap < 0) return io_fail(kErrorInvalidArgument, "singular_values: negative capacity");
    const std::string f = std::string(prefix) + "_singular_values.bin";
    std::vector<double> v;
    if (!read_all(f, v)) return io_fail(kErrorInvalidArgument, "Could not open file " + f);

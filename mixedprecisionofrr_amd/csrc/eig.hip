@@ -32,10 +32,14 @@ constexpr int kMaxSweeps = 30;
 constexpr int kBlkFull = 3;    // nn <= 128
 constexpr int kBlkPacked = 5;  // nn <= 192
 constexpr int kBlkGlobal = 9;  // nn <= 256
+constexpr int kBlkWide = 19;   // nn <= 384
+constexpr int kJacobiMaxN = 384;
+constexpr int kMaxPairs = kJacobiMaxN / 2;
 constexpr int kGroup = 3;      // blocks whose loads are in flight together
 static_assert(kBlkFull * kJacobiThreads >= (64 * 65) / 2, "blocks per thread, nn <= 128");
 static_assert(kBlkPacked * kJacobiThreads >= (96 * 97) / 2, "blocks per thread, nn <= 192");
 static_assert(kBlkGlobal * kJacobiThreads >= (128 * 129) / 2, "blocks per thread, nn <= 256");
+static_assert(kBlkWide * kJacobiThreads >= (kMaxPairs * (kMaxPairs + 1)) / 2, "blocks per thread, nn <= 384");
 
 // Pair s of round r in the circle schedule on nn indices (M = nn - 1): s = 0
 // pairs M with r, s > 0 pairs r + s with r - s (mod M); sorted so p < q.
@@ -85,8 +89,8 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
     int64_t strideLog, int32_t *__restrict__ rank, int32_t *__restrict__ nsweeps,
     int64_t strideInts, double *__restrict__ work, int64_t strideWork, int max_sweeps) {
    extern __shared__ double lds[];
-   __shared__ double2 rcs[128];  // (c, s) of pair slot s in this round
-   __shared__ int4 rpq[128];     // (p, q, off(p), off(q)) of pair slot s, p < q
+   __shared__ double2 rcs[kMaxPairs];  // (c, s) of pair slot s in this round
+   __shared__ int4 rpq[kMaxPairs];     // (p, q, off(p), off(q)) of pair slot s, p < q
    __shared__ int rotated, s_n;
    __shared__ double s_amax[16];
    const int b = blockIdx.x;
@@ -248,23 +252,25 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
 }
 // V = J_1 J_2 ... (all logged rotations) applied to the identity, one row of V
 // per wave.  The row lives in registers, arranged like the pairs of the circle
-// schedule: lane l holds the row's entries at the two indices of pair slot l
-// (and slot l + 64), its a-side (r + s mod M; M for s = 0) and b-side
-// (r - s mod M).  A round rotates each slot's two entries, then the indices move
-// to the next round's slots: every a-side entry one slot down, every b-side
-// entry one slot up, with a[1] -> b[0] and b[np-1] -> a[np-1] at the ends and
-// a[0] (index M) fixed.  The arithmetic per entry is the one of the rotation
-// kernel's logged (c, s) on sorted (p, q), in round order, as before.  The log
-// is staged through LDS kApplyChunk rounds at a time, the next chunk loaded into
-// registers while the current one is applied.
+// schedule: lane l holds the row's entries at the two indices of pair slots
+// l, l + 64, ..., l + 64 (S - 1), each slot's a-side (r + s mod M; M for s = 0)
+// and b-side (r - s mod M).  A round rotates each slot's two entries, then the
+// indices move to the next round's slots: every a-side entry one slot down,
+// every b-side entry one slot up, with a[1] -> b[0] and b[np-1] -> a[np-1] at
+// the ends and a[0] (index M) fixed.  The arithmetic per entry is the one of
+// the rotation kernel's logged (c, s) on sorted (p, q), in round order.  The
+// log is staged through LDS kApplyChunk rounds at a time, the next chunk loaded
+// into registers while the current one is applied.  S = 2 serves n <= 256,
+// S = 3 n <= 384.
 constexpr int kApplyChunk = 16;
-constexpr int kApplyPer = kApplyChunk * 128 / 64;  // log entries per lane and chunk
+template <int S>
 __global__ __launch_bounds__(64) void jacobi_apply_kernel(
     int n_max, const int32_t *__restrict__ n_dev, const double2 *__restrict__ rotlog,
     int64_t strideLog, const int32_t *__restrict__ rank, const int32_t *__restrict__ nsweeps,
     int64_t strideInts, double *__restrict__ V, int ldv, int64_t strideV) {
-   __shared__ double2 lg[kApplyChunk * 128];
-   __shared__ double row[256];
+   constexpr int kPer = kApplyChunk * S;  // log entries per lane and chunk
+   __shared__ double2 lg[kApplyChunk * 64 * S];
+   __shared__ double row[128 * S];
    const int b = blockIdx.y;
    const int n = n_dev ? min((int)n_dev[b], n_max) : n_max;
    const int r0 = blockIdx.x;
@@ -273,19 +279,24 @@ __global__ __launch_bounds__(64) void jacobi_apply_kernel(
    const int rounds = nsweeps[b * strideInts] * M;
    const double2 *log = rotlog + b * strideLog;
    const int lane = threadIdx.x;
-   const int sA = lane, sB = lane + 64;  // the lane's two slots
-   const bool hasA = sA < np, hasB = sB < np;
-   // round 0: slot 0 = (M, 0), slot s = (s, M - s)
-   int ia = (sA == 0) ? M : sA, ib = (sA == 0) ? 0 : M - sA;   // slot sA indices (a, b)
-   int ja = sB, jb = M - sB;                                     // slot sB indices (a, b)
-   double a0 = (hasA && ia == r0) ? 1.0 : 0.0, b0 = (hasA && ib == r0) ? 1.0 : 0.0;
-   double a1 = (hasB && ja == r0) ? 1.0 : 0.0, b1 = (hasB && jb == r0) ? 1.0 : 0.0;
-   double2 pre[kApplyPer];
+   bool has[S];
+   int ia[S], ib[S];  // indices (a, b) of slot lane + 64 u; round 0: slot 0 = (M, 0), slot s = (s, M - s)
+   double av[S], bv[S];
+#pragma unroll
+   for (int u = 0; u < S; ++u) {
+      const int sl = lane + 64 * u;
+      has[u] = sl < np;
+      ia[u] = (sl == 0) ? M : sl;
+      ib[u] = (sl == 0) ? 0 : M - sl;
+      av[u] = (has[u] && ia[u] == r0) ? 1.0 : 0.0;
+      bv[u] = (has[u] && ib[u] == r0) ? 1.0 : 0.0;
+   }
+   double2 pre[kPer];
    auto fetch = [&](int c0) {
       const int nr = min(kApplyChunk, rounds - c0);
       const double2 *src = log + (int64_t)c0 * np;
 #pragma unroll
-      for (int u = 0; u < kApplyPer; ++u) {
+      for (int u = 0; u < kPer; ++u) {
          const int i = lane + 64 * u;
          pre[u] = (i < nr * np) ? src[i] : make_double2(1.0, 0.0);
       }
@@ -304,43 +315,56 @@ __global__ __launch_bounds__(64) void jacobi_apply_kernel(
    for (int c0 = 0; c0 < rounds; c0 += kApplyChunk) {
       const int nr = min(kApplyChunk, rounds - c0);
 #pragma unroll
-      for (int u = 0; u < kApplyPer; ++u) lg[lane + 64 * u] = pre[u];
+      for (int u = 0; u < kPer; ++u) lg[lane + 64 * u] = pre[u];
       __syncthreads();
       if (c0 + kApplyChunk < rounds) fetch(c0 + kApplyChunk);  // in flight during this chunk
       for (int k = 0; k < nr; ++k) {
          const double2 *lr = lg + k * np;
-         if (hasA) rotate(lr[sA], ia, ib, a0, b0);
-         if (hasB) rotate(lr[sB], ja, jb, a1, b1);
+#pragma unroll
+         for (int u = 0; u < S; ++u)
+            if (has[u]) rotate(lr[lane + 64 * u], ia[u], ib[u], av[u], bv[u]);
          // move to the next round's slots
-         const double a0n = __shfl(a0, lane + 1, 64);       // a[s + 1], same set
-         const double a1n = __shfl(a1, lane + 1, 64);
-         const double a1f = __shfl(a1, 0, 64);              // a[64] for slot 63
-         const double b0p = __shfl(b0, lane - 1, 64);       // b[s - 1], same set
-         const double b1p = __shfl(b1, lane - 1, 64);
-         const double b0l = __shfl(b0, 63, 64);             // b[63] for slot 64
-         const double a01 = __shfl(a0, 1, 64);              // a[1] for slot 0
-         // slot sA
-         double na0, nb0;
-         if (sA == 0) { na0 = a0; nb0 = (np > 1) ? a01 : b0; }  // one slot: nothing moves
-         else {
-            na0 = (sA == np - 1) ? b0 : ((sA == 63) ? a1f : a0n);
-            nb0 = b0p;
+         double an[S], bp[S], af[S], bl[S];
+#pragma unroll
+         for (int u = 0; u < S; ++u) {
+            an[u] = __shfl(av[u], lane + 1, 64);  // a[s + 1], same set
+            bp[u] = __shfl(bv[u], lane - 1, 64);  // b[s - 1], same set
+            af[u] = __shfl(av[u], 0, 64);         // a[64 u]: a[s + 1] of lane 63 of set u - 1
+            bl[u] = __shfl(bv[u], 63, 64);        // b[64 u + 63]: b[s - 1] of lane 0 of set u + 1
          }
-         // slot sB (>= 64)
-         const double na1 = (sB == np - 1) ? b1 : a1n;
-         const double nb1 = (sB == 64) ? b0l : b1p;
-         a0 = na0; b0 = nb0; a1 = na1; b1 = nb1;
-         // indices of the next round
-         if (sA == 0) { ib = (ib + 1 >= M) ? 0 : ib + 1; }
-         else { ia = (ia + 1 == M) ? 0 : ia + 1; ib = (ib + 1 == M) ? 0 : ib + 1; }
-         ja = (ja + 1 == M) ? 0 : ja + 1;
-         jb = (jb + 1 == M) ? 0 : jb + 1;
+         const double a01 = __shfl(av[0], 1, 64);  // a[1] for slot 0
+         double na[S], nb[S];
+#pragma unroll
+         for (int u = 0; u < S; ++u) {
+            const int sl = lane + 64 * u;
+            if (sl == 0) {
+               na[u] = av[u];
+               nb[u] = (np > 1) ? a01 : bv[u];  // one slot: nothing moves
+            } else {
+               const double nxt = (lane == 63) ? ((u + 1 < S) ? af[u + 1 < S ? u + 1 : u] : av[u]) : an[u];
+               na[u] = (sl == np - 1) ? bv[u] : nxt;
+               nb[u] = (lane == 0) ? bl[u > 0 ? u - 1 : 0] : bp[u];
+            }
+         }
+#pragma unroll
+         for (int u = 0; u < S; ++u) {
+            av[u] = na[u];
+            bv[u] = nb[u];
+            // indices of the next round
+            if (lane + 64 * u == 0) {
+               ib[u] = (ib[u] + 1 >= M) ? 0 : ib[u] + 1;
+            } else {
+               ia[u] = (ia[u] + 1 == M) ? 0 : ia[u] + 1;
+               ib[u] = (ib[u] + 1 == M) ? 0 : ib[u] + 1;
+            }
+         }
       }
       __syncthreads();  // the chunk is consumed before the next one is stored
    }
    // scatter the row back to index order through LDS, then to its ranked columns
-   if (hasA) { row[ia] = a0; row[ib] = b0; }
-   if (hasB) { row[ja] = a1; row[jb] = b1; }
+#pragma unroll
+   for (int u = 0; u < S; ++u)
+      if (has[u]) { row[ia[u]] = av[u]; row[ib[u]] = bv[u]; }
    __syncthreads();
    double *Vb = V + b * strideV;
    const int32_t *rk = rank + b * strideInts;
@@ -367,7 +391,7 @@ int syev_batched(ofrr_ctx *ctx, int batch, int n, const int32_t *n_dev, const do
                  int64_t strideA, double *w, int64_t stridew, double *V, int ldv,
                  int64_t strideV, double *work /* nullable: workspace */) {
    if (n <= 0 || batch <= 0) return kSuccess;
-   if (n > 256) return set_error(ctx, kErrorBLASInvalidSize, "syev: n > 256 not supported");
+   if (n > kJacobiMaxN) return set_error(ctx, kErrorBLASInvalidSize, "syev: n > 384 not supported");
    const JacobiLayout L = jacobi_layout(n);
    const int64_t strideWork = (int64_t)L.total;
    if (!work) {
@@ -404,15 +428,23 @@ int syev_batched(ofrr_ctx *ctx, int batch, int n, const int32_t *n_dev, const do
       jacobi_rot_kernel<true, false, kBlkPacked><<<batch, kJacobiThreads, lds, ctx->stream>>>(
           n, n_dev, A, lda, strideA, w, stridew, rotlog, strideLog, rank, nsw, strideInts, work,
           strideWork, kMaxSweeps);
-   } else {
+   } else if (nn <= 256) {
       jacobi_rot_kernel<false, false, kBlkGlobal><<<batch, kJacobiThreads, 0, ctx->stream>>>(
+          n, n_dev, A, lda, strideA, w, stridew, rotlog, strideLog, rank, nsw, strideInts, work,
+          strideWork, kMaxSweeps);
+   } else {
+      jacobi_rot_kernel<false, false, kBlkWide><<<batch, kJacobiThreads, 0, ctx->stream>>>(
           n, n_dev, A, lda, strideA, w, stridew, rotlog, strideLog, rank, nsw, strideInts, work,
           strideWork, kMaxSweeps);
    }
    OFRR_LAUNCH(ctx);
    dim3 grid(n, batch);
-   jacobi_apply_kernel<<<grid, 64, 0, ctx->stream>>>(n, n_dev, rotlog, strideLog, rank, nsw,
-                                                    strideInts, V, ldv, strideV);
+   if (nn <= 256)
+      jacobi_apply_kernel<2><<<grid, 64, 0, ctx->stream>>>(n, n_dev, rotlog, strideLog, rank, nsw,
+                                                          strideInts, V, ldv, strideV);
+   else
+      jacobi_apply_kernel<3><<<grid, 64, 0, ctx->stream>>>(n, n_dev, rotlog, strideLog, rank, nsw,
+                                                          strideInts, V, ldv, strideV);
    OFRR_LAUNCH(ctx);
    return kSuccess;
 }
@@ -506,14 +538,18 @@ __global__ void rr_finish_kernel(int m, int nkeep, const int32_t *__restrict__ r
    if (threadIdx.x == 0) theta[c] = (c < r) ? th[r - 1 - c] : __builtin_nan("");
 }
 
+// workspace (bytes) rr_truncated asks for at basis dimension m
+size_t rr_workspace_bytes(int m) {
+   const size_t mm = (size_t)m * m;
+   return (m + 6 * mm + 2 * m + 10 + jacobi_work_doubles(m)) * sizeof(double);
+}
+
 int rr_truncated(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H, int ldh,
                  double tau, int nkeep, double *theta, double *Y, int ldy, int32_t *rank) {
    if (m <= 0 || nkeep <= 0 || nkeep > m) return set_error(ctx, kErrorInvalidArgument, "rr_truncated: sizes");
    // workspace: sc(m) Gs Hs U Bm T Hr Z (m*m each) lam th (m) rank(1) + jacobi work
    const size_t mm = (size_t)m * m;
-   const size_t jw = jacobi_work_doubles(m);
-   size_t need = (m + 6 * mm + 2 * m + 10 + jw) * sizeof(double);
-   double *base = (double *)workspace(ctx, need);
+   double *base = (double *)workspace(ctx, rr_workspace_bytes(m));
    if (!base) return set_error(ctx, kErrorCUDAMemory, "rr_truncated: workspace");
    double *sc = base, *Gs = sc + m, *Hs = Gs + mm, *U = Hs + mm, *Bm = U + mm, *T = Bm + mm;
    double *Hr = T + mm, *lam = Hr + mm, *th = lam + m;
@@ -663,14 +699,14 @@ int ofrr_sygv(ofrr_ctx *ctx, int n, const double *A, int lda, const double *B, i
    OFRR_CHECK_CTX(ctx);
    if (n <= 0 || lda < n || ldb < n || ldv < n || !A || !B || !w || !V)
       return set_error(ctx, kErrorInvalidArgument, "sygv: bad arguments");
-   if (n > 256) return set_error(ctx, kErrorBLASInvalidSize, "sygv: n > 256 not supported");
+   if (n > kJacobiMaxN) return set_error(ctx, kErrorBLASInvalidSize, "sygv: n > 384 not supported");
    return sygv_impl(ctx, n, A, lda, B, ldb, 0.0, 0, w, V, ldv);
 }
 
 int ofrr_rr_truncated(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H, int ldh,
                       double tau, int nkeep, double *theta, double *Y, int ldy, int32_t *rank) {
    OFRR_CHECK_CTX(ctx);
-   if (m <= 0 || m > 256 || ldg < m || ldh < m || ldy < m || !G || !H || !theta || !Y)
+   if (m <= 0 || m > kJacobiMaxN || ldg < m || ldh < m || ldy < m || !G || !H || !theta || !Y)
       return set_error(ctx, kErrorInvalidArgument, "rr_truncated: bad arguments");
    return rr_truncated(ctx, m, G, ldg, H, ldh, tau, nkeep, theta, Y, ldy, rank);
 }
@@ -678,7 +714,7 @@ int ofrr_rr_truncated(ofrr_ctx *ctx, int m, const double *G, int ldg, const doub
 int ofrr_rr_generalized(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H, int ldh,
                         double gtol, double *theta, double *Y, int ldy) {
    OFRR_CHECK_CTX(ctx);
-   if (m <= 0 || m > 256 || ldg < m || ldh < m || ldy < m || !G || !H || !theta || !Y)
+   if (m <= 0 || m > kJacobiMaxN || ldg < m || ldh < m || ldy < m || !G || !H || !theta || !Y)
       return set_error(ctx, kErrorInvalidArgument, "rr_generalized: bad arguments");
    return rr_generalized(ctx, m, G, ldg, H, ldh, gtol, theta, Y, ldy);
 }

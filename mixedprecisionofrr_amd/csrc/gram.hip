@@ -348,6 +348,16 @@ __global__ __launch_bounds__(256) void gram_lds_reduce_kernel(int nchunks, int k
    o.dst[item][(o.ro[item] + i) + (int64_t)(o.co[item] + j) * o.ld[item]] = t;
 }
 
+// Largest workspace (bytes) gram or gram_multi asks for at any n: gram's
+// partials are at most max(tiles, kGramChunks) chunk tiles of GT x GT doubles;
+// gram_multi's at most max(groups, 2 num_cus) chunks of kGramWaves half tiles
+size_t gram_workspace_bound(int num_cus) {
+   const size_t g1 = (size_t)std::max(kGramChunks, 36) * GT * GT * sizeof(double);
+   const int groups = (2 * kMaxGramItems + kGramWaves - 1) / kGramWaves;
+   const size_t g2 = (size_t)std::max(2 * num_cus, groups) * kGramWaves * (GT * GT / 2) * sizeof(double);
+   return std::max(g1, g2);
+}
+
 int gram_multi(ofrr_ctx *ctx, int64_t n, int k, int tb, int tz, GramMultiArgs &args) {
    if (args.nitems <= 0) return kSuccess;
    if (args.nitems > kMaxGramItems || k <= 0 || k > 128)

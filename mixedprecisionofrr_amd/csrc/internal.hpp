@@ -5,11 +5,13 @@
 namespace ofrr {
 
 // Y = alpha A X + beta Y (any layout; row-major contiguous blocks take the fast path).
+// n rows of A and Y; nx rows of X (the columns of A; -1: n).
 // bw: the operator's bandwidth max |col - row| when known (selects the strip
 // schedule of the row tiles, sched_block), 0 for row order
 int spmm_csr(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const void *va,
              int vdt, int k, const void *X, int xdt, int64_t xrs, int64_t xcs, double alpha,
-             double beta, void *Y, int ydt, int64_t yrs, int64_t ycs, int64_t bw = 0);
+             double beta, void *Y, int ydt, int64_t yrs, int64_t ycs, int64_t bw = 0,
+             int64_t nx = -1);
 // W = A X, R = W - X diag(theta) quantised by inv_sR into Rq (rdt), column max
 // |R| folded into colmax (atomic max), sums[2j] = ||r_j||^2, sums[2j+1] = ||x_j||^2.
 // X, W, Rq row-major with row stride k.
@@ -61,6 +63,13 @@ int gram_multi(ofrr_ctx *ctx, int64_t n, int k, int tb, int tz, GramMultiArgs &a
 int basis_update(ofrr_ctx *ctx, int64_t n, int k, int nb, const void *const *V, const int *vdt,
                  const int *kb, const int64_t *vrs, const int64_t *vcs, const double *const *C,
                  const int *ldc, void *Y, int ydt, int64_t yrs, int64_t ycs);
+// largest workspace of gram / gram_multi at any n (bytes)
+size_t gram_workspace_bound(int num_cus);
+// workspace of rr_truncated at basis dimension m (bytes)
+size_t rr_workspace_bytes(int m);
+// largest workspace a filtered solve over a CSR operator asks for (block width
+// <= 128, basis dimension <= 384, any n): the serving slots reserve it once
+size_t filtered_workspace_bound(int num_cus);
 // truncated Rayleigh-Ritz (device pointers, FP64)
 int rr_truncated(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H, int ldh,
                  double tau, int nkeep, double *theta, double *Y, int ldy, int32_t *rank);

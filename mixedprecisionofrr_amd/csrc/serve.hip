@@ -33,12 +33,11 @@ namespace {
 constexpr int kDefaultSlots = 4;
 constexpr int kDefaultDegree = 120;
 constexpr int kDefaultMaxIter = 200;
-// device bytes a slot's context holds beside the request memory: the
-// Rayleigh-Ritz workspace and Jacobi rotation log at basis dimension 256
-constexpr int64_t kSlotOverhead = 64ll << 20;
 
 struct Plan {
    int k = 0, nd = 0, degree = 0, maxiter = 0, lo_dt = OFRR_F64;
+   int refine = 0, refine_dtype = OFRR_F64;
+   double refine_tol = 0.0;
    int64_t n = 0, nnz = 0;
    size_t off_ci = 0, off_va = 0, off_lo = 0, off_x = 0, off_state = 0, total = 0;
    size_t state_bytes = 0;
@@ -46,23 +45,44 @@ struct Plan {
 
 inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// Block width: the smallest power of two >= 1.25 nwant (at least 16, at most
-// 128), so the block carries a margin of unwanted directions beyond the wanted
-// edge; two correction blocks while the basis dimension stays <= 256, else one.
+// Block width (unless the request fixes it): the smallest power of two >=
+// 1.25 nwant (at least 16, at most 128), so the block carries a margin of
+// unwanted directions beyond the wanted edge; two correction blocks while the
+// basis dimension stays <= 256, else one.
 int plan_request(const ofrr_eig_request &r, Plan &p) {
    if ((r.dim != 2 && r.dim != 3) || r.N < 4 || r.nwant < 1 || r.nwant > OFRR_SERVE_MAX_WANT ||
        !dtype_valid(r.basis_dtype) || !(r.tol > 0.0) || r.degree < 0 || r.maxiter < 0)
+      return kErrorInvalidArgument;
+   if (r.k < 0 || (r.k > 0 && (r.k < r.nwant || r.k > 128)) || r.nd < 0 || r.nd > 2 ||
+       r.refine < -1 || r.refine > 1 || (r.refine == 1 && !dtype_valid(r.refine_dtype)))
       return kErrorInvalidArgument;
    p.n = r.dim == 2 ? r.N * r.N : r.N * r.N * r.N;
    if (p.n >= (int64_t)1 << 31) return kErrorBLASInvalidSize;
    int64_t nnz = 0;
    if (ofrr_stencil_nnz(r.dim, r.N, &nnz) != kSuccess) return kErrorInvalidArgument;
    p.nnz = nnz;
-   const int want = (5 * r.nwant + 3) / 4;
-   p.k = 16;
-   while (p.k < want && p.k < 128) p.k *= 2;
+   if (r.k > 0) {
+      p.k = r.k;
+   } else {
+      const int want = (5 * r.nwant + 3) / 4;
+      p.k = 16;
+      while (p.k < want && p.k < 128) p.k *= 2;
+   }
    if (p.k > p.n) return kErrorInvalidArgument;
-   p.nd = (3 * p.k <= 256) ? 2 : 1;
+   p.nd = r.nd > 0 ? r.nd : ((3 * p.k <= 256) ? 2 : 1);
+   if (p.k * (1 + p.nd) > 384) return kErrorInvalidArgument;
+   // precision ladder: FP8 corrections gain at most ~2^4 per sweep, so by
+   // default an FP8 solve finishes in FP16 once the wanted residuals are below
+   // 1e-4 or stall
+   if (r.refine == 1) {
+      p.refine = 1;
+      p.refine_dtype = r.refine_dtype;
+      p.refine_tol = r.refine_tol;
+   } else if (r.refine == 0 && r.basis_dtype == OFRR_FP8) {
+      p.refine = 1;
+      p.refine_dtype = OFRR_F16;
+      p.refine_tol = 1e-4;
+   }
    p.degree = r.degree ? r.degree : kDefaultDegree;
    p.maxiter = r.maxiter ? r.maxiter : kDefaultMaxIter;
    // operator values of the growth products: FP64 for an FP64 basis, else a
@@ -75,10 +95,8 @@ int plan_request(const ofrr_eig_request &r, Plan &p) {
    o.nd = p.nd;
    o.degree = p.degree;
    o.basis_dtype = r.basis_dtype;
-   if (r.basis_dtype == OFRR_FP8) {  // the FP16 refinement of run_one
-      o.refine = 1;
-      o.refine_dtype = OFRR_F16;
-   }
+   o.refine = p.refine;
+   o.refine_dtype = p.refine_dtype;
    int64_t sb = 0;
    if (ofrr_filtered_eig_state_bytes(p.n, &o, &sb) != kSuccess) return kErrorInvalidArgument;
    p.state_bytes = (size_t)sb;
@@ -127,6 +145,10 @@ struct ofrr_server {
    std::chrono::steady_clock::time_point t0;
    std::vector<std::thread> workers;
    std::vector<int> worker_status;
+   // device bytes the slots' contexts hold for the whole life of the server:
+   // each reserves the largest workspace a solve asks for when it starts, so
+   // admission counts request memory against budget - slot_bytes
+   int64_t slot_bytes = 0;
 
    double now() const {
       return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -135,7 +157,7 @@ struct ofrr_server {
    bool head_fits() const {
       if (queue.empty()) return false;
       const Rec &r = recs.at(queue.front());
-      return bytes_running + r.footprint <= budget;
+      return bytes_running + r.footprint <= budget - slot_bytes;
    }
    void run_one(ofrr_ctx *ctx, Rec &r);
    void worker(int slot);
@@ -177,13 +199,9 @@ void ofrr_server::run_one(ofrr_ctx *ctx, Rec &r) {
       o.lower = 0.0;  // the diffusion operators are positive definite
       o.normA = 0.0;  // ||A||_2 = the largest Ritz value
       o.lock = 1;     // later sweeps filter only the columns after the converged pairs
-      if (q.basis_dtype == OFRR_FP8) {
-         // FP8 corrections gain at most ~2^4 per sweep: finish in FP16 once the
-         // wanted residuals are below 1e-4 or stall
-         o.refine = 1;
-         o.refine_dtype = OFRR_F16;
-         o.refine_tol = 1e-4;
-      }
+      o.refine = p.refine;
+      o.refine_dtype = p.refine_dtype;
+      o.refine_tol = p.refine_tol;
       std::vector<double> th(p.k), rs(p.k);
       ofrr_filter_stats stats{};
       ctx->state_mem = mem + p.off_state;
@@ -212,9 +230,12 @@ void ofrr_server::worker(int slot) {
    int st = ofrr_ctx_create(device, &ctx);
    if (!st && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) st = kErrorCUDAGeneral;
    if (!st) st = ofrr_ctx_set_stream(ctx, s);
+   // the workspace every solve of this slot reuses, reserved once up front
+   if (!st && !workspace(ctx, filtered_workspace_bound(ctx->num_cus))) st = kErrorCUDAMemory;
    {
       std::lock_guard<std::mutex> lk(mu);
       worker_status[slot] = st;
+      if (!st) slot_bytes += (int64_t)ctx->ws_bytes;
    }
    cv_done.notify_all();
    for (;;) {
@@ -256,7 +277,7 @@ int ofrr_request_footprint(const ofrr_eig_request *req, int *k, int *nd, int64_t
    if (st) return st;
    if (k) *k = p.k;
    if (nd) *nd = p.nd;
-   if (bytes) *bytes = (int64_t)p.total + kSlotOverhead;
+   if (bytes) *bytes = (int64_t)p.total;
    return kSuccess;
 }
 
@@ -335,11 +356,11 @@ int ofrr_server_submit(ofrr_server *s, const ofrr_eig_request *req, int64_t *id)
    r.req = *req;
    int st = plan_request(*req, r.plan);
    if (st) return st;
-   r.footprint = (int64_t)r.plan.total + kSlotOverhead;
+   r.footprint = (int64_t)r.plan.total;
    {
       std::lock_guard<std::mutex> lk(s->mu);
       if (s->stopping) return kErrorInvalidOperation;
-      if (r.footprint > s->budget) return kErrorOutOfMemory;
+      if (r.footprint > s->budget - s->slot_bytes) return kErrorOutOfMemory;
       r.res.t_submit = s->now();
       *id = s->next_id++;
       s->recs.emplace(*id, r);
@@ -386,6 +407,7 @@ int ofrr_server_stats_get(ofrr_server *s, ofrr_server_stats *out) {
    out->bytes_peak = s->bytes_peak;
    out->running_peak = s->running_peak;
    out->budget = s->budget;
+   out->slot_bytes = s->slot_bytes;
    return kSuccess;
 }
 

@@ -175,6 +175,10 @@ static thread_local int g_arena_dev = -1;
 // caller-owned memory of the current solve on this thread (null: g_arena)
 static thread_local char *g_ext_base = nullptr;
 static thread_local size_t g_ext_off = 0;
+// bytes the current solve reserved: a take past them returns null (and so does
+// every later take, since the offset only grows), so checking the last pointer
+// of a sequence of takes catches an undercounted reservation
+static thread_local size_t g_reserved = 0;
 
 static int arena_reserve(ofrr_ctx *ctx, size_t bytes) {
    if (ctx->state_mem) {
@@ -183,9 +187,11 @@ static int arena_reserve(ofrr_ctx *ctx, size_t bytes) {
          return set_error(ctx, kErrorOutOfMemory, "filtered_eig: caller state memory too small");
       g_ext_base = (char *)ctx->state_mem;
       g_ext_off = 0;
+      g_reserved = bytes;
       return kSuccess;
    }
    g_ext_base = nullptr;
+   g_reserved = bytes;
    if (g_arena.base && g_arena.bytes >= bytes && g_arena_dev == ctx->device) {
       g_arena.off = 0;
       return kSuccess;
@@ -202,14 +208,11 @@ static int arena_reserve(ofrr_ctx *ctx, size_t bytes) {
    return kSuccess;
 }
 static void *arena_take(size_t bytes) {
-   if (g_ext_base) {
-      const size_t a = (g_ext_off + 255) & ~(size_t)255;
-      g_ext_off = a + bytes;
-      return g_ext_base + a;
-   }
-   size_t a = (g_arena.off + 255) & ~(size_t)255;
-   g_arena.off = a + bytes;
-   return g_arena.base + a;
+   size_t &off = g_ext_base ? g_ext_off : g_arena.off;
+   char *base = g_ext_base ? g_ext_base : g_arena.base;
+   const size_t a = (off + 255) & ~(size_t)255;
+   off = a + bytes;
+   return (off <= g_reserved) ? base + a : nullptr;
 }
 
 static inline size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -279,6 +282,7 @@ int ofrr_arnoldi_hess_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *
    double *piv = (double *)arena_take((size_t)m * 8);
    double *coef = (double *)arena_take((size_t)m * 8);
    int32_t *perm = (int32_t *)arena_take((size_t)m * 4);
+   if (!perm) return set_error(ctx, kErrorInternalError, "arnoldi: state layout exceeds its reservation");
    double *hp = (double *)pinned(ctx, (size_t)m * 8 + 64);
    if (!hp) return set_error(ctx, kErrorCUDAMemory, "arnoldi: pinned");
    OFRR_HIP(ctx, hipMemcpyAsync(V, v0, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
@@ -358,6 +362,7 @@ int ofrr_subspace_iter_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t 
    double *piv = (double *)arena_take((size_t)m * 8);
    int32_t *perm = (int32_t *)arena_take((size_t)m * 4);
    int32_t *skip = (int32_t *)arena_take((size_t)m * 4);
+   if (!skip) return set_error(ctx, kErrorInternalError, "subspace: state layout exceeds its reservation");
    int32_t *hskip = (int32_t *)pinned(ctx, (size_t)m * 4 + (size_t)m * 8);
    if (!hskip) return set_error(ctx, kErrorCUDAMemory, "subspace: pinned");
    double *hth = (double *)(hskip + m);
@@ -543,6 +548,7 @@ int ofrr_svd_rr(ofrr_ctx *ctx, int64_t M, int64_t N, const double *A, int64_t ld
    double *wt = (double *)arena_take((size_t)tall * 8);
    double *small = (double *)arena_take(svd_small_doubles(ku, kv) * 8);
    double *sig = (double *)arena_take((size_t)(ku + kv) * 8);
+   if (!sig) return set_error(ctx, kErrorInternalError, "svd_rr: state layout exceeds its reservation");
    OFRR_HIP(ctx, hipMemcpy2DAsync(Ud, (size_t)M * 8, U, (size_t)ldu * 8, (size_t)M * 8, ku,
                                   hipMemcpyDeviceToDevice, ctx->stream));
    OFRR_HIP(ctx, hipMemcpy2DAsync(Vd, (size_t)N * 8, V, (size_t)ldv * 8, (size_t)N * 8, kv,
@@ -583,6 +589,7 @@ int ofrr_subspace_iter_svd(ofrr_ctx *ctx, int64_t M, int64_t N, const double *A,
    double *piv = (double *)arena_take((size_t)k * 8);
    int32_t *perm = (int32_t *)arena_take((size_t)k * 4);
    int32_t *skip = (int32_t *)arena_take((size_t)k * 4);
+   if (!skip) return set_error(ctx, kErrorInternalError, "subspace_iter_svd: state layout exceeds its reservation");
    int32_t *hskip = (int32_t *)pinned(ctx, (size_t)k * 4 + (size_t)k * 8);
    if (!hskip) return set_error(ctx, kErrorCUDAMemory, "subspace_iter_svd: pinned");
    double *hsig = (double *)(hskip + k);
@@ -853,77 +860,205 @@ struct AtaOp final : FilterOp {
    }
 };
 
-// device bytes of the filtered solver's state: X, X_new, W (FP64 n x k), the
-// quantised residual and four filter blocks (basis format), the projected-matrix
-// products Z (n x k nd), and the small m x m / per-column arrays
-static size_t filtered_state_bytes(int64_t n, const ofrr_filter_opts &o) {
-   const int k = o.k, nd = o.nd, d = o.degree, m = k * (1 + nd);
+// Storage width of the filter blocks: the smallest power of two >= k (the
+// lane layout of the step kernels and of the fused projection products).  For
+// a k that is not a power of two the blocks carry kp - k zero columns: X and W
+// hold zeros there, so R, every filter block and Z do too; the Rayleigh-Ritz
+// runs on the compacted Gram and projected matrices (the padded rows and
+// columns are dropped before the eigensolves, where their zero diagonal would
+// break the scaling) and its coefficients are scattered back with zeros for the
+// padding, so the padding never mixes into the real columns.
+static int filter_width(int k) {
+   int kp = 1;
+   while (kp < k) kp *= 2;
+   return kp;
+}
+
+// The filtered solver's device state.  One layout serves the sizing
+// (ofrr_filtered_eig_state_bytes and the serving footprint) and the solve, so
+// the two cannot disagree; a take past the reservation returns null.
+struct FiltBufs {
+   double *X, *Xn, *W;        // n x kp FP64 row-major: Ritz block, next block, A X
+   void *Rq, *Db[4], *Z;      // quantised residual and filter blocks (n x kp), Z (n x kp nd)
+   double *G, *H, *C;         // projected matrices (mp x mp), coefficients (mp x kp)
+   double *Gc, *Hc, *Cc;      // compacted m x m, m x k (padded widths only)
+   int32_t *cmap, *cinv;      // compacted -> padded basis index (m), padded -> compacted (mp)
+   double *th, *sums, *invsR, *sR, *scale[4], *sD1, *zall;
+   float *cmax[3], *cmaxR;
+   int32_t *rank;
+};
+struct Carve {
+   char *base;
+   size_t cap, off;
+   void *take(size_t b) {
+      const size_t a = al(off);
+      off = a + b;
+      return (base && off <= cap) ? base + a : nullptr;
+   }
+};
+static void filtered_layout(Carve &cv, int64_t n, const ofrr_filter_opts &o, FiltBufs &b) {
+   const int k = o.k, kp = filter_width(k), nd = o.nd, d = o.degree;
+   const int m = k * (1 + nd), mp = kp * (1 + nd);
    const int bdt = o.basis_dtype, rdt = o.refine ? o.refine_dtype : bdt;
    // blocks sized for the larger of the growth format and the refinement format
    const size_t bsz = std::max(dtype_size(bdt), dtype_size(rdt));
    const size_t zsz = (bdt == OFRR_F64 || rdt == OFRR_F64) ? 8 : 4;
-   return al((size_t)n * k * 8) * 3 + al((size_t)n * k * bsz) * 5 + al((size_t)n * k * nd * zsz) +
-          al((size_t)m * m * 8) * 2 + al((size_t)m * k * 8) + al((size_t)k * 8) * 14 +
-          al((size_t)d * k * 8) + al((size_t)k * 4) * 4 + 4096;
+   const size_t nk = (size_t)n * kp, kq = (size_t)kp;
+   b.X = (double *)cv.take(nk * 8);
+   b.Xn = (double *)cv.take(nk * 8);
+   b.W = (double *)cv.take(nk * 8);
+   b.Rq = cv.take(nk * bsz);
+   for (int i = 0; i < 4; ++i) b.Db[i] = cv.take(nk * bsz);
+   b.Z = cv.take(nk * nd * zsz);
+   b.G = (double *)cv.take((size_t)mp * mp * 8);
+   b.H = (double *)cv.take((size_t)mp * mp * 8);
+   b.C = (double *)cv.take((size_t)mp * kp * 8);
+   b.Gc = b.Hc = b.Cc = nullptr;
+   b.cmap = b.cinv = nullptr;
+   if (kp != k) {
+      b.Gc = (double *)cv.take((size_t)m * m * 8);
+      b.Hc = (double *)cv.take((size_t)m * m * 8);
+      b.Cc = (double *)cv.take((size_t)m * k * 8);
+      b.cmap = (int32_t *)cv.take((size_t)m * 4);
+      b.cinv = (int32_t *)cv.take((size_t)mp * 4);
+   }
+   b.th = (double *)cv.take(kq * 8);
+   b.sums = (double *)cv.take(2 * kq * 8);
+   b.invsR = (double *)cv.take(kq * 8);
+   b.sR = (double *)cv.take(kq * 8);
+   for (int i = 0; i < 4; ++i) b.scale[i] = (double *)cv.take(kq * 8);
+   b.sD1 = (double *)cv.take(kq * 8);
+   b.zall = (double *)cv.take((size_t)d * kq * 8);
+   for (int i = 0; i < 3; ++i) b.cmax[i] = (float *)cv.take(kq * 4);
+   b.cmaxR = (float *)cv.take(kq * 4);
+   b.rank = (int32_t *)cv.take(64);
+}
+static size_t filtered_state_bytes(int64_t n, const ofrr_filter_opts &o) {
+   Carve cv{nullptr, 0, 0};
+   FiltBufs b;
+   filtered_layout(cv, n, o, b);
+   return al(cv.off);
+}
+
+size_t filtered_workspace_bound(int num_cus) {
+   // gram / gram_multi partials, the Rayleigh-Ritz at basis 384, the
+   // back-transform's repacked coefficients (3 blocks of 128, width 128)
+   const size_t basis = (size_t)(3 * 128 / 16) * 128 * 16 * sizeof(double);
+   return std::max(std::max(gram_workspace_bound(num_cus), rr_workspace_bytes(384)),
+                   std::max(basis, (size_t)256));
+}
+
+// p[0 .. nwords) = 0 (4-byte words): a kernel rather than a memset, so the
+// solver's clears are ordinary kernel nodes under stream capture
+__global__ void zero_words_kernel(int64_t nwords, uint32_t *__restrict__ p) {
+   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nwords;
+        t += (int64_t)gridDim.x * blockDim.x)
+      p[t] = 0u;
+}
+static int zero_bytes(ofrr_ctx *ctx, void *p, size_t bytes) {
+   const int64_t w = (int64_t)(bytes / 4);
+   if (w <= 0) return kSuccess;
+   const unsigned g = (unsigned)std::min<int64_t>((w + 255) / 256, 65536);
+   zero_words_kernel<<<g, 256, 0, ctx->stream>>>(w, (uint32_t *)p);
+   OFRR_LAUNCH(ctx);
+   return kSuccess;
+}
+
+// Gc (m x m) = rows and columns map[0 .. m) of G (column-major, ld ldg)
+__global__ void sym_gather_kernel(int m, const int32_t *__restrict__ map, const double *__restrict__ G,
+                                  int ldg, double *__restrict__ Gc) {
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= m * m) return;
+   const int i = t % m, j = t / m;
+   Gc[t] = G[map[i] + (int64_t)map[j] * ldg];
+}
+// C (mp x kp, ld mp) = the compacted coefficients Cc (m x k, ld m) at the
+// padded rows inv^-1, zero in the padded rows and in the columns >= k
+__global__ void coef_scatter_kernel(int mp, int kp, int k, int m, const int32_t *__restrict__ inv,
+                                    const double *__restrict__ Cc, double *__restrict__ C) {
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= mp * kp) return;
+   const int p = t % mp, j = t / mp;
+   const int i = inv[p];
+   C[t] = (j < k && i >= 0) ? Cc[i + (int64_t)j * m] : 0.0;
 }
 
 static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o, double *Xio,
                          double *theta_host, double *res_host, ofrr_filter_stats *stats) {
    const int64_t n = op.n;
-   const int k = o->k, nd = o->nd, dmax = o->degree;
+   // kr: the Ritz block width asked for; k: its storage width (a power of two)
+   const int kr = o->k, nd = o->nd, dmax = o->degree;
    int bdt = o->basis_dtype;  // the growth format of the current iteration
    const int d = dmax;        // buffer sizing: the largest degree of any sweep
-   if (k < 2 || k > 128 || (k & (k - 1)) || nd < 1 || nd > 2 || d < 2 || o->nwant < 1 ||
-       o->nwant > k || !dtype_valid(bdt) || (int64_t)k > n)
+   if (kr < 2 || kr > 128 || nd < 1 || nd > 2 || d < 2 || o->nwant < 1 || o->nwant > kr ||
+       !dtype_valid(bdt) || (int64_t)filter_width(kr) > n)
       return set_error(ctx, kErrorInvalidArgument,
-                       "filtered_eig: need k a power of two in [2,128], nd in {1,2}, degree >= 2, 1 <= nwant <= k");
+                       "filtered_eig: need k in [2,128], nd in {1,2}, degree >= 2, 1 <= nwant <= k");
    if (o->refine && (!dtype_valid(o->refine_dtype) || !op.can_refine()))
       return set_error(ctx, kErrorInvalidArgument,
                        "filtered_eig: the refinement format is valid for a CSR operator only");
+   const int k = filter_width(kr);
+   const bool padded = k != kr;
    const int m = k * (1 + nd);
-   if (m > 256) return set_error(ctx, kErrorInvalidArgument, "filtered_eig: basis dimension > 256");
+   if (kr * (1 + nd) > 384) return set_error(ctx, kErrorInvalidArgument, "filtered_eig: basis dimension > 384");
    int zdt = (bdt == OFRR_F64) ? OFRR_F64 : OFRR_F32;
    size_t zsz = dtype_size(zdt);
-   // buffers hold the larger of the growth and refinement formats
-   const int rdt0 = o->refine ? o->refine_dtype : bdt;
-   const size_t bsz = std::max(dtype_size(bdt), dtype_size(rdt0));
-   const size_t zsz_max = (bdt == OFRR_F64 || rdt0 == OFRR_F64) ? 8 : 4;
    // ---- buffers
-   int st = arena_reserve(ctx, filtered_state_bytes(n, *o));
+   const size_t state_bytes = filtered_state_bytes(n, *o);
+   int st = arena_reserve(ctx, state_bytes);
    if (st) return st;
-   double *X = (double *)arena_take((size_t)n * k * 8);
-   double *Xn = (double *)arena_take((size_t)n * k * 8);
-   double *W = (double *)arena_take((size_t)n * k * 8);
-   void *Rq = arena_take((size_t)n * k * bsz);
-   void *Db[4];
-   for (int i = 0; i < 4; ++i) Db[i] = arena_take((size_t)n * k * bsz);
-   void *Z = arena_take((size_t)n * k * nd * zsz_max);
-   double *G = (double *)arena_take((size_t)m * m * 8);
-   double *H = (double *)arena_take((size_t)m * m * 8);
-   double *C = (double *)arena_take((size_t)m * k * 8);
-   double *th = (double *)arena_take((size_t)k * 8);
-   double *sums = (double *)arena_take((size_t)2 * k * 8);
-   double *invsR = (double *)arena_take((size_t)k * 8);
-   double *sR = (double *)arena_take((size_t)k * 8);
-   double *coef = (double *)arena_take((size_t)4 * k * 8);
-   double *scale[4];
-   for (int i = 0; i < 4; ++i) scale[i] = (double *)arena_take((size_t)k * 8);
-   double *invs = (double *)arena_take((size_t)k * 8);
-   double *sD1 = (double *)arena_take((size_t)k * 8);
-   double *zall = (double *)arena_take((size_t)d * k * 8);
-   float *cmax[3];
-   for (int i = 0; i < 3; ++i) cmax[i] = (float *)arena_take((size_t)k * 4);
-   int32_t *rank = (int32_t *)arena_take(64);
-   float *cmaxR = (float *)arena_take((size_t)k * 4);
-   unsigned long long *gub = (unsigned long long *)arena_take(8);
+   FiltBufs fb;
+   {
+      Carve cv{(char *)arena_take(state_bytes), state_bytes, 0};
+      if (!cv.base) return set_error(ctx, kErrorInternalError, "filtered_eig: state reservation");
+      filtered_layout(cv, n, *o, fb);
+      if (!fb.rank) return set_error(ctx, kErrorInternalError, "filtered_eig: state layout exceeds its reservation");
+   }
+   double *X = fb.X, *Xn = fb.Xn, *W = fb.W;
+   void *Rq = fb.Rq, *Z = fb.Z;
+   void *const *Db = fb.Db;
+   double *G = fb.G, *H = fb.H, *C = fb.C;
+   double *th = fb.th, *sums = fb.sums, *invsR = fb.invsR, *sR = fb.sR, *sD1 = fb.sD1, *zall = fb.zall;
+   double *const *scale = fb.scale;
+   float *const *cmax = fb.cmax;
+   float *cmaxR = fb.cmaxR;
+   int32_t *rank = fb.rank;
    double *sRd = sR;
-   // pinned host mirrors
-   const size_t hbytes = (size_t)(2 * k + k + k + d * k + 4 * k + 16) * 8;
+   // pinned host mirrors (the compaction maps after the rank)
+   const size_t hbytes = (size_t)(2 * k + k + k + d * k + 4 * k + 16) * 8 + (size_t)2 * m * 4;
    double *hp = (double *)pinned(ctx, hbytes);
    if (!hp) return set_error(ctx, kErrorCUDAMemory, "filtered_eig: pinned");
    double *h_sums = hp, *h_th = hp + 2 * k, *h_sR = h_th + k, *h_z = h_sR + k;
    float *h_cmaxR = (float *)(h_z + (size_t)d * k);
    int32_t *h_rank = (int32_t *)(h_cmaxR + 2 * k);
+   int32_t *h_cmap = (int32_t *)(hp + (size_t)(2 * k + k + k + d * k + 4 * k + 16));
+   int32_t *h_cinv = h_cmap + m;
+   // Rayleigh-Ritz of the basis [X | D_d (| D_mid)] with me = k + nd ka padded
+   // columns, of which the correction blocks hold kar = ka - (k - kr) real
+   // ones: eigensolves on the compacted matrices when the blocks are padded
+   auto rayleigh_ritz = [&](int me, int ka) -> int {
+      if (!padded) return rr_truncated(ctx, me, G, me, H, me, o->tau, kr, th, C, me, rank);
+      const int kar = ka - (k - kr), nb = (me - k) / std::max(1, ka);
+      int mr = 0;
+      for (int p = 0; p < me; ++p) h_cinv[p] = -1;
+      for (int j = 0; j < kr; ++j) { h_cinv[j] = mr; h_cmap[mr++] = j; }
+      for (int bb = 0; bb < nb; ++bb)
+         for (int j = 0; j < kar; ++j) {
+            const int p = k + bb * ka + j;
+            h_cinv[p] = mr;
+            h_cmap[mr++] = p;
+         }
+      OFRR_HIP(ctx, hipMemcpyAsync(fb.cmap, h_cmap, (size_t)mr * 4, hipMemcpyHostToDevice, ctx->stream));
+      OFRR_HIP(ctx, hipMemcpyAsync(fb.cinv, h_cinv, (size_t)me * 4, hipMemcpyHostToDevice, ctx->stream));
+      sym_gather_kernel<<<(mr * mr + 255) / 256, 256, 0, ctx->stream>>>(mr, fb.cmap, G, me, fb.Gc);
+      sym_gather_kernel<<<(mr * mr + 255) / 256, 256, 0, ctx->stream>>>(mr, fb.cmap, H, me, fb.Hc);
+      OFRR_LAUNCH(ctx);
+      int s2 = rr_truncated(ctx, mr, fb.Gc, mr, fb.Hc, mr, o->tau, kr, th, fb.Cc, mr, rank);
+      if (s2) return s2;
+      coef_scatter_kernel<<<(me * k + 255) / 256, 256, 0, ctx->stream>>>(me, k, kr, mr, fb.cinv, fb.Cc, C);
+      OFRR_LAUNCH(ctx);
+      return kSuccess;
+   };
 
    hipEvent_t ev[8];
    for (int i = 0; i < 8; ++i) OFRR_HIP(ctx, hipEventCreate(&ev[i]));
@@ -937,13 +1072,19 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    // Gershgorin bound for a CSR operator); without one, the top Ritz value
    double upper = 0.0;
    if ((st = op.upper(ctx, &upper))) return st;
-   // ---- initial Rayleigh-Ritz on the start block
-   OFRR_HIP(ctx, hipMemcpyAsync(X, Xio, (size_t)n * k * 8, hipMemcpyDeviceToDevice, ctx->stream));
+   // ---- initial Rayleigh-Ritz on the start block (zero padding columns)
+   if (padded) {
+      if ((st = zero_bytes(ctx, X, (size_t)n * k * 8))) return st;
+      if ((st = convert(ctx, n, kr, Xio, OFRR_F64, kr, 1, X, OFRR_F64, k, 1))) return st;
+   } else {
+      OFRR_HIP(ctx, hipMemcpyAsync(X, Xio, (size_t)n * k * 8, hipMemcpyDeviceToDevice, ctx->stream));
+   }
+   if ((st = zero_bytes(ctx, th, (size_t)k * 8))) return st;
    if ((st = op.apply64(ctx, k, X, W))) return st;
    ++spmm_calls;
    if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, X, OFRR_F64, k, 1, 1.0, 0.0, G, k))) return st;
    if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, W, OFRR_F64, k, 1, 1.0, 0.0, H, k))) return st;
-   if ((st = rr_truncated(ctx, k, G, k, H, k, o->tau, k, th, C, k, rank))) return st;
+   if ((st = rayleigh_ritz(k, k))) return st;
    {
       const void *vb[1] = {X};
       int dt[1] = {OFRR_F64}, kb[1] = {k}, ldc[1] = {k};
@@ -987,7 +1128,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
          for (int j = 0; j < k; ++j) inv[j] = 1.0 / h_sR[j];
          OFRR_HIP(ctx, hipMemcpyAsync(invsR, inv.data(), (size_t)k * 8, hipMemcpyHostToDevice, ctx->stream));
          OFRR_HIP(ctx, hipMemcpyAsync(sRd, h_sR, (size_t)k * 8, hipMemcpyHostToDevice, ctx->stream));
-         OFRR_HIP(ctx, hipMemsetAsync(cmaxR, 0, (size_t)k * 4, ctx->stream));
+         if ((st = zero_bytes(ctx, cmaxR, (size_t)k * 4))) return st;
          if ((st = op.resid(ctx, k, X, th, W, Rq, bdt, invsR, cmaxR, sums))) return st;
          ++spmm_calls;
          ++resid_passes;
@@ -1002,7 +1143,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
             const double cmj = (double)h_cmaxR[j];
             const double ratio = cmj / h_sR[j];
             const double want = cmj > 0 ? std::exp2(std::ceil(std::log2(cmj)) - tgt_resid) : 1.0;
-            if (ratio > 0.5 * dtype_max(bdt) || ratio < lo_ratio) redo = true;
+            if (j < kr && (ratio > 0.5 * dtype_max(bdt) || ratio < lo_ratio)) redo = true;
             pred[j] = want;
          }
          if (!redo || pass == 1) break;
@@ -1048,7 +1189,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       // Dh_{j+1} = (2/e) q_j (A - c) Dh_j - q_{j-1} q_j Dh_{j-1} + (2/e) zeta_j q_j R
       // with q_j = mu_j / mu_{j+1} and zeta_j = z_j / mu_j (all O(1): no overflow
       // at any degree).
-      const double cut = theta[k - 1], lo = o->lower;
+      const double cut = theta[kr - 1], lo = o->lower;
       const double e = 0.5 * (cut - lo), c0 = 0.5 * (cut + lo);
       for (int j = 0; j < k; ++j) Lth[j] = (theta[j] - c0) / e;
       const double top = (upper > 0.0) ? std::max(theta[0], upper) : theta[0] * (1.0 + 1e-3);
@@ -1123,7 +1264,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       // the column maxima rotate over three buffers: step s reads cmax[(s-1)%3],
       // accumulates into cmax[s%3] (cleared by step s-1, or here for s = 1) and
       // clears cmax[(s+1)%3] for step s+1
-      OFRR_HIP(ctx, hipMemsetAsync(cmax[1], 0, (size_t)k * 4, ctx->stream));
+      if ((st = zero_bytes(ctx, cmax[1], (size_t)k * 4))) return st;
       for (int s = 1; s < d; ++s) {
          int out = -1;
          for (int b = 0; b < 4; ++b)
@@ -1237,7 +1378,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       mirror_upper_kernel<<<(me * me + 255) / 256, 256, 0, ctx->stream>>>(me, H, me);
       OFRR_LAUNCH(ctx);
       OFRR_HIP(ctx, hipEventRecord(ev[6], ctx->stream));
-      if ((st = rr_truncated(ctx, me, G, me, H, me, o->tau, k, th, C, me, rank))) return st;
+      if ((st = rayleigh_ritz(me, ka))) return st;
       OFRR_HIP(ctx, hipEventRecord(ev[7], ctx->stream));
       {
          const void *vb[3] = {X, Dd, Dmid};
@@ -1265,19 +1406,24 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
          t_acc[3] += e2;
       }
       rank_min = std::min(rank_min, (int)h_rank[0]);
-      if (h_rank[0] < k) {
+      if (h_rank[0] < kr) {
          st = set_error(ctx, kErrorSolverDivergence, "filtered_eig: truncated rank below the block width");
          break;
       }
       for (int j = 0; j < k; ++j) theta[j] = h_th[j];
    }
    OFRR_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
-   OFRR_HIP(ctx, hipMemcpyAsync(Xio, X, (size_t)n * k * 8, hipMemcpyDeviceToDevice, ctx->stream));
+   if (padded) {
+      const int st2 = convert(ctx, n, kr, X, OFRR_F64, k, 1, Xio, OFRR_F64, kr, 1);
+      if (st2) return st2;
+   } else {
+      OFRR_HIP(ctx, hipMemcpyAsync(Xio, X, (size_t)n * k * 8, hipMemcpyDeviceToDevice, ctx->stream));
+   }
    OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
    float total;
    OFRR_HIP(ctx, hipEventElapsedTime(&total, ev[0], ev[2]));
    for (int i = 0; i < 8; ++i) (void)hipEventDestroy(ev[i]);
-   for (int j = 0; j < k; ++j) {
+   for (int j = 0; j < kr; ++j) {
       theta_host[j] = theta[j];
       res_host[j] = res[j];
    }

@@ -143,15 +143,17 @@ template <typename TV, typename TX, typename TY, int VEC, typename ACC, bool OVE
           typename CI, typename VA>
 __device__ inline void spmm_rows(int64_t r0, int nr, int grp, int groups, int lane_g, int G, int k,
                                  CI cs, VA vs, const int32_t *s_rp, const TX *__restrict__ X,
-                                 int64_t ldx, double alpha, double beta, TY *__restrict__ Y,
-                                 int64_t ldy) {
+                                 int64_t ldx, int64_t nx, double alpha, double beta,
+                                 TY *__restrict__ Y, int64_t ldy) {
    for (int c0 = lane_g * VEC; c0 < k; c0 += G * VEC) {
       for (int lr = grp; lr < nr; lr += groups) {
          const int64_t row = r0 + lr;
          ACC acc[VEC];
 #pragma unroll
          for (int v = 0; v < VEC; ++v) acc[v] = 0;
-         row_gather8<TV, TX, VEC, ACC, OVERREAD>(cs, vs, s_rp[lr], s_rp[lr + 1], X, ldx, row, c0,
+         // the padding slots gather a row of X that exists (X has nx rows)
+         const int64_t self = row < nx ? row : nx - 1;
+         row_gather8<TV, TX, VEC, ACC, OVERREAD>(cs, vs, s_rp[lr], s_rp[lr + 1], X, ldx, self, c0,
                                                  acc);
          TY *y = Y + row * ldy + c0;
          if (YVEC) {
@@ -182,8 +184,9 @@ __device__ inline void spmm_rows(int64_t r0, int nr, int grp, int groups, int la
 template <typename TV, typename TX, typename TY, int VEC, bool YVEC>
 __global__ __launch_bounds__(256) void spmm_rm_kernel(
     int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
-    const TV *__restrict__ va, int k, const TX *__restrict__ X, int64_t ldx, double alpha,
-    double beta, TY *__restrict__ Y, int64_t ldy, int lg, int64_t nblk, int64_t plane) {
+    const TV *__restrict__ va, int k, const TX *__restrict__ X, int64_t ldx, int64_t nx,
+    double alpha, double beta, TY *__restrict__ Y, int64_t ldy, int lg, int64_t nblk,
+    int64_t plane) {
    using ACC = typename std::conditional<std::is_same<TX, double>::value, double, float>::type;
    __shared__ CsrTile<TV> tile;
    const int64_t blk = sched_block(nblk, plane);
@@ -198,10 +201,11 @@ __global__ __launch_bounds__(256) void spmm_rm_kernel(
    __syncthreads();
    if (in_lds)
       spmm_rows<TV, TX, TY, VEC, ACC, true, YVEC>(r0, nr, grp, GROUPS, lane_g, G, k, tile.ci,
-                                                  tile.va, tile.rp, X, ldx, alpha, beta, Y, ldy);
+                                                  tile.va, tile.rp, X, ldx, nx, alpha, beta, Y, ldy);
    else
       spmm_rows<TV, TX, TY, VEC, ACC, false, YVEC>(r0, nr, grp, GROUPS, lane_g, G, k, ci + base,
-                                                   va + base, tile.rp, X, ldx, alpha, beta, Y, ldy);
+                                                   va + base, tile.rp, X, ldx, nx, alpha, beta, Y,
+                                                   ldy);
 }
 
 // ---------------------------------------------------------------------------
@@ -432,7 +436,7 @@ static inline bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; 
 template <typename TV, typename TX, typename TY>
 static int launch_spmm(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci,
                        const TV *va, int k, const TX *X, int64_t xrs, int64_t xcs, double alpha,
-                       double beta, TY *Y, int64_t yrs, int64_t ycs, int64_t bw) {
+                       double beta, TY *Y, int64_t yrs, int64_t ycs, int64_t bw, int64_t nx) {
    constexpr int VEC = 16 / sizeof(TX) > 0 ? 16 / sizeof(TX) : 1;
    bool fast = xcs == 1 && ycs == 1 && k % VEC == 0 && aligned16(X) &&
                (xrs * (int64_t)sizeof(TX)) % 16 == 0;
@@ -451,10 +455,10 @@ static int launch_spmm(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_
       const bool yvec = ((uintptr_t)Y % yb) == 0 && ((size_t)yrs * sizeof(TY)) % yb == 0;
       if (yvec)
          spmm_rm_kernel<TV, TX, TY, VEC, true><<<grid, 256, 0, ctx->stream>>>(
-             n, rp, ci, va, k, X, xrs, alpha, beta, Y, yrs, lg, nblk, pt);
+             n, rp, ci, va, k, X, xrs, nx, alpha, beta, Y, yrs, lg, nblk, pt);
       else
          spmm_rm_kernel<TV, TX, TY, VEC, false><<<grid, 256, 0, ctx->stream>>>(
-             n, rp, ci, va, k, X, xrs, alpha, beta, Y, yrs, lg, nblk, pt);
+             n, rp, ci, va, k, X, xrs, nx, alpha, beta, Y, yrs, lg, nblk, pt);
    } else {
       int64_t total = n * (int64_t)k;
       spmm_strided_kernel<TV, TX, TY><<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(
@@ -466,7 +470,8 @@ static int launch_spmm(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_
 
 int spmm_csr(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const void *va,
              int vdt, int k, const void *X, int xdt, int64_t xrs, int64_t xcs, double alpha,
-             double beta, void *Y, int ydt, int64_t yrs, int64_t ycs, int64_t bw) {
+             double beta, void *Y, int ydt, int64_t yrs, int64_t ycs, int64_t bw, int64_t nx) {
+   if (nx < 0) nx = n;
    if (vdt != OFRR_F64 && vdt != OFRR_F32 && vdt != OFRR_F16)
       return set_error(ctx, kErrorBLASUnsupportedType, "spmm: operator values must be FP64, FP32 or FP16");
    if (ydt == OFRR_FP8)
@@ -478,7 +483,7 @@ int spmm_csr(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, con
          return dispatch_dtype(ydt, [&](auto ty) -> int {
             using TY = decltype(ty);
             return launch_spmm<TV, TX, TY>(ctx, n, rp, ci, (const TV *)va, k, (const TX *)X, xrs,
-                                           xcs, alpha, beta, (TY *)Y, yrs, ycs, bw);
+                                           xcs, alpha, beta, (TY *)Y, yrs, ycs, bw, nx);
          });
       });
    });

@@ -9,6 +9,8 @@
  *   orc_stencil_csr     closed-form nnz counts (64^2 grid 20,224 / 128^3 grid 14,581,760), symmetry,
  *                       sine-mode eigenvectors A s = lambda s (closed form), scipy kron build
  *   orc_spmm_csr        brute-force dense A X, integer-valued exact results
+ *   orc_spmm_csr_t      bitwise orc_spmm_csr on scipy's row-stable transpose, brute-force
+ *                       dense A^T X, integer-valued exact results, <A^T y, x> = <y, A x>
  *   orc_scalecols       hand cases incl. ties and zero columns
  *   orc_iamax           reference KATs mvops_test.cpp:774-850 (index 72345 -> -8.0, 1-based)
  *   orc_matrix_iamax    reference KATs mvops_test.cpp:1151-1497 (0-based)
@@ -158,6 +160,21 @@ void orc_spmm_csr(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, con
             acc = fma(vals[p], X[(int64_t)col_idx[p] + j * ldx], acc);
          Y[i + j * ldy] = acc;
       }
+   }
+}
+
+void orc_spmm_csr_t(int64_t nrows, int64_t ncols, const int64_t *row_ptr, const int32_t *col_idx,
+                    const double *vals, int k, const double *X, int64_t ldx, double *Y, int64_t ldy) {
+   /* columns of X are independent: the scatter order within a column is the
+    * reference's row order */
+#pragma omp parallel for schedule(static)
+   for (int j = 0; j < k; ++j) {
+      double *y = Y + (int64_t)j * ldy;
+      const double *x = X + (int64_t)j * ldx;
+      for (int64_t c = 0; c < ncols; ++c) y[c] = 0.0;
+      for (int64_t i = 0; i < nrows; ++i)
+         for (int64_t p = row_ptr[i]; p < row_ptr[i + 1]; ++p)
+            y[col_idx[p]] = fma(vals[p], x[i], y[col_idx[p]]);
    }
 }
 

@@ -41,6 +41,11 @@ void orc_stencil_csr(int dim, int64_t N, uint64_t seed, int64_t *row_ptr, int32_
 /* Y = A X (csrmv.m:48-59 semantics, FP64, fused multiply-add in row order) */
 void orc_spmm_csr(int64_t n, const int64_t *row_ptr, const int32_t *col_idx, const double *vals,
                   int k, const double *X, int64_t ldx, double *Y, int64_t ldy);
+/* Y = A^T X for an nrows x ncols CSR A: the 'T' branch of csrmv.m:61-71 --
+ * Y = 0, then for each row i in increasing order and each stored entry a_ij,
+ * Y(j,:) += a_ij X(i,:) (fused multiply-add); X nrows x k, Y ncols x k */
+void orc_spmm_csr_t(int64_t nrows, int64_t ncols, const int64_t *row_ptr, const int32_t *col_idx,
+                    const double *vals, int k, const double *X, int64_t ldx, double *Y, int64_t ldy);
 /* divide each column by its signed max-abs entry (scalecols.m:26-33) */
 void orc_scalecols(int64_t m, int n, double *A, int64_t lda);
 /* 1-based index of the first max |x|, signed value (mvops.cu:607-946) */

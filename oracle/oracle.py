@@ -44,6 +44,8 @@ def lib():
         L.orc_stencil_csr.argtypes = [C.c_int, C.c_int64, C.c_uint64, _ip64, _ip32, _dp]
         L.orc_spmm_csr.argtypes = [C.c_int64, _ip64, _ip32, _dp, C.c_int, _dp, C.c_int64, _dp,
                                    C.c_int64]
+        L.orc_spmm_csr_t.argtypes = [C.c_int64, C.c_int64, _ip64, _ip32, _dp, C.c_int, _dp,
+                                     C.c_int64, _dp, C.c_int64]
         L.orc_scalecols.argtypes = [C.c_int64, C.c_int, _dp, C.c_int64]
         L.orc_iamax.argtypes = [C.c_int64, _dp, C.POINTER(C.c_double)]
         L.orc_matrix_iamax.argtypes = [C.c_int64, C.c_int, _dp, C.c_int64, _ip32]
@@ -101,8 +103,9 @@ def randu(seed, n, stream=0, offset=0):
 
 
 class CSR:
-    def __init__(self, n, row_ptr, col_idx, vals):
+    def __init__(self, n, row_ptr, col_idx, vals, ncols=None):
         self.n = n
+        self.ncols = n if ncols is None else ncols
         self.row_ptr = np.ascontiguousarray(row_ptr, np.int64)
         self.col_idx = np.ascontiguousarray(col_idx, np.int32)
         self.vals = np.asfortranarray(vals, np.float64)
@@ -113,7 +116,7 @@ class CSR:
 
     def to_scipy(self):
         import scipy.sparse as sp
-        return sp.csr_matrix((self.vals, self.col_idx, self.row_ptr), shape=(self.n, self.n))
+        return sp.csr_matrix((self.vals, self.col_idx, self.row_ptr), shape=(self.n, self.ncols))
 
 
 def stencil(dim, N, seed=0):
@@ -128,10 +131,20 @@ def stencil(dim, N, seed=0):
 
 
 def spmm(A, X):
-    X = _f(X.reshape(A.n, -1))
+    """A X (csrmv 'N'); X has A.ncols rows."""
+    X = _f(X.reshape(A.ncols, -1))
     k = X.shape[1]
     Y = np.zeros((A.n, k), order="F")
-    lib().orc_spmm_csr(A.n, A.row_ptr, A.col_idx, A.vals, k, X, A.n, Y, A.n)
+    lib().orc_spmm_csr(A.n, A.row_ptr, A.col_idx, A.vals, k, X, A.ncols, Y, A.n)
+    return Y
+
+
+def spmm_t(A, X):
+    """A^T X (csrmv 'T', csrmv.m:61-71); X has A.n rows."""
+    X = _f(X.reshape(A.n, -1))
+    k = X.shape[1]
+    Y = np.zeros((A.ncols, k), order="F")
+    lib().orc_spmm_csr_t(A.n, A.ncols, A.row_ptr, A.col_idx, A.vals, k, X, A.n, Y, A.ncols)
     return Y
 
 

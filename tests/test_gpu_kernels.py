@@ -271,7 +271,7 @@ def test_syev_bitwise_vs_oracle(ctx):
     (fixed reduction orders, explicit fused multiply-adds on both sides)."""
     from mixedprecisionofrr_amd import api
     rng = np.random.default_rng(21)
-    for m in (2, 3, 17, 64, 127, 128, 130, 191, 192, 193, 200, 256):
+    for m in (2, 3, 17, 64, 127, 128, 130, 191, 192, 193, 200, 256, 257, 300, 383, 384):
         M = rng.standard_normal((m, m))
         S = M + M.T
         w, V = api.syev_batched(ctx, _dev(S[None]))
