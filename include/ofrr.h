@@ -451,11 +451,17 @@ typedef struct {
    int64_t slot_bytes;                /* workspace the slots hold for the server's life
                                          (admission: bytes_running + footprint <=
                                          budget - slot_bytes) */
+   int64_t ws_regrowths;              /* solves whose workspace outgrew the slot's
+                                         reservation (counted into slot_bytes) */
 } ofrr_server_stats;
 
 /* the plan of a request (block width k, correction blocks nd) and its device
  * footprint in bytes; host only */
 OFRR_API int ofrr_request_footprint(const ofrr_eig_request *req, int *k, int *nd, int64_t *bytes);
+/* device bytes each slot of a server on `device` reserves for its whole life
+ * (the largest workspace a solve asks for); a server of s slots admits requests
+ * against budget - s * this.  Host only (reads the device's CU count). */
+OFRR_API int ofrr_server_slot_bytes(int device, int64_t *bytes);
 OFRR_API int ofrr_server_create(int device, const ofrr_server_opts *opt, ofrr_server **srv);
 /* enqueue; *id identifies the request.  1002 when its footprint exceeds the budget */
 OFRR_API int ofrr_server_submit(ofrr_server *srv, const ofrr_eig_request *req, int64_t *id);

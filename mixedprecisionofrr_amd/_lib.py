@@ -26,7 +26,7 @@ EXPORTS = [
     "ofrr_server_wait", "ofrr_server_stats_get", "ofrr_server_destroy",
     "ofrr_csr_file_info", "ofrr_csr_file_read", "ofrr_singular_values_file", "ofrr_io_last_error",
     "ofrr_svd_rr", "ofrr_subspace_iter_svd", "ofrr_fp8_check", "ofrr_gs_qr", "ofrr_gs_qr_step",
-    "ofrr_spmm_csr_rect", "ofrr_csr_transpose",
+    "ofrr_spmm_csr_rect", "ofrr_csr_transpose", "ofrr_server_slot_bytes",
 ]
 
 SERVE_MAX_WANT = 128
@@ -73,7 +73,8 @@ class EigResult(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("running", C.c_int64),
                 ("queued", C.c_int64), ("bytes_running", C.c_int64), ("bytes_peak", C.c_int64),
-                ("running_peak", C.c_int), ("budget", C.c_int64), ("slot_bytes", C.c_int64)]
+                ("running_peak", C.c_int), ("budget", C.c_int64), ("slot_bytes", C.c_int64),
+                ("ws_regrowths", C.c_int64)]
 
 
 _lib = None
@@ -104,6 +105,7 @@ def _decl(L):
     L.ofrr_spmm_csr_rect.argtypes = [vp, i32, i64, i64, i64, vp, vp, vp, i32, i32, vp, i32, i64,
                                      i64, dbl, dbl, vp, i32, i64, i64]
     L.ofrr_csr_transpose.argtypes = [vp, i64, i64, i64, vp, vp, vp, i32, vp, vp, vp]
+    L.ofrr_server_slot_bytes.argtypes = [i32, C.POINTER(i64)]
     L.ofrr_iamax.argtypes = [vp, i32, i64, vp, i64, vp, vp]
     L.ofrr_matrix_iamax.argtypes = [vp, i32, i64, i32, vp, i64, i64, vp]
     L.ofrr_scalecols.argtypes = [vp, i32, i64, i32, vp, i64, i64, vp]

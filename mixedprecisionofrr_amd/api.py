@@ -652,11 +652,29 @@ def _sym_shift(rp, ci, va, n, symmetrize, shift):
 
 # ---- serving -----------------------------------------------------------------
 def request(dim, N, nwant, coef_seed=0, start_seed=0, basis_dtype=torch.float16, tol=1e-8,
-            degree=0, maxiter=0):
+            degree=0, maxiter=0, k=0, nd=0, refine_dtype=None, refine_tol=0.0, refine=None):
     """An eigen request: the nwant largest eigenpairs of the dim-D diffusion
-    stencil on an N^dim grid (coef_seed 0: the Laplacian)."""
+    stencil on an N^dim grid (coef_seed 0: the Laplacian).  k / nd fix the
+    block width and correction blocks (0: the server's plan); refine_dtype sets
+    the precision ladder's refinement format (refine=False: no ladder; default:
+    FP8 finishes in FP16)."""
+    if refine_dtype is not None:
+        rf, rdt = 1, dtype_code(refine_dtype)
+    else:
+        rf, rdt = (-1 if refine is False else 0), F64
     return _lib.EigRequest(dim, N, coef_seed, start_seed, nwant, dtype_code(basis_dtype), tol,
-                           degree, maxiter)
+                           degree, maxiter, k, nd, rf, rdt, refine_tol)
+
+
+def server_slot_bytes(device=0):
+    """Device bytes each server slot reserves for its lifetime (its workspace);
+    a server of s slots admits requests against budget - s * this."""
+    L = _lib.load()
+    b = C.c_int64()
+    st = L.ofrr_server_slot_bytes(device, C.byref(b))
+    if st != 0:
+        raise OfrrError(st, "server_slot_bytes")
+    return b.value
 
 
 def request_footprint(req):

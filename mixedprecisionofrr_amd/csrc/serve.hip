@@ -149,6 +149,7 @@ struct ofrr_server {
    // each reserves the largest workspace a solve asks for when it starts, so
    // admission counts request memory against budget - slot_bytes
    int64_t slot_bytes = 0;
+   int64_t ws_regrowths = 0;  // solves whose workspace outgrew the slot's reservation
 
    double now() const {
       return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -232,10 +233,14 @@ void ofrr_server::worker(int slot) {
    if (!st) st = ofrr_ctx_set_stream(ctx, s);
    // the workspace every solve of this slot reuses, reserved once up front
    if (!st && !workspace(ctx, filtered_workspace_bound(ctx->num_cus))) st = kErrorCUDAMemory;
+   int64_t held = 0;
    {
       std::lock_guard<std::mutex> lk(mu);
       worker_status[slot] = st;
-      if (!st) slot_bytes += (int64_t)ctx->ws_bytes;
+      if (!st) {
+         held = (int64_t)ctx->ws_bytes;
+         slot_bytes += held;
+      }
    }
    cv_done.notify_all();
    for (;;) {
@@ -255,6 +260,12 @@ void ofrr_server::worker(int slot) {
       cv_admit.notify_all();  // the next head may fit as well
       run_one(ctx, r);
       lk.lock();
+      if ((int64_t)ctx->ws_bytes > held) {
+         // the bound missed a workspace request: count the growth as held
+         slot_bytes += (int64_t)ctx->ws_bytes - held;
+         held = (int64_t)ctx->ws_bytes;
+         ++ws_regrowths;
+      }
       r.res.t_done = now();
       r.state = kDone;
       bytes_running -= r.footprint;
@@ -278,6 +289,16 @@ int ofrr_request_footprint(const ofrr_eig_request *req, int *k, int *nd, int64_t
    if (k) *k = p.k;
    if (nd) *nd = p.nd;
    if (bytes) *bytes = (int64_t)p.total;
+   return kSuccess;
+}
+
+int ofrr_server_slot_bytes(int device, int64_t *bytes) {
+   if (!bytes) return kErrorInvalidArgument;
+   int cus = 0;
+   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+      return kErrorCUDAGeneral;
+   const size_t b = filtered_workspace_bound(cus);
+   *bytes = (int64_t)(b + b / 4);  // what workspace() allocates for it
    return kSuccess;
 }
 
@@ -408,6 +429,7 @@ int ofrr_server_stats_get(ofrr_server *s, ofrr_server_stats *out) {
    out->running_peak = s->running_peak;
    out->budget = s->budget;
    out->slot_bytes = s->slot_bytes;
+   out->ws_regrowths = s->ws_regrowths;
    return kSuccess;
 }
 

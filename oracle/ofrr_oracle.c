@@ -16,7 +16,9 @@
  *   orc_matrix_iamax    reference KATs mvops_test.cpp:1151-1497 (0-based)
  *   orc_gemm_tn/nn      reference gemm KAT mvops_test.cpp:455-721 and numpy on random data
  *   orc_hessenberg      A = QR, Q(perm(i),i) = 1, Q(perm(j),i) = 0 (j<i), |Q| <= 1, and the
- *                       duplicated-column skip pattern of hessenberg_test.cpp:73-91
+ *                       duplicated-column skip pattern of hessenberg_test.cpp:73-91; with
+ *                       dropped columns A = Q R~ (R~ = R with a unit diagonal where dropped)
+ *                       for both variants, which differ exactly there
  *   orc_syev            reference KAT mvops_test.cpp:2292-2447 (tridiag(-1,2,-1), n = 10)
  *   orc_sygv            reference KAT mvops_test.cpp:2450-2626 (B = diag(1..10))
  *   orc_rr_truncated    exact Ritz values for sine-mode bases, planted rank drop
@@ -278,14 +280,19 @@ void orc_gemm_nn(int64_t m, int n, int k, const double *A, int64_t lda, const do
 /* ------------------------------------------------------------------------- */
 void orc_hessenberg(int64_t m, int n, double *Q, int64_t ldq, double *R, int ldr,
                     double select_tol, int variant, int *select, int *perm) {
+   /* the argmax row of every column, dropped ones included: C++ v3 takes it
+    * from iamax before the drop decision (hessenberg.cu:783-795) */
+   int *prow = (int *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int));
    for (int j = 0; j < n; ++j)
       for (int i = 0; i < n; ++i) R[i + (int64_t)j * ldr] = 0.0;
    for (int i = 0; i < n; ++i) {
       double *v = Q + (int64_t)i * ldq;
       for (int j = 0; j < i; ++j) {
+         /* variant 0 leaves dropped columns out (hessenberg.m:58, C++ v1
+          * hessenberg.cu:520); variant 1 eliminates with every earlier column,
+          * a dropped one with its unscaled residual (v3, hessenberg.cu:768-779) */
          if (!select[j] && variant == 0) continue;
-         if (!select[j]) continue; /* a dropped column has no pivot row */
-         double c = v[perm[j]];
+         double c = v[prow[j]];
          R[j + (int64_t)i * ldr] = c;
          const double *q = Q + (int64_t)j * ldq;
          for (int64_t r = 0; r < m; ++r) v[r] = fma(-c, q[r], v[r]);
@@ -293,6 +300,7 @@ void orc_hessenberg(int64_t m, int n, double *Q, int64_t ldq, double *R, int ldr
       double piv;
       int k1 = orc_iamax(m, v, &piv);
       R[i + (int64_t)i * ldr] = piv;
+      prow[i] = k1 - 1;
       if (fabs(piv) < select_tol) {
          select[i] = 0;
          perm[i] = -1;
@@ -302,6 +310,7 @@ void orc_hessenberg(int64_t m, int n, double *Q, int64_t ldq, double *R, int ldr
          for (int64_t r = 0; r < m; ++r) v[r] = v[r] / piv;
       }
    }
+   free(prow);
 }
 
 /* ------------------------------------------------------------------------- */

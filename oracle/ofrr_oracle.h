@@ -63,7 +63,8 @@ void orc_gemm_nn(int64_t m, int n, int k, const double *A, int64_t lda, const do
  * Q: m x n in (the matrix A) / out (the basis Q); R: n x n (zeroed here);
  * select[i] = 1 kept, 0 dropped; perm[i] = 0-based pivot row (-1 if dropped).
  * variant 0: skip dropped columns in later eliminations (MATLAB, C++ v1)
- * variant 1: C++ v3 order, dropped columns still eliminated (hessenberg.cu:771)
+ * variant 1: C++ v3, dropped columns still eliminated with their argmax row and
+ *            unscaled residual (hessenberg.cu:768-779)
  */
 void orc_hessenberg(int64_t m, int n, double *Q, int64_t ldq, double *R, int ldr,
                     double select_tol, int variant, int *select, int *perm);

@@ -72,10 +72,13 @@ def test_serve_same_alone_and_in_mix(api):
 def test_serve_admission_respects_budget(api):
     reqs = _requests(api)
     fps = [api.request_footprint(r)[2] for r in reqs]
-    budget = int(max(fps) * 1.2)  # no two of the largest fit together
-    out, st = _run(api, reqs, 4, budget=budget)
-    assert st["budget"] == budget
-    assert st["bytes_peak"] <= budget
+    slots = 4
+    reserve = slots * api.server_slot_bytes(0)
+    budget = reserve + int(max(fps) * 1.2)  # no two of the largest fit together
+    out, st = _run(api, reqs, slots, budget=budget)
+    assert st["budget"] == budget and st["slot_bytes"] == reserve
+    assert st["ws_regrowths"] == 0  # the slots' reservation covered every solve
+    assert st["bytes_peak"] <= budget - reserve
     assert all(o["status"] == 0 for o in out)
     # requests are admitted in arrival order
     admits = [o["t_admit"] for o in out]

@@ -150,6 +150,36 @@ def test_hessenberg_invariants_and_skip(m, n, kats):
     assert list(np.nonzero(skip)[0]) == [3, 4, 5]
 
 
+def _rtilde(R, sel):
+    Rt = R.copy()
+    for j in np.nonzero(sel == 0)[0]:
+        Rt[j, j] = 1.0  # a dropped column keeps its unscaled residual
+    return Rt
+
+
+def test_hessenberg_variants_with_dropped_columns(kats):
+    """Variant 0 (MATLAB / C++ v1) leaves dropped columns out of later
+    eliminations; variant 1 (C++ v3, hessenberg.cu:768-779) eliminates with them,
+    using their argmax row and unscaled residual.  Both satisfy A = Q R~ (R~ = R
+    with a unit diagonal at dropped columns); they agree bit for bit without
+    drops and differ, with nonzero R(j, i) for a dropped j, with them."""
+    rng = np.random.default_rng(31)
+    A = rng.uniform(-1, 1, (400, 30))
+    Q0, R0, s0, p0 = O.hessenberg(A, 2.0 ** -52, variant=0)
+    Q1, R1, s1, p1 = O.hessenberg(A, 2.0 ** -52, variant=1)
+    assert s0.all() and np.array_equal(Q0, Q1) and np.array_equal(R0, R1)
+    A2 = A.copy()
+    A2[:, 7] = A2[:, 2] + 1e-9 * rng.uniform(-1, 1, 400)   # nearly dependent: dropped at 1e-6
+    Q0, R0, s0, p0 = O.hessenberg(A2, 1e-6, variant=0)
+    Q1, R1, s1, p1 = O.hessenberg(A2, 1e-6, variant=1)
+    assert list(np.nonzero(s0 == 0)[0]) == [7] and list(np.nonzero(s1 == 0)[0]) == [7]
+    for Q, R, sl in ((Q0, R0, s0), (Q1, R1, s1)):
+        assert np.abs(A2 - Q @ _rtilde(R, sl)).max() < 1e-12
+    assert np.all(R0[7, 8:] == 0.0) and np.any(R1[7, 8:] != 0.0)
+    assert not np.array_equal(Q0[:, 8:], Q1[:, 8:])
+    np.testing.assert_array_equal(Q0[:, :8], Q1[:, :8])
+
+
 def test_syev_sygv_reference_kats(kats):
     n = 10
     A = np.diag(2 * np.ones(n)) - np.diag(np.ones(n - 1), 1) - np.diag(np.ones(n - 1), -1)

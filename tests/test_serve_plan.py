@@ -57,3 +57,31 @@ def test_invalid_requests_are_refused(bad):
 def test_bad_tolerance_refused():
     with pytest.raises(api.OfrrError):
         api.request_footprint(api.request(2, 100, 8, tol=0.0))
+
+
+def test_fixed_plan_fields():
+    # the configured large FP8 solve: 100 pairs, basis 300 = k 100 x (1 + 2 blocks)
+    k, nd, fp = api.request_footprint(api.request(3, 64, 100, k=100, nd=2,
+                                                  basis_dtype=torch.float8_e4m3fn,
+                                                  refine_dtype=torch.float64, refine_tol=1e-4))
+    assert (k, nd) == (100, 2)
+    n = 64 ** 3
+    # blocks are stored 128 wide (power-of-two storage of k = 100)
+    assert fp >= 3 * n * 128 * 8
+    assert api.request_footprint(api.request(3, 64, 16, nd=1))[:2] == (32, 1)
+
+
+@pytest.mark.parametrize("bad", [dict(k=8), dict(k=129), dict(nd=3), dict(refine_dtype=7)])
+def test_invalid_plan_fields(bad):
+    args = dict(dim=3, N=64, nwant=16)
+    args.update(bad)
+    rdt = args.pop("refine_dtype", None)
+    req = api.request(**args)
+    if rdt is not None:
+        req.refine, req.refine_dtype = 1, rdt
+    with pytest.raises(api.OfrrError):
+        api.request_footprint(req)
+
+
+def test_largest_basis_is_384():
+    assert api.request_footprint(api.request(3, 64, 100, k=128, nd=2))[:2] == (128, 2)
