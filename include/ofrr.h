@@ -131,6 +131,29 @@ OFRR_API int ofrr_hessenberg(ofrr_ctx *ctx, int q_dtype, int64_t m, int n, void 
                              int64_t rs, int64_t cs, double select_tol, double *R_dev, int ldr,
                              int32_t *perm_dev, int32_t *skip_dev);
 
+/* The C++ v3 variant (hessenberg_v3, CPP/linalg/factorization/hessenberg.cu:
+ * 617-801): as ofrr_hessenberg, except that a dropped column still takes part
+ * in the later eliminations, with its argmax row and its unscaled residual
+ * (the v3 step eliminates against every earlier column, hessenberg.cu:768-779).
+ * Without dropped columns the two give the same result bit for bit. */
+OFRR_API int ofrr_hessenberg_v3(ofrr_ctx *ctx, int q_dtype, int64_t m, int n, void *Q_dev,
+                                int64_t rs, int64_t cs, double select_tol, double *R_dev, int ldr,
+                                int32_t *perm_dev, int32_t *skip_dev);
+/* One column of the process (hessenberg_v3_step, hessenberg.hpp:93-104):
+ * column i of Q against its finished columns 0..i-1 (variant 1: every one, as
+ * v3; variant 0: the kept ones, as hessenberg.m), then its pivot: R(0..i, i)
+ * (FP64, device), prow1_dev[i] = the 1-based argmax row (written for dropped
+ * columns too; prow1_dev[0..i) must hold those of the finished columns),
+ * skip_dev[i] and *skipped_host (nullable; when given, the call synchronizes).
+ * Left-looking: the coefficients follow by forward substitution over the pivot
+ * rows, then one pass over the rows -- the same rounded operations as the full
+ * factorization, so columns stepped one at a time equal ofrr_hessenberg(_v3)
+ * bit for bit. */
+OFRR_API int ofrr_hessenberg_step(ofrr_ctx *ctx, int variant, int q_dtype, int64_t m, int n,
+                                  void *Q_dev, int64_t rs, int64_t cs, int i, double select_tol,
+                                  double *R_dev, int ldr, int32_t *prow1_dev, int32_t *skip_dev,
+                                  int32_t *skipped_host);
+
 /* ---- Gram-Schmidt QR, the orthogonalizing baselines (CPP/linalg/
  * factorization/qr.hpp:9-141, MATLAB/+src/+qr/mgs.m, cgs.m).  In place on Q
  * (m x n, m >= n, dtype q_dtype: FP64, FP32, FP16, BF16); dot products and
@@ -311,6 +334,24 @@ OFRR_API int ofrr_filtered_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int6
 OFRR_API int ofrr_dense_gemm(ofrr_ctx *ctx, int64_t R, int64_t K, int nc, const void *P_dev,
                              int p_dtype, int64_t ldp, const void *Q_dev, int q_dtype, int64_t ldq,
                              double alpha, double beta, void *C_dev, int c_dtype, int64_t ldc);
+
+/* C = alpha op(A) op(B) + beta C, column-major BLAS form (gemm<T_I, T_O,
+ * T_COMPUTE> of CPP/linalg/blas/mvops.hpp:79-82, mvops.cu:423-605, any sizes and
+ * transposes): op(A) m x k, op(B) k x n, C m x n with leading dimensions lda,
+ * ldb, ldc.  A and B share a format (FP64, FP32, FP16, BF16); C any of these.
+ * compute_dtype OFRR_F64 accumulates in FP64, OFRR_F32 / OFRR_F16 / OFRR_BF16 in
+ * FP32 (an FP16 compute rounds once at the end).  Fixed summation order.
+ * beta = 0 does not read C. */
+OFRR_API int ofrr_gemm(ofrr_ctx *ctx, int transA, int transB, int64_t m, int64_t n, int64_t k,
+                       double alpha, const void *A_dev, int a_dtype, int64_t lda, const void *B_dev,
+                       int b_dtype, int64_t ldb, double beta, void *C_dev, int c_dtype, int64_t ldc,
+                       int compute_dtype);
+/* X = alpha X + shift element-wise over a rows x cols block (element (i, j) at
+ * X[i * rs + j * cs]; FP64, FP32, FP16, BF16): Vector / Matrix::scale
+ * (CPP/containers/vector.hpp:140, matrix.hpp:145) with shift = 0, and the map of
+ * U[0,1) draws to [lo, hi) for fill_random. */
+OFRR_API int ofrr_axpb(ofrr_ctx *ctx, int dtype, int64_t rows, int64_t cols, void *X_dev, int64_t rs,
+                       int64_t cs, double alpha, double shift);
 
 /* Y = X converted element-wise to another format / layout: element (i, j) is
  * X[i * xrs + j * xcs] and Y[i * yrs + j * ycs] (Matrix::cast, matrix.cu:178-187,

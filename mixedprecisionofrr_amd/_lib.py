@@ -27,6 +27,7 @@ EXPORTS = [
     "ofrr_csr_file_info", "ofrr_csr_file_read", "ofrr_singular_values_file", "ofrr_io_last_error",
     "ofrr_svd_rr", "ofrr_subspace_iter_svd", "ofrr_fp8_check", "ofrr_gs_qr", "ofrr_gs_qr_step",
     "ofrr_spmm_csr_rect", "ofrr_csr_transpose", "ofrr_server_slot_bytes",
+    "ofrr_hessenberg_v3", "ofrr_hessenberg_step", "ofrr_gemm", "ofrr_axpb",
 ]
 
 SERVE_MAX_WANT = 128
@@ -106,6 +107,12 @@ def _decl(L):
                                      i64, dbl, dbl, vp, i32, i64, i64]
     L.ofrr_csr_transpose.argtypes = [vp, i64, i64, i64, vp, vp, vp, i32, vp, vp, vp]
     L.ofrr_server_slot_bytes.argtypes = [i32, C.POINTER(i64)]
+    L.ofrr_gemm.argtypes = [vp, i32, i32, i64, i64, i64, dbl, vp, i32, i64, vp, i32, i64, dbl, vp,
+                            i32, i64, i32]
+    L.ofrr_axpb.argtypes = [vp, i32, i64, i64, vp, i64, i64, dbl, dbl]
+    L.ofrr_hessenberg_v3.argtypes = [vp, i32, i64, i32, vp, i64, i64, dbl, vp, i32, vp, vp]
+    L.ofrr_hessenberg_step.argtypes = [vp, i32, i32, i64, i32, vp, i64, i64, i32, dbl, vp, i32,
+                                       vp, vp, C.POINTER(i32)]
     L.ofrr_iamax.argtypes = [vp, i32, i64, vp, i64, vp, vp]
     L.ofrr_matrix_iamax.argtypes = [vp, i32, i64, i32, vp, i64, i64, vp]
     L.ofrr_scalecols.argtypes = [vp, i32, i64, i32, vp, i64, i64, vp]

@@ -277,19 +277,41 @@ def check_special(ctx, A):
     return ctx.lib.ofrr_check_special(ctx.h, dtype_code(A), m, n, _p(A), A.stride(0), A.stride(1))
 
 
-def hessenberg(ctx, A, select_tol=None):
-    """[Q, R, perm, skip] of the Hessenberg process (hessenberg.m:55-74) on a copy of A."""
+def hessenberg(ctx, A, select_tol=None, variant=0):
+    """[Q, R, perm, skip] of the Hessenberg process on a copy of A: variant 0
+    (hessenberg.m:55-74, C++ hessenberg / hessenberg_v2) leaves dropped columns
+    out of later eliminations, variant 1 (C++ hessenberg_v3) eliminates with
+    them."""
     Q = A.clone() if A.stride(0) == 1 else A.t().contiguous().t().clone()
     m, n = Q.shape
     if select_tol is None:
-        select_tol = {torch.float64: 2.0 ** -52, torch.float32: 2.0 ** -23,
-                      torch.float16: 2.0 ** -10, torch.bfloat16: 2.0 ** -7}[Q.dtype]
+        select_tol = _EPS[Q.dtype]
     R = colmajor(n, n, torch.float64, Q.device)
     perm = torch.zeros(n, dtype=torch.int32, device=Q.device)
     skip = torch.zeros(n, dtype=torch.int32, device=Q.device)
-    ctx.check(ctx.lib.ofrr_hessenberg(ctx.h, dtype_code(Q), m, n, _p(Q), Q.stride(0), Q.stride(1),
-                                      select_tol, _p(R), n, _p(perm), _p(skip)), "hessenberg")
+    fn = ctx.lib.ofrr_hessenberg_v3 if variant == 1 else ctx.lib.ofrr_hessenberg
+    ctx.check(fn(ctx.h, dtype_code(Q), m, n, _p(Q), Q.stride(0), Q.stride(1), select_tol, _p(R), n,
+                 _p(perm), _p(skip)), "hessenberg")
     return Q, R, perm, skip
+
+
+def hessenberg_by_steps(ctx, A, select_tol=None, variant=1):
+    """The same factorization built one column at a time with
+    ofrr_hessenberg_step (hessenberg_v3_step): [Q, R, prow1 (1-based argmax
+    rows, dropped columns included), skip]."""
+    Q = A.clone() if A.stride(0) == 1 else A.t().contiguous().t().clone()
+    m, n = Q.shape
+    if select_tol is None:
+        select_tol = _EPS[Q.dtype]
+    R = torch.zeros(n, n, dtype=torch.float64, device=Q.device).t()
+    prow = torch.zeros(n, dtype=torch.int32, device=Q.device)
+    skip = torch.zeros(n, dtype=torch.int32, device=Q.device)
+    dropped = C.c_int32()
+    for i in range(n):
+        ctx.check(ctx.lib.ofrr_hessenberg_step(ctx.h, variant, dtype_code(Q), m, n, _p(Q), Q.stride(0),
+                                               Q.stride(1), i, select_tol, _p(R), n, _p(prow),
+                                               _p(skip), C.byref(dropped)), "hessenberg_step")
+    return Q, R, prow, skip
 
 
 QR_VARIANTS = {"mgs": 0, "mgs_v2": 1, "cgs": 2, "cgs2": 3}

@@ -57,12 +57,13 @@ SELFTEST = os.path.join(HERE, "msvd_selftest")
 
 def build_selftest(verbose=True):
     """The C++ adapter's self-test (tests/cpp/msvd_selftest.cpp over
-    include/ofrr/msvd.hpp), host code linked against libofrr.so."""
+    include/ofrr/msvd.hpp), host code linked against libofrr.so.  Built with
+    hipcc (clang), which gives the adapter the reference's __half element type."""
     src = os.path.join(ROOT, "tests", "cpp", "msvd_selftest.cpp")
     tmp = SELFTEST + ".tmp"
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-           "-I", os.path.join(ROOT, "include"), src, "-L", HERE, "-lofrr", "-L/opt/rocm/lib",
-           "-lamdhip64", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp]
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include"),
+           src, "-L", HERE,
+           "-lofrr", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"self-test build failed:\n{r.stdout}\n{r.stderr}")

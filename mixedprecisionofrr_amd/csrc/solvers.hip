@@ -668,6 +668,8 @@ struct FilterOp {
    virtual void set_precise(bool) {}
    // whether a sweep may filter a trailing column range only (soft locking)
    virtual bool can_lock() const { return false; }
+   // the narrowest block storage the operator's products take
+   virtual int min_width() const { return 1; }
 };
 
 struct CsrOp final : FilterOp {
@@ -756,6 +758,7 @@ static int64_t round8(int64_t x) { return (x + 7) / 8 * 8; }
 // a 16-bit copy of A on MFMA with FP32 accumulation (or A itself for an FP64
 // basis); the residual uses A in FP64.
 struct DenseSymOp final : FilterOp {
+   int min_width() const override { return 16; }
    const double *A = nullptr;
    int64_t lda = 0;
    const void *Alo = nullptr;  // row-major n x n, row stride ldlo, type lo_dt (16-bit)
@@ -803,6 +806,7 @@ struct DenseSymOp final : FilterOp {
 // the growth steps are BF16 MFMA products with FP32 accumulation; the residual
 // and the final vectors use the FP32 A with FP64 accumulation.
 struct AtaOp final : FilterOp {
+   int min_width() const override { return 16; }
    int64_t M = 0;
    const float *Acm = nullptr;  // column-major, lda: as P of A^T Y
    int64_t lda = 0;
@@ -868,10 +872,18 @@ struct AtaOp final : FilterOp {
 // columns are dropped before the eigensolves, where their zero diagonal would
 // break the scaling) and its coefficients are scattered back with zeros for the
 // padding, so the padding never mixes into the real columns.
-static int filter_width(int k) {
+static int filter_width(int k, int minw = 1) {
    int kp = 1;
-   while (kp < k) kp *= 2;
+   while (kp < k || kp < minw) kp *= 2;
    return kp;
+}
+// the narrowest storage the kernels take: the step kernels read 16 bytes per
+// lane (8 FP16 / 16 FP8 columns), the residual kernel two FP64 columns, and
+// the dense operators' MFMA products 16 columns (op_minw)
+static int filter_min_width(const ofrr_filter_opts &o, int op_minw) {
+   const int rdt = o.refine ? o.refine_dtype : o.basis_dtype;
+   const size_t es = std::min(dtype_size(o.basis_dtype), dtype_size(rdt));
+   return std::max(std::max(2, (int)(16 / std::max<size_t>(1, es))), op_minw);
 }
 
 // The filtered solver's device state.  One layout serves the sizing
@@ -896,8 +908,8 @@ struct Carve {
       return (base && off <= cap) ? base + a : nullptr;
    }
 };
-static void filtered_layout(Carve &cv, int64_t n, const ofrr_filter_opts &o, FiltBufs &b) {
-   const int k = o.k, kp = filter_width(k), nd = o.nd, d = o.degree;
+static void filtered_layout(Carve &cv, int64_t n, const ofrr_filter_opts &o, int op_minw, FiltBufs &b) {
+   const int k = o.k, kp = filter_width(k, filter_min_width(o, op_minw)), nd = o.nd, d = o.degree;
    const int m = k * (1 + nd), mp = kp * (1 + nd);
    const int bdt = o.basis_dtype, rdt = o.refine ? o.refine_dtype : bdt;
    // blocks sized for the larger of the growth format and the refinement format
@@ -933,10 +945,10 @@ static void filtered_layout(Carve &cv, int64_t n, const ofrr_filter_opts &o, Fil
    b.cmaxR = (float *)cv.take(kq * 4);
    b.rank = (int32_t *)cv.take(64);
 }
-static size_t filtered_state_bytes(int64_t n, const ofrr_filter_opts &o) {
+static size_t filtered_state_bytes(int64_t n, const ofrr_filter_opts &o, int op_minw = 1) {
    Carve cv{nullptr, 0, 0};
    FiltBufs b;
-   filtered_layout(cv, n, o, b);
+   filtered_layout(cv, n, o, op_minw, b);
    return al(cv.off);
 }
 
@@ -990,28 +1002,29 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    const int kr = o->k, nd = o->nd, dmax = o->degree;
    int bdt = o->basis_dtype;  // the growth format of the current iteration
    const int d = dmax;        // buffer sizing: the largest degree of any sweep
+   const int minw = filter_min_width(*o, op.min_width());
    if (kr < 2 || kr > 128 || nd < 1 || nd > 2 || d < 2 || o->nwant < 1 || o->nwant > kr ||
-       !dtype_valid(bdt) || (int64_t)filter_width(kr) > n)
+       !dtype_valid(bdt) || (int64_t)filter_width(kr, minw) > n)
       return set_error(ctx, kErrorInvalidArgument,
                        "filtered_eig: need k in [2,128], nd in {1,2}, degree >= 2, 1 <= nwant <= k");
    if (o->refine && (!dtype_valid(o->refine_dtype) || !op.can_refine()))
       return set_error(ctx, kErrorInvalidArgument,
                        "filtered_eig: the refinement format is valid for a CSR operator only");
-   const int k = filter_width(kr);
+   const int k = filter_width(kr, minw);
    const bool padded = k != kr;
    const int m = k * (1 + nd);
    if (kr * (1 + nd) > 384) return set_error(ctx, kErrorInvalidArgument, "filtered_eig: basis dimension > 384");
    int zdt = (bdt == OFRR_F64) ? OFRR_F64 : OFRR_F32;
    size_t zsz = dtype_size(zdt);
    // ---- buffers
-   const size_t state_bytes = filtered_state_bytes(n, *o);
+   const size_t state_bytes = filtered_state_bytes(n, *o, op.min_width());
    int st = arena_reserve(ctx, state_bytes);
    if (st) return st;
    FiltBufs fb;
    {
       Carve cv{(char *)arena_take(state_bytes), state_bytes, 0};
       if (!cv.base) return set_error(ctx, kErrorInternalError, "filtered_eig: state reservation");
-      filtered_layout(cv, n, *o, fb);
+      filtered_layout(cv, n, *o, op.min_width(), fb);
       if (!fb.rank) return set_error(ctx, kErrorInternalError, "filtered_eig: state layout exceeds its reservation");
    }
    double *X = fb.X, *Xn = fb.Xn, *W = fb.W;
@@ -1484,8 +1497,9 @@ int ofrr_filtered_eig_dense(ofrr_ctx *ctx, int64_t n, const double *A, int64_t l
    if (!o || n <= 0 || !A || lda < n || !Xio || !theta_host || !res_host)
       return set_error(ctx, kErrorInvalidArgument, "filtered_eig_dense: bad arguments");
    const int k = o->k, bdt = o->basis_dtype;
-   if (k < 16 || k > 128 || (k & (k - 1)))
-      return set_error(ctx, kErrorInvalidArgument, "filtered_eig_dense: k must be a power of two in [16, 128]");
+   if (k < 2 || k > 128)
+      return set_error(ctx, kErrorInvalidArgument, "filtered_eig_dense: k must lie in [2, 128]");
+   const int kp = filter_width(k, 16);  // the operator's products take the stored width
    if (bdt != OFRR_F64 && bdt != OFRR_F16 && bdt != OFRR_BF16)
       return set_error(ctx, kErrorBLASUnsupportedType, "filtered_eig_dense: basis must be FP64, FP16 or BF16");
    DenseSymOp op;
@@ -1503,7 +1517,7 @@ int ofrr_filtered_eig_dense(ofrr_ctx *ctx, int64_t n, const double *A, int64_t l
       op.Alo = alo.p;
    }
    DevBuf d64;
-   if (!tb.alloc((size_t)n * k * 8) || !d64.alloc((size_t)n * k * 8))
+   if (!tb.alloc((size_t)n * kp * 8) || !d64.alloc((size_t)n * kp * 8))
       return set_error(ctx, kErrorCUDAMemory, "filtered_eig_dense: product");
    op.T = tb.p;
    op.D64 = (double *)d64.p;
@@ -1519,8 +1533,9 @@ int ofrr_svd_filtered(ofrr_ctx *ctx, int64_t M, int64_t N, const float *A, int64
    if (!o || M <= 0 || N <= 0 || !A || lda < M || !Vio || !sigma_host || !res_host)
       return set_error(ctx, kErrorInvalidArgument, "svd_filtered: bad arguments");
    const int k = o->k, bdt = o->basis_dtype;
-   if (k < 16 || k > 128 || (k & (k - 1)) || k > N)
-      return set_error(ctx, kErrorInvalidArgument, "svd_filtered: k must be a power of two in [16, 128], <= N");
+   const int kp = filter_width(k, 16);  // the operator's products take the stored width
+   if (k < 2 || k > 128 || kp > N)
+      return set_error(ctx, kErrorInvalidArgument, "svd_filtered: k must lie in [2, 128], its storage width <= N");
    if (bdt != OFRR_F64 && bdt != OFRR_BF16)
       return set_error(ctx, kErrorBLASUnsupportedType, "svd_filtered: basis must be BF16 or FP64");
    AtaOp op;
@@ -1530,8 +1545,8 @@ int ofrr_svd_filtered(ofrr_ctx *ctx, int64_t M, int64_t N, const float *A, int64
    op.lda = lda;
    DevBuf arm, brm, bcm, y64, ylo, tb, d64;
    int st;
-   if (!arm.alloc((size_t)M * N * 4) || !y64.alloc((size_t)M * k * 8) || !tb.alloc((size_t)N * k * 8) ||
-       !d64.alloc((size_t)N * k * 8))
+   if (!arm.alloc((size_t)M * N * 4) || !y64.alloc((size_t)M * kp * 8) || !tb.alloc((size_t)N * kp * 8) ||
+       !d64.alloc((size_t)N * kp * 8))
       return set_error(ctx, kErrorCUDAMemory, "svd_filtered: operator copies");
    op.D64 = (double *)d64.p;
    if ((st = convert(ctx, M, N, A, OFRR_F32, 1, lda, arm.p, OFRR_F32, N, 1))) return st;
@@ -1542,7 +1557,7 @@ int ofrr_svd_filtered(ofrr_ctx *ctx, int64_t M, int64_t N, const float *A, int64
       op.ldr = round8(N);
       op.ldc = round8(M);
       if (!brm.alloc((size_t)M * op.ldr * 2) || !bcm.alloc((size_t)N * op.ldc * 2) ||
-          !ylo.alloc((size_t)M * k * 2))
+          !ylo.alloc((size_t)M * kp * 2))
          return set_error(ctx, kErrorCUDAMemory, "svd_filtered: BF16 copies");
       if ((st = convert(ctx, M, N, A, OFRR_F32, 1, lda, brm.p, OFRR_BF16, op.ldr, 1))) return st;
       if ((st = convert(ctx, M, N, A, OFRR_F32, 1, lda, bcm.p, OFRR_BF16, 1, op.ldc))) return st;
@@ -1559,7 +1574,18 @@ int ofrr_svd_filtered(ofrr_ctx *ctx, int64_t M, int64_t N, const float *A, int64
    for (int j = 0; j < k; ++j) sigma_host[j] = std::sqrt(std::max(0.0, th[j]));
    if (U) {
       // U = A V diag(1 / sigma), FP64 accumulation
-      int st2 = dense_gemm(ctx, M, N, k, op.Arm, OFRR_F32, N, Vio, OFRR_F64, k, 1.0, 0.0, U, OFRR_F64, k);
+      int st2;
+      if (kp == k) {
+         st2 = dense_gemm(ctx, M, N, k, op.Arm, OFRR_F32, N, Vio, OFRR_F64, k, 1.0, 0.0, U, OFRR_F64, k);
+      } else {
+         // the product takes the stored width: V zero-padded to kp columns, A V
+         // into the M x kp scratch, its first k columns to U
+         st2 = zero_bytes(ctx, op.D64, (size_t)N * kp * 8);
+         if (!st2) st2 = convert(ctx, N, k, Vio, OFRR_F64, k, 1, op.D64, OFRR_F64, kp, 1);
+         if (!st2)
+            st2 = dense_gemm(ctx, M, N, kp, op.Arm, OFRR_F32, N, op.D64, OFRR_F64, kp, 1.0, 0.0, op.Y64, OFRR_F64, kp);
+         if (!st2) st2 = convert(ctx, M, k, op.Y64, OFRR_F64, kp, 1, U, OFRR_F64, k, 1);
+      }
       if (st2) return st2;
       std::vector<double> inv(k);
       for (int j = 0; j < k; ++j) inv[j] = sigma_host[j] > 0.0 ? 1.0 / sigma_host[j] : 0.0;

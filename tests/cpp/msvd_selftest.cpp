@@ -5,8 +5,10 @@
 // hessenberg_test.cpp tolerance and skip pattern, and closed-form Laplacian
 // spectra for the eigensolver drivers.  Prints one line per check; exit code 0
 // when every check passes.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "ofrr/msvd.hpp"
@@ -330,6 +332,231 @@ int main() {
       for (int i = 0; i < 16; ++i) err2 = std::fmax(err2, std::fabs(S2(i, i) - S(i, i)) / S(i, i));
       check(st == MSVDStatus::kSuccess && err2 < 1e-10, "rr_svd on the converged basis", err2);
    }
+   // ---- gemm: transposes and the reference's exceptions (mvops_test.cpp:622-769)
+   {
+      Matrix<double> A(2, 3, Location::kHOST), B(4, 2, Location::kHOST), C(3, 4, Location::kHOST);
+      const double at[2][3] = {{1, 2, 5}, {3, 4, 6}};
+      const double bt[4][2] = {{1, 5}, {2, 6}, {3, 7}, {4, 8}};
+      const double c[3][4] = {{16, 20, 24, 28}, {22, 28, 34, 40}, {35, 46, 57, 68}};
+      for (int i = 0; i < 2; ++i) for (int j = 0; j < 3; ++j) A(i, j) = at[i][j];
+      for (int i = 0; i < 4; ++i) for (int j = 0; j < 2; ++j) B(i, j) = bt[i][j];
+      C.fill(0.0);
+      MSVDStatus st = gemm<double, double, double>(true, true, 1.0, A, B, 0.0, C);
+      bool same = st == MSVDStatus::kSuccess;
+      for (int i = 0; i < 3; ++i) for (int j = 0; j < 4; ++j) same = same && C(i, j) == c[i][j];
+      check(same, "gemm A^T B^T KAT (host matrices, exact)");
+      // beta on the device: C = 2 A^T B^T - C
+      Matrix<double> Ad = A, Bd = B, Cd = C;
+      Ad.to_device(); Bd.to_device(); Cd.to_device();
+      st = gemm<double, double, double>(true, true, 2.0, Ad, Bd, -1.0, Cd);
+      Cd.to_host();
+      same = st == MSVDStatus::kSuccess;
+      for (int i = 0; i < 3; ++i) for (int j = 0; j < 4; ++j) same = same && Cd(i, j) == c[i][j];
+      check(same, "gemm beta on the device (no host round trip)");
+      auto msg = [](auto &&f) {
+         try { f(); } catch (const std::runtime_error &e) { return std::string(e.what()); }
+         return std::string("no exception");
+      };
+      Matrix<float> Af(3, 2, Location::kHOST), Bf(2, 4, Location::kDEVICE), Cf(3, 4, Location::kHOST);
+      check(msg([&] { gemm<float, float, float>(false, false, 1.0f, Af, Bf, 0.0f, Cf); }) ==
+                "All matrices must be in the same location (CPU or GPU)", "gemm location message");
+      Matrix<float> A2(3, 3, Location::kHOST), B2(2, 4, Location::kHOST), C2(3, 4, Location::kHOST);
+      check(msg([&] { gemm<float, float, float>(false, false, 1.0f, A2, B2, 0.0f, C2); }) ==
+                "Matrix dimensions do not match for multiplication", "gemm dimension message");
+      Matrix<float> A3(3, 2, Location::kHOST), B3(2, 4, Location::kHOST), C3(3, 3, Location::kHOST);
+      check(msg([&] { gemm<float, float, float>(false, false, 1.0f, A3, B3, 0.0f, C3); }) ==
+                "Output matrix C has wrong dimensions", "gemm output message");
+      Matrix<double> Ah(3, 2, Location::kHOST), Bh(2, 4, Location::kHOST);
+      Matrix<float> Ch(3, 4, Location::kHOST);
+      check(msg([&] { gemm<double, float, double>(false, false, 1.0, Ah, Bh, 0.0, Ch); }) ==
+                "For CPU GEMM, input, output, and compute types must be the same", "gemm CPU mixed-type message");
+      // a general-size product against a host FP64 loop (both transposes)
+      const int m = 300, n = 170, kk = 211;
+      Matrix<double> P(kk, m), Qm(n, kk), G(m, n);
+      P.fill_random(21); Qm.fill_random(22);
+      st = gemm<double, double, double>(true, true, 1.0, P, Qm, 0.0, G);
+      auto hp = host_of(P), hq = host_of(Qm), hg = host_of(G);
+      double err = 0;
+      for (int i = 0; i < m; i += 7)
+         for (int j = 0; j < n; j += 5) {
+            double s2 = 0;
+            for (int p = 0; p < kk; ++p) s2 += hp[p + (size_t)i * kk] * hq[j + (size_t)p * n];
+            err = std::fmax(err, std::fabs(s2 - hg[i + (size_t)j * m]));
+         }
+      check(st == MSVDStatus::kSuccess && err < 1e-12, "gemm 300x170x211 A^T B^T", err);
+   }
+   // ---- containers: scale, cast, submatrix, fill_random range
+   {
+      Matrix<double> A(50, 40);
+      A.fill_random(3);
+      auto h0 = host_of(A);
+      double lo = 1, hi = -1;
+      for (double v : h0) { lo = std::fmin(lo, v); hi = std::fmax(hi, v); }
+      check(lo >= -1.0 && hi < 1.0 && lo < -0.9 && hi > 0.9, "fill_random uniform in [-1, 1)", lo);
+      Matrix<double> S = A.submatrix(10, 5, 20, 30);
+      S.scale(-2.0);
+      auto h1 = host_of(A);
+      bool ok = S.ld() == 50 && S.rows() == 20 && S.cols() == 30 && S.location() == Location::kDEVICE;
+      for (int j = 0; j < 40; ++j)
+         for (int i = 0; i < 50; ++i) {
+            const bool in = i >= 10 && i < 30 && j >= 5 && j < 35;
+            ok = ok && h1[i + 50 * j] == (in ? -2.0 * h0[i + 50 * j] : h0[i + 50 * j]);
+         }
+      check(ok, "submatrix view + scale on the device");
+      Matrix<float> F = A.cast<float>();
+      auto hf = host_of(F);
+      ok = F.rows() == 50 && F.cols() == 40 && F.location() == Location::kDEVICE;
+      for (size_t t = 0; t < hf.size(); ++t) ok = ok && hf[t] == (float)h1[t];
+      check(ok, "Matrix::cast<float> on the device");
+      Vector<double> v(100, Location::kHOST);
+      for (int i = 0; i < 100; ++i) v[i] = i;
+      v.scale(0.5);
+      Vector<float> vf = v.cast<float>();
+      check(v[7] == 3.5 && vf[9] == 4.5f && vf.location() == Location::kHOST, "Vector::scale / cast on the host");
+      bool threw = false;
+      try { A.submatrix(45, 0, 10, 1); } catch (const std::runtime_error &) { threw = true; }
+      check(threw, "submatrix out of bounds throws");
+   }
+   // ---- hessenberg_v2 / v3 / v3_step (hessenberg_test.cpp: 2000 x 200, 1e-10)
+   {
+      const int m = 2000, n = 200;
+      Matrix<double> A(m, n), Q2(m, n), Q3(m, n), Qs(m, n), R2(n, n, Location::kHOST), R3(n, n, Location::kHOST);
+      A.fill_random(17);
+      {
+         auto h = host_of(A);
+         for (int j = 3; j <= 5; ++j)
+            for (int i = 0; i < m; ++i) h[i + (size_t)j * m] = h[i + (size_t)(j - 3) * m];
+         (void)hipMemcpy(A.data(), h.data(), h.size() * 8, hipMemcpyHostToDevice);
+      }
+      std::vector<int> s2 = hessenberg_v2<double, double, double>(A, Q2, R2, 1e-12);
+      std::vector<int> s3 = hessenberg_v3<double, double, double>(A, Q3, R3, 1e-12);
+      bool pattern = s2[3] && s2[4] && s2[5] && !s2[2] && !s2[6] && s3 == s2;
+      // v3 reproduced column by column with the step entry point
+      (void)hipMemcpy(Qs.data(), A.data(), (size_t)m * n * 8, hipMemcpyDeviceToDevice);
+      Matrix<double> Rdummy(n, n, Location::kHOST), sf(n, n);
+      Vector<int> perm(n), skipd(n);
+      int skiph = 0;
+      std::vector<int> ss(n);
+      for (int i = 0; i < n; ++i)
+         ss[i] = hessenberg_v3_step<double, double>(Qs, Rdummy, &skiph, skipd.data(), i, perm.data(), sf,
+                                                    (double *)nullptr, 1e-12);
+      auto hq3 = host_of(Q3), hqs = host_of(Qs), hsf = host_of(sf);
+      bool same = ss == s3;
+      for (size_t t = 0; t < hq3.size(); ++t) same = same && hq3[t] == hqs[t];
+      for (int j = 0; j < n; ++j)
+         for (int i = 0; i <= j; ++i) same = same && R3(i, j) == hsf[i + (size_t)j * n];
+      auto ha = host_of(A), hq2 = host_of(Q2);
+      double err = 0;
+      for (int j = 0; j < n; ++j)
+         for (int i = 0; i < m; i += 7) {
+            double s = 0;
+            for (int p = 0; p <= j; ++p) if (!s2[p]) s += hq2[i + (size_t)p * m] * R2(p, j);
+            if (!s2[j]) err = std::fmax(err, std::fabs(s - ha[i + (size_t)j * m]));
+         }
+      check(err < 1e-10 && pattern, "hessenberg_v2 A = QR and skip pattern, v3 same drops", err);
+      check(same, "hessenberg_v3_step sequence = hessenberg_v3 (bitwise)");
+   }
+#ifdef OFRR_MSVD_HALF
+   // ---- half precision (hessenberg_test.cpp:161-224, mvops_test.cpp:455-721)
+   {
+      Matrix<__half> A(3, 2, Location::kHOST), B(2, 4, Location::kHOST), C(3, 4, Location::kDEVICE);
+      const float a[3][2] = {{1, 3}, {2, 4}, {5, 6}};
+      const float b[2][4] = {{1, 2, 3, 4}, {5, 6, 7, 8}};
+      const float c[3][4] = {{16, 20, 24, 28}, {22, 28, 34, 40}, {35, 46, 57, 68}};
+      for (int i = 0; i < 3; ++i) for (int j = 0; j < 2; ++j) A(i, j) = __float2half(a[i][j]);
+      for (int i = 0; i < 2; ++i) for (int j = 0; j < 4; ++j) B(i, j) = __float2half(b[i][j]);
+      A.to_device(); B.to_device();
+      C.fill(__float2half(0.0f));
+      MSVDStatus st = gemm<__half, __half, float>(false, false, 1.0f, A, B, 0.0f, C);
+      C.to_host();
+      float e1 = 0;
+      for (int i = 0; i < 3; ++i) for (int j = 0; j < 4; ++j) e1 = std::fmax(e1, std::fabs(__half2float(C(i, j)) - c[i][j]));
+      check(st == MSVDStatus::kSuccess && e1 <= 0.1f, "gemm<__half,__half,float> KAT", e1);
+      Matrix<__half> C2(3, 4, Location::kDEVICE);
+      st = gemm<__half, __half, __half>(false, false, __float2half(1.0f), A, B, __float2half(0.0f), C2);
+      C2.to_host();
+      float e2 = 0;
+      for (int i = 0; i < 3; ++i) for (int j = 0; j < 4; ++j) e2 = std::fmax(e2, std::fabs(__half2float(C2(i, j)) - c[i][j]));
+      check(st == MSVDStatus::kSuccess && e2 <= 0.2f, "gemm<__half,__half,__half> KAT", e2);
+   }
+   {
+      const size_t m = 2000, n = 100;
+      Matrix<__half> A(m, n), Q(m, n), R(n, n, Location::kHOST);
+      A.fill_random();
+      std::vector<int> skip = hessenberg<__half, __half, float>(A, Q, R);
+      bool clean = check_special_values(Q) == MSVDStatus::kSuccess;
+      Matrix<__half> QR(m, n);
+      R.to_device();
+      clean = clean && check_special_values(R) == MSVDStatus::kSuccess;
+      gemm<__half, __half, float>(false, false, 1.0f, Q, R, 0.0f, QR);
+      A.to_host();
+      QR.to_host();
+      float md = 0;
+      for (size_t j = 0; j < n; ++j)
+         for (size_t i = 0; i < m; ++i) md = std::fmax(md, std::fabs(__half2float(A(i, j)) - __half2float(QR(i, j))));
+      check(clean && md < 5e-2f, "hessenberg<__half,__half,float> 2000x100: A = QR within 5e-2", md);
+      A.fill_random();
+      A.to_device();
+      R.to_host();
+      (void)hipMemcpy(&A(0, 3), &A(0, 0), 3 * sizeof(__half) * m, hipMemcpyDeviceToDevice);
+      std::vector<int> skip2 = hessenberg<__half, __half, float>(A, Q, R, __float2half(1e-2f));
+      bool pattern = true;
+      for (size_t i = 0; i < n; ++i) pattern = pattern && skip2[i] == ((i == 3 || i == 4 || i == 5) ? 1 : 0);
+      check(pattern, "hessenberg<__half> duplicated columns 3..5 dropped at 1e-2");
+      std::vector<int> skip3 = hessenberg_v3<__half, __half, float>(A, Q, R, __float2half(1e-2f));
+      check(skip3 == skip2, "hessenberg_v3<__half> same drops");
+   }
+   {
+      // ofrr_eig / rr_eig / si_eig with half-precision operands on the 2D Laplacian 16 x 16
+      const int N = 16, n = N * N;
+      Matrix<double> A(n, n, Location::kHOST);
+      for (int y = 0; y < N; ++y)
+         for (int x = 0; x < N; ++x) {
+            const int i = x + N * y;
+            A(i, i) = 4.0;
+            if (x > 0) A(i, i - 1) = -1.0;
+            if (x + 1 < N) A(i, i + 1) = -1.0;
+            if (y > 0) A(i, i - N) = -1.0;
+            if (y + 1 < N) A(i, i + N) = -1.0;
+         }
+      std::vector<double> ex;
+      for (int p = 1; p <= N; ++p)
+         for (int q = 1; q <= N; ++q)
+            ex.push_back(4 * std::pow(std::sin(p * M_PI / (2.0 * (N + 1))), 2) + 4 * std::pow(std::sin(q * M_PI / (2.0 * (N + 1))), 2));
+      std::sort(ex.begin(), ex.end(), [](double a, double b) { return a > b; });
+      Matrix<__half> Ah = A.cast<__half>();  // the stencil is exact in FP16
+      Ah.to_device();
+      Matrix<__half> V(n, 24), D(24, 24, Location::kHOST);
+      V.fill_random(9);
+      MSVDStatus st = si_eig<__half, __half, float>(Ah, V, D, 200, 1e-8f, false, 1e-12f);
+      double err = 0;
+      for (int i = 0; i < 12; ++i) err = std::fmax(err, std::fabs((double)__half2float(D(i, i)) - ex[i]) / ex[i]);
+      check(st == MSVDStatus::kSuccess && err < 1e-3, "si_eig<__half,__half,float> (FP16 growth blocks) top 12", err);
+      Matrix<float> Vf(n, 24), Df(24, 24, Location::kHOST);
+      Vf.fill_random(9);
+      st = si_eig<__half, float, float>(Ah, Vf, Df, 200, 1e-8f, false, 1e-12f);
+      double errf = 0;
+      for (int i = 0; i < 12; ++i) errf = std::fmax(errf, std::fabs((double)Df(i, i) - ex[i]) / ex[i]);
+      check(st == MSVDStatus::kSuccess && errf < 1e-6, "si_eig<__half,float,float> top 12", errf);
+      // ofrr_eig on half-precision projected matrices of an exact invariant basis
+      Matrix<float> Dr(12, 12, Location::kHOST);
+      Matrix<float> Vr = Vf.submatrix(0, 0, n, 12).cast<float>();
+      st = rr_eig<__half, float, float>(Ah, Vr, Dr, false, 1e-10f);
+      double err2 = 0;
+      for (int i = 0; i < 6; ++i) err2 = std::fmax(err2, std::fabs((double)Dr(i, i) - ex[i]) / ex[i]);
+      check(st == MSVDStatus::kSuccess && err2 < 1e-5, "rr_eig<__half,float,float> on a converged basis", err2);
+      Matrix<__half> Gh(3, 3, Location::kHOST), Hh(3, 3, Location::kHOST);
+      for (int i = 0; i < 3; ++i)
+         for (int j = 0; j < 3; ++j) {
+            Gh(i, j) = __float2half(i == j ? 2.0f : 0.0f);
+            Hh(i, j) = __float2half(i == j ? (float)(2 * (i + 1)) : 0.0f);
+         }
+      Matrix<float> Ve(3, 3, Location::kHOST), De(3, 3, Location::kHOST);
+      st = ofrr_eig<__half, float, float>(Hh, Gh, Ve, De, 1e-10f);
+      check(st == MSVDStatus::kSuccess && De(0, 0) == 3.0f && De(1, 1) == 2.0f && De(2, 2) == 1.0f,
+            "ofrr_eig<__half,float,float> pencil (diag(2,4,6), 2 I)", De(0, 0));
+   }
+#endif
    // ---- special values
    {
       Matrix<double> A(8, 8, Location::kHOST);

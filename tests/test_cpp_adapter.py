@@ -21,6 +21,17 @@ def test_header_compiles_as_host_cpp():
     assert r.returncode == 0, r.stderr[-2000:]
 
 
+def test_header_compiles_with_half_types():
+    """Under clang (hipcc) the adapter has the reference's __half element type:
+    the self-test's half-precision sections compile too."""
+    src = os.path.join(ROOT, "tests", "cpp", "msvd_selftest.cpp")
+    r = subprocess.run(["/opt/rocm/bin/hipcc", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                        "-Wno-unused-command-line-argument", "--offload-arch=gfx950", "-I",
+                        os.path.join(ROOT, "include"), src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+
+
 @pytest.mark.gpu
 def test_msvd_selftest():
     assert os.path.exists(BIN), "build() makes the self-test binary"

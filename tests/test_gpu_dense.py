@@ -121,3 +121,30 @@ def test_svd_filtered_lowrank(ctx, bdt):
     AV = Ad @ V[:, :nwant]
     assert float((AV - U[:, :nwant] * torch.from_numpy(sig[:nwant]).to(dev)[None, :]).abs().max()) < 1e-10
     assert np.allclose(U[:, :nwant].norm(dim=0).cpu().numpy(), 1.0, atol=1e-8)
+
+
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32, torch.float16])
+def test_hessenberg_v3_and_steps(ctx, dt):
+    """C++ v3 (dropped columns still eliminated) against the oracle's variant 1,
+    and the column-by-column step entry point bit for bit equal to the full
+    factorizations of both variants."""
+    from mixedprecisionofrr_amd import api
+    rng = np.random.default_rng(44)
+    A = rng.uniform(-1, 1, (700, 40))
+    A[:, 9] = A[:, 3] + 1e-9 * rng.uniform(-1, 1, 700)   # dropped at select_tol 1e-6
+    tol = 1e-6 if dt != torch.float16 else 1e-2
+    Ad = torch.as_tensor(A, device="cuda:0").to(dt)
+    for variant in (0, 1):
+        print(f"hessenberg variant={variant} dtype={dt} full and by steps", flush=True)
+        Q, R, perm, skip = api.hessenberg(ctx, Ad, tol, variant=variant)
+        Qs, Rs, prow, skips = api.hessenberg_by_steps(ctx, Ad, tol, variant=variant)
+        assert torch.equal(Q, Qs), (Q.double() - Qs.double()).abs().max()
+        assert torch.equal(skip, skips)
+        assert np.array_equal(np.triu(R.cpu().numpy()), np.triu(Rs.cpu().numpy()))
+        kept = skip.cpu().numpy() == 0
+        assert np.array_equal(perm.cpu().numpy()[kept], prow.cpu().numpy()[kept] - 1)
+        if dt == torch.float64:
+            Qo, Ro, so, po = O.hessenberg(A, tol, variant=variant)
+            assert list(np.nonzero(so == 0)[0]) == [9]
+            assert np.array_equal(Q.cpu().numpy(), Qo)
+            assert np.array_equal(R.cpu().numpy(), Ro)
