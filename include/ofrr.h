@@ -245,7 +245,7 @@ OFRR_API int ofrr_residuals(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t
 /* ---- drivers ------------------------------------------------------------ */
 /* Arnoldi-Hessenberg eigensolver (arnoldi_eig of CPP/linalg/solvers/eig/
  * arnoldi_eig.hpp:25-28 over a CSR operator, semantics of MATLAB/+src/+krylov/
- * arnoldi_hess_eig.m:43-107) in FP64.  v0_dev: start vector (n).  theta_host
+ * arnoldi_hess_eig.m:43-107) in FP64; kdim + 1 <= 384.  v0_dev: start vector (n).  theta_host
  * (k, descending) and X_dev (n x k column-major, ldx) out.  Synchronous.
  * *built (host, nullable) receives the basis columns of the last sweep. */
 OFRR_API int ofrr_arnoldi_hess_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz,
@@ -253,6 +253,20 @@ OFRR_API int ofrr_arnoldi_hess_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz,
                                    const double *vals_dev, int k, int kdim, int maxiter,
                                    double select_tol, double gtol, const double *v0_dev,
                                    double *theta_host, double *X_dev, int64_t ldx, int *built);
+
+/* The same solver with a low-precision Krylov basis (arnoldi_hess_eig.m:9-31
+ * precision options): the basis stored in basis_dtype (precision_output: FP64,
+ * FP32, FP16 or BF16), its arithmetic and the sweep's operator products in FP32
+ * (FP64 for an FP64 basis; precision_compute / precision_A_matvec_*), the
+ * Rayleigh-Ritz in FP64 (the basis widened, V^T V and V^T A V with FP64
+ * accumulation: precision_eigenvalues double).  The pivot decisions stay on the
+ * device: a sweep of kdim columns synchronizes once, not once per column.
+ * theta_host and X_dev (FP64) as for ofrr_arnoldi_hess_eig; kdim + 1 <= 384. */
+OFRR_API int ofrr_arnoldi_hess_eig_mp(ofrr_ctx *ctx, int basis_dtype, int64_t n, int64_t nnz,
+                                      const int64_t *row_ptr_dev, const int32_t *col_idx_dev,
+                                      const double *vals_dev, int k, int kdim, int maxiter,
+                                      double select_tol, double gtol, const double *v0_dev,
+                                      double *theta_host, double *X_dev, int64_t ldx, int *built);
 
 /* Block subspace iteration (si_eig of CPP/linalg/solvers/eig/si_eig.hpp:24-27
  * over a CSR operator, semantics of MATLAB/+src/+subspace/subspace_iter_eig.m:

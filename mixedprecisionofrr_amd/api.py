@@ -456,18 +456,29 @@ def residuals(ctx, A, X, theta, normA):
     return np.array(res[:])
 
 
-def arnoldi_hess_eig(ctx, A, k, kdim, maxiter, v0, select_tol=2.0 ** -52, gtol=0.0):
-    """arnoldi_hess_eig.m in FP64 on the GPU: (theta desc, X col-major, built)."""
+def arnoldi_hess_eig(ctx, A, k, kdim, maxiter, v0, select_tol=2.0 ** -52, gtol=0.0,
+                     basis_dtype=torch.float64):
+    """arnoldi_hess_eig.m on the GPU: (theta desc, X col-major FP64, built).
+    basis_dtype is precision_output (the stored Krylov basis: FP64, FP32, FP16
+    or BF16; FP32 arithmetic for the 32- and 16-bit formats); the
+    Rayleigh-Ritz runs in FP64."""
     A.require_square("arnoldi_hess_eig")
     if v0.numel() != A.n:
         raise ValueError(f"arnoldi_hess_eig: the start vector needs {A.n} entries")
     th = (C.c_double * k)()
     X = colmajor(A.n, k, torch.float64, v0.device)
     built = C.c_int()
-    ctx.check(ctx.lib.ofrr_arnoldi_hess_eig(ctx.h, A.n, A.nnz, _p(A.row_ptr), _p(A.col_idx),
-                                            _p(A.vals), k, kdim, maxiter, select_tol, gtol,
-                                            _p(v0.contiguous()), th, _p(X), A.n,
-                                            C.byref(built)), "arnoldi_hess_eig")
+    v = v0.to(torch.float64).contiguous()
+    if basis_dtype == torch.float64:
+        st = ctx.lib.ofrr_arnoldi_hess_eig(ctx.h, A.n, A.nnz, _p(A.row_ptr), _p(A.col_idx),
+                                           _p(A.vals), k, kdim, maxiter, select_tol, gtol, _p(v), th,
+                                           _p(X), A.n, C.byref(built))
+    else:
+        st = ctx.lib.ofrr_arnoldi_hess_eig_mp(ctx.h, dtype_code(basis_dtype), A.n, A.nnz,
+                                              _p(A.row_ptr), _p(A.col_idx), _p(A.vals), k, kdim,
+                                              maxiter, select_tol, gtol, _p(v), th, _p(X), A.n,
+                                              C.byref(built))
+    ctx.check(st, "arnoldi_hess_eig")
     return np.array(th[:]), X, built.value
 
 

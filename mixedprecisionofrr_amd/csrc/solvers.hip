@@ -18,35 +18,6 @@
 namespace ofrr {
 
 // ---- small device kernels used by the drivers ------------------------------
-__global__ void scale_vec_kernel(int64_t n, double *__restrict__ x, const double *__restrict__ piv) {
-   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (i < n) x[i] = x[i] / *piv;
-}
-
-// Arnoldi-Hessenberg elimination coefficients for the new column w against
-// columns 0..i of V (pivot rows perm): c_j = w[perm_j] - sum_{j'<j} c_j' V(perm_j, j'),
-// accumulated with fused multiply-adds in the order the sequential process uses
-__global__ void arnoldi_coef_kernel(int i, int64_t n, const double *__restrict__ V,
-                                    const double *__restrict__ w, const int32_t *__restrict__ perm,
-                                    double *__restrict__ c) {
-   if (threadIdx.x != 0) return;
-   for (int j = 0; j <= i; ++j) {
-      const int64_t pj = perm[j];
-      double t = w[pj];
-      for (int jj = 0; jj < j; ++jj) t = fma(-c[jj], V[pj + (int64_t)jj * n], t);
-      c[j] = t;
-   }
-}
-
-__global__ void arnoldi_elim_kernel(int i, int64_t n, const double *__restrict__ V,
-                                    double *__restrict__ w, const double *__restrict__ c) {
-   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (r >= n) return;
-   double t = w[r];
-   for (int j = 0; j <= i; ++j) t = fma(-c[j], V[r + (int64_t)j * n], t);
-   w[r] = t;
-}
-
 // per-column sums of (Ax - theta x)^2 and x^2 for strided X, partial per block
 __global__ __launch_bounds__(256) void resid_cols_kernel(int64_t n, const double *__restrict__ AX,
                                                          int64_t ars, int64_t acs,
@@ -259,74 +230,90 @@ int ofrr_residuals(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp, con
 }
 
 // ---------------------------------------------------------------------------
-int ofrr_arnoldi_hess_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp,
-                          const int32_t *ci, const double *va, int k, int kdim, int maxiter,
-                          double select_tol, double gtol, const double *v0, double *theta_host,
-                          double *X, int64_t ldx, int *built_out) {
-   OFRR_CHECK_CTX(ctx);
-   (void)nnz;
-   if (n <= 0 || k <= 0 || kdim <= 0 || kdim + 1 > 256 || !rp || !ci || !va || !v0 || !theta_host || !X || ldx < n)
+// Arnoldi-Hessenberg (arnoldi_hess_eig.m:43-107) with the Krylov basis stored
+// in vdt (precision_output: FP64, FP32, FP16 or BF16), its arithmetic in FP64
+// for FP64 and FP32 otherwise (precision_compute), the operator products of
+// the sweep accumulated in that arithmetic (precision_A_matvec_*), and the
+// Rayleigh-Ritz in FP64: the basis widened to FP64, A V, V^T V and V^T A V
+// with FP64 accumulation, eig(V^T A V, V^T V + gtol I).  Each new column is
+// one left-looking Hessenberg step against the earlier ones (the pivot-row
+// forward substitution, ofrr_hessenberg_step), so a sweep of kdim columns runs
+// on the device without a host round trip per column: a column whose pivot is
+// below select_tol is flagged there, and one read after the sweep finds where
+// the process stopped (the columns after it are never used).
+static int arnoldi_impl(ofrr_ctx *ctx, int vdt, int64_t n, const int64_t *rp, const int32_t *ci,
+                        const double *va, int k, int kdim, int maxiter, double select_tol, double gtol,
+                        const double *v0, double *theta_host, double *X, int64_t ldx, int *built_out) {
+   if (n <= 0 || k <= 0 || kdim <= 0 || kdim + 1 > 384 || !rp || !ci || !va || !v0 || !theta_host || !X || ldx < n)
       return set_error(ctx, kErrorInvalidArgument, "arnoldi_hess_eig: bad arguments");
+   if (vdt != OFRR_F64 && vdt != OFRR_F32 && vdt != OFRR_F16 && vdt != OFRR_BF16)
+      return set_error(ctx, kErrorBLASUnsupportedType, "arnoldi_hess_eig: basis FP64, FP32, FP16 or BF16");
    if (k > n) k = (int)n;
    const int m = kdim + 1;
-   size_t need = al((size_t)n * m * 8) * 3 + al((size_t)m * m * 8) * 3 + al((size_t)m * 8) * 3 + al((size_t)m * 4);
+   const bool wide = vdt != OFRR_F64;
+   const size_t es = dtype_size(vdt);
+   size_t need = al((size_t)n * m * es) + al((size_t)n * m * 8) * (wide ? 3 : 2) + al((size_t)m * m * 8) * 4 +
+                 al((size_t)m * 8) + al((size_t)m * 4) * 2;
    int st = arena_reserve(ctx, need);
    if (st) return st;
-   double *V = (double *)arena_take((size_t)n * m * 8);
+   void *V = arena_take((size_t)n * m * es);
+   double *V64 = wide ? (double *)arena_take((size_t)n * m * 8) : (double *)V;
    double *AV = (double *)arena_take((size_t)n * m * 8);
    double *U = (double *)arena_take((size_t)n * m * 8);
    double *G = (double *)arena_take((size_t)m * m * 8);
    double *H = (double *)arena_take((size_t)m * m * 8);
    double *Y = (double *)arena_take((size_t)m * m * 8);
+   double *R = (double *)arena_take((size_t)m * m * 8);
    double *th = (double *)arena_take((size_t)m * 8);
-   double *piv = (double *)arena_take((size_t)m * 8);
-   double *coef = (double *)arena_take((size_t)m * 8);
-   int32_t *perm = (int32_t *)arena_take((size_t)m * 4);
-   if (!perm) return set_error(ctx, kErrorInternalError, "arnoldi: state layout exceeds its reservation");
-   double *hp = (double *)pinned(ctx, (size_t)m * 8 + 64);
+   int32_t *prow1 = (int32_t *)arena_take((size_t)m * 4);
+   int32_t *skip = (int32_t *)arena_take((size_t)m * 4);
+   if (!skip) return set_error(ctx, kErrorInternalError, "arnoldi: state layout exceeds its reservation");
+   double *hp = (double *)pinned(ctx, (size_t)m * 8 + (size_t)m * 4 + 64);
    if (!hp) return set_error(ctx, kErrorCUDAMemory, "arnoldi: pinned");
-   OFRR_HIP(ctx, hipMemcpyAsync(V, v0, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
-   const unsigned nb = (unsigned)((n + 255) / 256);
+   int32_t *hskip = (int32_t *)(hp + m);
+   const int64_t ldv = n;
+   auto col = [&](int j) { return (char *)V + (size_t)j * ldv * es; };
+   if ((st = convert(ctx, n, 1, v0, OFRR_F64, 1, n, V, vdt, 1, ldv))) return st;
    int tits = 1, built = 0, have = 0;
    while (tits < maxiter) {
-      if ((st = argmax_columns(ctx, OFRR_F64, n, 1, V, 1, 0, 0, perm, piv))) return st;
-      OFRR_HIP(ctx, hipMemcpyAsync(hp, piv, 8, hipMemcpyDeviceToHost, ctx->stream));
-      OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      if (std::fabs(hp[0]) < select_tol) break;
-      scale_vec_kernel<<<nb, 256, 0, ctx->stream>>>(n, V, piv);
-      int ncols = 1;
+      // first column: its pivot and scale (a tiny start vector ends the solve)
+      int32_t dropped = 0;
+      if ((st = ofrr_hessenberg_step(ctx, 0, vdt, n, m, V, 1, ldv, 0, select_tol, R, m, prow1, skip, &dropped)))
+         return st;
+      if (dropped) break;
       for (int i = 0; i < kdim; ++i) {
-         double *w = V + (int64_t)(i + 1) * n;
-         if ((st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, 1, V + (int64_t)i * n, OFRR_F64, 1, n, 1.0, 0.0, w, OFRR_F64, 1, n))) return st;
-         arnoldi_coef_kernel<<<1, 64, 0, ctx->stream>>>(i, n, V, w, perm, coef);
-         arnoldi_elim_kernel<<<nb, 256, 0, ctx->stream>>>(i, n, V, w, coef);
-         OFRR_LAUNCH(ctx);
-         if ((st = argmax_columns(ctx, OFRR_F64, n, 1, w, 1, 0, 0, perm + i + 1, piv + i + 1))) return st;
-         OFRR_HIP(ctx, hipMemcpyAsync(hp, piv + i + 1, 8, hipMemcpyDeviceToHost, ctx->stream));
-         OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-         if (std::fabs(hp[0]) < select_tol) { tits = maxiter; break; }
-         scale_vec_kernel<<<nb, 256, 0, ctx->stream>>>(n, w, piv + i + 1);
-         OFRR_LAUNCH(ctx);
-         ++ncols;
-         ++tits;
+         // w = A v_i in the basis arithmetic, stored in the basis format
+         if ((st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, 1, col(i), vdt, 1, ldv, 1.0, 0.0, col(i + 1), vdt, 1, ldv)))
+            return st;
+         if ((st = ofrr_hessenberg_step(ctx, 0, vdt, n, m, V, 1, ldv, i + 1, select_tol, R, m, prow1, skip, nullptr)))
+            return st;
       }
+      OFRR_HIP(ctx, hipMemcpyAsync(hskip, skip, (size_t)m * 4, hipMemcpyDeviceToHost, ctx->stream));
+      OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      int ncols = m;
+      for (int j = 1; j < m; ++j)
+         if (hskip[j]) { ncols = j; break; }
+      if (ncols < m) tits = maxiter;  // the process stopped: this is the last Rayleigh-Ritz
+      else tits += kdim;
       built = ncols;
-      if ((st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, ncols, V, OFRR_F64, 1, n, 1.0, 0.0, AV, OFRR_F64, 1, n))) return st;
-      if ((st = gram(ctx, n, ncols, ncols, V, OFRR_F64, 1, n, AV, OFRR_F64, 1, n, 1.0, 0.0, H, ncols))) return st;
-      if ((st = gram(ctx, n, ncols, ncols, V, OFRR_F64, 1, n, V, OFRR_F64, 1, n, 1.0, 0.0, G, ncols))) return st;
+      // Rayleigh-Ritz in FP64
+      if (wide && (st = convert(ctx, n, ncols, V, vdt, 1, ldv, V64, OFRR_F64, 1, n))) return st;
+      if ((st = spmm_csr(ctx, n, rp, ci, va, OFRR_F64, ncols, V64, OFRR_F64, 1, n, 1.0, 0.0, AV, OFRR_F64, 1, n))) return st;
+      if ((st = gram(ctx, n, ncols, ncols, V64, OFRR_F64, 1, n, AV, OFRR_F64, 1, n, 1.0, 0.0, H, ncols))) return st;
+      if ((st = gram(ctx, n, ncols, ncols, V64, OFRR_F64, 1, n, V64, OFRR_F64, 1, n, 1.0, 0.0, G, ncols))) return st;
       st = rr_generalized(ctx, ncols, G, ncols, H, ncols, gtol, th, Y, ncols);
       if (st == kErrorSingularMatrix) break;
       if (st) return st;
       {
-         const void *vb[1] = {V};
+         const void *vb[1] = {V64};
          int dt[1] = {OFRR_F64}, kb[1] = {ncols}, ldc[1] = {ncols};
          int64_t rs[1] = {1}, cs[1] = {n};
          const double *cb[1] = {Y};
          if ((st = basis_update(ctx, n, ncols, 1, vb, dt, kb, rs, cs, cb, ldc, U, OFRR_F64, 1, n))) return st;
       }
       have = ncols;
-      if (tits < maxiter)
-         OFRR_HIP(ctx, hipMemcpyAsync(V, U, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+      // restart from the top Ritz vector, in the basis format
+      if (tits < maxiter && (st = convert(ctx, n, 1, U, OFRR_F64, 1, n, V, vdt, 1, ldv))) return st;
    }
    int kk = std::min(k, have);
    if (kk > 0) {
@@ -338,6 +325,26 @@ int ofrr_arnoldi_hess_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *
    for (int c = 0; c < k; ++c) theta_host[c] = (c < kk) ? hp[c] : NAN;
    if (built_out) *built_out = built;
    return kSuccess;
+}
+
+int ofrr_arnoldi_hess_eig(ofrr_ctx *ctx, int64_t n, int64_t nnz, const int64_t *rp,
+                          const int32_t *ci, const double *va, int k, int kdim, int maxiter,
+                          double select_tol, double gtol, const double *v0, double *theta_host,
+                          double *X, int64_t ldx, int *built_out) {
+   OFRR_CHECK_CTX(ctx);
+   (void)nnz;
+   return arnoldi_impl(ctx, OFRR_F64, n, rp, ci, va, k, kdim, maxiter, select_tol, gtol, v0, theta_host, X, ldx,
+                       built_out);
+}
+
+int ofrr_arnoldi_hess_eig_mp(ofrr_ctx *ctx, int basis_dtype, int64_t n, int64_t nnz, const int64_t *rp,
+                             const int32_t *ci, const double *va, int k, int kdim, int maxiter,
+                             double select_tol, double gtol, const double *v0, double *theta_host,
+                             double *X, int64_t ldx, int *built_out) {
+   OFRR_CHECK_CTX(ctx);
+   (void)nnz;
+   return arnoldi_impl(ctx, basis_dtype, n, rp, ci, va, k, kdim, maxiter, select_tol, gtol, v0, theta_host, X,
+                       ldx, built_out);
 }
 
 // ---------------------------------------------------------------------------

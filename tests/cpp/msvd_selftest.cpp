@@ -60,7 +60,7 @@ int main() {
    }
    // ---- gemm KAT (mvops_test.cpp:455-721): C = A B, A 3x2, B 2x4
    {
-      Matrix<double> A(3, 2, Location::kHOST), B(2, 4, Location::kHOST), C(3, 4, Location::kDEVICE);
+      Matrix<double> A(3, 2, Location::kHOST), B(2, 4, Location::kHOST), C(3, 4, Location::kHOST);
       const double a[3][2] = {{1, 3}, {2, 4}, {5, 6}};
       const double b[2][4] = {{1, 2, 3, 4}, {5, 6, 7, 8}};
       const double c[3][4] = {{16, 20, 24, 28}, {22, 28, 34, 40}, {35, 46, 57, 68}};

@@ -69,3 +69,38 @@ def test_filtered_mixed_on_1138_bus(ctx):
     res_o = O.residuals(Ah, X.cpu().numpy(), th, ref[0])
     assert np.max(res_o[:20]) <= 2e-8
     assert np.max(np.abs(th[:20] - ref[:20]) / ref[:20]) < 1e-10
+
+
+_EPS = {torch.float32: 2.0 ** -23, torch.float16: 2.0 ** -10, torch.bfloat16: 2.0 ** -7}
+
+
+@pytest.mark.parametrize("name", ["bcsstk01", "bcsstk03", "1138_bus"])
+@pytest.mark.parametrize("bdt,gtol", [(torch.float32, 0.0), (torch.float16, 1e-10)])
+def test_arnoldi_hessenberg_mixed_precision(ctx, name, bdt, gtol):
+    """test4's single and half Arnoldi-Hessenberg runs (test4/test.m:102-111:
+    compute single, output single / half, eigenvalues double, generalized_tol
+    0 / 1e-10, select_tol = eps(output)) with the experiment's k, kdim and
+    maxiter: the top Ritz value and every Ritz value whose FP64 residual is at
+    the basis format's level match LAPACK's eigenvalues (golden values)."""
+    from mixedprecisionofrr_amd import api
+    A, Ah, m = _load(ctx, name)
+    k, kdim, maxiter = m["k"], m["kdim"], m["maxiter"]
+    v0 = torch.empty(A.n, dtype=torch.float64, device="cuda:0")
+    api.fill_randn(ctx, v0, seed=42)
+    print(f"test4 {name}: mixed Arnoldi basis={bdt} k={k} kdim={kdim} maxiter={maxiter}", flush=True)
+    th, X, built = api.arnoldi_hess_eig(ctx, A, k, kdim, maxiter, v0, select_tol=_EPS[bdt], gtol=gtol,
+                                        basis_dtype=bdt)
+    ref = np.array(m["top_eigenvalues"])
+    res = O.residuals(Ah, X.cpu().numpy(), th, ref[0])
+    ok = np.isfinite(th)
+    err = [float(np.min(np.abs(ref - t) / ref[0])) for t in th[ok]]
+    print(f"  built {built} residuals {np.array2string(res[ok], precision=1)} "
+          f"eig errors {np.array2string(np.array(err), precision=1)}", flush=True)
+    u = _EPS[bdt]
+    assert abs(th[0] - ref[0]) / ref[0] < 50 * u
+    # a Ritz pair with residual r has its value within ~r of an eigenvalue
+    # (symmetric A: |theta - lambda| <= ||A x - theta x|| / ||x||)
+    conv = np.where(ok & (res <= 10 * u))[0]
+    assert len(conv) >= 1
+    for j in conv:
+        assert np.min(np.abs(ref - th[j]) / ref[0]) <= 1.01 * res[j] + 1e-12, (name, j, th[j], res[j])
