@@ -94,8 +94,9 @@ def test_arnoldi_hessenberg_mixed_precision(ctx, name, bdt, gtol):
     res = O.residuals(Ah, X.cpu().numpy(), th, ref[0])
     ok = np.isfinite(th)
     err = [float(np.min(np.abs(ref - t) / ref[0])) for t in th[ok]]
-    print(f"  built {built} residuals {np.array2string(res[ok], precision=1)} "
-          f"eig errors {np.array2string(np.array(err), precision=1)}", flush=True)
+    fmt = {"float_kind": lambda x: f"{x:.1e}"}
+    print(f"  built {built} residuals {np.array2string(res[ok], formatter=fmt)} "
+          f"eig errors {np.array2string(np.array(err), formatter=fmt)}", flush=True)
     u = _EPS[bdt]
     assert abs(th[0] - ref[0]) / ref[0] < 50 * u
     # a Ritz pair with residual r has its value within ~r of an eigenvalue
