@@ -201,6 +201,21 @@ OFRR_API int ofrr_gs_qr_step(ofrr_ctx *ctx, int variant, int q_dtype, int64_t m,
 OFRR_API int ofrr_gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V_dev, int v_dtype,
                        int64_t v_rs, int64_t v_cs, const void *W_dev, int w_dtype, int64_t w_rs,
                        int64_t w_cs, double alpha, double beta, double *G_dev, int ldg);
+/* ofrr_gram without device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * semantics and FP64 accumulation as ofrr_gram; the per-chunk partial sums
+ * live in a static 8 MiB device array per GPU (set up by ofrr_ctx_create),
+ * so it calls no hipMalloc / hipFree.  Limit: ceil(k1/64) * ceil(k2/64) <= 256
+ * output tiles (k1 = k2 <= 1024), else 2005 with a message.  Its chunking
+ * differs from ofrr_gram's (about 256 chunk tiles instead of 2048), so the two
+ * agree to FP64 rounding, not bit for bit.  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, other host
+ * threads) race. */
+OFRR_API int ofrr_gram_noalloc(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V_dev,
+                               int v_dtype, int64_t v_rs, int64_t v_cs, const void *W_dev,
+                               int w_dtype, int64_t w_rs, int64_t w_cs, double alpha, double beta,
+                               double *G_dev, int ldg);
 /* Y = sum_b V_b C_b over nb blocks (Y n x k FP64 or FP32; V_b n x kb of any
  * dtype; C_b kb x k FP64 column-major): the back-transform V <- V U of
  * eig_rr.m:56 for a basis whose column blocks have different dtypes. */

@@ -28,7 +28,7 @@ EXPORTS = [
     "ofrr_svd_rr", "ofrr_subspace_iter_svd", "ofrr_fp8_check", "ofrr_gs_qr", "ofrr_gs_qr_step",
     "ofrr_spmm_csr_rect", "ofrr_csr_transpose", "ofrr_server_slot_bytes",
     "ofrr_hessenberg_v3", "ofrr_hessenberg_step", "ofrr_gemm", "ofrr_axpb",
-    "ofrr_arnoldi_hess_eig_mp",
+    "ofrr_arnoldi_hess_eig_mp", "ofrr_gram_noalloc",
 ]
 
 SERVE_MAX_WANT = 128
@@ -111,6 +111,8 @@ def _decl(L):
     L.ofrr_gemm.argtypes = [vp, i32, i32, i64, i64, i64, dbl, vp, i32, i64, vp, i32, i64, dbl, vp,
                             i32, i64, i32]
     L.ofrr_axpb.argtypes = [vp, i32, i64, i64, vp, i64, i64, dbl, dbl]
+    L.ofrr_gram_noalloc.argtypes = [vp, i64, i32, i32, vp, i32, i64, i64, vp, i32, i64, i64, dbl,
+                                    dbl, vp, i32]
     L.ofrr_arnoldi_hess_eig_mp.argtypes = [vp, i32, i64, i64, vp, vp, vp, i32, i32, i32, dbl, dbl, vp,
                                            C.POINTER(dbl), vp, i64, C.POINTER(i32)]
     L.ofrr_hessenberg_v3.argtypes = [vp, i32, i64, i32, vp, i64, i64, dbl, vp, i32, vp, vp]

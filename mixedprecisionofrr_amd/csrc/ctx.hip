@@ -2,7 +2,7 @@
 // The context replaces the reference's CUDAHandler singleton
 // (CPP/core/utils/cuda_handler.cu:9-45): it owns a stream, a grow-only device
 // workspace and a pinned host staging buffer, one context per host thread.
-#include "common.hpp"
+#include "internal.hpp"
 
 namespace ofrr {
 
@@ -103,6 +103,8 @@ int ofrr_ctx_create(int device, ofrr_ctx **out) {
    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return kErrorCUDAGeneral;
    if (device < 0 || device >= n) return kErrorInvalidArgument;
    if (hipSetDevice(device) != hipSuccess) return kErrorCUDAGeneral;
+   // once per device, with it current: the scratch of ofrr_gram_noalloc
+   if (gram_scratch_init(device) != kSuccess) return kErrorCUDAGeneral;
    ofrr_ctx *c = new ofrr_ctx();
    c->device = device;
    // a blocking stream: ordered with the null stream that frameworks default to

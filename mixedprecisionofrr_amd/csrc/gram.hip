@@ -118,19 +118,26 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(int nchunks, int ntile
    *g = (beta == 0.0) ? alpha * t : fma(alpha, t, beta * *g);
 }
 
-int gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64_t vrs,
-         int64_t vcs, const void *W, int wdt, int64_t wrs, int64_t wcs, double alpha, double beta,
-         double *G, int ldg) {
-   if (k1 <= 0 || k2 <= 0) return kSuccess;
-   const int ti = (k1 + GT - 1) / GT, tj = (k2 + GT - 1) / GT;
-   // about kGramChunks waves per tile pair, each with a multiple of 4 rows
-   int64_t want = std::max<int64_t>(1, kGramChunks / (ti * tj));
-   int64_t rows = (n + want - 1) / want;
+// rows per chunk and chunk count of a product over n rows with ti x tj output
+// tiles: about `chunk_tiles` (chunk, tile) waves, each chunk a multiple of 4
+// rows; the partials take nchunks x ti x tj tiles of GT x GT doubles, at most
+// max(chunk_tiles, ti tj) of them
+static void gram_chunks(int64_t n, int ti, int tj, int64_t chunk_tiles, int64_t &rows, int &nchunks) {
+   const int64_t want = std::max<int64_t>(1, chunk_tiles / ((int64_t)ti * tj));
+   rows = (n + want - 1) / want;
    rows = std::max<int64_t>(64, ((rows + 3) / 4) * 4);
-   const int nchunks = (int)((n + rows - 1) / rows);
+   nchunks = (int)std::max<int64_t>(1, (n + rows - 1) / rows);
+}
+
+// the product into `part` (nchunks x ti x tj partial tiles, gram_chunks)
+static int gram_run(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64_t vrs,
+                    int64_t vcs, const void *W, int wdt, int64_t wrs, int64_t wcs, double alpha,
+                    double beta, double *G, int ldg, double *part, int64_t chunk_tiles) {
+   const int ti = (k1 + GT - 1) / GT, tj = (k2 + GT - 1) / GT;
+   int64_t rows;
+   int nchunks;
+   gram_chunks(n, ti, tj, chunk_tiles, rows, nchunks);
    const int64_t per_chunk = (int64_t)ti * tj * GT * GT;
-   double *part = (double *)workspace(ctx, (size_t)nchunks * per_chunk * sizeof(double));
-   if (!part) return set_error(ctx, kErrorCUDAMemory, "gram: workspace");
    int st = dispatch_dtype(vdt, [&](auto tv) -> int {
       using TV = decltype(tv);
       return dispatch_dtype(wdt, [&](auto tw) -> int {
@@ -146,6 +153,51 @@ int gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64
        nchunks, ti, tj, k1, k2, part, alpha, beta, G, ldg);
    OFRR_LAUNCH(ctx);
    return kSuccess;
+}
+
+int gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64_t vrs,
+         int64_t vcs, const void *W, int wdt, int64_t wrs, int64_t wcs, double alpha, double beta,
+         double *G, int ldg) {
+   if (k1 <= 0 || k2 <= 0) return kSuccess;
+   const int ti = (k1 + GT - 1) / GT, tj = (k2 + GT - 1) / GT;
+   int64_t rows;
+   int nchunks;
+   gram_chunks(n, ti, tj, kGramChunks, rows, nchunks);
+   const int64_t per_chunk = (int64_t)ti * tj * GT * GT;
+   double *part = (double *)workspace(ctx, (size_t)nchunks * per_chunk * sizeof(double));
+   if (!part) return set_error(ctx, kErrorCUDAMemory, "gram: workspace");
+   return gram_run(ctx, n, k1, k2, V, vdt, vrs, vcs, W, wdt, wrs, wcs, alpha, beta, G, ldg, part,
+                   kGramChunks);
+}
+
+// ---- the allocation-free Gram product (ofrr_gram_noalloc) -----------------
+// Its chunk partials live in `scratch`, a file-scope device array of
+// kScratchChunkTiles tiles of GT x GT doubles (8 MiB on every GPU the library
+// runs on, held until the process ends).  Host code never passes the array's
+// name to a kernel: ofrr_ctx_create looks up its device address once per
+// device (gram_scratch_init, with that device current) and the entry point
+// passes that address.  Every chunk writes its whole partial tile before the
+// reduction reads it, so nothing needs clearing between calls.
+constexpr int64_t kScratchChunkTiles = 256;
+constexpr int kScratchMaxDevices = 64;
+static __device__ __align__(256) double scratch[kScratchChunkTiles * GT * GT];
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[kScratchMaxDevices] = {};
+
+int gram_scratch_init(int device) {
+   if (device < 0 || device >= kScratchMaxDevices) return kErrorInvalidArgument;
+   std::lock_guard<std::mutex> lk(g_scratch_mu);
+   if (g_scratch_ptr[device]) return kSuccess;
+   double *p = nullptr;
+   if (hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return kErrorCUDAGeneral;
+   g_scratch_ptr[device] = p;
+   return kSuccess;
+}
+
+static double *gram_scratch(int device) {
+   if (device < 0 || device >= kScratchMaxDevices) return nullptr;
+   std::lock_guard<std::mutex> lk(g_scratch_mu);
+   return g_scratch_ptr[device];
 }
 
 // ---------------------------------------------------------------------------
@@ -708,6 +760,24 @@ int ofrr_gram(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, 
    if (k1 == 0 || k2 == 0) return kSuccess;
    if (!V || !W || !G) return set_error(ctx, kErrorInvalidArgument, "gram: null pointer");
    return gram(ctx, n, k1, k2, V, vdt, vrs, vcs, W, wdt, wrs, wcs, alpha, beta, G, ldg);
+}
+
+int ofrr_gram_noalloc(ofrr_ctx *ctx, int64_t n, int k1, int k2, const void *V, int vdt, int64_t vrs,
+                      int64_t vcs, const void *W, int wdt, int64_t wrs, int64_t wcs, double alpha,
+                      double beta, double *G, int ldg) {
+   OFRR_CHECK_CTX(ctx);
+   if (n < 0 || k1 < 0 || k2 < 0 || ldg < k1) return set_error(ctx, kErrorInvalidArgument, "gram_noalloc: bad sizes");
+   if (k1 == 0 || k2 == 0) return kSuccess;
+   if (!V || !W || !G) return set_error(ctx, kErrorInvalidArgument, "gram_noalloc: null pointer");
+   const int64_t tiles = (int64_t)((k1 + GT - 1) / GT) * ((k2 + GT - 1) / GT);
+   if (tiles > kScratchChunkTiles)
+      return set_error(ctx, kErrorBLASInvalidSize,
+                       "gram_noalloc: ceil(k1/64) * ceil(k2/64) exceeds the 256 tiles of its scratch");
+   if (!dtype_valid(vdt) || !dtype_valid(wdt)) return set_error(ctx, kErrorBLASUnsupportedType, "gram_noalloc: dtype");
+   double *part = gram_scratch(ctx->device);
+   if (!part) return set_error(ctx, kErrorInternalError, "gram_noalloc: scratch address not set up for this device");
+   return gram_run(ctx, n, k1, k2, V, vdt, vrs, vcs, W, wdt, wrs, wcs, alpha, beta, G, ldg, part,
+                   kScratchChunkTiles);
 }
 
 int ofrr_basis_update(ofrr_ctx *ctx, int64_t n, int k, int nb, const void *const *V,

@@ -63,6 +63,9 @@ int gram_multi(ofrr_ctx *ctx, int64_t n, int k, int tb, int tz, GramMultiArgs &a
 int basis_update(ofrr_ctx *ctx, int64_t n, int k, int nb, const void *const *V, const int *vdt,
                  const int *kb, const int64_t *vrs, const int64_t *vcs, const double *const *C,
                  const int *ldc, void *Y, int ydt, int64_t yrs, int64_t ycs);
+// the device address of the allocation-free Gram product's scratch on
+// `device` (which must be current), looked up once per device
+int gram_scratch_init(int device);
 // largest workspace of gram / gram_multi at any n (bytes)
 size_t gram_workspace_bound(int num_cus);
 // workspace of rr_truncated at basis dimension m (bytes)
