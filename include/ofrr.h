@@ -320,6 +320,13 @@ typedef struct {
                         columns: width the smallest power of two >= k - p, halved from
                         k, at least 16 (the Ritz block keeps all k columns, so locked
                         pairs stay in every Rayleigh-Ritz); 0: every sweep filters k */
+   int mid_cols;     /* nd = 2 only.  0: the mid-filter block enters the basis whole;
+                        w > 0: only its trailing w columns do (the ones nearest the
+                        block edge), so the basis dimension is k + ka + min(w, ka)
+                        with ka the filtered width -- a Ritz block wider than nwant
+                        (guard columns) at a given basis dimension, e.g. k = 128,
+                        w = 44 for 100 pairs in a basis of 300.  Every Ritz column
+                        still gets its full correction block D_d. */
 } ofrr_filter_opts;
 
 typedef struct {
@@ -501,6 +508,8 @@ typedef struct {
    int refine_dtype;   /* refinement format when refine = 1 (OFRR_F64: the growth
                           products then also use the FP64 operator values) */
    double refine_tol;  /* switch once the wanted residuals are below it (or stall) */
+   int mid_cols;       /* ofrr_filter_opts.mid_cols (0: the whole mid-filter block);
+                          basis dimension k + k + mid_cols with nd = 2 */
 } ofrr_eig_request;
 
 typedef struct {

@@ -40,7 +40,7 @@ class FilterOpts(C.Structure):
                 ("maxiter", C.c_int), ("basis_dtype", C.c_int), ("tol", C.c_double),
                 ("tau", C.c_double), ("lower", C.c_double), ("normA", C.c_double),
                 ("refine", C.c_int), ("refine_dtype", C.c_int), ("refine_tol", C.c_double),
-                ("lock", C.c_int)]
+                ("lock", C.c_int), ("mid_cols", C.c_int)]
 
 
 class FilterStats(C.Structure):
@@ -61,7 +61,7 @@ class EigRequest(C.Structure):
                 ("start_seed", C.c_uint64), ("nwant", C.c_int), ("basis_dtype", C.c_int),
                 ("tol", C.c_double), ("degree", C.c_int), ("maxiter", C.c_int),
                 ("k", C.c_int), ("nd", C.c_int), ("refine", C.c_int), ("refine_dtype", C.c_int),
-                ("refine_tol", C.c_double)]
+                ("refine_tol", C.c_double), ("mid_cols", C.c_int)]
 
 
 class EigResult(C.Structure):

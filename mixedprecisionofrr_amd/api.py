@@ -498,7 +498,7 @@ def subspace_iter_eig(ctx, A, V0, maxiter, step_size=1, select_tol=2.0 ** -52, g
 
 def filtered_eig(ctx, A, X0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=torch.float16,
                  tol=1e-8, tau=1e-12, lower=0.0, normA=0.0, use_lo_values=True,
-                 refine_dtype=None, refine_tol=0.0, lock=True):
+                 refine_dtype=None, refine_tol=0.0, lock=True, mid_cols=0):
     """Filtered OFRR solve; X0 (n x k, FP64) is copied.  refine_dtype: the
     precision ladder's refinement format (None: off).  lock: soft locking (a
     sweep filters only the columns after the leading converged wanted pairs).
@@ -511,7 +511,7 @@ def filtered_eig(ctx, A, X0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=to
     o = FilterOpts(k, nwant, nd, degree, maxiter, bdt, tol, tau, lower, normA,
                    0 if refine_dtype is None else 1,
                    0 if refine_dtype is None else dtype_code(refine_dtype), refine_tol,
-                   1 if lock else 0)
+                   1 if lock else 0, mid_cols)
     th = (C.c_double * k)()
     res = (C.c_double * k)()
     stats = FilterStats()
@@ -685,10 +685,13 @@ def _sym_shift(rp, ci, va, n, symmetrize, shift):
 
 # ---- serving -----------------------------------------------------------------
 def request(dim, N, nwant, coef_seed=0, start_seed=0, basis_dtype=torch.float16, tol=1e-8,
-            degree=0, maxiter=0, k=0, nd=0, refine_dtype=None, refine_tol=0.0, refine=None):
+            degree=0, maxiter=0, k=0, nd=0, refine_dtype=None, refine_tol=0.0, refine=None,
+            mid_cols=0):
     """An eigen request: the nwant largest eigenpairs of the dim-D diffusion
     stencil on an N^dim grid (coef_seed 0: the Laplacian).  k / nd fix the
-    block width and correction blocks (0: the server's plan); refine_dtype sets
+    block width and correction blocks (0: the server's plan), mid_cols the
+    mid-filter block's columns kept in the basis (0: all; basis k + k +
+    mid_cols with nd = 2); refine_dtype sets
     the precision ladder's refinement format (refine=False: no ladder; default:
     FP8 finishes in FP16)."""
     if refine_dtype is not None:
@@ -696,7 +699,7 @@ def request(dim, N, nwant, coef_seed=0, start_seed=0, basis_dtype=torch.float16,
     else:
         rf, rdt = (-1 if refine is False else 0), F64
     return _lib.EigRequest(dim, N, coef_seed, start_seed, nwant, dtype_code(basis_dtype), tol,
-                           degree, maxiter, k, nd, rf, rdt, refine_tol)
+                           degree, maxiter, k, nd, rf, rdt, refine_tol, mid_cols)
 
 
 def server_slot_bytes(device=0):

@@ -36,7 +36,7 @@ constexpr int kDefaultMaxIter = 200;
 
 struct Plan {
    int k = 0, nd = 0, degree = 0, maxiter = 0, lo_dt = OFRR_F64;
-   int refine = 0, refine_dtype = OFRR_F64;
+   int refine = 0, refine_dtype = OFRR_F64, mid_cols = 0;
    double refine_tol = 0.0;
    int64_t n = 0, nnz = 0;
    size_t off_ci = 0, off_va = 0, off_lo = 0, off_x = 0, off_state = 0, total = 0;
@@ -54,7 +54,7 @@ int plan_request(const ofrr_eig_request &r, Plan &p) {
        !dtype_valid(r.basis_dtype) || !(r.tol > 0.0) || r.degree < 0 || r.maxiter < 0)
       return kErrorInvalidArgument;
    if (r.k < 0 || (r.k > 0 && (r.k < r.nwant || r.k > 128)) || r.nd < 0 || r.nd > 2 ||
-       r.refine < -1 || r.refine > 1 || (r.refine == 1 && !dtype_valid(r.refine_dtype)))
+       r.refine < -1 || r.refine > 1 || (r.refine == 1 && !dtype_valid(r.refine_dtype)) || r.mid_cols < 0)
       return kErrorInvalidArgument;
    p.n = r.dim == 2 ? r.N * r.N : r.N * r.N * r.N;
    if (p.n >= (int64_t)1 << 31) return kErrorBLASInvalidSize;
@@ -69,8 +69,9 @@ int plan_request(const ofrr_eig_request &r, Plan &p) {
       while (p.k < want && p.k < 128) p.k *= 2;
    }
    if (p.k > p.n) return kErrorInvalidArgument;
-   p.nd = r.nd > 0 ? r.nd : ((3 * p.k <= 256) ? 2 : 1);
-   if (p.k * (1 + p.nd) > 384) return kErrorInvalidArgument;
+   p.mid_cols = (r.mid_cols > 0 && r.mid_cols < p.k) ? r.mid_cols : 0;
+   p.nd = r.nd > 0 ? r.nd : ((3 * p.k <= 256 || p.mid_cols) ? 2 : 1);
+   if (p.k * p.nd + (p.nd == 2 && p.mid_cols ? p.mid_cols : p.k) > 384) return kErrorInvalidArgument;
    // precision ladder: FP8 corrections gain at most ~2^4 per sweep, so by
    // default an FP8 solve finishes in FP16 once the wanted residuals are below
    // 1e-4 or stall
@@ -97,6 +98,7 @@ int plan_request(const ofrr_eig_request &r, Plan &p) {
    o.basis_dtype = r.basis_dtype;
    o.refine = p.refine;
    o.refine_dtype = p.refine_dtype;
+   o.mid_cols = p.mid_cols;
    int64_t sb = 0;
    if (ofrr_filtered_eig_state_bytes(p.n, &o, &sb) != kSuccess) return kErrorInvalidArgument;
    p.state_bytes = (size_t)sb;
@@ -203,6 +205,7 @@ void ofrr_server::run_one(ofrr_ctx *ctx, Rec &r) {
       o.refine = p.refine;
       o.refine_dtype = p.refine_dtype;
       o.refine_tol = p.refine_tol;
+      o.mid_cols = p.mid_cols;
       std::vector<double> th(p.k), rs(p.k);
       ofrr_filter_stats stats{};
       ctx->state_mem = mem + p.off_state;

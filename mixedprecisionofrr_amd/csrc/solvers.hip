@@ -934,7 +934,7 @@ static void filtered_layout(Carve &cv, int64_t n, const ofrr_filter_opts &o, int
    b.C = (double *)cv.take((size_t)mp * kp * 8);
    b.Gc = b.Hc = b.Cc = nullptr;
    b.cmap = b.cinv = nullptr;
-   if (kp != k) {
+   if (kp != k || (nd == 2 && o.mid_cols > 0)) {
       b.Gc = (double *)cv.take((size_t)m * m * 8);
       b.Hc = (double *)cv.take((size_t)m * m * 8);
       b.Cc = (double *)cv.take((size_t)m * k * 8);
@@ -1011,16 +1011,19 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    const int d = dmax;        // buffer sizing: the largest degree of any sweep
    const int minw = filter_min_width(*o, op.min_width());
    if (kr < 2 || kr > 128 || nd < 1 || nd > 2 || d < 2 || o->nwant < 1 || o->nwant > kr ||
-       !dtype_valid(bdt) || (int64_t)filter_width(kr, minw) > n)
+       !dtype_valid(bdt) || (int64_t)filter_width(kr, minw) > n || o->mid_cols < 0)
       return set_error(ctx, kErrorInvalidArgument,
                        "filtered_eig: need k in [2,128], nd in {1,2}, degree >= 2, 1 <= nwant <= k");
    if (o->refine && (!dtype_valid(o->refine_dtype) || !op.can_refine()))
       return set_error(ctx, kErrorInvalidArgument,
                        "filtered_eig: the refinement format is valid for a CSR operator only");
    const int k = filter_width(kr, minw);
-   const bool padded = k != kr;
+   // mid: the trailing columns of the mid-filter block kept in the basis (0: all)
+   const int mid_keep = (nd == 2 && o->mid_cols > 0 && o->mid_cols < kr) ? o->mid_cols : 0;
+   const bool padded = k != kr || mid_keep > 0;  // columns left out: compacted Rayleigh-Ritz
    const int m = k * (1 + nd);
-   if (kr * (1 + nd) > 384) return set_error(ctx, kErrorInvalidArgument, "filtered_eig: basis dimension > 384");
+   if (kr * nd + (mid_keep ? mid_keep : (nd == 2 ? kr : 0)) > 384)
+      return set_error(ctx, kErrorInvalidArgument, "filtered_eig: basis dimension > 384");
    int zdt = (bdt == OFRR_F64) ? OFRR_F64 : OFRR_F32;
    size_t zsz = dtype_size(zdt);
    // ---- buffers
@@ -1062,12 +1065,15 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       int mr = 0;
       for (int p = 0; p < me; ++p) h_cinv[p] = -1;
       for (int j = 0; j < kr; ++j) { h_cinv[j] = mr; h_cmap[mr++] = j; }
-      for (int bb = 0; bb < nb; ++bb)
-         for (int j = 0; j < kar; ++j) {
+      for (int bb = 0; bb < nb; ++bb) {
+         // the mid-filter block (bb = 1) keeps its trailing mid_keep real columns
+         const int j0 = (bb == 1 && mid_keep) ? std::max(0, kar - mid_keep) : 0;
+         for (int j = j0; j < kar; ++j) {
             const int p = k + bb * ka + j;
             h_cinv[p] = mr;
             h_cmap[mr++] = p;
          }
+      }
       OFRR_HIP(ctx, hipMemcpyAsync(fb.cmap, h_cmap, (size_t)mr * 4, hipMemcpyHostToDevice, ctx->stream));
       OFRR_HIP(ctx, hipMemcpyAsync(fb.cinv, h_cinv, (size_t)me * 4, hipMemcpyHostToDevice, ctx->stream));
       sym_gather_kernel<<<(mr * mr + 255) / 256, 256, 0, ctx->stream>>>(mr, fb.cmap, G, me, fb.Gc);

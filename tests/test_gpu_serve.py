@@ -105,3 +105,40 @@ def test_serve_eigenvalues_match_dense_fp64(api):
     lap = O.laplacian_eigs(3, 12)[:8]
     assert out[1]["status"] == 0
     assert np.max(np.abs(out[1]["theta"] - lap) / lap) < 1e-10
+
+
+def _fp8_basis300(api):
+    """The large FP8 solve's configuration on its scaled twin: 3-D
+    variable-coefficient diffusion (coefficient seed 5) on a 32^3 grid, the 100
+    largest pairs, basis dimension 300 = a 128-column Ritz block (100 wanted +
+    28 guard columns), the 128-column FP8 E4M3 correction block and the
+    trailing 44 columns of the FP8 mid-filter block (per-column power-of-two
+    scales, FP32 accumulation), FP64 Gram / truncated Rayleigh-Ritz /
+    residuals, and the precision ladder finishing the growth blocks in FP16
+    below 1e-4."""
+    return api.request(3, 32, 100, coef_seed=5, start_seed=7, basis_dtype=torch.float8_e4m3fn,
+                       tol=1e-8, k=128, nd=2, mid_cols=44, refine_dtype=torch.float16, refine_tol=1e-4)
+
+
+def test_serve_fp8_basis300_beside_a_mix(api):
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "fp8_twin.json")))
+    print("serve: FP8 basis-300 solve (32^3 twin) beside the mix on 4 slots", flush=True)
+    mix, st = _run(api, _requests(api)[:4] + [_fp8_basis300(api)] + _requests(api)[4:], 4)
+    print("serve: the FP8 basis-300 solve alone", flush=True)
+    alone, _ = _run(api, [_fp8_basis300(api)], 1)
+    b, a = mix[4], alone[0]
+    print(f"  status {b['status']} k {b['k']} nd {b['nd']} iterations {b['iterations']} "
+          f"max residual {np.max(b['res']):.2e}", flush=True)
+    assert b["status"] == 0 and (b["k"], b["nd"]) == (128, 2)
+    assert b["converged"] == 100 and np.max(b["res"]) <= 1e-8
+    ref = np.array(gold["top_eigenvalues"][:100])
+    assert np.max(np.abs(b["theta"] - ref) / ref[0]) < 1e-10
+    # the same result alone and in the mix, bit for bit
+    assert a["iterations"] == b["iterations"]
+    assert np.array_equal(a["theta"], b["theta"]) and np.array_equal(a["res"], b["res"])
+    assert st["running_peak"] > 1 and st["completed"] == 9
+    for o in mix:
+        assert o["status"] == 0

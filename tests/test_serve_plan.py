@@ -60,15 +60,19 @@ def test_bad_tolerance_refused():
 
 
 def test_fixed_plan_fields():
-    # the configured large FP8 solve: 100 pairs, basis 300 = k 100 x (1 + 2 blocks)
-    k, nd, fp = api.request_footprint(api.request(3, 64, 100, k=100, nd=2,
+    # the configured large FP8 solve: 100 pairs, basis 300 = 128 + 128 + 44
+    k, nd, fp = api.request_footprint(api.request(3, 64, 100, k=128, nd=2, mid_cols=44,
                                                   basis_dtype=torch.float8_e4m3fn,
-                                                  refine_dtype=torch.float64, refine_tol=1e-4))
-    assert (k, nd) == (100, 2)
+                                                  refine_dtype=torch.float16, refine_tol=1e-4))
+    assert (k, nd) == (128, 2)
     n = 64 ** 3
-    # blocks are stored 128 wide (power-of-two storage of k = 100)
     assert fp >= 3 * n * 128 * 8
+    # a block width that is not a power of two is stored at the next one
+    k, nd, fp = api.request_footprint(api.request(3, 64, 100, k=100, nd=2))
+    assert (k, nd) == (100, 2) and fp >= 3 * n * 128 * 8
     assert api.request_footprint(api.request(3, 64, 16, nd=1))[:2] == (32, 1)
+    # mid_cols >= k keeps the whole mid-filter block (basis 384, the largest)
+    assert api.request_footprint(api.request(3, 64, 100, k=128, nd=2, mid_cols=129))[:2] == (128, 2)
 
 
 @pytest.mark.parametrize("bad", [dict(k=8), dict(k=129), dict(nd=3), dict(refine_dtype=7)])
