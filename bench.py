@@ -352,8 +352,14 @@ def serve_stream(api, count, rate, slots, seed, big=False):
     import numpy as np
     import torch
     reqs, meta = serve_requests(api, count, seed)
+    # the large FP8 solve as configured: basis dimension 300 = a 128-column Ritz
+    # block (100 wanted + 28 guard columns), the 128-column FP8 correction block
+    # and the trailing 44 columns of the FP8 mid-filter block; FP64 Gram,
+    # truncated Rayleigh-Ritz and residuals; the precision ladder finishes the
+    # growth blocks in FP16 once the wanted residuals fall below 1e-4 or stall
     big_req = api.request(3, 256, 100, coef_seed=5, start_seed=7, basis_dtype=torch.float8_e4m3fn,
-                          tol=1e-8) if big else None
+                          tol=1e-8, k=128, nd=2, mid_cols=44, refine_dtype=torch.float16,
+                          refine_tol=1e-4) if big else None
     big_id = big_out = None
     arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, size=count))
     arr -= arr[0]
@@ -393,9 +399,13 @@ def serve_stream(api, count, rate, slots, seed, big=False):
             log(f"serve: large FP8 solve status {b['status']} iterations {b['iterations']} "
                 f"{(b['t_done'] - b['t_submit']):.1f} s")
             big_out = {"workload": "3-D 7-point variable-coefficient diffusion, 256^3 grid "
-                                   "(n = 16,777,216), 100 largest pairs, block 128 + one FP8 E4M3 "
-                                   "correction block (basis 256), tol 1e-8, submitted during the "
-                                   "serving stream",
+                                   "(n = 16,777,216), 100 largest pairs, basis dimension 300 "
+                                   "(128-column FP64 Ritz block + 128-column FP8 E4M3 correction "
+                                   "block + 44 columns of the FP8 mid-filter block, per-column "
+                                   "power-of-two scales, FP32 accumulation; FP16 ladder below "
+                                   "1e-4), FP64 Gram / truncated Rayleigh-Ritz / residuals, tol "
+                                   "1e-8, submitted during the serving stream",
+                       "basis_dim": 300, "k": b["k"], "nd": b["nd"],
                        "status": b["status"], "iterations": b["iterations"],
                        "filter_steps": b["filter_steps"], "converged": b["converged"],
                        "latency_s": b["t_done"] - b["t_submit"],
