@@ -89,30 +89,67 @@ def run_solver(api, ctx, A, X0, args, basis, label):
     return st, th, X, res, stats
 
 
-def cpu_baseline(args, gpu_iters, gpu_col_steps, A_dims):
+def host_info():
+    """The CPU the baseline runs on: lscpu's model, sockets, cores per socket and
+    threads per core, and the CPUs this process may use (its affinity set)."""
+    info = {"nproc_affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for line in out.splitlines():
+            name, _, val = line.partition(":")
+            if name.strip() in keys:
+                info[keys[name.strip()]] = val.strip()
+    except (OSError, ValueError) as e:
+        info["lscpu_error"] = repr(e)
+    return info
+
+
+def baseline_threads(info):
+    """Physical cores among the CPUs this process may use: the affinity set
+    divided by the threads per core.  A host that sets OMP_NUM_THREADS for the
+    GPU's share of its cores caps it there."""
+    try:
+        tpc = max(1, int(info.get("threads_per_core", "1")))
+    except ValueError:
+        tpc = 1
+    cores = max(1, info["nproc_affinity"] // tpc)
+    env = info.get("omp_num_threads_env")
+    if env and env.isdigit() and int(env) > 0:
+        cores = min(cores, int(env))
+    return cores
+
+
+def cpu_baseline(args, gpu_iters, gpu_col_steps, A_dims, reps=3):
     """FP64 oracle on the same operator and start block: one outer iteration at
-    two filter degrees (bounded sample), extrapolated to the GPU solve: the
-    per-iteration remainder times its iterations plus the cost of a k-column
-    filter step times its filtered columns / k (soft locking narrows the later
-    sweeps)."""
+    two filter degrees (bounded sample, each timed `reps` times, the median
+    taken), extrapolated to the GPU solve: the per-iteration remainder times its
+    iterations plus the cost of a k-column filter step times its filtered
+    columns / k (soft locking narrows the later sweeps)."""
     import numpy as np
 
+    info = host_info()
+    cores = baseline_threads(info)
+    os.environ["OMP_NUM_THREADS"] = str(cores)
     from oracle import oracle as O
-    cores = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
     dim, N = A_dims
     t0 = time.time()
     Ah = O.stencil(dim, N)
     X0 = O.randn(args.seed, Ah.n * args.k).reshape(Ah.n, args.k)  # row-major draws
     X0 = np.asfortranarray(X0)
     t_build = time.time() - t0
-    times = {}
-    for deg in (2, 4):
-        t = time.time()
-        O.filtered_ofrr(Ah, X0, args.nwant, nd=min(args.nd, 1 if deg < 4 else args.nd), degree=deg,
-                        maxiter=1, tol=1e-30, tau=args.tau)
-        times[deg] = time.time() - t
-        log(f"cpu baseline degree {deg}: {times[deg]:.1f} s")
+    samples = {2: [], 4: []}
+    for r in range(reps):
+        for deg in (2, 4):
+            t = time.time()
+            O.filtered_ofrr(Ah, X0, args.nwant, nd=min(args.nd, 1 if deg < 4 else args.nd),
+                            degree=deg, maxiter=1, tol=1e-30, tau=args.tau)
+            samples[deg].append(time.time() - t)
+            log(f"cpu baseline rep {r} degree {deg}: {samples[deg][-1]:.1f} s")
+    times = {d: float(np.median(v)) for d, v in samples.items()}
     step = max(0.0, (times[4] - times[2]) / 2.0)
     fixed = max(0.0, times[2] - step)          # initial RR + 1 iteration at degree 2 - 1 step
     col_steps = gpu_col_steps / float(args.k)
@@ -122,14 +159,60 @@ def cpu_baseline(args, gpu_iters, gpu_col_steps, A_dims):
         "unit": "eigenpairs/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"FP64 oracle (oracle/ofrr_oracle.c, same algorithm and start block) on the "
-                   f"full {N}^3 operator: one outer iteration at filter degree 2 and 4 "
+        "host": info,
+        "samples_s": {str(d): v for d, v in samples.items()},
+        "sample": (f"FP64 oracle (oracle/ofrr_oracle.c, OpenMP, {cores} threads = the physical "
+                   f"cores of this process's CPU set, capped by OMP_NUM_THREADS; same algorithm "
+                   f"and start block, Jacobi Rayleigh-Ritz) on the full {N}^3 operator: one outer "
+                   f"iteration at filter degree 2 and 4, median of {reps} runs each "
                    f"({times[2]:.1f} s, {times[4]:.1f} s) -> {step:.2f} s per filter step, "
                    f"{fixed:.2f} s per-iteration remainder; solve time = remainder x the GPU's "
                    f"{gpu_iters} iterations + step x its {col_steps:.1f} k-column-equivalent filter "
-                   f"steps = {solve_s:.0f} s; "
-                   f"operator build {t_build:.1f} s excluded"),
+                   f"steps = {solve_s:.0f} s; operator build {t_build:.1f} s excluded"),
     }
+
+
+def arnoldi_small(api, ctx, reps=5):
+    """The plain FP64 OFRR case on the 2-D 64 x 64 Dirichlet Laplacian (n = 4096):
+    Arnoldi with the Hessenberg-process basis, basis dimension 60, the 20 largest
+    eigenvalues, N(0,1) start vector from seed 0, three restart cycles
+    (maxiter 181).  The GPU path (ofrr_arnoldi_hess_eig) and the CPU oracle run
+    the same operations; each is timed `reps` times (median), and their Ritz
+    values and residuals are reported side by side."""
+    import numpy as np
+    import torch
+
+    from oracle import oracle as O
+    N, k, kdim, maxiter = 64, 20, 60, 1 + 60 * 3
+    Ah = O.stencil(2, N)
+    A = api.CSRMatrix.stencil(ctx, 2, N)
+    v0 = torch.empty(Ah.n, dtype=torch.float64, device="cuda:0")
+    api.fill_randn(ctx, v0, seed=0)
+    v0h = v0.cpu().numpy()
+    exact = laplacian_top(2, N, k)
+    gpu_ms, cpu_ms = [], []
+    for _ in range(reps + 1):  # the first GPU run is a warmup
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        th, X, built = api.arnoldi_hess_eig(ctx, A, k, kdim, maxiter, v0)
+        torch.cuda.synchronize()
+        gpu_ms.append((time.perf_counter() - t) * 1e3)
+    gpu_ms = gpu_ms[1:]
+    for _ in range(reps):
+        t = time.perf_counter()
+        th_o, X_o, built_o = O.arnoldi_hess_eig(Ah, k, kdim, maxiter, v0h)
+        cpu_ms.append((time.perf_counter() - t) * 1e3)
+    res = O.residuals(Ah, X.cpu().numpy(), th, exact[0])
+    res_o = O.residuals(Ah, X_o, th_o, exact[0])
+    return {"workload": "plain FP64 OFRR (Hessenberg-process Arnoldi), 2-D 5-point Dirichlet Laplacian "
+                        "64 x 64 (n = 4096), basis dimension 60, 20 largest, N(0,1) start from seed 0, "
+                        "maxiter 181",
+            "gpu_ms_median": float(np.median(gpu_ms)), "cpu_ms_median": float(np.median(cpu_ms)),
+            "cpu_threads": int(os.environ.get("OMP_NUM_THREADS", "1")), "reps": reps,
+            "gpu_converged": int(np.sum(res <= 1e-8)), "cpu_converged": int(np.sum(res_o <= 1e-8)),
+            "gpu_max_residual": float(np.max(res)), "cpu_max_residual": float(np.max(res_o)),
+            "max_ritz_rel_diff_gpu_vs_cpu": float(np.max(np.abs(th - th_o) / np.abs(th_o))),
+            "max_eig_rel_err_vs_closed_form": float(np.max(np.abs(th - exact) / exact))}
 
 
 def main():
@@ -169,6 +252,9 @@ def main():
     p.add_argument("--qr-dtypes", default="f64,f32,f16")
     p.add_argument("--svd-m", type=int, default=262144)
     p.add_argument("--svd-n", type=int, default=4096)
+    p.add_argument("--svd-k", type=int, default=96,
+                   help="svd workload: Ritz block width (64 wanted + guard columns); the "
+                        "basis is 2 k (one correction block)")
     p.add_argument("--copy-probe", action="store_true",
                    help="run a plain 1 GiB device copy first (calibrates HBM byte counters)")
     args = p.parse_args()
@@ -281,6 +367,13 @@ def main():
         except Exception as e:  # reported, never silently replaced
             cpu = {"value": None, "unit": "eigenpairs/s", "cores": 0, "kind": "port",
                    "sample": f"failed: {e!r}"}
+    # ---- the plain FP64 Arnoldi case (64 x 64 Laplacian): GPU and CPU timings
+    small = None
+    if not args.no_cpu:
+        try:
+            small = arnoldi_small(api, ctx)
+        except Exception as e:
+            small = {"error": repr(e)}
 
     out = {
         "metric": METRIC,
@@ -321,6 +414,7 @@ def main():
                      "traffic_source": "profiles/cheb_step_f16_laplacian128_k64.md",
                      "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_launch_ms * 1e3},
         "cpu_baseline": cpu,
+        "arnoldi_64x64": small,
     }
     print(json.dumps(out), flush=True)
 
@@ -450,7 +544,7 @@ def svd_main(args):
     dev = "cuda:0"
     ctx = api.Context(0)
     M, N, nwant = args.svd_m, args.svd_n, 64
-    k = 128
+    k = args.svd_k
     log(f"building the {M} x {N} test matrix on the GPU")
     with torch.no_grad():
         g = torch.Generator(device=dev)
@@ -488,10 +582,14 @@ def svd_main(args):
     elapsed = time.perf_counter() - t0
     value = pairs / elapsed
     # FP64-basis path of the same algorithm
-    ts = time.perf_counter()
-    st64, sv64, _, _, res64, stats64 = solve(torch.float64)
-    torch.cuda.synchronize()
-    t64 = time.perf_counter() - ts
+    fp64 = None
+    if not args.no_fp64:
+        ts = time.perf_counter()
+        st64, sv64, _, _, res64, stats64 = solve(torch.float64)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - ts
+        fp64 = {"value": int(np.sum(res64[:nwant] <= args.tol)) / t64, "ms_per_solve": t64 * 1e3,
+                "iterations": stats64["iterations"], "status": st64}
     exact = np.exp(-np.arange(nwant) / 64.0)
     out = {
         "metric": "converged singular triplets/sec at resid<=1e-8 (BF16 growth vs FP64 GPU)",
@@ -509,8 +607,7 @@ def svd_main(args):
                   "max_sigma_rel_err_vs_construction": float(np.max(np.abs(sv[:nwant] - exact) / exact)),
                   "t_filter_ms": stats["t_filter_ms"], "t_rr_ms": stats["t_rr_ms"],
                   "t_resid_ms": stats["t_resid_ms"], "t_gram_ms": stats["t_gram_ms"]},
-        "fp64_gpu": {"value": int(np.sum(res64[:nwant] <= args.tol)) / t64, "ms_per_solve": t64 * 1e3,
-                     "iterations": stats64["iterations"], "status": st64},
+        "fp64_gpu": fp64,
     }
     print(json.dumps(out), flush=True)
 

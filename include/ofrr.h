@@ -499,10 +499,10 @@ typedef struct {
    double tol;         /* ||A x - theta x|| / (||A|| ||x||) */
    int degree;         /* filter degree per outer iteration (0: by size) */
    int maxiter;        /* outer iterations (0: 200) */
-   int k;              /* Ritz block width, nwant..128 (0: the smallest power of two
-                          >= 1.25 nwant, at least 16) */
+   int k;              /* Ritz block width, nwant..128 (0: the smallest multiple of 16
+                          >= 1.25 nwant, at least 16; stored padded to a power of two) */
    int nd;             /* correction blocks, 1 or 2 (0: 2 while the basis dimension
-                          k (1 + nd) stays <= 256, else 1); basis dimension <= 384 */
+                          k (1 + nd) stays <= 192, else 1); basis dimension <= 384 */
    int refine;         /* precision ladder: 0 default (FP8 growth finishes in FP16 below
                           1e-4), 1 use refine_dtype / refine_tol, -1 off */
    int refine_dtype;   /* refinement format when refine = 1 (OFRR_F64: the growth

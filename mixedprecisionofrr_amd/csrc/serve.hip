@@ -45,10 +45,12 @@ struct Plan {
 
 inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// Block width (unless the request fixes it): the smallest power of two >=
-// 1.25 nwant (at least 16, at most 128), so the block carries a margin of
-// unwanted directions beyond the wanted edge; two correction blocks while the
-// basis dimension stays <= 256, else one.
+// Block width (unless the request fixes it): the smallest multiple of 16 >=
+// 1.25 nwant (at least 16, at most 128, and >= nwant), so the block carries a
+// margin of unwanted directions beyond the wanted edge; a width that is not a
+// power of two is stored padded to the next one.  Two correction blocks while
+// the basis dimension k (1 + nd) stays <= 192, else one: up to 192 the
+// Rayleigh-Ritz eigensolves keep their matrix in LDS.
 int plan_request(const ofrr_eig_request &r, Plan &p) {
    if ((r.dim != 2 && r.dim != 3) || r.N < 4 || r.nwant < 1 || r.nwant > OFRR_SERVE_MAX_WANT ||
        !dtype_valid(r.basis_dtype) || !(r.tol > 0.0) || r.degree < 0 || r.maxiter < 0)
@@ -65,12 +67,11 @@ int plan_request(const ofrr_eig_request &r, Plan &p) {
       p.k = r.k;
    } else {
       const int want = (5 * r.nwant + 3) / 4;
-      p.k = 16;
-      while (p.k < want && p.k < 128) p.k *= 2;
+      p.k = std::max(r.nwant, std::min(128, std::max(16, (want + 15) / 16 * 16)));
    }
    if (p.k > p.n) return kErrorInvalidArgument;
    p.mid_cols = (r.mid_cols > 0 && r.mid_cols < p.k) ? r.mid_cols : 0;
-   p.nd = r.nd > 0 ? r.nd : ((3 * p.k <= 256 || p.mid_cols) ? 2 : 1);
+   p.nd = r.nd > 0 ? r.nd : ((3 * p.k <= 192 || p.mid_cols) ? 2 : 1);
    if (p.k * p.nd + (p.nd == 2 && p.mid_cols ? p.mid_cols : p.k) > 384) return kErrorInvalidArgument;
    // precision ladder: FP8 corrections gain at most ~2^4 per sweep, so by
    // default an FP8 solve finishes in FP16 once the wanted residuals are below

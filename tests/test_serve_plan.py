@@ -16,12 +16,16 @@ def _state_bytes(n, k, nd, degree, bdt):
 
 
 @pytest.mark.parametrize("nwant,k,nd", [(1, 16, 2), (8, 16, 2), (13, 32, 2), (16, 32, 2),
-                                        (32, 64, 2), (50, 64, 2), (52, 128, 1), (64, 128, 1),
+                                        (32, 48, 2), (50, 64, 2), (52, 80, 1), (64, 80, 1),
                                         (100, 128, 1), (128, 128, 1)])
 def test_block_width_and_corrections(nwant, k, nd):
+    """k: the smallest multiple of 16 >= 1.25 nwant (16..128, >= nwant); two
+    correction blocks while the basis stays <= 192 (the LDS-resident eigensolves)."""
     kk, ndd, _ = api.request_footprint(api.request(3, 64, nwant))
     assert (kk, ndd) == (k, nd)
     assert kk * (1 + ndd) <= 256
+    if kk <= 64:
+        assert kk * (1 + ndd) <= 192
 
 
 def test_footprint_covers_operator_block_and_state():
