@@ -327,6 +327,11 @@ typedef struct {
                         (guard columns) at a given basis dimension, e.g. k = 128,
                         w = 44 for 100 pairs in a basis of 300.  Every Ritz column
                         still gets its full correction block D_d. */
+   int graph;        /* CSR operator only.  1: each filter sweep (its degree - 1
+                        Chebyshev steps) is recorded once as a HIP graph and replayed
+                        while its shape repeats (the context keeps the graphs; a sweep
+                        of the same shape on other buffers updates the graph's
+                        arguments); results are the same as with plain launches */
 } ofrr_filter_opts;
 
 typedef struct {
@@ -345,6 +350,8 @@ typedef struct {
    double t_resid_ms;    /* FP64 operator application + residual (incl. host checks) */
    int filter_col_steps; /* sum over the Chebyshev steps of the columns filtered */
    int filter_first_cols;/* sum over the sweeps of the columns of the first step */
+   int graph_replays;    /* sweeps launched as one captured graph (opts.graph) */
+   int graph_captures;   /* of those, sweeps recorded anew (first of a shape, or new buffers) */
 } ofrr_filter_stats;
 
 /* X_dev: n x k FP64 row-major (row stride k), start block in / Ritz block out;
@@ -486,6 +493,8 @@ typedef struct {
    int slots;          /* concurrent solves (0: 4) */
    int64_t budget;     /* device bytes the running solves may hold together
                           (0: 90% of the free device memory at creation) */
+   int graphs;         /* filter sweeps as captured HIP graphs (ofrr_filter_opts.graph),
+                          kept per slot and shape across requests: 0 or 1 on, -1 off */
 } ofrr_server_opts;
 
 #define OFRR_SERVE_MAX_WANT 128

@@ -40,7 +40,7 @@ class FilterOpts(C.Structure):
                 ("maxiter", C.c_int), ("basis_dtype", C.c_int), ("tol", C.c_double),
                 ("tau", C.c_double), ("lower", C.c_double), ("normA", C.c_double),
                 ("refine", C.c_int), ("refine_dtype", C.c_int), ("refine_tol", C.c_double),
-                ("lock", C.c_int), ("mid_cols", C.c_int)]
+                ("lock", C.c_int), ("mid_cols", C.c_int), ("graph", C.c_int)]
 
 
 class FilterStats(C.Structure):
@@ -49,11 +49,12 @@ class FilterStats(C.Structure):
                 ("t_total_ms", C.c_double), ("t_filter_ms", C.c_double),
                 ("t_zspmm_ms", C.c_double), ("t_gram_ms", C.c_double), ("t_rr_ms", C.c_double),
                 ("t_update_ms", C.c_double), ("t_resid_ms", C.c_double),
-                ("filter_col_steps", C.c_int), ("filter_first_cols", C.c_int)]
+                ("filter_col_steps", C.c_int), ("filter_first_cols", C.c_int),
+                ("graph_replays", C.c_int), ("graph_captures", C.c_int)]
 
 
 class ServerOpts(C.Structure):
-    _fields_ = [("slots", C.c_int), ("budget", C.c_int64)]
+    _fields_ = [("slots", C.c_int), ("budget", C.c_int64), ("graphs", C.c_int)]
 
 
 class EigRequest(C.Structure):

@@ -498,10 +498,12 @@ def subspace_iter_eig(ctx, A, V0, maxiter, step_size=1, select_tol=2.0 ** -52, g
 
 def filtered_eig(ctx, A, X0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=torch.float16,
                  tol=1e-8, tau=1e-12, lower=0.0, normA=0.0, use_lo_values=True,
-                 refine_dtype=None, refine_tol=0.0, lock=True, mid_cols=0):
+                 refine_dtype=None, refine_tol=0.0, lock=True, mid_cols=0, graph=False):
     """Filtered OFRR solve; X0 (n x k, FP64) is copied.  refine_dtype: the
     precision ladder's refinement format (None: off).  lock: soft locking (a
     sweep filters only the columns after the leading converged wanted pairs).
+    graph: each filter sweep runs as a captured HIP graph, replayed while its
+    shape repeats (same results as plain launches).
     Returns (status, theta, X row-major, res, stats dict)."""
     A.require_square("filtered_eig")
     _check_rows(X0, A.n, "filtered_eig", "X0")
@@ -511,7 +513,7 @@ def filtered_eig(ctx, A, X0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=to
     o = FilterOpts(k, nwant, nd, degree, maxiter, bdt, tol, tau, lower, normA,
                    0 if refine_dtype is None else 1,
                    0 if refine_dtype is None else dtype_code(refine_dtype), refine_tol,
-                   1 if lock else 0, mid_cols)
+                   1 if lock else 0, mid_cols, 1 if graph else 0)
     th = (C.c_double * k)()
     res = (C.c_double * k)()
     stats = FilterStats()
@@ -737,10 +739,11 @@ class Server:
     solves run at once on their own streams, admitted in arrival order against
     a device-byte budget (0: 90% of free memory)."""
 
-    def __init__(self, device=0, slots=4, budget=0):
+    def __init__(self, device=0, slots=4, budget=0, graphs=True):
         self.lib = _lib.load()
         h = C.c_void_p()
-        st = self.lib.ofrr_server_create(device, C.byref(_lib.ServerOpts(slots, budget)),
+        st = self.lib.ofrr_server_create(device, C.byref(_lib.ServerOpts(slots, budget,
+                                                                         1 if graphs else -1)),
                                          C.byref(h))
         if st != 0:
             raise OfrrError(st, f"ofrr_server_create: {self.lib.ofrr_status_string(st).decode()}")

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/ofrr.h"
 
@@ -52,6 +53,19 @@ struct ofrr_ctx {
    // solve (the serving path); null: the grow-only per-thread arena
    void *state_mem = nullptr;
    size_t state_bytes = 0;
+   // captured filter sweeps, one executable graph per sweep shape (size, width,
+   // degree, formats); a later sweep of the same shape on other buffers (the
+   // next request of a serving slot) updates the graph's kernel arguments
+   // in place instead of instantiating it again
+   struct SweepGraph {
+      uint64_t key = 0, sig = 0, used = 0;
+      hipGraphExec_t exec = nullptr;
+   };
+   std::vector<SweepGraph> graphs;
+   uint64_t graph_clock = 0;
+   // a graph cannot be recorded on the null stream: such a context replays
+   // on own_stream, ordered with the caller's stream by these two events
+   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
    std::string last_error;
 };
 

@@ -123,6 +123,10 @@ int ofrr_ctx_destroy(ofrr_ctx *c) {
    if (!c) return kSuccess;
    (void)hipSetDevice(c->device);
    (void)hipStreamSynchronize(c->stream);
+   for (auto &g : c->graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
    if (c->ws) (void)hipFree(c->ws);
    if (c->pinned) (void)hipHostFree(c->pinned);
    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);

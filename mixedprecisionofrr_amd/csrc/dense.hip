@@ -396,10 +396,12 @@ __global__ __launch_bounds__(256) void cheb_combine_kernel(
    const int c = threadIdx.x % k, rstep = 256 / k;
    const float mc = ca_.maxc[c];
    const int se = (mc > 0.f) ? ceil_log2_2m_d(mc) - ca_.target_exp : 0;
-   const ACC ca = (ACC)ldexp(ca_.alpha * ca_.sc[c], -se);
-   const ACC cg = (ACC)(Dp ? ldexp(-ca_.beta * ca_.sp[c], -se) : 0.0);
+   const double alpha = ca_.step ? ca_.step[0] : ca_.alpha;
+   const double beta = ca_.step ? ca_.step[1] : ca_.beta;
+   const ACC ca = (ACC)ldexp(alpha * ca_.sc[c], -se);
+   const ACC cg = (ACC)(Dp ? ldexp(-beta * ca_.sp[c], -se) : 0.0);
    const ACC cd = (ACC)ldexp(ca_.w[c] * ca_.sR[c], -se);
-   const ACC cshift = (ACC)ca_.c;
+   const ACC cshift = (ACC)(ca_.step ? ca_.step[2] : ca_.c);
    if (blockIdx.x == 0 && threadIdx.x < k) {
       ca_.sn_out[c] = ldexp(1.0, se);
       if (ca_.zero_next) ca_.zero_next[c] = 0.f;

@@ -25,6 +25,8 @@ int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, c
 // sn = 2^(ceil(log2(2 max|Dc|)) - target_exp) written to sn_out
 struct ChebArgs {
    double alpha, beta, c;     // step scalars and the centre of the damped interval
+   const double *step;        // nullable: the same three scalars in device memory (a
+                              // captured sweep replays with new values)
    const double *w;           // per-column weight of the residual block (k values)
    const double *sR, *sc, *sp;// scales of R, Dc, Dp (sp may be null: no Dp)
    const float *maxc;         // max |D_j| per column

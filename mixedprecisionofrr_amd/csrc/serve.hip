@@ -153,6 +153,7 @@ struct ofrr_server {
    // admission counts request memory against budget - slot_bytes
    int64_t slot_bytes = 0;
    int64_t ws_regrowths = 0;  // solves whose workspace outgrew the slot's reservation
+   bool graphs = true;        // filter sweeps as captured graphs, kept per slot and shape
 
    double now() const {
       return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -207,6 +208,7 @@ void ofrr_server::run_one(ofrr_ctx *ctx, Rec &r) {
       o.refine_dtype = p.refine_dtype;
       o.refine_tol = p.refine_tol;
       o.mid_cols = p.mid_cols;
+      o.graph = graphs ? 1 : 0;
       std::vector<double> th(p.k), rs(p.k);
       ofrr_filter_stats stats{};
       ctx->state_mem = mem + p.off_state;
@@ -317,6 +319,7 @@ int ofrr_server_create(int device, const ofrr_server_opts *opt, ofrr_server **ou
    if (hipSetDevice(device) != hipSuccess) return kErrorCUDAGeneral;
    ofrr_server *s = new ofrr_server();
    s->device = device;
+   s->graphs = !(opt && opt->graphs < 0);
    s->t0 = std::chrono::steady_clock::now();
    if (opt && opt->budget > 0) {
       s->budget = opt->budget;

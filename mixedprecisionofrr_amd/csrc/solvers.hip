@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "internal.hpp"
@@ -677,6 +678,10 @@ struct FilterOp {
    virtual bool can_lock() const { return false; }
    // the narrowest block storage the operator's products take
    virtual int min_width() const { return 1; }
+   // whether a sweep's launches may be recorded as a graph, and what of the
+   // operator a recorded sweep depends on (its buffers and value format)
+   virtual bool can_graph() const { return false; }
+   virtual void sweep_sig(std::vector<uint64_t> &) const {}
 };
 
 struct CsrOp final : FilterOp {
@@ -719,6 +724,12 @@ struct CsrOp final : FilterOp {
    bool can_refine() const override { return true; }
    void set_precise(bool p) override { precise = p; }
    bool can_lock() const override { return true; }
+   bool can_graph() const override { return true; }
+   void sweep_sig(std::vector<uint64_t> &v) const override {
+      v.insert(v.end(), {(uint64_t)(uintptr_t)rp, (uint64_t)(uintptr_t)ci,
+                         (uint64_t)(uintptr_t)(precise ? (const void *)va : vlo),
+                         (uint64_t)(precise ? OFRR_F64 : vlo_dt), (uint64_t)bw});
+   }
 };
 
 // Y (n x wr 8-byte words, row-major) = words [c0, c0 + wr) of every row of X
@@ -902,7 +913,7 @@ struct FiltBufs {
    double *G, *H, *C;         // projected matrices (mp x mp), coefficients (mp x kp)
    double *Gc, *Hc, *Cc;      // compacted m x m, m x k (padded widths only)
    int32_t *cmap, *cinv;      // compacted -> padded basis index (m), padded -> compacted (mp)
-   double *th, *sums, *invsR, *sR, *scale[4], *sD1, *zall;
+   double *th, *sums, *invsR, *sR, *scale[4], *sD1, *zall, *steps;
    float *cmax[3], *cmaxR;
    int32_t *rank;
 };
@@ -948,6 +959,7 @@ static void filtered_layout(Carve &cv, int64_t n, const ofrr_filter_opts &o, int
    for (int i = 0; i < 4; ++i) b.scale[i] = (double *)cv.take(kq * 8);
    b.sD1 = (double *)cv.take(kq * 8);
    b.zall = (double *)cv.take((size_t)d * kq * 8);
+   b.steps = (double *)cv.take((size_t)3 * d * 8);
    for (int i = 0; i < 3; ++i) b.cmax[i] = (float *)cv.take(kq * 4);
    b.cmaxR = (float *)cv.take(kq * 4);
    b.rank = (int32_t *)cv.take(64);
@@ -1002,6 +1014,103 @@ __global__ void coef_scatter_kernel(int mp, int kp, int k, int m, const int32_t 
    C[t] = (j < k && i >= 0) ? Cc[i + (int64_t)j * m] : 0.0;
 }
 
+// 64-bit FNV-1a over a list of words: graph keys and argument signatures
+static uint64_t hash_words(const std::vector<uint64_t> &v) {
+   uint64_t h = 1469598103934665603ull;
+   for (uint64_t w : v)
+      for (int b = 0; b < 8; ++b) {
+         h ^= (w >> (8 * b)) & 0xffu;
+         h *= 1099511628211ull;
+      }
+   return h;
+}
+
+// Launch a filter sweep through the context's graph of its shape `key`:
+// replayed as is when its kernel arguments (`sig`) repeat; otherwise the
+// sweep is recorded again (enqueue() under stream capture) and either updates
+// the executable graph of that shape in place or, for a new shape, is
+// instantiated (the least recently used of kMaxSweepGraphs is dropped).
+constexpr size_t kMaxSweepGraphs = 8;
+static int sweep_graph_on(ofrr_ctx *ctx, uint64_t key, uint64_t sig, const std::function<int()> &enqueue,
+                          int &captured) {
+   captured = 0;
+   ofrr_ctx::SweepGraph *hit = nullptr;
+   for (auto &g : ctx->graphs)
+      if (g.key == key && g.exec) { hit = &g; break; }
+   if (hit && hit->sig == sig) {
+      hit->used = ++ctx->graph_clock;
+      OFRR_HIP(ctx, hipGraphLaunch(hit->exec, ctx->stream));
+      return kSuccess;
+   }
+   hipGraph_t graph = nullptr;
+   OFRR_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+   const int st = enqueue();
+   const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+   if (st || ec != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return st ? st : hip_status(ctx, ec, "filter sweep: stream capture");
+   }
+   captured = 1;
+   bool updated = false;
+   if (hit) {
+      hipGraphNode_t bad = nullptr;
+      hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+      if (hipGraphExecUpdate(hit->exec, graph, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+         updated = true;
+      } else {
+         (void)hipGetLastError();
+         (void)hipGraphExecDestroy(hit->exec);
+         hit->exec = nullptr;
+      }
+   }
+   if (!updated) {
+      hipGraphExec_t ex = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+      if (ei != hipSuccess) {
+         (void)hipGraphDestroy(graph);
+         return hip_status(ctx, ei, "filter sweep: graph instantiation");
+      }
+      if (!hit) {
+         if (ctx->graphs.size() >= kMaxSweepGraphs) {
+            auto lru = ctx->graphs.begin();
+            for (auto it = ctx->graphs.begin(); it != ctx->graphs.end(); ++it)
+               if (it->used < lru->used) lru = it;
+            if (lru->exec) (void)hipGraphExecDestroy(lru->exec);
+            ctx->graphs.erase(lru);
+         }
+         ctx->graphs.push_back(ofrr_ctx::SweepGraph{});
+         hit = &ctx->graphs.back();
+         hit->key = key;
+      }
+      hit->exec = ex;
+   }
+   (void)hipGraphDestroy(graph);
+   hit->sig = sig;
+   hit->used = ++ctx->graph_clock;
+   OFRR_HIP(ctx, hipGraphLaunch(hit->exec, ctx->stream));
+   return kSuccess;
+}
+// the same on the context's stream, or (null stream) on its own stream between
+// a fork and a join event; enqueue() launches on ctx->stream, which points to
+// the recording stream meanwhile
+static int sweep_graph(ofrr_ctx *ctx, uint64_t key, uint64_t sig, const std::function<int()> &enqueue,
+                       int &captured) {
+   hipStream_t user = ctx->stream;
+   if (user) return sweep_graph_on(ctx, key, sig, enqueue, captured);
+   if (!ctx->own_stream) return set_error(ctx, kErrorInvalidOperation, "filter sweep graph: no stream");
+   if (!ctx->ev_fork) OFRR_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+   if (!ctx->ev_join) OFRR_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+   OFRR_HIP(ctx, hipEventRecord(ctx->ev_fork, user));
+   OFRR_HIP(ctx, hipStreamWaitEvent(ctx->own_stream, ctx->ev_fork, 0));
+   ctx->stream = ctx->own_stream;
+   const int st = sweep_graph_on(ctx, key, sig, enqueue, captured);
+   ctx->stream = user;
+   if (st) return st;
+   OFRR_HIP(ctx, hipEventRecord(ctx->ev_join, ctx->own_stream));
+   OFRR_HIP(ctx, hipStreamWaitEvent(user, ctx->ev_join, 0));
+   return kSuccess;
+}
+
 static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o, double *Xio,
                          double *theta_host, double *res_host, ofrr_filter_stats *stats) {
    const int64_t n = op.n;
@@ -1048,7 +1157,8 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    int32_t *rank = fb.rank;
    double *sRd = sR;
    // pinned host mirrors (the compaction maps after the rank)
-   const size_t hbytes = (size_t)(2 * k + k + k + d * k + 4 * k + 16) * 8 + (size_t)2 * m * 4;
+   const size_t hbytes = (size_t)(2 * k + k + k + d * k + 4 * k + 16) * 8 + (size_t)2 * m * 4 +
+                         (size_t)3 * d * 8;
    double *hp = (double *)pinned(ctx, hbytes);
    if (!hp) return set_error(ctx, kErrorCUDAMemory, "filtered_eig: pinned");
    double *h_sums = hp, *h_th = hp + 2 * k, *h_sR = h_th + k, *h_z = h_sR + k;
@@ -1056,6 +1166,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    int32_t *h_rank = (int32_t *)(h_cmaxR + 2 * k);
    int32_t *h_cmap = (int32_t *)(hp + (size_t)(2 * k + k + k + d * k + 4 * k + 16));
    int32_t *h_cinv = h_cmap + m;
+   double *h_step = (double *)(h_cinv + m);  // 3 scalars per filter step
    // Rayleigh-Ritz of the basis [X | D_d (| D_mid)] with me = k + nd ka padded
    // columns, of which the correction blocks hold kar = ka - (k - kr) real
    // ones: eigensolves on the compacted matrices when the blocks are padded
@@ -1090,6 +1201,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    for (int i = 0; i < 8; ++i) OFRR_HIP(ctx, hipEventCreate(&ev[i]));
    float t_acc[7] = {0, 0, 0, 0, 0, 0, 0};
    int filter_steps = 0, resid_passes = 0, filter_col_steps = 0, filter_first_cols = 0;
+   int graph_replays = 0, graph_captures = 0;
    auto rec = [&](int i) { return hipEventRecord(ev[i], ctx->stream); };
    OFRR_HIP(ctx, rec(0));
 
@@ -1271,56 +1383,97 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       // itself, or (locked columns present) a compact copy in the second half
       // of Db[0], whose first half is all a step writes (n ka <= n k / 2)
       const void *Ra = Rq;
-      if (c > 0) {
-         const int es = (int)dtype_size(bdt);
-         const int wr = ka * es / 8;
-         void *dst = (char *)Db[0] + (size_t)n * ka * es;
-         const unsigned g = (unsigned)std::min<int64_t>((n * wr + 255) / 256, 65536);
-         copy_cols_kernel<<<g, 256, 0, ctx->stream>>>(n, wr, (int64_t)k * es / 8, (int64_t)c * es / 8,
-                                                      (const uint64_t *)Rq, (uint64_t *)dst);
-         OFRR_LAUNCH(ctx);
-         Ra = dst;
-      }
-      // recurrence state: current / previous stored block and their scales;
-      // Db[b] has scale scale[b]; D_1 lives in Rq with scale sD1
-      const void *Dc = Ra, *Dp = nullptr;
-      double *sc = sD1, *sp = nullptr;
-      int curIdx = -1, prevIdx = -1, snap = -1;
+      const int es = (int)dtype_size(bdt);
+      if (c > 0) Ra = (char *)Db[0] + (size_t)n * ka * es;
+      // The buffers of every step (host bookkeeping): the recurrence keeps the
+      // current and previous stored blocks and their scales; Db[b] has scale
+      // scale[b]; D_1 lives in Ra with scale sD1.  The column maxima rotate
+      // over three buffers: step s reads cmax[(s-1)%3], accumulates into
+      // cmax[s%3] (cleared by step s-1, or before the sweep for s = 1) and
+      // clears cmax[(s+1)%3] for step s+1.
+      struct StepBufs { int out; const void *Dc, *Dp; double *sc, *sp; };
+      std::vector<StepBufs> sb(d);
+      const void *Dc = Ra;
       const void *Dmid = (nd == 2 && mid == 1) ? Ra : nullptr;
-      // the column maxima rotate over three buffers: step s reads cmax[(s-1)%3],
-      // accumulates into cmax[s%3] (cleared by step s-1, or here for s = 1) and
-      // clears cmax[(s+1)%3] for step s+1
-      if ((st = zero_bytes(ctx, cmax[1], (size_t)k * 4))) return st;
-      for (int s = 1; s < d; ++s) {
-         int out = -1;
-         for (int b = 0; b < 4; ++b)
-            if (b != curIdx && b != prevIdx && b != snap) { out = b; break; }
-         ChebArgs ca;
-         ca.alpha = (2.0 / e) * q[s];
-         ca.beta = q[s - 1] * q[s];
-         ca.c = c0;
-         ca.w = zall + (size_t)(s - 1) * ka;
-         ca.sR = sRd + c;
-         ca.sc = sc;
-         ca.sp = sp;
-         ca.maxc = cmax[(s - 1) % 3];
-         ca.maxn = cmax[s % 3];
-         ca.zero_next = (s + 1 < d) ? cmax[(s + 1) % 3] : nullptr;
-         ca.sn_out = scale[out];
-         ca.target_exp = tgt_step;
-         if ((st = op.cheb(ctx, ka, Dc, Dp, Ra, bdt, ca, Db[out]))) return st;
-         ++spmm_calls;
-         ++filter_steps;
-         filter_col_steps += ka;
-         if (s == 1) filter_first_cols += ka;
-         prevIdx = curIdx;
-         Dp = Dc;
-         sp = sc;
-         curIdx = out;
-         Dc = Db[out];
-         sc = scale[out];
-         if (nd == 2 && s + 1 == mid) { snap = out; Dmid = Db[out]; }
+      int curIdx = -1, snap = -1;
+      {
+         const void *Dp = nullptr;
+         double *sc = sD1, *sp = nullptr;
+         int prevIdx = -1;
+         for (int s = 1; s < d; ++s) {
+            int out = -1;
+            for (int b = 0; b < 4; ++b)
+               if (b != curIdx && b != prevIdx && b != snap) { out = b; break; }
+            sb[s] = StepBufs{out, Dc, Dp, sc, sp};
+            prevIdx = curIdx;
+            Dp = Dc;
+            sp = sc;
+            curIdx = out;
+            Dc = Db[out];
+            sc = scale[out];
+            if (nd == 2 && s + 1 == mid) { snap = out; Dmid = Db[out]; }
+         }
       }
+      // step scalars in device memory (a replayed sweep reads this iteration's)
+      for (int s = 1; s < d; ++s) {
+         h_step[3 * (s - 1)] = (2.0 / e) * q[s];
+         h_step[3 * (s - 1) + 1] = q[s - 1] * q[s];
+         h_step[3 * (s - 1) + 2] = c0;
+      }
+      OFRR_HIP(ctx, hipMemcpyAsync(fb.steps, h_step, (size_t)(d - 1) * 3 * 8, hipMemcpyHostToDevice,
+                                   ctx->stream));
+      auto enqueue_sweep = [&]() -> int {
+         if (c > 0) {
+            const int wr = ka * es / 8;
+            const unsigned g = (unsigned)std::min<int64_t>((n * wr + 255) / 256, 65536);
+            copy_cols_kernel<<<g, 256, 0, ctx->stream>>>(n, wr, (int64_t)k * es / 8, (int64_t)c * es / 8,
+                                                         (const uint64_t *)Rq, (uint64_t *)Ra);
+            OFRR_LAUNCH(ctx);
+         }
+         int st2 = zero_bytes(ctx, cmax[1], (size_t)k * 4);
+         if (st2) return st2;
+         for (int s = 1; s < d; ++s) {
+            ChebArgs ca;
+            ca.alpha = h_step[3 * (s - 1)];
+            ca.beta = h_step[3 * (s - 1) + 1];
+            ca.c = c0;
+            ca.step = fb.steps + 3 * (s - 1);
+            ca.w = zall + (size_t)(s - 1) * ka;
+            ca.sR = sRd + c;
+            ca.sc = sb[s].sc;
+            ca.sp = sb[s].sp;
+            ca.maxc = cmax[(s - 1) % 3];
+            ca.maxn = cmax[s % 3];
+            ca.zero_next = (s + 1 < d) ? cmax[(s + 1) % 3] : nullptr;
+            ca.sn_out = scale[sb[s].out];
+            ca.target_exp = tgt_step;
+            if ((st2 = op.cheb(ctx, ka, sb[s].Dc, sb[s].Dp, Ra, bdt, ca, Db[sb[s].out]))) return st2;
+         }
+         return kSuccess;
+      };
+      if (o->graph > 0 && op.can_graph()) {
+         // the shape of the sweep (its kernels, grids and scalar arguments) and
+         // every buffer it touches
+         std::vector<uint64_t> kv = {(uint64_t)n, (uint64_t)k, (uint64_t)ka, (uint64_t)c, (uint64_t)d,
+                                     (uint64_t)bdt, (uint64_t)tgt_step, (uint64_t)nd, (uint64_t)mid};
+         std::vector<uint64_t> sv;
+         op.sweep_sig(sv);
+         kv.push_back(sv.size() > 3 ? sv[3] : 0);  // the operator's value format
+         const void *bufs[] = {Rq, Ra, Db[0], Db[1], Db[2], Db[3], zall, sRd, sD1, fb.steps};
+         for (const void *ptr : bufs) sv.push_back((uint64_t)(uintptr_t)ptr);
+         for (int i = 0; i < 4; ++i) sv.push_back((uint64_t)(uintptr_t)scale[i]);
+         for (int i = 0; i < 3; ++i) sv.push_back((uint64_t)(uintptr_t)cmax[i]);
+         int captured = 0;
+         if ((st = sweep_graph(ctx, hash_words(kv), hash_words(sv), enqueue_sweep, captured))) return st;
+         ++graph_replays;
+         graph_captures += captured;
+      } else if ((st = enqueue_sweep())) {
+         return st;
+      }
+      spmm_calls += d - 1;
+      filter_steps += d - 1;
+      filter_col_steps += ka * (d - 1);
+      filter_first_cols += ka;
       OFRR_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
       // ---- projection: V = [X | Dd (| Dmid)], AV = [W | Z]
       const void *Dd = Dc;
@@ -1461,6 +1614,8 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       stats->filter_steps = filter_steps;
       stats->filter_col_steps = filter_col_steps;
       stats->filter_first_cols = filter_first_cols;
+      stats->graph_replays = graph_replays;
+      stats->graph_captures = graph_captures;
       stats->resid_passes = resid_passes;
       stats->t_total_ms = total;
       stats->t_filter_ms = t_acc[0];

@@ -371,12 +371,14 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
    // scale 2^-se folded in (exact: a power of two):
    // T / s = (alpha sc / s)(A Dc - c Dc) + (-beta sp / s) Dp + (w sR / s) R
    // They live in LDS and are re-read per row (in registers they cost occupancy).
+   const double alpha = ca_.step ? ca_.step[0] : ca_.alpha;
+   const double beta = ca_.step ? ca_.step[1] : ca_.beta;
    for (int j = threadIdx.x; j < k; j += 256) {
       const float m = ca_.maxc[j];
       // stored scale s = 2^(ceil(log2(2 max|T|)) - target): the stored maximum lands in [2^(t-1), 2^t]
       const int se = (m > 0.f) ? ceil_log2_2m(m) - ca_.target_exp : 0;
-      s_coef[0][j] = (ACC)ldexp(ca_.alpha * ca_.sc[j], -se);
-      s_coef[1][j] = (ACC)(HAS_DP ? ldexp(-ca_.beta * ca_.sp[j], -se) : 0.0);
+      s_coef[0][j] = (ACC)ldexp(alpha * ca_.sc[j], -se);
+      s_coef[1][j] = (ACC)(HAS_DP ? ldexp(-beta * ca_.sp[j], -se) : 0.0);
       s_coef[2][j] = (ACC)ldexp(ca_.w[j] * ca_.sR[j], -se);
       s_se[j] = se;
       if (blk == 0) {
@@ -387,7 +389,7 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
    float mx[VEC];
 #pragma unroll
    for (int v = 0; v < VEC; ++v) mx[v] = 0.f;
-   const ACC cshift = (ACC)ca_.c;
+   const ACC cshift = (ACC)(ca_.step ? ca_.step[2] : ca_.c);
    __syncthreads();
    if (in_lds)
       cheb_rows<TV, TB, VEC, ACC, HAS_DP, true>(r0, nr, grp, GROUPS, c0, k, tile.ci, tile.va,

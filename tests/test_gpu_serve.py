@@ -30,8 +30,8 @@ def _requests(api):
                         tol=1e-8) for i, (d, N, w, s) in enumerate(MIX)]
 
 
-def _run(api, reqs, slots, budget=0):
-    with api.Server(0, slots=slots, budget=budget) as srv:
+def _run(api, reqs, slots, budget=0, graphs=True):
+    with api.Server(0, slots=slots, budget=budget, graphs=graphs) as srv:
         if slots == 1:
             out = []
             for r in reqs:  # strictly one after the other
@@ -142,3 +142,18 @@ def test_serve_fp8_basis300_beside_a_mix(api):
     assert st["running_peak"] > 1 and st["completed"] == 9
     for o in mix:
         assert o["status"] == 0
+
+
+def test_serve_graph_sweeps_match_plain_launches(api):
+    """The slots' filter sweeps run as captured graphs kept per slot and shape
+    (re-recorded into the same executable graphs for the next request of a
+    shape); the same stream with plain launches gives bitwise equal results."""
+    reqs = _requests(api)[:6]
+    print("serve: graphs on", flush=True)
+    on, _ = _run(api, reqs, 2)
+    print("serve: graphs off", flush=True)
+    off, _ = _run(api, _requests(api)[:6], 2, graphs=False)
+    for a, b in zip(on, off):
+        assert a["status"] == 0 and b["status"] == 0
+        assert a["iterations"] == b["iterations"]
+        assert np.array_equal(a["theta"], b["theta"]) and np.array_equal(a["res"], b["res"])

@@ -289,3 +289,46 @@ def test_filtered_fp8_precision_ladder_reaches_1e8(ctx, refine):
     assert np.max(np.abs(th[:nwant] - exact) / exact) < 1e-11
     res_o = O.residuals(O.stencil(3, N), X.cpu().numpy(), th, exact[0])
     assert np.max(res_o[:nwant]) < 2e-8
+
+
+@pytest.mark.parametrize("case", ["f16_locking", "fp8_ladder", "f16_padded"])
+def test_filtered_graph_sweeps_match_plain_launches(ctx, case):
+    """Filter sweeps recorded as HIP graphs and replayed give bitwise the same
+    solve as plain launches: every iteration replays (or re-records) a sweep
+    between the solver's own eager work and host synchronisations, so any
+    replay that differed would change the Ritz values, residuals or block.
+    Soft locking narrows the sweeps (new shapes), the FP8 ladder switches the
+    format, and a padded width compacts the Rayleigh-Ritz.  A second graph
+    solve on new buffers re-records through the context's graphs of the same
+    shapes and matches too."""
+    from mixedprecisionofrr_amd import api
+    N, k, nwant, deg, kw = 28, 64, 40, 40, {}
+    bdt = torch.float16
+    if case == "fp8_ladder":
+        N, k, nwant, deg, bdt = 40, 32, 20, 60, torch.float8_e4m3fn
+        kw = dict(refine_dtype=torch.float16, refine_tol=1e-4)
+    if case == "f16_padded":
+        k, nwant = 48, 36
+    A = api.CSRMatrix.stencil(ctx, 3, N, seed=4).with_low_precision(torch.float32)
+    X0 = torch.empty(A.n, k, dtype=torch.float64, device="cuda:0")
+    api.fill_randn(ctx, X0, seed=2)
+    print(f"graph sweeps vs plain: {case}", flush=True)
+    args = dict(nd=2, degree=deg, maxiter=200, basis_dtype=bdt, tol=1e-8, **kw)
+    plain = api.filtered_eig(ctx, A, X0, nwant, graph=False, **args)
+    g1 = api.filtered_eig(ctx, A, X0, nwant, graph=True, **args)
+    g2 = api.filtered_eig(ctx, A, X0.clone(), nwant, graph=True, **args)
+    assert plain[0] == 0, plain[4]
+    assert plain[4]["graph_replays"] == 0
+    for g in (g1, g2):
+        assert g[0] == plain[0] and g[4]["iterations"] == plain[4]["iterations"]
+        assert np.array_equal(g[1], plain[1]) and np.array_equal(g[3], plain[3])
+        assert torch.equal(g[2], plain[2])
+        st = g[4]
+        # one graph launch per sweep; fewer recordings than sweeps
+        assert st["graph_replays"] == st["iterations"]
+        assert st["graph_captures"] < st["graph_replays"]
+        assert st["filter_steps"] == plain[4]["filter_steps"]
+    # the first graph solve records each shape; the second finds them in the
+    # context (same shapes, and here the same solver buffers)
+    assert g1[4]["graph_captures"] >= 1
+    assert g2[4]["graph_captures"] <= g1[4]["graph_captures"]
