@@ -495,6 +495,11 @@ typedef struct {
                           (0: 90% of the free device memory at creation) */
    int graphs;         /* filter sweeps as captured HIP graphs (ofrr_filter_opts.graph),
                           kept per slot and shape across requests: 0 or 1 on, -1 off */
+   int rr_group_us;    /* grouped Rayleigh-Ritz: the truncated Rayleigh-Ritz jobs that
+                          running solves hand in within this many microseconds of each
+                          other run as one batched launch per stage on the server's own
+                          stream (each job's result is the one it gets alone); 0: 200,
+                          -1: every solve runs its own */
 } ofrr_server_opts;
 
 #define OFRR_SERVE_MAX_WANT 128
@@ -541,6 +546,8 @@ typedef struct {
                                          budget - slot_bytes) */
    int64_t ws_regrowths;              /* solves whose workspace outgrew the slot's
                                          reservation (counted into slot_bytes) */
+   int64_t rr_groups, rr_group_jobs;  /* grouped Rayleigh-Ritz launches and the jobs in them */
+   int rr_group_max;                  /* most jobs in one group */
 } ofrr_server_stats;
 
 /* the plan of a request (block width k, correction blocks nd) and its device

@@ -54,7 +54,8 @@ class FilterStats(C.Structure):
 
 
 class ServerOpts(C.Structure):
-    _fields_ = [("slots", C.c_int), ("budget", C.c_int64), ("graphs", C.c_int)]
+    _fields_ = [("slots", C.c_int), ("budget", C.c_int64), ("graphs", C.c_int),
+                ("rr_group_us", C.c_int)]
 
 
 class EigRequest(C.Structure):
@@ -77,7 +78,8 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("running", C.c_int64),
                 ("queued", C.c_int64), ("bytes_running", C.c_int64), ("bytes_peak", C.c_int64),
                 ("running_peak", C.c_int), ("budget", C.c_int64), ("slot_bytes", C.c_int64),
-                ("ws_regrowths", C.c_int64)]
+                ("ws_regrowths", C.c_int64), ("rr_groups", C.c_int64), ("rr_group_jobs", C.c_int64),
+                ("rr_group_max", C.c_int)]
 
 
 _lib = None

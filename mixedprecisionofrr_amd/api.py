@@ -739,12 +739,13 @@ class Server:
     solves run at once on their own streams, admitted in arrival order against
     a device-byte budget (0: 90% of free memory)."""
 
-    def __init__(self, device=0, slots=4, budget=0, graphs=True):
+    def __init__(self, device=0, slots=4, budget=0, graphs=True, rr_group_us=0):
+        """graphs: filter sweeps as captured HIP graphs; rr_group_us: window of the
+        grouped Rayleigh-Ritz (0: default, -1: off)."""
         self.lib = _lib.load()
         h = C.c_void_p()
-        st = self.lib.ofrr_server_create(device, C.byref(_lib.ServerOpts(slots, budget,
-                                                                         1 if graphs else -1)),
-                                         C.byref(h))
+        opts = _lib.ServerOpts(slots, budget, 1 if graphs else -1, rr_group_us)
+        st = self.lib.ofrr_server_create(device, C.byref(opts), C.byref(h))
         if st != 0:
             raise OfrrError(st, f"ofrr_server_create: {self.lib.ofrr_status_string(st).decode()}")
         self.h = h

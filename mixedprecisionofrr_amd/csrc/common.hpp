@@ -66,6 +66,12 @@ struct ofrr_ctx {
    // a graph cannot be recorded on the null stream: such a context replays
    // on own_stream, ordered with the caller's stream by these two events
    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+   // grouped Rayleigh-Ritz (serving slots): when set, the filtered solver hands
+   // its truncated Rayleigh-Ritz job to rr_group (which orders it after
+   // ev_rr_ready on this stream and makes this stream wait for ev_rr_done)
+   int (*rr_group)(void *arg, ofrr_ctx *slot, const void *job) = nullptr;
+   void *rr_group_arg = nullptr;
+   hipEvent_t ev_rr_ready = nullptr, ev_rr_done = nullptr;
    std::string last_error;
 };
 

@@ -127,6 +127,8 @@ int ofrr_ctx_destroy(ofrr_ctx *c) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+   if (c->ev_rr_ready) (void)hipEventDestroy(c->ev_rr_ready);
+   if (c->ev_rr_done) (void)hipEventDestroy(c->ev_rr_done);
    if (c->ws) (void)hipFree(c->ws);
    if (c->pinned) (void)hipHostFree(c->pinned);
    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);

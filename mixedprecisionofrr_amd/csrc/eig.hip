@@ -575,6 +575,151 @@ int rr_truncated(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H
 }
 
 // ---------------------------------------------------------------------------
+// Grouped truncated Rayleigh-Ritz: the jobs of several solves (each its own G,
+// H, m, tau, outputs) in one launch per stage, job = blockIdx.y (or .z).  Each
+// job's work area holds the arrays of rr_truncated with leading dimension
+// mx (the largest m of the group); every job runs the arithmetic of
+// rr_truncated on its own m in the same order, so its results do not depend
+// on the other jobs of its group.
+struct RRJobArgs {
+   int n, mx;
+   int64_t slab;  // doubles per job area
+   RRJob j[kMaxRRJobs];
+};
+// job area: sc (mx) | Gs Hs U Bm T Hr (mx^2 each) | lam th (mx)
+__device__ inline double *rrj_area(double *base, const RRJobArgs &a, int b) { return base + b * a.slab; }
+
+__global__ void rrg_scale_kernel(RRJobArgs a, double *base, int32_t *mdev) {
+   const int b = blockIdx.y, m = a.j[b].m, mx = a.mx;
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t == 0) mdev[b] = m;
+   if (t >= m * m) return;
+   const int i = t % m, j = t / m;
+   const double *G = a.j[b].G, *H = a.j[b].H;
+   const int ldg = a.j[b].ldg, ldh = a.j[b].ldh;
+   double *sc = rrj_area(base, a, b), *Gs = sc + mx, *Hs = Gs + (int64_t)mx * mx;
+   double gi = G[i + (int64_t)i * ldg], gj = G[j + (int64_t)j * ldg];
+   double si = gi > 0.0 ? 1.0 / sqrt(gi) : 0.0, sj = gj > 0.0 ? 1.0 / sqrt(gj) : 0.0;
+   if (j == 0) sc[i] = si;
+   double g = 0.5 * (G[i + (int64_t)j * ldg] + G[j + (int64_t)i * ldg]);
+   double h = 0.5 * (H[i + (int64_t)j * ldh] + H[j + (int64_t)i * ldh]);
+   Gs[i + (int64_t)j * mx] = g * si * sj;
+   Hs[i + (int64_t)j * mx] = h * si * sj;
+}
+
+__global__ void rrg_trunc_kernel(RRJobArgs a, double *base, int32_t *rdev) {
+   __shared__ int s_r;
+   const int b = blockIdx.x, m = a.j[b].m, mx = a.mx;
+   const int64_t mm = (int64_t)mx * mx;
+   double *sc = rrj_area(base, a, b), *U = sc + mx + 2 * mm, *Bm = U + mm, *lam = sc + mx + 6 * mm;
+   if (threadIdx.x == 0) {
+      double lmax = lam[m - 1];
+      int r = 0;
+      for (int i = 0; i < m; ++i) r += (lam[i] > a.j[b].tau * lmax) ? 1 : 0;
+      s_r = r;
+      rdev[b] = r;
+      *a.j[b].rank = r;
+   }
+   __syncthreads();
+   const int r = s_r;
+   for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+      int i = t % m, c = t / m;
+      double val = 0.0;
+      if (c < r) {
+         int src = m - r + c;
+         val = U[i + (int64_t)src * mx] / sqrt(lam[src]);
+      }
+      Bm[i + (int64_t)c * mx] = val;
+   }
+}
+
+// stage 0: T = Hs Bm; stage 1: Hr = Bm^T T (the small_gemm arithmetic, ld mx)
+__global__ void rrg_gemm_kernel(RRJobArgs a, double *base, int stage) {
+   const int b = blockIdx.z, m = a.j[b].m, mx = a.mx;
+   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+   if (i >= m || j >= m) return;
+   const int64_t mm = (int64_t)mx * mx;
+   double *sc = rrj_area(base, a, b), *Hs = sc + mx + mm, *Bm = sc + mx + 3 * mm, *T = Bm + mm;
+   double *Hr = T + mm;
+   double s = 0.0;
+   if (stage == 0) {
+      for (int p = 0; p < m; ++p) s = fma(Hs[i + (int64_t)p * mx], Bm[p + (int64_t)j * mx], s);
+      T[i + (int64_t)j * mx] = 1.0 * s;
+   } else {
+      for (int p = 0; p < m; ++p) s = fma(Bm[p + (int64_t)i * mx], T[p + (int64_t)j * mx], s);
+      Hr[i + (int64_t)j * mx] = 1.0 * s;
+   }
+}
+
+__global__ void rrg_finish_kernel(RRJobArgs a, double *base, const int32_t *rdev) {
+   const int b = blockIdx.y, m = a.j[b].m, mx = a.mx, c = blockIdx.x;
+   if (c >= a.j[b].nkeep) return;
+   const int64_t mm = (int64_t)mx * mx;
+   const double *sc = rrj_area(base, a, b), *Z = sc + mx, *Bm = sc + mx + 3 * mm;
+   const double *th = sc + mx + 6 * mm + mx;
+   const int r = rdev[b];
+   double *Y = a.j[b].Y;
+   const int ldy = a.j[b].ldy;
+   for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      double s = 0.0;
+      if (c < r) {
+         int src = r - 1 - c;
+         for (int p = 0; p < r; ++p) s = fma(Bm[i + (int64_t)p * mx], Z[p + (int64_t)src * mx], s);
+         s *= sc[i];
+      }
+      Y[i + (int64_t)c * ldy] = s;
+   }
+   if (threadIdx.x == 0) a.j[b].theta[c] = (c < r) ? th[r - 1 - c] : __builtin_nan("");
+}
+
+size_t rr_group_workspace_bytes(int njobs, int mx) {
+   const size_t slab = ((size_t)mx + 6 * (size_t)mx * mx + 2 * (size_t)mx + 1) & ~(size_t)1;
+   return ((size_t)njobs * slab + (size_t)njobs * jacobi_work_doubles(mx) + (size_t)njobs + 8) *
+          sizeof(double);
+}
+
+int rr_truncated_group(ofrr_ctx *ctx, int njobs, const RRJob *jobs) {
+   if (njobs <= 0 || njobs > kMaxRRJobs) return set_error(ctx, kErrorInvalidArgument, "rr group: job count");
+   RRJobArgs a{};
+   a.n = njobs;
+   int mx = 0, kx = 0;
+   for (int b = 0; b < njobs; ++b) {
+      if (jobs[b].m <= 0 || jobs[b].nkeep <= 0 || jobs[b].nkeep > jobs[b].m || jobs[b].m > kJacobiMaxN)
+         return set_error(ctx, kErrorInvalidArgument, "rr group: job sizes");
+      a.j[b] = jobs[b];
+      mx = std::max(mx, jobs[b].m);
+      kx = std::max(kx, jobs[b].nkeep);
+   }
+   a.mx = mx;
+   a.slab = (int64_t)(((size_t)mx + 6 * (size_t)mx * mx + 2 * (size_t)mx + 1) & ~(size_t)1);
+   double *base = (double *)workspace(ctx, rr_group_workspace_bytes(njobs, mx));
+   if (!base) return set_error(ctx, kErrorCUDAMemory, "rr group: workspace");
+   double *jwork = base + (size_t)njobs * a.slab;
+   int32_t *mdev = (int32_t *)(jwork + (size_t)njobs * jacobi_work_doubles(mx));
+   int32_t *rdev = mdev + njobs;
+   const int64_t mm = (int64_t)mx * mx;
+   // per job: Gs at sc + mx, U at sc + mx + 2 mm, lam at sc + mx + 6 mm;
+   // Hr at sc + mx + 5 mm, th at lam + mx, Z over Gs (dead after its eigensolve)
+   double *Gs0 = base + mx, *U0 = base + mx + 2 * mm, *lam0 = base + mx + 6 * mm;
+   double *Hr0 = base + mx + 5 * mm, *th0 = lam0 + mx;
+   rrg_scale_kernel<<<dim3((unsigned)((mm + 255) / 256), njobs), 256, 0, ctx->stream>>>(a, base, mdev);
+   OFRR_LAUNCH(ctx);
+   int st = syev_batched(ctx, njobs, mx, mdev, Gs0, mx, a.slab, lam0, a.slab, U0, mx, a.slab, jwork);
+   if (st) return st;
+   rrg_trunc_kernel<<<njobs, 256, 0, ctx->stream>>>(a, base, rdev);
+   OFRR_LAUNCH(ctx);
+   for (int stage = 0; stage < 2; ++stage) {
+      rrg_gemm_kernel<<<dim3((mx + 63) / 64, mx, njobs), 64, 0, ctx->stream>>>(a, base, stage);
+      OFRR_LAUNCH(ctx);
+   }
+   st = syev_batched(ctx, njobs, mx, rdev, Hr0, mx, a.slab, th0, a.slab, Gs0, mx, a.slab, jwork);
+   if (st) return st;
+   rrg_finish_kernel<<<dim3(kx, njobs), 256, 0, ctx->stream>>>(a, base, rdev);
+   OFRR_LAUNCH(ctx);
+   return kSuccess;
+}
+
+// ---------------------------------------------------------------------------
 // generalised problem H x = lambda (G + gtol I) x via Cholesky (one workgroup)
 __global__ __launch_bounds__(1024) void chol_transform_kernel(int m, const double *__restrict__ G,
                                                               int ldg, const double *__restrict__ H,

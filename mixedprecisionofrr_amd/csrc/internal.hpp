@@ -80,6 +80,20 @@ int rr_truncated(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H
                  double tau, int nkeep, double *theta, double *Y, int ldy, int32_t *rank);
 int rr_generalized(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H, int ldh,
                    double gtol, double *theta, double *Y, int ldy);
+// One job of a grouped truncated Rayleigh-Ritz: rr_truncated's arguments.
+constexpr int kMaxRRJobs = 16;
+struct RRJob {
+   const double *G, *H;
+   int ldg, ldh, m, nkeep;
+   double tau;
+   double *theta, *Y;
+   int ldy;
+   int32_t *rank;
+};
+// rr_truncated for each of njobs jobs (m <= 384 each) in one launch per stage on
+// ctx's stream; each job's results equal rr_truncated's on its own arguments
+int rr_truncated_group(ofrr_ctx *ctx, int njobs, const RRJob *jobs);
+size_t rr_group_workspace_bytes(int njobs, int mx);
 // argmax |x| (1-based index + signed value, device outputs)
 int iamax(ofrr_ctx *ctx, int dtype, int64_t n, const void *x, int64_t inc, int32_t *idx1,
           double *val);

@@ -17,6 +17,17 @@
 //
 // Every reduction of a solve has a fixed order and the solves share no device
 // memory, so a request's result is the same alone and beside other requests.
+//
+// Same-phase work of the running solves is grouped: a solve that reaches its
+// truncated Rayleigh-Ritz hands the job (its Gram and projected matrices) to
+// the server's Rayleigh-Ritz group, and the jobs handed in within a short
+// window (or once every running solve has handed one in) run as one batched
+// launch per stage on the server's own stream, ordered after each job's
+// inputs by an event on its slot's stream and before its slot's next work by
+// another.  Each job's arithmetic is the one it gets alone (rr_truncated_group).
+// The filter sweeps of every slot run as captured HIP graphs, one per sweep
+// shape, kept by the slot across requests.
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -133,6 +144,94 @@ struct Rec {
 
 using namespace ofrr;
 
+namespace ofrr {
+namespace {
+// The grouped Rayleigh-Ritz of a server.  The first solve to hand in a job
+// leads: it waits up to `window` (or until every running solve has a job in),
+// takes the jobs in arrival order and launches them together; the others wait
+// until their job is launched.  The leader's own GPU work is still queued
+// meanwhile: the host runs ahead of its stream, so the window costs the
+// device nothing unless the job's inputs are ready before it ends.
+struct RRGroupExec {
+   struct Pending {
+      RRJob job;
+      ofrr_ctx *slot = nullptr;
+      int status = 0;
+      bool launched = false;
+   };
+   ofrr_ctx *gctx = nullptr;
+   hipStream_t stream = nullptr;
+   std::chrono::microseconds window{200};
+   int max_jobs = 1;
+   std::atomic<int> *running = nullptr;  // solves running in the server
+   std::mutex mu;
+   std::condition_variable cv;
+   std::deque<Pending *> pending;
+   bool leader = false;
+   int64_t groups = 0, group_jobs = 0;
+   int group_max = 0;
+
+   int launch(std::vector<Pending *> &batch) {
+      std::vector<RRJob> jobs;
+      for (Pending *p : batch) {
+         OFRR_HIP(gctx, hipStreamWaitEvent(stream, p->slot->ev_rr_ready, 0));
+         jobs.push_back(p->job);
+      }
+      int st = rr_truncated_group(gctx, (int)jobs.size(), jobs.data());
+      if (st) return st;
+      for (Pending *p : batch) OFRR_HIP(gctx, hipEventRecord(p->slot->ev_rr_done, stream));
+      return kSuccess;
+   }
+
+   int submit(ofrr_ctx *slot, const RRJob &job) {
+      if (!slot->ev_rr_ready) OFRR_HIP(slot, hipEventCreateWithFlags(&slot->ev_rr_ready, hipEventDisableTiming));
+      if (!slot->ev_rr_done) OFRR_HIP(slot, hipEventCreateWithFlags(&slot->ev_rr_done, hipEventDisableTiming));
+      OFRR_HIP(slot, hipEventRecord(slot->ev_rr_ready, slot->stream));
+      Pending me;
+      me.job = job;
+      me.slot = slot;
+      std::unique_lock<std::mutex> lk(mu);
+      pending.push_back(&me);
+      cv.notify_all();
+      while (!me.launched) {
+         if (leader) {
+            cv.wait(lk, [&] { return me.launched || !leader; });
+            continue;
+         }
+         leader = true;
+         cv.wait_for(lk, window, [&] {
+            return (int)pending.size() >= std::min(max_jobs, std::max(1, running->load()));
+         });
+         std::vector<Pending *> batch;
+         while (!pending.empty() && (int)batch.size() < max_jobs) {
+            batch.push_back(pending.front());
+            pending.pop_front();
+         }
+         lk.unlock();
+         const int st = launch(batch);
+         lk.lock();
+         for (Pending *p : batch) {
+            p->status = st;
+            p->launched = true;
+         }
+         ++groups;
+         group_jobs += (int64_t)batch.size();
+         group_max = std::max(group_max, (int)batch.size());
+         leader = false;
+         cv.notify_all();
+      }
+      lk.unlock();
+      if (me.status) return set_error(slot, me.status, "grouped Rayleigh-Ritz: " + gctx->last_error);
+      OFRR_HIP(slot, hipStreamWaitEvent(slot->stream, slot->ev_rr_done, 0));
+      return kSuccess;
+   }
+   static int entry(void *arg, ofrr_ctx *slot, const void *job) {
+      return static_cast<RRGroupExec *>(arg)->submit(slot, *static_cast<const RRJob *>(job));
+   }
+};
+}  // namespace
+}  // namespace ofrr
+
 struct ofrr_server {
    int device = 0;
    int64_t budget = 0;
@@ -154,6 +253,8 @@ struct ofrr_server {
    int64_t slot_bytes = 0;
    int64_t ws_regrowths = 0;  // solves whose workspace outgrew the slot's reservation
    bool graphs = true;        // filter sweeps as captured graphs, kept per slot and shape
+   std::atomic<int> running_now{0};
+   RRGroupExec *rr = nullptr;  // grouped Rayleigh-Ritz (null: each solve runs its own)
 
    double now() const {
       return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -239,6 +340,10 @@ void ofrr_server::worker(int slot) {
    if (!st) st = ofrr_ctx_set_stream(ctx, s);
    // the workspace every solve of this slot reuses, reserved once up front
    if (!st && !workspace(ctx, filtered_workspace_bound(ctx->num_cus))) st = kErrorCUDAMemory;
+   if (!st && rr) {
+      ctx->rr_group = &RRGroupExec::entry;
+      ctx->rr_group_arg = rr;
+   }
    int64_t held = 0;
    {
       std::lock_guard<std::mutex> lk(mu);
@@ -261,6 +366,7 @@ void ofrr_server::worker(int slot) {
       bytes_running += r.footprint;
       bytes_peak = std::max(bytes_peak, bytes_running);
       ++running;
+      running_now.store(running);
       running_peak = std::max(running_peak, running);
       lk.unlock();
       cv_admit.notify_all();  // the next head may fit as well
@@ -276,6 +382,7 @@ void ofrr_server::worker(int slot) {
       r.state = kDone;
       bytes_running -= r.footprint;
       --running;
+      running_now.store(running);
       ++completed;
       lk.unlock();
       cv_done.notify_all();
@@ -344,6 +451,27 @@ int ofrr_server_create(int device, const ofrr_server_opts *opt, ofrr_server **ou
    // to the driver at every synchronisation
    uint64_t keep = UINT64_MAX;
    (void)hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold, &keep);
+   if (!(opt && opt->rr_group_us < 0) && slots > 1) {
+      // the group's context, stream and workspace (every slot's largest job)
+      auto *g = new RRGroupExec();
+      g->window = std::chrono::microseconds((opt && opt->rr_group_us > 0) ? opt->rr_group_us : 200);
+      g->max_jobs = std::min(slots, kMaxRRJobs);
+      g->running = &s->running_now;
+      int gst = ofrr_ctx_create(device, &g->gctx);
+      if (!gst && hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) gst = kErrorCUDAGeneral;
+      if (!gst) gst = ofrr_ctx_set_stream(g->gctx, g->stream);
+      if (!gst && !workspace(g->gctx, rr_group_workspace_bytes(g->max_jobs, 384))) gst = kErrorCUDAMemory;
+      if (gst) {
+         if (g->gctx) ofrr_ctx_destroy(g->gctx);
+         if (g->stream) (void)hipStreamDestroy(g->stream);
+         delete g;
+         (void)hipMemPoolDestroy(s->pool);
+         delete s;
+         return gst;
+      }
+      s->rr = g;
+      s->slot_bytes += (int64_t)g->gctx->ws_bytes;
+   }
    s->worker_status.assign(slots, -1);
    try {
       for (int i = 0; i < slots; ++i) s->workers.emplace_back([s, i] { s->worker(i); });
@@ -354,8 +482,7 @@ int ofrr_server_create(int device, const ofrr_server_opts *opt, ofrr_server **ou
       }
       s->cv_admit.notify_all();
       for (auto &t : s->workers) t.join();
-      (void)hipMemPoolDestroy(s->pool);
-      delete s;
+      ofrr_server_destroy(s);
       return kErrorInternalError;
    }
    // every worker has its context and stream before the server is handed out
@@ -437,6 +564,14 @@ int ofrr_server_stats_get(ofrr_server *s, ofrr_server_stats *out) {
    out->budget = s->budget;
    out->slot_bytes = s->slot_bytes;
    out->ws_regrowths = s->ws_regrowths;
+   out->rr_groups = out->rr_group_jobs = 0;
+   out->rr_group_max = 0;
+   if (s->rr) {
+      std::lock_guard<std::mutex> lg(s->rr->mu);
+      out->rr_groups = s->rr->groups;
+      out->rr_group_jobs = s->rr->group_jobs;
+      out->rr_group_max = s->rr->group_max;
+   }
    return kSuccess;
 }
 
@@ -450,6 +585,11 @@ int ofrr_server_destroy(ofrr_server *s) {
    for (auto &t : s->workers)
       if (t.joinable()) t.join();
    (void)hipSetDevice(s->device);
+   if (s->rr) {
+      ofrr_ctx_destroy(s->rr->gctx);
+      if (s->rr->stream) (void)hipStreamDestroy(s->rr->stream);
+      delete s->rr;
+   }
    if (s->pool) (void)hipMemPoolDestroy(s->pool);
    delete s;
    return kSuccess;

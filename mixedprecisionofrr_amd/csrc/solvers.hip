@@ -1170,8 +1170,15 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    // Rayleigh-Ritz of the basis [X | D_d (| D_mid)] with me = k + nd ka padded
    // columns, of which the correction blocks hold kar = ka - (k - kr) real
    // ones: eigensolves on the compacted matrices when the blocks are padded
+   // the truncated Rayleigh-Ritz itself, or (a serving slot) its job handed
+   // to the server's grouped Rayleigh-Ritz
+   auto rr_run = [&](int mm, const double *Gm, const double *Hm, double *Y) -> int {
+      if (!ctx->rr_group) return rr_truncated(ctx, mm, Gm, mm, Hm, mm, o->tau, kr, th, Y, mm, rank);
+      const RRJob job{Gm, Hm, mm, mm, mm, kr, o->tau, th, Y, mm, rank};
+      return ctx->rr_group(ctx->rr_group_arg, ctx, &job);
+   };
    auto rayleigh_ritz = [&](int me, int ka) -> int {
-      if (!padded) return rr_truncated(ctx, me, G, me, H, me, o->tau, kr, th, C, me, rank);
+      if (!padded) return rr_run(me, G, H, C);
       const int kar = ka - (k - kr), nb = (me - k) / std::max(1, ka);
       int mr = 0;
       for (int p = 0; p < me; ++p) h_cinv[p] = -1;
@@ -1190,7 +1197,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       sym_gather_kernel<<<(mr * mr + 255) / 256, 256, 0, ctx->stream>>>(mr, fb.cmap, G, me, fb.Gc);
       sym_gather_kernel<<<(mr * mr + 255) / 256, 256, 0, ctx->stream>>>(mr, fb.cmap, H, me, fb.Hc);
       OFRR_LAUNCH(ctx);
-      int s2 = rr_truncated(ctx, mr, fb.Gc, mr, fb.Hc, mr, o->tau, kr, th, fb.Cc, mr, rank);
+      int s2 = rr_run(mr, fb.Gc, fb.Hc, fb.Cc);
       if (s2) return s2;
       coef_scatter_kernel<<<(me * k + 255) / 256, 256, 0, ctx->stream>>>(me, k, kr, mr, fb.cinv, fb.Cc, C);
       OFRR_LAUNCH(ctx);

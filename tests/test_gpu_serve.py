@@ -30,8 +30,8 @@ def _requests(api):
                         tol=1e-8) for i, (d, N, w, s) in enumerate(MIX)]
 
 
-def _run(api, reqs, slots, budget=0, graphs=True):
-    with api.Server(0, slots=slots, budget=budget, graphs=graphs) as srv:
+def _run(api, reqs, slots, budget=0, graphs=True, rr_group_us=0):
+    with api.Server(0, slots=slots, budget=budget, graphs=graphs, rr_group_us=rr_group_us) as srv:
         if slots == 1:
             out = []
             for r in reqs:  # strictly one after the other
@@ -73,7 +73,10 @@ def test_serve_admission_respects_budget(api):
     reqs = _requests(api)
     fps = [api.request_footprint(r)[2] for r in reqs]
     slots = 4
-    reserve = slots * api.server_slot_bytes(0)
+    # what the slots and the grouped Rayleigh-Ritz hold for the server's life
+    with api.Server(0, slots=slots) as probe:
+        reserve = probe.stats()["slot_bytes"]
+    assert reserve >= slots * api.server_slot_bytes(0)
     budget = reserve + int(max(fps) * 1.2)  # no two of the largest fit together
     out, st = _run(api, reqs, slots, budget=budget)
     assert st["budget"] == budget and st["slot_bytes"] == reserve
@@ -157,3 +160,36 @@ def test_serve_graph_sweeps_match_plain_launches(api):
         assert a["status"] == 0 and b["status"] == 0
         assert a["iterations"] == b["iterations"]
         assert np.array_equal(a["theta"], b["theta"]) and np.array_equal(a["res"], b["res"])
+
+
+def test_serve_grouped_rr_matches_separate_and_direct(api):
+    """The grouped Rayleigh-Ritz (jobs of several running solves in one launch
+    per stage) gives each solve bitwise the result of its own Rayleigh-Ritz:
+    the same stream with grouping off, and the direct ofrr_filtered_eig call on
+    the request's operator and start block, agree exactly.  A wide window makes
+    the solves' jobs meet, so groups of more than one job form."""
+    reqs = [api.request(2, 60 + 10 * i, 8 + 4 * (i % 3), coef_seed=20 + i, start_seed=50 + i,
+                        basis_dtype=torch.float16, tol=1e-8) for i in range(8)]
+    print("serve: grouped Rayleigh-Ritz (window 2 ms)", flush=True)
+    grouped, st = _run(api, reqs, 4, rr_group_us=2000)
+    print("serve: separate Rayleigh-Ritz", flush=True)
+    sep, st0 = _run(api, reqs, 4, rr_group_us=-1)
+    assert st["rr_group_max"] > 1 and st["rr_group_jobs"] > st["rr_groups"]
+    assert st0["rr_groups"] == 0
+    for a, b in zip(grouped, sep):
+        assert a["status"] == 0 and b["status"] == 0
+        assert a["iterations"] == b["iterations"]
+        assert np.array_equal(a["theta"], b["theta"]) and np.array_equal(a["res"], b["res"])
+    # the direct call with the server's options (degree 120, tau 1e-12, soft
+    # locking, FP32 operator values for variable coefficients)
+    from mixedprecisionofrr_amd import api as A
+    ctx = A.Context(0)
+    for r, g in zip(reqs[:3], grouped[:3]):
+        k = g["k"]
+        M = A.CSRMatrix.stencil(ctx, r.dim, r.N, seed=r.coef_seed).with_low_precision(torch.float32)
+        X0 = torch.empty(M.n, k, dtype=torch.float64, device="cuda:0")
+        A.fill_randn(ctx, X0, seed=r.start_seed)
+        out = A.filtered_eig(ctx, M, X0, r.nwant, nd=g["nd"], degree=120, maxiter=200,
+                             basis_dtype=torch.float16, tol=1e-8, tau=1e-12)
+        assert out[0] == 0 and out[4]["iterations"] == g["iterations"]
+        assert np.array_equal(out[1][:r.nwant], g["theta"]) and np.array_equal(out[3][:r.nwant], g["res"])
