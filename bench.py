@@ -234,7 +234,7 @@ def main():
                    help="growth-block format of the mixed path (the workload names FP16)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
-    p.add_argument("--workload", choices=["eig", "svd", "serve", "qr"], default="eig",
+    p.add_argument("--workload", choices=["eig", "svd", "serve", "qr", "graphs"], default="eig",
                    help="eig: the 128^3 Laplacian eigensolve (default); svd: truncated SVD of "
                         "the dense 262,144 x 4,096 low-rank-plus-noise matrix; serve: a stream "
                         "of eigen requests with Poisson arrivals through the serving loop")
@@ -243,6 +243,11 @@ def main():
                    help="Poisson arrival rate of the serving stream (requests/s)")
     p.add_argument("--serve-slots", type=int, default=4)
     p.add_argument("--serve-seed", type=int, default=0)
+    p.add_argument("--serve-graphs", choices=["on", "off"], default="off",
+                   help="serve workload: filter sweeps as captured HIP graphs")
+    p.add_argument("--serve-rr-group-us", type=int, default=0,
+                   help="serve workload: grouped Rayleigh-Ritz (0: the jobs that meet, w > 0: "
+                        "also wait up to w microseconds for every running solve, -1: off)")
     p.add_argument("--serve-big", action="store_true",
                    help="also submit the large FP8 solve (3-D variable-coefficient diffusion on "
                         "a 256^3 grid, 100 largest pairs, FP8 E4M3 growth blocks) a quarter of "
@@ -273,9 +278,12 @@ def main():
         return svd_main(args)
     if args.workload == "qr":
         return qr_main(args)
+    if args.workload == "graphs":
+        return graphs_main(args)
     if args.workload == "serve":
         r = serve_stream(api, args.serve_requests, args.serve_rate, args.serve_slots,
-                         args.serve_seed, big=args.serve_big)
+                         args.serve_seed, big=args.serve_big, graphs=args.serve_graphs == "on",
+                         rr_group_us=args.serve_rr_group_us)
         out = {"metric": "serving: converged eigenpairs/s and p50/p99 request latency",
                "value": r["pairs_per_s"], "unit": "eigenpairs/s", "n_gpus": 1,
                "steps": args.serve_requests, "warmup": 1, "ms_per_step": r["wall_s"] * 1e3 / max(1, args.serve_requests),
@@ -439,10 +447,12 @@ def serve_requests(api, count, seed):
     return reqs, meta
 
 
-def serve_stream(api, count, rate, slots, seed, big=False):
+def serve_stream(api, count, rate, slots, seed, big=False, graphs=False, rr_group_us=0):
     """Submit `count` requests at seeded Poisson arrival times through a
     `slots`-slot server; latency = completion - arrival on the server's clock.
-    big: the large FP8 solve joins the stream after a quarter of the arrivals."""
+    big: the large FP8 solve joins the stream after a quarter of the arrivals.
+    graphs: filter sweeps as captured HIP graphs; rr_group_us: the grouped
+    Rayleigh-Ritz window (0: the server's default, -1: off)."""
     import numpy as np
     import torch
     reqs, meta = serve_requests(api, count, seed)
@@ -457,7 +467,7 @@ def serve_stream(api, count, rate, slots, seed, big=False):
     big_id = big_out = None
     arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, size=count))
     arr -= arr[0]
-    with api.Server(0, slots=slots) as srv:
+    with api.Server(0, slots=slots, graphs=graphs, rr_group_us=rr_group_us) as srv:
         # first use of every kernel on the box, outside the stream
         srv.wait(srv.submit(api.request(3, 24, 8, coef_seed=1, start_seed=1)))
         log(f"serve: {count} requests, Poisson {rate:g}/s, {slots} slots")
@@ -518,8 +528,13 @@ def serve_stream(api, count, rate, slots, seed, big=False):
                      f"(seed {seed}), 2-D 5-point / 3-D 7-point variable-coefficient diffusion "
                      f"(face coefficients log-uniform in [0.1, 10]), n log-uniform in [1e4, 4e6], "
                      f"nwant in {{8,16,32,64}}, tol 1e-8, FP16 growth blocks, {slots} slots, "
-                     f"HBM-footprint admission"),
+                     f"HBM-footprint admission, filter sweeps as HIP graphs "
+                     f"{'on' if graphs else 'off'}, grouped Rayleigh-Ritz "
+                     f"{'off' if rr_group_us < 0 else 'on'}"),
         "arrival_rate_per_s": rate, "requests": count, "slots": slots,
+        "graphs": bool(graphs), "rr_group_us": rr_group_us,
+        "rr_groups": stats["rr_groups"], "rr_group_jobs": stats["rr_group_jobs"],
+        "rr_group_max": stats["rr_group_max"],
         "completed_ok": ok, "converged_pairs": pairs, "wall_s": wall,
         "pairs_per_s": pairs / wall, "requests_per_s": count / wall,
         "latency_ms": {"p50": pct(list(lat), 50), "p99": pct(list(lat), 99),
@@ -610,6 +625,42 @@ def svd_main(args):
         "fp64_gpu": fp64,
     }
     print(json.dumps(out), flush=True)
+
+
+def graphs_main(args):
+    """Filter sweeps as captured HIP graphs against plain launches: the same
+    filtered solve (variable-coefficient diffusion, FP16 blocks, degree 120,
+    soft locking) with and without opts.graph, twice each (the second graph
+    solve replays the context's graphs without recording); device time per
+    Chebyshev step from the solver's events.  Results are bitwise equal."""
+    import numpy as np
+    import torch
+
+    from mixedprecisionofrr_amd import api
+    ctx = api.Context(0)
+    rows = []
+    for dim, N, k, nwant in [(2, 100, 16, 8), (2, 300, 32, 16), (3, 64, 64, 32)]:
+        A = api.CSRMatrix.stencil(ctx, dim, N, seed=3).with_low_precision(torch.float32)
+        X0 = torch.empty(A.n, k, dtype=torch.float64, device="cuda:0")
+        api.fill_randn(ctx, X0, seed=1)
+        ref = None
+        for graph in (False, True, False, True):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            st, th, X, res, s = api.filtered_eig(ctx, A, X0, nwant, nd=2, degree=120, maxiter=200,
+                                                 basis_dtype=torch.float16, tol=1e-8, graph=graph)
+            torch.cuda.synchronize()
+            wall = (time.perf_counter() - t) * 1e3
+            if ref is None:
+                ref = th
+            rows.append({"n": A.n, "k": k, "graph": graph, "status": st, "iterations": s["iterations"],
+                         "wall_ms": wall, "t_filter_ms": s["t_filter_ms"], "filter_steps": s["filter_steps"],
+                         "us_per_step": 1e3 * s["t_filter_ms"] / max(1, s["filter_steps"]),
+                         "graph_replays": s["graph_replays"], "graph_captures": s["graph_captures"],
+                         "same_as_plain": bool(np.array_equal(th, ref))})
+            log(json.dumps(rows[-1]))
+    print(json.dumps({"metric": "filter sweep device time per Chebyshev step, graph replay vs plain launches",
+                      "unit": "us", "rows": rows}), flush=True)
 
 
 def qr_main(args):

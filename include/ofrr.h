@@ -493,13 +493,16 @@ typedef struct {
    int slots;          /* concurrent solves (0: 4) */
    int64_t budget;     /* device bytes the running solves may hold together
                           (0: 90% of the free device memory at creation) */
-   int graphs;         /* filter sweeps as captured HIP graphs (ofrr_filter_opts.graph),
-                          kept per slot and shape across requests: 0 or 1 on, -1 off */
-   int rr_group_us;    /* grouped Rayleigh-Ritz: the truncated Rayleigh-Ritz jobs that
-                          running solves hand in within this many microseconds of each
-                          other run as one batched launch per stage on the server's own
-                          stream (each job's result is the one it gets alone); 0: 200,
-                          -1: every solve runs its own */
+   int graphs;         /* 1: filter sweeps as captured HIP graphs (ofrr_filter_opts.graph),
+                          kept per slot and shape across requests; 0 or -1: plain launches
+                          (the default: on ROCm 7.2 a replayed sweep measured slower per step
+                          than its launches) */
+   int rr_group_us;    /* grouped Rayleigh-Ritz: the truncated Rayleigh-Ritz jobs of the
+                          running solves that meet run as one batched launch per stage on
+                          the server's own stream (each job's result is the one it gets
+                          alone).  0: the jobs already handed in when a group is formed;
+                          w > 0: a group also waits up to w microseconds for every running
+                          solve's job; -1: every solve runs its own */
 } ofrr_server_opts;
 
 #define OFRR_SERVE_MAX_WANT 128

@@ -739,9 +739,10 @@ class Server:
     solves run at once on their own streams, admitted in arrival order against
     a device-byte budget (0: 90% of free memory)."""
 
-    def __init__(self, device=0, slots=4, budget=0, graphs=True, rr_group_us=0):
-        """graphs: filter sweeps as captured HIP graphs; rr_group_us: window of the
-        grouped Rayleigh-Ritz (0: default, -1: off)."""
+    def __init__(self, device=0, slots=4, budget=0, graphs=False, rr_group_us=0):
+        """graphs: filter sweeps as captured HIP graphs; rr_group_us: the grouped
+        Rayleigh-Ritz (0: the jobs that meet, w > 0: also wait up to w us for
+        every running solve's job, -1: off)."""
         self.lib = _lib.load()
         h = C.c_void_p()
         opts = _lib.ServerOpts(slots, budget, 1 if graphs else -1, rr_group_us)

@@ -20,13 +20,13 @@
 //
 // Same-phase work of the running solves is grouped: a solve that reaches its
 // truncated Rayleigh-Ritz hands the job (its Gram and projected matrices) to
-// the server's Rayleigh-Ritz group, and the jobs handed in within a short
-// window (or once every running solve has handed one in) run as one batched
-// launch per stage on the server's own stream, ordered after each job's
+// the server's Rayleigh-Ritz group, and the jobs that meet there (optionally
+// within a window) run as one batched launch per stage on the server's own
+// stream, ordered after each job's
 // inputs by an event on its slot's stream and before its slot's next work by
 // another.  Each job's arithmetic is the one it gets alone (rr_truncated_group).
-// The filter sweeps of every slot run as captured HIP graphs, one per sweep
-// shape, kept by the slot across requests.
+// On request (ofrr_server_opts.graphs) the filter sweeps of every slot run as
+// captured HIP graphs, one per sweep shape, kept by the slot across requests.
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -147,11 +147,13 @@ using namespace ofrr;
 namespace ofrr {
 namespace {
 // The grouped Rayleigh-Ritz of a server.  The first solve to hand in a job
-// leads: it waits up to `window` (or until every running solve has a job in),
-// takes the jobs in arrival order and launches them together; the others wait
-// until their job is launched.  The leader's own GPU work is still queued
-// meanwhile: the host runs ahead of its stream, so the window costs the
-// device nothing unless the job's inputs are ready before it ends.
+// leads: it takes the jobs handed in so far (with a window, it first waits up
+// to `window` or until every running solve has a job in), in arrival order,
+// and launches them together; the others wait until their job is launched.
+// The leader's own GPU work is still queued meanwhile: the host runs ahead of
+// its stream, so a window costs the device nothing unless the job's inputs are
+// ready before it ends.  Without a window (the default) a group holds the jobs
+// that met while the previous group was being launched.
 struct RRGroupExec {
    struct Pending {
       RRJob job;
@@ -161,7 +163,7 @@ struct RRGroupExec {
    };
    ofrr_ctx *gctx = nullptr;
    hipStream_t stream = nullptr;
-   std::chrono::microseconds window{200};
+   std::chrono::microseconds window{0};
    int max_jobs = 1;
    std::atomic<int> *running = nullptr;  // solves running in the server
    std::mutex mu;
@@ -199,9 +201,10 @@ struct RRGroupExec {
             continue;
          }
          leader = true;
-         cv.wait_for(lk, window, [&] {
-            return (int)pending.size() >= std::min(max_jobs, std::max(1, running->load()));
-         });
+         if (window.count() > 0)
+            cv.wait_for(lk, window, [&] {
+               return (int)pending.size() >= std::min(max_jobs, std::max(1, running->load()));
+            });
          std::vector<Pending *> batch;
          while (!pending.empty() && (int)batch.size() < max_jobs) {
             batch.push_back(pending.front());
@@ -252,7 +255,7 @@ struct ofrr_server {
    // admission counts request memory against budget - slot_bytes
    int64_t slot_bytes = 0;
    int64_t ws_regrowths = 0;  // solves whose workspace outgrew the slot's reservation
-   bool graphs = true;        // filter sweeps as captured graphs, kept per slot and shape
+   bool graphs = false;       // filter sweeps as captured graphs, kept per slot and shape
    std::atomic<int> running_now{0};
    RRGroupExec *rr = nullptr;  // grouped Rayleigh-Ritz (null: each solve runs its own)
 
@@ -426,7 +429,7 @@ int ofrr_server_create(int device, const ofrr_server_opts *opt, ofrr_server **ou
    if (hipSetDevice(device) != hipSuccess) return kErrorCUDAGeneral;
    ofrr_server *s = new ofrr_server();
    s->device = device;
-   s->graphs = !(opt && opt->graphs < 0);
+   s->graphs = opt && opt->graphs > 0;
    s->t0 = std::chrono::steady_clock::now();
    if (opt && opt->budget > 0) {
       s->budget = opt->budget;
@@ -454,7 +457,7 @@ int ofrr_server_create(int device, const ofrr_server_opts *opt, ofrr_server **ou
    if (!(opt && opt->rr_group_us < 0) && slots > 1) {
       // the group's context, stream and workspace (every slot's largest job)
       auto *g = new RRGroupExec();
-      g->window = std::chrono::microseconds((opt && opt->rr_group_us > 0) ? opt->rr_group_us : 200);
+      g->window = std::chrono::microseconds((opt && opt->rr_group_us > 0) ? opt->rr_group_us : 0);
       g->max_jobs = std::min(slots, kMaxRRJobs);
       g->running = &s->running_now;
       int gst = ofrr_ctx_create(device, &g->gctx);

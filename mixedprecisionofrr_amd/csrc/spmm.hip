@@ -347,7 +347,73 @@ __device__ inline void cheb_rows(int64_t r0, int nr, int grp, int groups, int c0
    }
 }
 
-template <typename TV, typename TB, int VEC, bool HAS_DP>
+// The same row loop for blocks of fewer than 2^31 bytes: every address is a
+// 32-bit byte offset from a kernel-argument base, so a gather costs one integer
+// multiply-add and one VGPR (base + offset addressing) instead of a 64-bit
+// multiply and a register pair (94 VGPRs instead of 116 for FP16 blocks: five
+// waves per SIMD instead of four).  The arithmetic (order of the fused
+// multiply-adds, padding slots of the last group of 8 gathering the row itself
+// with weight 0) is the one of cheb_rows.
+template <typename TV, typename TB, int VEC, typename ACC, bool HAS_DP, bool OVERREAD>
+__device__ inline void cheb_rows32(int64_t r0, int nr, int grp, int groups, int c0, uint32_t rowb,
+                                   const int32_t *cs, const TV *vs, const int32_t *s_rp,
+                                   const ACC (*s_coef)[kChebMaxK], ACC cshift,
+                                   const char *__restrict__ Dcb, const char *__restrict__ Dpb,
+                                   const char *__restrict__ Rb, char *__restrict__ Dnb, float mx[VEC]) {
+   const uint32_t c0b = (uint32_t)c0 * (uint32_t)sizeof(TB);
+   for (int lr = grp; lr < nr; lr += groups) {
+      const uint32_t self = (uint32_t)(r0 + lr) * rowb + c0b;
+      const VecT<TB, VEC> rr = vload_nt<TB, VEC>(reinterpret_cast<const TB *>(Rb + self));
+      VecT<TB, VEC> dp;
+      if (HAS_DP) dp = vload_nt<TB, VEC>(reinterpret_cast<const TB *>(Dpb + self));
+      const VecT<TB, VEC> dc = vload<TB, VEC>(reinterpret_cast<const TB *>(Dcb + self));
+      ACC acc[VEC];
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) acc[v] = 0;
+      const int e = s_rp[lr + 1];
+      for (int p = s_rp[lr]; p < e; p += 8) {
+         uint32_t off[8];
+         ACC a[8];
+#pragma unroll
+         for (int u = 0; u < 8; ++u) {
+            const bool ok = p + u < e;
+            if (OVERREAD) {  // the LDS tile is padded by 8 entries
+               const int32_t cl = cs[p + u];
+               const ACC al = (ACC)Elem<TV>::load(vs[p + u]);
+               off[u] = ok ? (uint32_t)cl * rowb + c0b : self;
+               a[u] = ok ? al : (ACC)0;
+            } else {
+               off[u] = ok ? (uint32_t)cs[p + u] * rowb + c0b : self;
+               a[u] = ok ? (ACC)Elem<TV>::load(vs[p + u]) : (ACC)0;
+            }
+         }
+         VecT<TB, VEC> xv[8];
+#pragma unroll
+         for (int u = 0; u < 8; ++u) xv[u] = vload<TB, VEC>(reinterpret_cast<const TB *>(Dcb + off[u]));
+#pragma unroll
+         for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) acc[v] = fma(a[u], (ACC)Elem<TB>::load(xv[u].v[v]), acc[v]);
+      }
+      asm volatile("" ::: "memory");  // keep the coefficient reads inside the loop
+      const VecT<ACC, VEC> ca = *reinterpret_cast<const VecT<ACC, VEC> *>(&s_coef[0][c0]);
+      const VecT<ACC, VEC> cd = *reinterpret_cast<const VecT<ACC, VEC> *>(&s_coef[2][c0]);
+      VecT<ACC, VEC> cg;
+      if (HAS_DP) cg = *reinterpret_cast<const VecT<ACC, VEC> *>(&s_coef[1][c0]);
+      VecT<TB, VEC> out;
+#pragma unroll
+      for (int v = 0; v < VEC; ++v) {
+         ACC t = ca.v[v] * fma(-cshift, (ACC)Elem<TB>::load(dc.v[v]), acc[v]);
+         if (HAS_DP) t = fma(cg.v[v], (ACC)Elem<TB>::load(dp.v[v]), t);
+         t = fma(cd.v[v], (ACC)Elem<TB>::load(rr.v[v]), t);
+         mx[v] = fmaxf(mx[v], (float)fabs(t));
+         out.v[v] = store_acc<TB>(t);
+      }
+      vstore_nt<TB, VEC>(reinterpret_cast<TB *>(Dnb + self), out);
+   }
+}
+
+template <typename TV, typename TB, int VEC, bool HAS_DP, bool OFF32>
 __global__ __launch_bounds__(256) void cheb_step_kernel(
     int64_t n, const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
     const TV *__restrict__ va, int k, const TB *__restrict__ Dc, const TB *__restrict__ Dp,
@@ -391,7 +457,18 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
    for (int v = 0; v < VEC; ++v) mx[v] = 0.f;
    const ACC cshift = (ACC)(ca_.step ? ca_.step[2] : ca_.c);
    __syncthreads();
-   if (in_lds)
+   if (OFF32) {
+      const uint32_t rowb = (uint32_t)(k * sizeof(TB));
+      if (in_lds)
+         cheb_rows32<TV, TB, VEC, ACC, HAS_DP, true>(r0, nr, grp, GROUPS, c0, rowb, tile.ci, tile.va,
+                                                     tile.rp, s_coef, cshift, (const char *)Dc,
+                                                     (const char *)Dp, (const char *)R, (char *)Dn, mx);
+      else
+         cheb_rows32<TV, TB, VEC, ACC, HAS_DP, false>(r0, nr, grp, GROUPS, c0, rowb, ci + base,
+                                                      va + base, tile.rp, s_coef, cshift,
+                                                      (const char *)Dc, (const char *)Dp,
+                                                      (const char *)R, (char *)Dn, mx);
+   } else if (in_lds)
       cheb_rows<TV, TB, VEC, ACC, HAS_DP, true>(r0, nr, grp, GROUPS, c0, k, tile.ci, tile.va,
                                                 tile.rp, s_coef, cshift, Dc, Dp, R, Dn, mx);
    else  // more than kTileCap entries in this block's rows: indices from global memory
@@ -525,7 +602,9 @@ int cheb_step(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, co
       using TV = decltype(tv);
       return dispatch_dtype(bdt, [&](auto tb) -> int {
          using TB = decltype(tb);
-         constexpr int VEC = 16 / sizeof(TB);
+         // 16-byte lane accesses; 8 bytes (8 elements) for the 1-byte format,
+         // whose 16-element lanes need too many registers for 3 waves per SIMD
+         constexpr int VEC = sizeof(TB) == 1 ? 8 : 16 / sizeof(TB);
          if (k % VEC != 0 || k > kChebMaxK)
             return set_error(ctx, kErrorInvalidArgument, "cheb_step: k must be a multiple of the vector width and <= 256");
          int G = k / VEC;
@@ -537,14 +616,21 @@ int cheb_step(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, co
          dim3 grid((unsigned)sched_grid(nblk, pt));
          int lg = 0;
          while ((1 << lg) < G) ++lg;
-         if (Dp)
-            cheb_step_kernel<TV, TB, VEC, true><<<grid, 256, 0, ctx->stream>>>(
-                n, rp, ci, (const TV *)va, k, (const TB *)Dc, (const TB *)Dp, (const TB *)R, args,
-                (TB *)Dn, lg, nblk, pt);
-         else
-            cheb_step_kernel<TV, TB, VEC, false><<<grid, 256, 0, ctx->stream>>>(
-                n, rp, ci, (const TV *)va, k, (const TB *)Dc, nullptr, (const TB *)R, args,
-                (TB *)Dn, lg, nblk, pt);
+         // 32-bit byte offsets while every block stays below 2^31 bytes
+         const bool off32 = (uint64_t)n * (uint64_t)k * sizeof(TB) < ((uint64_t)1 << 31);
+         auto go = [&](auto has_dp, auto o32) {
+            cheb_step_kernel<TV, TB, VEC, decltype(has_dp)::value, decltype(o32)::value>
+                <<<grid, 256, 0, ctx->stream>>>(n, rp, ci, (const TV *)va, k, (const TB *)Dc,
+                                                (const TB *)Dp, (const TB *)R, args, (TB *)Dn, lg,
+                                                nblk, pt);
+         };
+         using T_ = std::true_type;
+         using F_ = std::false_type;
+         if (Dp) {
+            if (off32) go(T_{}, T_{}); else go(T_{}, F_{});
+         } else {
+            if (off32) go(F_{}, T_{}); else go(F_{}, F_{});
+         }
          OFRR_LAUNCH(ctx);
          return kSuccess;
       });

@@ -30,7 +30,7 @@ def _requests(api):
                         tol=1e-8) for i, (d, N, w, s) in enumerate(MIX)]
 
 
-def _run(api, reqs, slots, budget=0, graphs=True, rr_group_us=0):
+def _run(api, reqs, slots, budget=0, graphs=False, rr_group_us=0):
     with api.Server(0, slots=slots, budget=budget, graphs=graphs, rr_group_us=rr_group_us) as srv:
         if slots == 1:
             out = []
@@ -153,7 +153,7 @@ def test_serve_graph_sweeps_match_plain_launches(api):
     shape); the same stream with plain launches gives bitwise equal results."""
     reqs = _requests(api)[:6]
     print("serve: graphs on", flush=True)
-    on, _ = _run(api, reqs, 2)
+    on, _ = _run(api, reqs, 2, graphs=True)
     print("serve: graphs off", flush=True)
     off, _ = _run(api, _requests(api)[:6], 2, graphs=False)
     for a, b in zip(on, off):
