@@ -499,8 +499,8 @@ typedef struct {
                           than its launches) */
    int rr_group_us;    /* grouped Rayleigh-Ritz: the truncated Rayleigh-Ritz jobs of the
                           running solves that meet run as one batched launch per stage on
-                          the server's own stream (each job's result is the one it gets
-                          alone).  0: the jobs already handed in when a group is formed;
+                          the stream of the solve that formed the group (each job's result
+                          is the one it gets alone).  0: the jobs already handed in when a group is formed;
                           w > 0: a group also waits up to w microseconds for every running
                           solve's job; -1: every solve runs its own */
 } ofrr_server_opts;

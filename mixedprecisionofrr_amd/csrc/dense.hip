@@ -8,15 +8,17 @@
 //   * A X for a symmetric dense A in either storage.
 // They are HBM-bound (arithmetic intensity = nc flops per element of P), so P is
 // streamed once, straight from HBM into MFMA fragments with 16-byte loads, and
-// the small operand Q is staged through LDS, transposed, once per 128-long K
-// chunk and shared by the workgroup's four waves (16 rows of P each).
+// the small operand Q is staged through LDS once per K chunk and shared by the
+// workgroup's four waves (64 rows of P each in the low-precision kernel, whose
+// Q is first transposed once per product so that the staging reads 16-byte
+// rows; 16 rows each in the FP64 kernel).
 //
 //   low   P, Q in BF16 or FP16, FP32 accumulation: v_mfma_f32_16x16x32_{bf16,f16}
 //         (lane l holds P[l%16][8(l/16)+e], Q[8(l/16)+e][l%16]; result register g
 //         of lane l is C[4(l/16)+g][l%16], checked on gfx950 with a probe)
 //   f64   P in FP32 or FP64, Q in FP64, FP64 accumulation: v_mfma_f64_16x16x4
-//         (lane l holds P[l%16][l/16], Q[l/16][l%16]; register g of lane l is
-//         C[4g + l/16][l%16])
+//         (lane l holds one P[l%16][k] and Q[k][l%16] with k from lane group
+//         l/16; register g of lane l is C[4g + l/16][l%16])
 //
 // A long K (A^T Y over the rows of a tall A) is split into S ranges whose
 // partial products are summed in a fixed order by a second kernel, so the
@@ -30,8 +32,6 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kDRows = 64;    // rows of P per workgroup (4 waves x 16)
-constexpr int kDChunk = 128;  // K per LDS stage of Q
 constexpr int kDMaxNc = 128;
 
 template <typename T> struct LowMfma;
@@ -65,137 +65,215 @@ __device__ inline typename LowMfma<T>::vec load8(const T *__restrict__ row, int6
 }
 
 // ---------------------------------------------------------------------------
+// Qt (nc x KT, row c = column c of Q, zero for k >= K) from Q (K x nc row-major):
+// the low-precision kernel reads its B fragments (8 consecutive k of one column)
+// as 16-byte rows of Qt.  Q is the small operand (K nc elements), so this costs
+// one read and one write of it per product.
+template <typename T>
+__global__ void q_transpose_kernel(int64_t K, int nc, const T *__restrict__ Q, int64_t ldq, int64_t KT,
+                                   T *__restrict__ Qt) {
+   __shared__ T tile[64][65];
+   const int64_t k0 = (int64_t)blockIdx.x * 64;
+   const int c0 = blockIdx.y * 64;
+   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+   for (int r = ty; r < 64; r += 4) {
+      const int64_t k = k0 + r;
+      const int c = c0 + tx;
+      tile[r][tx] = (k < K && c < nc) ? Q[k * ldq + c] : T(0);
+   }
+   __syncthreads();
+   for (int r = ty; r < 64; r += 4) {
+      const int c = c0 + r;
+      const int64_t k = k0 + tx;
+      if (c < nc && k < KT) Qt[(int64_t)c * KT + k] = tile[tx][r];
+   }
+}
+
 // low precision: out = alpha * P Q over K range [kb, ke) (+ beta out), or the
-// raw FP32 partial into part[split] when part != nullptr
+// raw FP32 partial into part[split] when part != nullptr.  A workgroup owns
+// 256 rows of P, each of its four waves 64 (four 16-row MFMA tiles), against
+// all nc columns: per 32-long K step a lane loads 16 bytes of each of its four
+// P rows straight from HBM (P is read once) and 16 bytes of Qt per column tile
+// from the LDS chunk the workgroup staged (64 K x nc), and issues 4 ntile MFMAs.
+constexpr int kTallRows = 256;
+constexpr int kTallKC = 64;
 template <typename T, typename TO>
-__global__ __launch_bounds__(256) void dense_low_kernel(
-    int64_t R, int64_t K, int nc, const T *__restrict__ P, int64_t ldp, const T *__restrict__ Q,
-    int64_t ldq, int64_t kper, double alpha, double beta, TO *__restrict__ C, int64_t ldc,
+__global__ __launch_bounds__(256, 2) void dense_low_kernel(
+    int64_t R, int64_t K, int nc, const T *__restrict__ P, int64_t ldp, const T *__restrict__ Qt,
+    int64_t KT, int64_t kper, double alpha, double beta, TO *__restrict__ C, int64_t ldc,
     float *__restrict__ part) {
-   // Q^T chunk: row c holds Q[kc .. kc + 128][c]; rows padded by 16 B so that
-   // the 16 lanes of a fragment read (different c) fall in distinct banks
-   constexpr int QLD = kDChunk + 8;
-   __shared__ __align__(16) T qt[kDMaxNc * QLD];
+   constexpr int QLD = kTallKC + 8;  // 16-byte pad: the 16 columns of a fragment read spread over banks
+   __shared__ __align__(16) T qs[kDMaxNc * QLD];
    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
    const int li = lane & 15, lg = lane >> 4;
-   const int64_t r0 = (int64_t)blockIdx.x * kDRows + 16 * w;
+   const int64_t r0 = (int64_t)blockIdx.x * kTallRows + 64 * w;
    const int64_t kb = (int64_t)blockIdx.y * kper, ke = min(K, kb + kper);
    const int ntile = nc >> 4;
-   f32x4 acc[kDMaxNc / 16];
+   f32x4 acc[4][kDMaxNc / 16];
 #pragma unroll
-   for (int t = 0; t < kDMaxNc / 16; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-   const int64_t prow = min(r0 + li, R - 1);
-   const T *prw = P + prow * ldp;
-   for (int64_t kc = kb; kc < ke; kc += kDChunk) {
+   for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int t = 0; t < kDMaxNc / 16; ++t) acc[rt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+   const T *prw[4];
+#pragma unroll
+   for (int rt = 0; rt < 4; ++rt) prw[rt] = P + min(r0 + 16 * rt + li, R - 1) * ldp;
+   for (int64_t kc = kb; kc < ke; kc += kTallKC) {
       __syncthreads();  // the previous chunk is consumed
-      // stage Q[kc .. kc+128][0 .. nc) transposed (zero past ke)
-      for (int e = threadIdx.x; e < kDChunk * nc; e += 256) {
-         const int kk = e / nc, c = e - kk * nc;
-         const int64_t k = kc + kk;
-         qt[c * QLD + kk] = (k < ke) ? Q[k * ldq + c] : T(0);
+      // Qt[0 .. nc)[kc .. kc + 64) -> qs, 16 bytes per access (Qt is zero past K)
+      for (int e = threadIdx.x; e < nc * (kTallKC / 8); e += 256) {
+         const int c = e / (kTallKC / 8), q = e - c * (kTallKC / 8);
+         *reinterpret_cast<uint4 *>(&qs[c * QLD + 8 * q]) =
+             *reinterpret_cast<const uint4 *>(&Qt[(int64_t)c * KT + kc + 8 * q]);
       }
       __syncthreads();
 #pragma unroll
-      for (int s = 0; s < kDChunk / 32; ++s) {
+      for (int s = 0; s < kTallKC / 32; ++s) {
          const int64_t k = kc + 32 * s + 8 * lg;
-         // past the range end the fragment is zero (both operands)
-         const auto a = (k < ke) ? load8<T>(prw, k, ke) : typename LowMfma<T>::vec{};
+         typename LowMfma<T>::vec a[4];
+#pragma unroll
+         for (int rt = 0; rt < 4; ++rt)
+            a[rt] = (k < ke) ? load8<T>(prw[rt], k, ke) : typename LowMfma<T>::vec{};
 #pragma unroll
          for (int t = 0; t < kDMaxNc / 16; ++t) {
             if (t >= ntile) break;
             typename LowMfma<T>::vec b;
-            const uint4 u = *reinterpret_cast<const uint4 *>(&qt[(16 * t + li) * QLD + 32 * s + 8 * lg]);
+            const uint4 u = *reinterpret_cast<const uint4 *>(&qs[(16 * t + li) * QLD + 32 * s + 8 * lg]);
             __builtin_memcpy(&b, &u, 16);
-            acc[t] = LowMfma<T>::mma(a, b, acc[t]);
+#pragma unroll
+            for (int rt = 0; rt < 4; ++rt) acc[rt][t] = LowMfma<T>::mma(a[rt], b, acc[rt][t]);
          }
       }
    }
-   // register g of lane l: row 4 lg + g, column 16 t + li
+   // register g of lane l: row 4 lg + g, column 16 t + li (of row tile rt)
 #pragma unroll
-   for (int t = 0; t < kDMaxNc / 16; ++t) {
-      if (t >= ntile) break;
+   for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-         const int64_t r = r0 + 4 * lg + g;
-         if (r >= R) continue;
-         const int c = 16 * t + li;
-         if (part) {
-            part[((int64_t)blockIdx.y * R + r) * nc + c] = acc[t][g];
-         } else {
-            TO *o = C + r * ldc + c;
-            double v = alpha * (double)acc[t][g];
-            if (beta != 0.0) v = fma(beta, (double)Elem<TO>::load(*o), v);
-            *o = Elem<TO>::store(v);
+      for (int t = 0; t < kDMaxNc / 16; ++t) {
+         if (t >= ntile) break;
+#pragma unroll
+         for (int g = 0; g < 4; ++g) {
+            const int64_t r = r0 + 16 * rt + 4 * lg + g;
+            if (r >= R) continue;
+            const int c = 16 * t + li;
+            if (part) {
+               part[((int64_t)blockIdx.y * R + r) * nc + c] = acc[rt][t][g];
+            } else {
+               TO *o = C + r * ldc + c;
+               double v = alpha * (double)acc[rt][t][g];
+               if (beta != 0.0) v = fma(beta, (double)Elem<TO>::load(*o), v);
+               *o = Elem<TO>::store(v);
+            }
          }
       }
-   }
 }
 
 // ---------------------------------------------------------------------------
-// FP64 accumulation: P in FP32 or FP64, Q in FP64
+// FP64 accumulation: P in FP32 or FP64, Q in FP64.  A workgroup owns 128 rows
+// of P, each wave 32 (two 16-row tiles), against all nc columns; Q is staged
+// through LDS in its own row-major layout, 32 K at a time, with 16-byte loads.
+// In MFMA step q of a 16-long K block s, lane group g supplies K index
+// 16 s + 4 g + q on both operands, so a lane reads 4 consecutive elements of
+// each of its P rows with one vector load, and the B operand of a step (16
+// consecutive columns of one Q row per lane group) is a conflict-free LDS read.
+constexpr int kF64Rows = 128;
+constexpr int kF64KC = 32;
+template <typename TP> struct P4;
+template <> struct P4<float> {
+   __device__ static void load(const float *p, double a[4]) {
+      const float4 v = *reinterpret_cast<const float4 *>(p);
+      a[0] = v.x; a[1] = v.y; a[2] = v.z; a[3] = v.w;
+   }
+};
+template <> struct P4<double> {
+   __device__ static void load(const double *p, double a[4]) {
+      const double2 u = *reinterpret_cast<const double2 *>(p);
+      const double2 v = *reinterpret_cast<const double2 *>(p + 2);
+      a[0] = u.x; a[1] = u.y; a[2] = v.x; a[3] = v.y;
+   }
+};
 template <typename TP, typename TO>
-__global__ __launch_bounds__(256) void dense_f64_kernel(
+__global__ __launch_bounds__(256, 2) void dense_f64_kernel(
     int64_t R, int64_t K, int nc, const TP *__restrict__ P, int64_t ldp,
     const double *__restrict__ Q, int64_t ldq, int64_t kper, double alpha, double beta,
-    TO *__restrict__ C, int64_t ldc, double *__restrict__ part) {
-   constexpr int KC = 64;        // K per LDS stage
-   constexpr int QLD = KC + 1;   // odd stride: conflict-free 8-byte reads
-   __shared__ double qt[kDMaxNc * QLD];
+    TO *__restrict__ C, int64_t ldc, double *__restrict__ part, int pvec) {
+   constexpr int QLD = kDMaxNc + 2;  // 16-byte aligned rows, staggered banks between rows
+   __shared__ __align__(16) double qs[kF64KC * QLD];
    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
    const int li = lane & 15, lg = lane >> 4;
-   const int64_t r0 = (int64_t)blockIdx.x * kDRows + 16 * w;
+   const int64_t r0 = (int64_t)blockIdx.x * kF64Rows + 32 * w;
    const int64_t kb = (int64_t)blockIdx.y * kper, ke = min(K, kb + kper);
    const int ntile = nc >> 4;
-   f64x4 acc[kDMaxNc / 16];
+   f64x4 acc[2][kDMaxNc / 16];
 #pragma unroll
-   for (int t = 0; t < kDMaxNc / 16; ++t) acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
-   const int64_t prow = min(r0 + li, R - 1);
-   const TP *prw = P + prow * ldp;
-   for (int64_t kc = kb; kc < ke; kc += KC) {
-      __syncthreads();
-      for (int e = threadIdx.x; e < KC * nc; e += 256) {
-         const int kk = e / nc, c = e - kk * nc;
+   for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int t = 0; t < kDMaxNc / 16; ++t) acc[rt][t] = f64x4{0.0, 0.0, 0.0, 0.0};
+   const TP *prw[2];
+#pragma unroll
+   for (int rt = 0; rt < 2; ++rt) prw[rt] = P + min(r0 + 16 * rt + li, R - 1) * ldp;
+   const bool qvec = (ldq % 2 == 0) && (((uintptr_t)Q & 15u) == 0);
+   // 4 P elements from K index k on (zero past ke), one vector load when aligned
+   auto load_p4 = [&](const TP *row, int64_t k, double a[4]) {
+      if (pvec && k + 4 <= ke) {
+         P4<TP>::load(row + k, a);
+      } else {
+#pragma unroll
+         for (int q = 0; q < 4; ++q) a[q] = (k + q < ke) ? (double)row[k + q] : 0.0;
+      }
+   };
+   for (int64_t kc = kb; kc < ke; kc += kF64KC) {
+      __syncthreads();  // the previous chunk is consumed
+      // Q[kc .. kc + 32)[0 .. nc) -> qs (zero past ke), two doubles per access
+      for (int e = threadIdx.x; e < kF64KC * (nc / 2); e += 256) {
+         const int kk = e / (nc / 2), c = 2 * (e - kk * (nc / 2));
          const int64_t k = kc + kk;
-         qt[c * QLD + kk] = (k < ke) ? Q[k * ldq + c] : 0.0;
+         double2 v = make_double2(0.0, 0.0);
+         if (k < ke) {
+            if (qvec) v = *reinterpret_cast<const double2 *>(&Q[k * ldq + c]);
+            else v = make_double2(Q[k * ldq + c], Q[k * ldq + c + 1]);
+         }
+         *reinterpret_cast<double2 *>(&qs[kk * QLD + c]) = v;
       }
       __syncthreads();
-      // lane group lg takes columns kc + 16 s + 4 lg + {0..3} of its P row; the
-      // k slot of step q is column kc + 16 s + 4 lg + q on both operands
 #pragma unroll
-      for (int s = 0; s < KC / 16; ++s) {
+      for (int s = 0; s < kF64KC / 16; ++s) {
          const int64_t k = kc + 16 * s + 4 * lg;
-         double a[4];
+         double a[2][4];
 #pragma unroll
-         for (int q = 0; q < 4; ++q) a[q] = (k + q < ke) ? (double)prw[k + q] : 0.0;
+         for (int rt = 0; rt < 2; ++rt) load_p4(prw[rt], k, a[rt]);
 #pragma unroll
          for (int q = 0; q < 4; ++q)
 #pragma unroll
             for (int t = 0; t < kDMaxNc / 16; ++t) {
                if (t >= ntile) break;
-               const double b = qt[(16 * t + li) * QLD + 16 * s + 4 * lg + q];
-               acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[q], b, acc[t], 0, 0, 0);
+               const double b = qs[(16 * s + 4 * lg + q) * QLD + 16 * t + li];
+#pragma unroll
+               for (int rt = 0; rt < 2; ++rt)
+                  acc[rt][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt][q], b, acc[rt][t], 0, 0, 0);
             }
       }
    }
-   // register g of lane l: row 4 g + lg, column 16 t + li
+   // register g of lane l: row 4 g + lg, column 16 t + li (of row tile rt)
 #pragma unroll
-   for (int t = 0; t < kDMaxNc / 16; ++t) {
-      if (t >= ntile) break;
+   for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-         const int64_t r = r0 + 4 * g + lg;
-         if (r >= R) continue;
-         const int c = 16 * t + li;
-         if (part) {
-            part[((int64_t)blockIdx.y * R + r) * nc + c] = acc[t][g];
-         } else {
-            TO *o = C + r * ldc + c;
-            double v = alpha * acc[t][g];
-            if (beta != 0.0) v = fma(beta, (double)Elem<TO>::load(*o), v);
-            *o = Elem<TO>::store(v);
+      for (int t = 0; t < kDMaxNc / 16; ++t) {
+         if (t >= ntile) break;
+#pragma unroll
+         for (int g = 0; g < 4; ++g) {
+            const int64_t r = r0 + 16 * rt + 4 * g + lg;
+            if (r >= R) continue;
+            const int c = 16 * t + li;
+            if (part) {
+               part[((int64_t)blockIdx.y * R + r) * nc + c] = acc[rt][t][g];
+            } else {
+               TO *o = C + r * ldc + c;
+               double v = alpha * acc[rt][t][g];
+               if (beta != 0.0) v = fma(beta, (double)Elem<TO>::load(*o), v);
+               *o = Elem<TO>::store(v);
+            }
          }
       }
-   }
 }
 
 // C = alpha * sum_s part[s] (+ beta C), splits summed in order
@@ -216,8 +294,8 @@ __global__ void dense_split_reduce_kernel(int64_t R, int nc, int S, const TA *__
 }
 
 // number of K ranges: enough workgroups to fill the chip when R is short
-static int64_t split_count(int64_t R, int64_t K, int64_t chunk) {
-   const int64_t row_blocks = (R + kDRows - 1) / kDRows;
+static int64_t split_count(int64_t R, int64_t K, int64_t chunk, int64_t rows) {
+   const int64_t row_blocks = (R + rows - 1) / rows;
    int64_t S = 1;
    while (row_blocks * S < 2048 && K / (S * 2) >= 4 * chunk) S *= 2;
    return S;
@@ -238,38 +316,59 @@ int dense_gemm(ofrr_ctx *ctx, int64_t R, int64_t K, int nc, const void *P, int p
                        "dense_gemm: operand types must be (BF16, BF16), (FP16, FP16) or (FP32|FP64, FP64)");
    if (low && (ldp % 8 != 0 || ((uintptr_t)P & 15u) != 0))
       return set_error(ctx, kErrorInvalidArgument, "dense_gemm: 16-bit P needs 16-byte aligned rows");
-   const int64_t chunk = low ? kDChunk : 64;
-   const int64_t S = split_count(R, K, chunk);
+   const int64_t chunk = low ? kTallKC : kF64KC;
+   const int64_t rows = low ? kTallRows : kF64Rows;
+   const int64_t S = split_count(R, K, chunk, rows);
    const int64_t kper = ((K + S - 1) / S + chunk - 1) / chunk * chunk;
    const int64_t Sr = (K + kper - 1) / kper;
-   const int64_t rb = (R + kDRows - 1) / kDRows;
+   const int64_t rb = (R + rows - 1) / rows;
    dim3 grid((unsigned)rb, (unsigned)std::max<int64_t>(1, Sr));
    const size_t esz = low ? 4 : 8;
-   void *part = nullptr;
-   if (Sr > 1) {
-      part = workspace(ctx, (size_t)Sr * R * nc * esz);
-      if (!part) return set_error(ctx, kErrorCUDAMemory, "dense_gemm: workspace");
+   // workspace: [Qt (low) | split partials]
+   const int64_t KT = (K + kTallKC - 1) / kTallKC * kTallKC;
+   const size_t qt_bytes = low ? ((size_t)nc * KT * 2 + 255) / 256 * 256 : 0;
+   const size_t part_bytes = Sr > 1 ? (size_t)Sr * R * nc * esz : 0;
+   char *ws = nullptr;
+   if (qt_bytes + part_bytes) {
+      ws = (char *)workspace(ctx, qt_bytes + part_bytes);
+      if (!ws) return set_error(ctx, kErrorCUDAMemory, "dense_gemm: workspace");
+   }
+   void *part = part_bytes ? (void *)(ws + qt_bytes) : nullptr;
+   void *Qt = low ? (void *)ws : nullptr;
+   if (low) {
+      const dim3 tg((unsigned)(KT / 64), (unsigned)((nc + 63) / 64));
+      if (pdt == OFRR_BF16)
+         q_transpose_kernel<__hip_bfloat16><<<tg, 256, 0, ctx->stream>>>(
+             K, nc, (const __hip_bfloat16 *)Q, ldq, KT, (__hip_bfloat16 *)Qt);
+      else
+         q_transpose_kernel<__half><<<tg, 256, 0, ctx->stream>>>(K, nc, (const __half *)Q, ldq, KT,
+                                                                  (__half *)Qt);
+      OFRR_LAUNCH(ctx);
    }
    return dispatch_dtype(cdt, [&](auto to) -> int {
       using TO = decltype(to);
       if (low) {
          if (pdt == OFRR_BF16)
             dense_low_kernel<__hip_bfloat16, TO><<<grid, 256, 0, ctx->stream>>>(
-                R, K, nc, (const __hip_bfloat16 *)P, ldp, (const __hip_bfloat16 *)Q, ldq, kper,
+                R, K, nc, (const __hip_bfloat16 *)P, ldp, (const __hip_bfloat16 *)Qt, KT, kper,
                 alpha, beta, (TO *)C, ldc, (float *)part);
          else
             dense_low_kernel<__half, TO><<<grid, 256, 0, ctx->stream>>>(
-                R, K, nc, (const __half *)P, ldp, (const __half *)Q, ldq, kper, alpha, beta,
+                R, K, nc, (const __half *)P, ldp, (const __half *)Qt, KT, kper, alpha, beta,
                 (TO *)C, ldc, (float *)part);
       } else {
+         // vector loads of 4 P elements when the rows keep 16-byte (FP32) / 8-byte
+         // (FP64, two 16-byte halves need 16) alignment
+         const size_t pes = pdt == OFRR_F32 ? 4 : 8;
+         const int pvec = (((uintptr_t)P % 16) == 0 && ((size_t)ldp * pes) % 16 == 0) ? 1 : 0;
          if (pdt == OFRR_F32)
             dense_f64_kernel<float, TO><<<grid, 256, 0, ctx->stream>>>(
                 R, K, nc, (const float *)P, ldp, (const double *)Q, ldq, kper, alpha, beta,
-                (TO *)C, ldc, (double *)part);
+                (TO *)C, ldc, (double *)part, pvec);
          else
             dense_f64_kernel<double, TO><<<grid, 256, 0, ctx->stream>>>(
                 R, K, nc, (const double *)P, ldp, (const double *)Q, ldq, kper, alpha, beta,
-                (TO *)C, ldc, (double *)part);
+                (TO *)C, ldc, (double *)part, pvec);
       }
       OFRR_LAUNCH(ctx);
       if (Sr > 1) {

@@ -21,10 +21,10 @@
 // Same-phase work of the running solves is grouped: a solve that reaches its
 // truncated Rayleigh-Ritz hands the job (its Gram and projected matrices) to
 // the server's Rayleigh-Ritz group, and the jobs that meet there (optionally
-// within a window) run as one batched launch per stage on the server's own
-// stream, ordered after each job's
-// inputs by an event on its slot's stream and before its slot's next work by
-// another.  Each job's arithmetic is the one it gets alone (rr_truncated_group).
+// within a window) run as one batched launch per stage on the stream of the
+// solve that formed the group, ordered after each other job's inputs by an
+// event on its slot's stream and before that slot's next work by another.
+// Each job's arithmetic is the one it gets alone (rr_truncated_group).
 // On request (ofrr_server_opts.graphs) the filter sweeps of every slot run as
 // captured HIP graphs, one per sweep shape, kept by the slot across requests.
 #include <atomic>
@@ -161,8 +161,6 @@ struct RRGroupExec {
       int status = 0;
       bool launched = false;
    };
-   ofrr_ctx *gctx = nullptr;
-   hipStream_t stream = nullptr;
    std::chrono::microseconds window{0};
    int max_jobs = 1;
    std::atomic<int> *running = nullptr;  // solves running in the server
@@ -173,15 +171,19 @@ struct RRGroupExec {
    int64_t groups = 0, group_jobs = 0;
    int group_max = 0;
 
-   int launch(std::vector<Pending *> &batch) {
+   // the group runs on the leader's stream and workspace: a group of one is the
+   // leader's own Rayleigh-Ritz, and groups of different leaders do not queue
+   // behind each other
+   int launch(ofrr_ctx *lead, std::vector<Pending *> &batch) {
       std::vector<RRJob> jobs;
       for (Pending *p : batch) {
-         OFRR_HIP(gctx, hipStreamWaitEvent(stream, p->slot->ev_rr_ready, 0));
+         if (p->slot != lead) OFRR_HIP(lead, hipStreamWaitEvent(lead->stream, p->slot->ev_rr_ready, 0));
          jobs.push_back(p->job);
       }
-      int st = rr_truncated_group(gctx, (int)jobs.size(), jobs.data());
+      int st = rr_truncated_group(lead, (int)jobs.size(), jobs.data());
       if (st) return st;
-      for (Pending *p : batch) OFRR_HIP(gctx, hipEventRecord(p->slot->ev_rr_done, stream));
+      for (Pending *p : batch)
+         if (p->slot != lead) OFRR_HIP(lead, hipEventRecord(p->slot->ev_rr_done, lead->stream));
       return kSuccess;
    }
 
@@ -190,6 +192,7 @@ struct RRGroupExec {
       if (!slot->ev_rr_done) OFRR_HIP(slot, hipEventCreateWithFlags(&slot->ev_rr_done, hipEventDisableTiming));
       OFRR_HIP(slot, hipEventRecord(slot->ev_rr_ready, slot->stream));
       Pending me;
+      bool led = false;
       me.job = job;
       me.slot = slot;
       std::unique_lock<std::mutex> lk(mu);
@@ -211,7 +214,8 @@ struct RRGroupExec {
             pending.pop_front();
          }
          lk.unlock();
-         const int st = launch(batch);
+         led = true;
+         const int st = launch(slot, batch);
          lk.lock();
          for (Pending *p : batch) {
             p->status = st;
@@ -224,8 +228,8 @@ struct RRGroupExec {
          cv.notify_all();
       }
       lk.unlock();
-      if (me.status) return set_error(slot, me.status, "grouped Rayleigh-Ritz: " + gctx->last_error);
-      OFRR_HIP(slot, hipStreamWaitEvent(slot->stream, slot->ev_rr_done, 0));
+      if (me.status) return set_error(slot, me.status, "grouped Rayleigh-Ritz failed");
+      if (!led) OFRR_HIP(slot, hipStreamWaitEvent(slot->stream, slot->ev_rr_done, 0));
       return kSuccess;
    }
    static int entry(void *arg, ofrr_ctx *slot, const void *job) {
@@ -342,7 +346,10 @@ void ofrr_server::worker(int slot) {
    if (!st && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) st = kErrorCUDAGeneral;
    if (!st) st = ofrr_ctx_set_stream(ctx, s);
    // the workspace every solve of this slot reuses, reserved once up front
-   if (!st && !workspace(ctx, filtered_workspace_bound(ctx->num_cus))) st = kErrorCUDAMemory;
+   // (a slot that leads a grouped Rayleigh-Ritz runs the group in its workspace)
+   size_t reserve = filtered_workspace_bound(ctx ? ctx->num_cus : 256);
+   if (rr) reserve = std::max(reserve, rr_group_workspace_bytes(rr->max_jobs, 384));
+   if (!st && !workspace(ctx, reserve)) st = kErrorCUDAMemory;
    if (!st && rr) {
       ctx->rr_group = &RRGroupExec::entry;
       ctx->rr_group_arg = rr;
@@ -455,25 +462,11 @@ int ofrr_server_create(int device, const ofrr_server_opts *opt, ofrr_server **ou
    uint64_t keep = UINT64_MAX;
    (void)hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold, &keep);
    if (!(opt && opt->rr_group_us < 0) && slots > 1) {
-      // the group's context, stream and workspace (every slot's largest job)
       auto *g = new RRGroupExec();
       g->window = std::chrono::microseconds((opt && opt->rr_group_us > 0) ? opt->rr_group_us : 0);
       g->max_jobs = std::min(slots, kMaxRRJobs);
       g->running = &s->running_now;
-      int gst = ofrr_ctx_create(device, &g->gctx);
-      if (!gst && hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) gst = kErrorCUDAGeneral;
-      if (!gst) gst = ofrr_ctx_set_stream(g->gctx, g->stream);
-      if (!gst && !workspace(g->gctx, rr_group_workspace_bytes(g->max_jobs, 384))) gst = kErrorCUDAMemory;
-      if (gst) {
-         if (g->gctx) ofrr_ctx_destroy(g->gctx);
-         if (g->stream) (void)hipStreamDestroy(g->stream);
-         delete g;
-         (void)hipMemPoolDestroy(s->pool);
-         delete s;
-         return gst;
-      }
       s->rr = g;
-      s->slot_bytes += (int64_t)g->gctx->ws_bytes;
    }
    s->worker_status.assign(slots, -1);
    try {
@@ -588,11 +581,7 @@ int ofrr_server_destroy(ofrr_server *s) {
    for (auto &t : s->workers)
       if (t.joinable()) t.join();
    (void)hipSetDevice(s->device);
-   if (s->rr) {
-      ofrr_ctx_destroy(s->rr->gctx);
-      if (s->rr->stream) (void)hipStreamDestroy(s->rr->stream);
-      delete s->rr;
-   }
+   delete s->rr;
    if (s->pool) (void)hipMemPoolDestroy(s->pool);
    delete s;
    return kSuccess;
