@@ -234,7 +234,7 @@ def main():
                    help="growth-block format of the mixed path (the workload names FP16)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
-    p.add_argument("--workload", choices=["eig", "svd", "serve", "qr", "graphs"], default="eig",
+    p.add_argument("--workload", choices=["eig", "svd", "serve", "qr", "graphs", "syev"], default="eig",
                    help="eig: the 128^3 Laplacian eigensolve (default); svd: truncated SVD of "
                         "the dense 262,144 x 4,096 low-rank-plus-noise matrix; serve: a stream "
                         "of eigen requests with Poisson arrivals through the serving loop")
@@ -280,6 +280,8 @@ def main():
         return qr_main(args)
     if args.workload == "graphs":
         return graphs_main(args)
+    if args.workload == "syev":
+        return syev_main(args)
     if args.workload == "serve":
         r = serve_stream(api, args.serve_requests, args.serve_rate, args.serve_slots,
                          args.serve_seed, big=args.serve_big, graphs=args.serve_graphs == "on",
@@ -661,6 +663,46 @@ def graphs_main(args):
             log(json.dumps(rows[-1]))
     print(json.dumps({"metric": "filter sweep device time per Chebyshev step, graph replay vs plain launches",
                       "unit": "us", "rows": rows}), flush=True)
+
+
+def syev_main(args):
+    """Device eigensolver (the Rayleigh-Ritz Jacobi) by size: a graded symmetric
+    matrix per m (eigenvalues log-spaced over 1e-9 .. 60 with seeded random
+    orthogonal eigenvectors, scaled to unit diagonal like the solver's Gram
+    matrix), one ofrr_syev_batched call timed (best of 3 after a warmup).  The
+    sweep count is the oracle's, whose Jacobi rotates identically."""
+    import numpy as np
+    import torch
+
+    from mixedprecisionofrr_amd import api
+    from oracle import oracle as O
+    ctx = api.Context(0)
+    rows = []
+    for m in (96, 128, 160, 192, 256, 300, 384):
+        rng = np.random.default_rng(m)
+        Qm = np.linalg.qr(rng.standard_normal((m, m)))[0]
+        A = (Qm * np.logspace(-9, np.log10(60.0), m)) @ Qm.T
+        sc = 1.0 / np.sqrt(np.diag(A))
+        A = A * sc[:, None] * sc[None, :]
+        A = 0.5 * (A + A.T)
+        At = torch.tensor(A, device="cuda:0")
+        api.syev_batched(ctx, At)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            t = time.perf_counter()
+            api.syev_batched(ctx, At)
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t) * 1e3)
+        w = np.zeros(m)
+        V = np.zeros((m, m), order="F")
+        sweeps = O.lib().orc_syev(m, np.asfortranarray(A), m, w, V, m)
+        ms = min(ts)
+        rows.append({"m": m, "ms": ms, "sweeps": int(sweeps),
+                     "us_per_round": 1e3 * ms / max(1, (sweeps + 1) * (m - 1 + (m & 1)))})
+        log(json.dumps(rows[-1]))
+    print(json.dumps({"metric": "symmetric eigensolve time by size (Jacobi, one workgroup)",
+                      "unit": "ms", "rows": rows}), flush=True)
 
 
 def qr_main(args):
