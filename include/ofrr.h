@@ -244,6 +244,18 @@ OFRR_API int ofrr_sygv(ofrr_ctx *ctx, int n, const double *A_dev, int lda, const
 OFRR_API int ofrr_rr_truncated(ofrr_ctx *ctx, int m, const double *G_dev, int ldg,
                                const double *H_dev, int ldh, double tau, int nkeep,
                                double *theta_dev, double *Y_dev, int ldy, int32_t *rank_dev);
+/* The same Rayleigh-Ritz for a basis V = [X | D] whose first kx columns
+ * (kx <= 128) are a previous Ritz block, in split form (the filtered solver's
+ * default): G_XX = L L^T; S = G_DD - G_XD^T G_XX^{-1} G_XD scaled to unit
+ * diagonal; eig(S) = U L, keep L_i > tau * max(1, max L); W = [[L^{-T}, -G_XX^{-1}
+ * G_XD B_D], [0, B_D]] with B_D = s U_r L_r^{-1/2}; eig of the D block of W^T H W,
+ * then of the coupled matrix -> theta (nkeep, descending), Y (m x nkeep) with
+ * Y^T G Y = I.  rank_dev: kx + the kept D rank, or -1 when G_XX is not positive
+ * definite (theta and Y are then not written).  No counterpart in the reference:
+ * a faster route to rr_eig's pencil for the solver's bases. */
+OFRR_API int ofrr_rr_split(ofrr_ctx *ctx, int m, int kx, const double *G_dev, int ldg,
+                           const double *H_dev, int ldh, double tau, int nkeep, double *theta_dev,
+                           double *Y_dev, int ldy, int32_t *rank_dev);
 /* the reference's generalised form eig(H, G + gtol I) (eig_rr.m:52), sorted
  * descending; all m pairs */
 OFRR_API int ofrr_rr_generalized(ofrr_ctx *ctx, int m, const double *G_dev, int ldg,
@@ -332,6 +344,13 @@ typedef struct {
                         while its shape repeats (the context keeps the graphs; a sweep
                         of the same shape on other buffers updates the graph's
                         arguments); results are the same as with plain launches */
+   int rr_whole;     /* 1: every Rayleigh-Ritz eigendecomposes the Gram matrix of the
+                        whole basis [X | D] (ofrr_rr_truncated).  0 (default): after
+                        the first, the split form -- the Ritz block X is whitened by
+                        the Cholesky factor of X^T X and only the Gram matrix of D with
+                        its X component removed is eigendecomposed (size m - k), then
+                        the D block of the projected matrix before the coupled one; the
+                        whole form is used where X^T X is not positive definite */
 } ofrr_filter_opts;
 
 typedef struct {
@@ -352,6 +371,8 @@ typedef struct {
    int filter_first_cols;/* sum over the sweeps of the columns of the first step */
    int graph_replays;    /* sweeps launched as one captured graph (opts.graph) */
    int graph_captures;   /* of those, sweeps recorded anew (first of a shape, or new buffers) */
+   int rr_fallbacks;     /* split Rayleigh-Ritz steps redone in the whole form (X^T X
+                            not positive definite) */
 } ofrr_filter_stats;
 
 /* X_dev: n x k FP64 row-major (row stride k), start block in / Ritz block out;

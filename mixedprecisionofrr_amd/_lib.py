@@ -19,7 +19,7 @@ EXPORTS = [
     "ofrr_fill_randn", "ofrr_fill_randu", "ofrr_stencil_nnz", "ofrr_stencil_csr",
     "ofrr_spmm_csr", "ofrr_iamax", "ofrr_matrix_iamax", "ofrr_scalecols", "ofrr_check_special",
     "ofrr_hessenberg", "ofrr_gram", "ofrr_basis_update", "ofrr_syev_batched", "ofrr_sygv",
-    "ofrr_rr_truncated", "ofrr_rr_generalized", "ofrr_residuals", "ofrr_arnoldi_hess_eig",
+    "ofrr_rr_truncated", "ofrr_rr_split", "ofrr_rr_generalized", "ofrr_residuals", "ofrr_arnoldi_hess_eig",
     "ofrr_subspace_iter_eig", "ofrr_filtered_eig", "ofrr_dense_gemm", "ofrr_convert",
     "ofrr_filtered_eig_dense", "ofrr_svd_filtered", "ofrr_filtered_eig_state_bytes",
     "ofrr_request_footprint", "ofrr_server_create", "ofrr_server_submit", "ofrr_server_poll",
@@ -40,7 +40,8 @@ class FilterOpts(C.Structure):
                 ("maxiter", C.c_int), ("basis_dtype", C.c_int), ("tol", C.c_double),
                 ("tau", C.c_double), ("lower", C.c_double), ("normA", C.c_double),
                 ("refine", C.c_int), ("refine_dtype", C.c_int), ("refine_tol", C.c_double),
-                ("lock", C.c_int), ("mid_cols", C.c_int), ("graph", C.c_int)]
+                ("lock", C.c_int), ("mid_cols", C.c_int), ("graph", C.c_int),
+                ("rr_whole", C.c_int)]
 
 
 class FilterStats(C.Structure):
@@ -50,7 +51,8 @@ class FilterStats(C.Structure):
                 ("t_zspmm_ms", C.c_double), ("t_gram_ms", C.c_double), ("t_rr_ms", C.c_double),
                 ("t_update_ms", C.c_double), ("t_resid_ms", C.c_double),
                 ("filter_col_steps", C.c_int), ("filter_first_cols", C.c_int),
-                ("graph_replays", C.c_int), ("graph_captures", C.c_int)]
+                ("graph_replays", C.c_int), ("graph_captures", C.c_int),
+                ("rr_fallbacks", C.c_int)]
 
 
 class ServerOpts(C.Structure):
@@ -137,6 +139,7 @@ def _decl(L):
     L.ofrr_syev_batched.argtypes = [vp, i32, i32, vp, i32, i64, vp, i64, vp, i32, i64]
     L.ofrr_sygv.argtypes = [vp, i32, vp, i32, vp, i32, vp, vp, i32]
     L.ofrr_rr_truncated.argtypes = [vp, i32, vp, i32, vp, i32, dbl, i32, vp, vp, i32, vp]
+    L.ofrr_rr_split.argtypes = [vp, i32, i32, vp, i32, vp, i32, dbl, i32, vp, vp, i32, vp]
     L.ofrr_rr_generalized.argtypes = [vp, i32, vp, i32, vp, i32, dbl, vp, vp, i32]
     L.ofrr_residuals.argtypes = [vp, i64, i64, vp, vp, vp, i32, vp, i64, i64,
                                  C.POINTER(dbl), dbl, C.POINTER(dbl)]

@@ -429,6 +429,21 @@ def rr_truncated(ctx, G, H, tau, nkeep):
     return th, Y, int(rk.item())
 
 
+def rr_split(ctx, G, H, kx, tau, nkeep):
+    """Split truncated Rayleigh-Ritz over a basis whose first kx columns are a
+    previous Ritz block: (theta desc, Y, rank); rank -1 when G[:kx, :kx] is not
+    positive definite (theta and Y are then not written)."""
+    m = G.shape[0]
+    Gc = G.t().contiguous().t() if G.stride(0) != 1 else G
+    Hc = H.t().contiguous().t() if H.stride(0) != 1 else H
+    th = torch.full((nkeep,), float("nan"), dtype=torch.float64, device=G.device)
+    Y = colmajor(m, nkeep, torch.float64, G.device)
+    rk = torch.zeros(1, dtype=torch.int32, device=G.device)
+    ctx.check(ctx.lib.ofrr_rr_split(ctx.h, m, kx, _p(Gc), Gc.stride(1), _p(Hc), Hc.stride(1), tau,
+                                    nkeep, _p(th), _p(Y), m, _p(rk)), "rr_split")
+    return th, Y, int(rk.item())
+
+
 def rr_generalized(ctx, G, H, gtol=0.0):
     """eig(H, G + gtol I) sorted descending (eig_rr.m:52)."""
     m = G.shape[0]
@@ -498,12 +513,15 @@ def subspace_iter_eig(ctx, A, V0, maxiter, step_size=1, select_tol=2.0 ** -52, g
 
 def filtered_eig(ctx, A, X0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=torch.float16,
                  tol=1e-8, tau=1e-12, lower=0.0, normA=0.0, use_lo_values=True,
-                 refine_dtype=None, refine_tol=0.0, lock=True, mid_cols=0, graph=False):
+                 refine_dtype=None, refine_tol=0.0, lock=True, mid_cols=0, graph=False,
+                 rr_whole=False):
     """Filtered OFRR solve; X0 (n x k, FP64) is copied.  refine_dtype: the
     precision ladder's refinement format (None: off).  lock: soft locking (a
     sweep filters only the columns after the leading converged wanted pairs).
     graph: each filter sweep runs as a captured HIP graph, replayed while its
-    shape repeats (same results as plain launches).
+    shape repeats (same results as plain launches).  rr_whole: every
+    Rayleigh-Ritz eigendecomposes the Gram matrix of the whole basis instead of
+    the split form (X whitened by Cholesky, the D block's Schur complement).
     Returns (status, theta, X row-major, res, stats dict)."""
     A.require_square("filtered_eig")
     _check_rows(X0, A.n, "filtered_eig", "X0")
@@ -513,7 +531,7 @@ def filtered_eig(ctx, A, X0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=to
     o = FilterOpts(k, nwant, nd, degree, maxiter, bdt, tol, tau, lower, normA,
                    0 if refine_dtype is None else 1,
                    0 if refine_dtype is None else dtype_code(refine_dtype), refine_tol,
-                   1 if lock else 0, mid_cols, 1 if graph else 0)
+                   1 if lock else 0, mid_cols, 1 if graph else 0, 1 if rr_whole else 0)
     th = (C.c_double * k)()
     res = (C.c_double * k)()
     stats = FilterStats()

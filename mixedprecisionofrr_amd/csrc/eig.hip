@@ -720,6 +720,416 @@ int rr_truncated_group(ofrr_ctx *ctx, int njobs, const RRJob *jobs) {
 }
 
 // ---------------------------------------------------------------------------
+// Split truncated Rayleigh-Ritz for a basis V = [X | D] whose first kx columns
+// are the previous Ritz block (X^T X ~ I): the same pencil (G, H), reached by
+// a congruence that keeps every eigendecomposition small.
+//   G_XX = L L^T (Cholesky); C = G_XX^{-1} G_XD; S = G_DD - G_XD^T C, the Gram
+//   matrix of D with its X component removed, scaled to unit diagonal by s;
+//   eig(s S s) = U Lam, kept Lam_i > tau max(1, max Lam) (the X directions
+//   have unit scaled eigenvalues) -> B_D = s U_r Lam_r^{-1/2};
+//   W = [[L^{-T}, -C B_D], [0, B_D]] so that W^T G W = I on the kept directions;
+//   Hr = W^T H W; eig(Hr_DD) = Q hd first (Hr_XX is already near diag(theta) as
+//   X holds Ritz vectors); Hr2 = [[Hr_XX, Hr_XD Q], [., diag(hd)]];
+//   eig(Hr2) = Z th; Y = W blkdiag(I, Q) Z for the top nkeep, descending.
+// The D-block Gram eigendecomposition is of size m - kx instead of m, and the
+// coupled one starts from diagonal blocks, so it needs few sweeps.  Jobs of a
+// group run one launch per stage (job = blockIdx.y / .z) in their own work
+// areas (leading dimension mx); each job's arithmetic is the oracle's
+// orc_rr_split: every sum a sequential FMA chain in increasing index order.
+// A job whose G_XX is not positive definite writes rank -1 and skips its
+// eigensolves (the caller falls back to rr_truncated).
+struct RRSplitArgs {
+   int n, mx, kxm;
+   int64_t slab;  // doubles per job area
+   RRJob j[kMaxRRJobs];
+};
+// job area (ld mx): L (kxm x kxm) | Cc (kxm x mx) | S (S, then T, then H2) |
+// Ss (Ss, then Hr, then Z) | U (U, then Q) | W | W2 (Hd, then W2) | sd lam hd th
+// (mx each)
+struct RRSplitArea {
+   double *L, *Cc, *S, *Ss, *U, *W, *W2, *sd, *lam, *hd, *th;
+};
+__host__ __device__ inline RRSplitArea rrs_area(double *base, int mx, int kxm, int64_t slab, int b) {
+   RRSplitArea a;
+   const int64_t mm = (int64_t)mx * mx;
+   a.L = base + b * slab;
+   a.Cc = a.L + (int64_t)kxm * kxm;
+   a.S = a.Cc + (int64_t)kxm * mx;
+   a.Ss = a.S + mm;
+   a.U = a.Ss + mm;
+   a.W = a.U + mm;
+   a.W2 = a.W + mm;
+   a.sd = a.W2 + mm;
+   a.lam = a.sd + mx;
+   a.hd = a.lam + mx;
+   a.th = a.hd + mx;
+   return a;
+}
+__host__ __device__ inline int64_t rrs_slab(int mx, int kxm) {
+   return ((int64_t)kxm * kxm + (int64_t)kxm * mx + 5 * (int64_t)mx * mx + 4 * (int64_t)mx + 1) &
+          ~(int64_t)1;
+}
+// per-job device ints: fail | md | r | mr (each njobs long)
+__device__ inline double rrs_g(const RRJob &J, int i, int j) {
+   return 0.5 * (J.G[i + (int64_t)j * J.ldg] + J.G[j + (int64_t)i * J.ldg]);
+}
+__device__ inline double rrs_h(const RRJob &J, int i, int j) {
+   return 0.5 * (J.H[i + (int64_t)j * J.ldh] + J.H[j + (int64_t)i * J.ldh]);
+}
+
+// Cholesky of G_XX in LDS (right-looking, the oracle's order); L (lower, ld
+// kxm) to the job area
+__global__ __launch_bounds__(1024) void rrs_chol_kernel(RRSplitArgs a, double *base, int32_t *ints) {
+   extern __shared__ double Ls[];
+   __shared__ int s_fail;
+   const int b = blockIdx.x, nj = a.n;
+   const RRJob &J = a.j[b];
+   const int kx = J.kx, m = J.m, md = m - kx;
+   const RRSplitArea A = rrs_area(base, a.mx, a.kxm, a.slab, b);
+#define LS(i, j) Ls[(i) + (j) * kx]
+   if (threadIdx.x == 0) s_fail = 0;
+   for (int t = threadIdx.x; t < kx * kx; t += blockDim.x) {
+      const int i = t % kx, j = t / kx;
+      Ls[t] = (i >= j) ? rrs_g(J, i, j) : 0.0;
+   }
+   __syncthreads();
+   for (int j = 0; j < kx; ++j) {
+      const double d = LS(j, j);  // every thread reads it before anyone writes column j
+      if (!(d > 0.0)) {
+         if (threadIdx.x == 0) s_fail = 1;
+         break;  // uniform: d is the same LDS value for every thread
+      }
+      const double sq = sqrt(d);
+      __syncthreads();
+      for (int i = j + threadIdx.x; i < kx; i += blockDim.x) LS(i, j) = (i == j) ? sq : LS(i, j) / sq;
+      __syncthreads();
+      const int w = kx - j - 1;
+      for (int t = threadIdx.x; t < w * w; t += blockDim.x) {
+         const int i = j + 1 + t % w, c = j + 1 + t / w;
+         if (i >= c) LS(i, c) = fma(-LS(i, j), LS(c, j), LS(i, c));
+      }
+      __syncthreads();
+   }
+   __syncthreads();
+   const int fail = s_fail;
+   if (threadIdx.x == 0) {
+      ints[b] = fail;
+      ints[nj + b] = fail ? 0 : md;  // the D-block eigensolve's size
+      if (fail) *J.rank = -1;
+   }
+   if (fail) return;
+   for (int t = threadIdx.x; t < kx * kx; t += blockDim.x) {
+      const int i = t % kx, j = t / kx;
+      A.L[i + (int64_t)j * a.kxm] = Ls[t];
+   }
+#undef LS
+}
+
+// Triangular solves with L, one wave per column, right-looking with the
+// column in registers (lane l holds rows l and l + 64): column c < md gives
+// C(:, c) = L^{-T} L^{-1} G_XD(:, c); column md + c' gives W(0:kx, c') =
+// L^{-T} e_c'.  The forward pass adds the terms of each row in increasing
+// index order, the backward pass in decreasing order: the oracle's chains.
+__global__ __launch_bounds__(64) void rrs_solve_kernel(RRSplitArgs a, double *base, const int32_t *ints) {
+   const int b = blockIdx.y;
+   if (ints[b]) return;
+   const RRJob &J = a.j[b];
+   const int kx = J.kx, md = J.m - kx, col = blockIdx.x;
+   if (col >= md + kx) return;
+   const RRSplitArea A = rrs_area(base, a.mx, a.kxm, a.slab, b);
+   const int ldl = a.kxm, lane = threadIdx.x;
+   const double *L = A.L;
+   const bool rhs = col < md;
+   double x0 = 0.0, x1 = 0.0;  // rows lane, lane + 64
+   if (rhs) {
+      if (lane < kx) x0 = rrs_g(J, lane, kx + col);
+      if (lane + 64 < kx) x1 = rrs_g(J, lane + 64, kx + col);
+      for (int i = 0; i < kx; ++i) {
+         const double piv = L[i + (int64_t)i * ldl];
+         double xi = (i < 64) ? __shfl(x0, i, 64) : __shfl(x1, i - 64, 64);
+         xi = xi / piv;
+         if (lane == (i & 63)) {
+            if (i < 64) x0 = xi;
+            else x1 = xi;
+         }
+         if (lane > i && lane < kx) x0 = fma(-L[lane + (int64_t)i * ldl], xi, x0);
+         if (lane + 64 > i && lane + 64 < kx) x1 = fma(-L[lane + 64 + (int64_t)i * ldl], xi, x1);
+      }
+   } else {
+      const int c = col - md;
+      x0 = (lane == c) ? 1.0 : 0.0;
+      x1 = (lane + 64 == c) ? 1.0 : 0.0;
+   }
+   for (int p = kx - 1; p >= 0; --p) {
+      const double piv = L[p + (int64_t)p * ldl];
+      double xp = (p < 64) ? __shfl(x0, p, 64) : __shfl(x1, p - 64, 64);
+      xp = xp / piv;
+      if (lane == (p & 63)) {
+         if (p < 64) x0 = xp;
+         else x1 = xp;
+      }
+      if (lane < p) x0 = fma(-L[p + (int64_t)lane * ldl], xp, x0);
+      if (lane + 64 < p) x1 = fma(-L[p + (int64_t)(lane + 64) * ldl], xp, x1);
+   }
+   double *out = rhs ? A.Cc + (int64_t)col * a.kxm : A.W + (int64_t)(col - md) * a.mx;
+   if (lane < kx) out[lane] = x0;
+   if (lane + 64 < kx) out[lane + 64] = x1;
+}
+
+// S(i, j) = G_DD(i, j) - G_XD(:, i)^T C(:, j), upper triangle (i <= j)
+__global__ void rrs_schur_kernel(RRSplitArgs a, double *base, const int32_t *ints) {
+   const int b = blockIdx.y;
+   if (ints[b]) return;
+   const RRJob &J = a.j[b];
+   const int kx = J.kx, md = J.m - kx, mx = a.mx;
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= md * md) return;
+   const int i = t % md, j = t / md;
+   if (i > j) return;
+   const RRSplitArea A = rrs_area(base, mx, a.kxm, a.slab, b);
+   const double *cj = A.Cc + (int64_t)j * a.kxm;
+   double s2 = rrs_g(J, kx + i, kx + j);
+   for (int p = 0; p < kx; ++p) s2 = fma(-rrs_g(J, p, kx + i), cj[p], s2);
+   A.S[i + (int64_t)j * mx] = s2;
+}
+
+// sd = diag(S)^{-1/2} (0 for a non-positive entry); Ss = S(i, j) sd_i sd_j (upper)
+__global__ void rrs_sscale_kernel(RRSplitArgs a, double *base, const int32_t *ints) {
+   const int b = blockIdx.y;
+   if (ints[b]) return;
+   const RRJob &J = a.j[b];
+   const int md = J.m - J.kx, mx = a.mx;
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= md * md) return;
+   const int i = t % md, j = t / md;
+   if (i > j) return;
+   const RRSplitArea A = rrs_area(base, mx, a.kxm, a.slab, b);
+   const double di = A.S[i + (int64_t)i * mx], dj = A.S[j + (int64_t)j * mx];
+   const double si = di > 0.0 ? 1.0 / sqrt(di) : 0.0, sj = dj > 0.0 ? 1.0 / sqrt(dj) : 0.0;
+   if (i == j) A.sd[i] = si;
+   A.Ss[i + (int64_t)j * mx] = A.S[i + (int64_t)j * mx] * si * sj;
+}
+
+// kept D rank r; W(kx:, kx + c) = U(:, src) / sqrt(lam_src) * sd, W(kx:, 0:kx) = 0
+__global__ void rrs_trunc_kernel(RRSplitArgs a, double *base, int32_t *ints) {
+   __shared__ int s_r;
+   const int b = blockIdx.x, nj = a.n;
+   if (ints[b]) {
+      if (threadIdx.x == 0) ints[2 * nj + b] = ints[3 * nj + b] = 0;
+      return;
+   }
+   const RRJob &J = a.j[b];
+   const int kx = J.kx, md = J.m - kx, mx = a.mx;
+   const RRSplitArea A = rrs_area(base, mx, a.kxm, a.slab, b);
+   if (threadIdx.x == 0) {
+      const double lmax = (md > 0 && A.lam[md - 1] > 1.0) ? A.lam[md - 1] : 1.0;
+      int r = 0;
+      for (int i = 0; i < md; ++i) r += (A.lam[i] > J.tau * lmax) ? 1 : 0;
+      s_r = r;
+      ints[2 * nj + b] = r;
+      ints[3 * nj + b] = kx + r;
+      *J.rank = kx + r;
+   }
+   __syncthreads();
+   const int r = s_r;
+   for (int t = threadIdx.x; t < md * (kx + r); t += blockDim.x) {
+      const int i = t % md, c = t / md;
+      double v = 0.0;
+      if (c >= kx) {
+         const int src = md - r + (c - kx);
+         v = A.U[i + (int64_t)src * mx] / sqrt(A.lam[src]) * A.sd[i];
+      }
+      A.W[kx + i + (int64_t)c * mx] = v;
+   }
+}
+
+// W(0:kx, kx + c) = -C B_D(:, c)
+__global__ void rrs_wxd_kernel(RRSplitArgs a, double *base, const int32_t *ints) {
+   const int b = blockIdx.y, nj = a.n;
+   if (ints[b]) return;
+   const RRJob &J = a.j[b];
+   const int kx = J.kx, md = J.m - kx, mx = a.mx, r = ints[2 * nj + b];
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= kx * r) return;
+   const int i = t % kx, c = t / kx;
+   const RRSplitArea A = rrs_area(base, mx, a.kxm, a.slab, b);
+   const double *bd = A.W + kx + (int64_t)(kx + c) * mx;
+   double s2 = 0.0;
+   for (int p = 0; p < md; ++p) s2 = fma(A.Cc[i + (int64_t)p * a.kxm], bd[p], s2);
+   A.W[i + (int64_t)(kx + c) * mx] = -s2;
+}
+
+// stage 0: T = H W (m x mr, into S); stage 1: Hr = W^T T (mr x mr, into Ss), and
+// its D block copied to W2 for the eigensolver
+__global__ void rrs_gemm_kernel(RRSplitArgs a, double *base, const int32_t *ints, int stage) {
+   const int b = blockIdx.z, nj = a.n;
+   if (ints[b]) return;
+   const RRJob &J = a.j[b];
+   const int m = J.m, kx = J.kx, mx = a.mx, mr = ints[3 * nj + b];
+   const int i = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+   const RRSplitArea A = rrs_area(base, mx, a.kxm, a.slab, b);
+   double s2 = 0.0;
+   if (stage == 0) {
+      if (i >= m || c >= mr) return;
+      const double *w = A.W + (int64_t)c * mx;
+      for (int p = 0; p < m; ++p) s2 = fma(rrs_h(J, i, p), w[p], s2);
+      A.S[i + (int64_t)c * mx] = s2;
+   } else {
+      if (i >= mr || c >= mr) return;
+      const double *w = A.W + (int64_t)i * mx, *tc = A.S + (int64_t)c * mx;
+      for (int p = 0; p < m; ++p) s2 = fma(w[p], tc[p], s2);
+      A.Ss[i + (int64_t)c * mx] = s2;
+      if (i >= kx && c >= kx) A.W2[(i - kx) + (int64_t)(c - kx) * mx] = s2;
+   }
+}
+
+// Hr2 (upper, into S): [[Hr_XX, Hr_XD Q], [., diag(hd)]]; W2 = W blkdiag(I, Q)
+__global__ void rrs_couple_kernel(RRSplitArgs a, double *base, const int32_t *ints) {
+   const int b = blockIdx.y, nj = a.n;
+   if (ints[b]) return;
+   const RRJob &J = a.j[b];
+   const int m = J.m, kx = J.kx, mx = a.mx, r = ints[2 * nj + b], mr = kx + r;
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= m * mr) return;
+   const int i = t % m, c = t / m;
+   const RRSplitArea A = rrs_area(base, mx, a.kxm, a.slab, b);
+   const double *Hr = A.Ss, *Q = A.U;
+   if (i <= c && i < mr) {
+      double v;
+      if (c < kx) {
+         v = Hr[i + (int64_t)c * mx];
+      } else if (i < kx) {
+         double s2 = 0.0;
+         for (int p = 0; p < r; ++p) s2 = fma(Hr[i + (int64_t)(kx + p) * mx], Q[p + (int64_t)(c - kx) * mx], s2);
+         v = s2;
+      } else {
+         v = (i == c) ? A.hd[i - kx] : 0.0;
+      }
+      A.S[i + (int64_t)c * mx] = v;
+   }
+   double v = A.W[i + (int64_t)c * mx];
+   if (c >= kx) {
+      double s2 = 0.0;
+      for (int p = 0; p < r; ++p) s2 = fma(A.W[i + (int64_t)(kx + p) * mx], Q[p + (int64_t)(c - kx) * mx], s2);
+      v = s2;
+   }
+   A.W2[i + (int64_t)c * mx] = v;
+}
+
+// theta (descending) and Y = W2 Z for the top nkeep
+__global__ void rrs_finish_kernel(RRSplitArgs a, double *base, const int32_t *ints) {
+   const int b = blockIdx.y, nj = a.n, c = blockIdx.x;
+   const RRJob &J = a.j[b];
+   if (c >= J.nkeep || ints[b]) return;
+   const int m = J.m, mx = a.mx, mr = ints[3 * nj + b];
+   const RRSplitArea A = rrs_area(base, mx, a.kxm, a.slab, b);
+   const double *Z = A.Ss;
+   for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      double s2 = 0.0;
+      if (c < mr) {
+         const int src = mr - 1 - c;
+         for (int p = 0; p < mr; ++p) s2 = fma(A.W2[i + (int64_t)p * mx], Z[p + (int64_t)src * mx], s2);
+      }
+      J.Y[i + (int64_t)c * J.ldy] = s2;
+   }
+   if (threadIdx.x == 0) J.theta[c] = (c < mr) ? A.th[mr - 1 - c] : __builtin_nan("");
+}
+
+size_t rr_split_workspace_bytes(int njobs, int mx, int kxm) {
+   return ((size_t)njobs * rrs_slab(mx, kxm) + (size_t)njobs * jacobi_work_doubles(mx) +
+           (size_t)2 * njobs + 8) * sizeof(double);
+}
+
+int rr_split_group(ofrr_ctx *ctx, int njobs, const RRJob *jobs) {
+   if (njobs <= 0 || njobs > kMaxRRJobs) return set_error(ctx, kErrorInvalidArgument, "rr split: job count");
+   RRSplitArgs a{};
+   a.n = njobs;
+   int mx = 0, kxm = 0, kx_min = 1 << 30, mdx = 0, nk = 0;
+   for (int b = 0; b < njobs; ++b) {
+      const RRJob &J = jobs[b];
+      if (J.m <= 0 || J.m > kJacobiMaxN || J.kx <= 0 || J.kx > J.m || J.kx > 128 || J.nkeep <= 0 ||
+          J.nkeep > J.m)
+         return set_error(ctx, kErrorInvalidArgument, "rr split: job sizes");
+      a.j[b] = J;
+      mx = std::max(mx, J.m);
+      kxm = std::max(kxm, J.kx);
+      kx_min = std::min(kx_min, J.kx);
+      mdx = std::max(mdx, J.m - J.kx);
+      nk = std::max(nk, J.nkeep);
+   }
+   a.mx = mx;
+   a.kxm = kxm;
+   a.slab = rrs_slab(mx, kxm);
+   double *base = (double *)workspace(ctx, rr_split_workspace_bytes(njobs, mx, kxm));
+   if (!base) return set_error(ctx, kErrorCUDAMemory, "rr split: workspace");
+   double *jwork = base + (size_t)njobs * a.slab;
+   int32_t *ints = (int32_t *)(jwork + (size_t)njobs * jacobi_work_doubles(mx));
+   const RRSplitArea A0 = rrs_area(base, mx, kxm, a.slab, 0);
+   static std::once_flag attr_once;
+   static hipError_t attr_err = hipSuccess;
+   std::call_once(attr_once, [] {
+      attr_err = hipFuncSetAttribute((const void *)rrs_chol_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 128 * 8);
+   });
+   OFRR_HIP(ctx, attr_err);
+   rrs_chol_kernel<<<njobs, 1024, (size_t)kxm * kxm * sizeof(double), ctx->stream>>>(a, base, ints);
+   OFRR_LAUNCH(ctx);
+   rrs_solve_kernel<<<dim3(mdx + kxm, njobs), 64, 0, ctx->stream>>>(a, base, ints);
+   OFRR_LAUNCH(ctx);
+   const unsigned gmd = (unsigned)(((int64_t)mdx * mdx + 255) / 256);
+   if (mdx > 0) {
+      rrs_schur_kernel<<<dim3(gmd, njobs), 256, 0, ctx->stream>>>(a, base, ints);
+      OFRR_LAUNCH(ctx);
+      rrs_sscale_kernel<<<dim3(gmd, njobs), 256, 0, ctx->stream>>>(a, base, ints);
+      OFRR_LAUNCH(ctx);
+      // eig of the scaled D-block Gram matrix (size md per job, 0 for a failed job)
+      int st = syev_batched(ctx, njobs, mdx, ints + njobs, A0.Ss, mx, a.slab, A0.lam, a.slab, A0.U,
+                            mx, a.slab, jwork);
+      if (st) return st;
+   }
+   rrs_trunc_kernel<<<njobs, 256, 0, ctx->stream>>>(a, base, ints);
+   OFRR_LAUNCH(ctx);
+   if (mdx > 0) {
+      rrs_wxd_kernel<<<dim3((unsigned)(((int64_t)kxm * mdx + 255) / 256), njobs), 256, 0, ctx->stream>>>(
+          a, base, ints);
+      OFRR_LAUNCH(ctx);
+   }
+   for (int stage = 0; stage < 2; ++stage) {
+      rrs_gemm_kernel<<<dim3((mx + 63) / 64, mx, njobs), 64, 0, ctx->stream>>>(a, base, ints, stage);
+      OFRR_LAUNCH(ctx);
+   }
+   if (mdx > 0) {
+      // eig of Hr's D block (size r per job, copied into W2)
+      int st = syev_batched(ctx, njobs, mdx, ints + 2 * njobs, A0.W2, mx, a.slab, A0.hd, a.slab, A0.U,
+                            mx, a.slab, jwork);
+      if (st) return st;
+   }
+   rrs_couple_kernel<<<dim3((unsigned)(((int64_t)mx * mx + 255) / 256), njobs), 256, 0, ctx->stream>>>(
+       a, base, ints);
+   OFRR_LAUNCH(ctx);
+   int st = syev_batched(ctx, njobs, mx, ints + 3 * njobs, A0.S, mx, a.slab, A0.th, a.slab, A0.Ss, mx,
+                         a.slab, jwork);
+   if (st) return st;
+   rrs_finish_kernel<<<dim3(nk, njobs), 256, 0, ctx->stream>>>(a, base, ints);
+   OFRR_LAUNCH(ctx);
+   (void)kx_min;
+   return kSuccess;
+}
+
+int rr_group_run(ofrr_ctx *ctx, int njobs, const RRJob *jobs) {
+   RRJob whole[kMaxRRJobs], split[kMaxRRJobs];
+   int nw = 0, ns = 0;
+   for (int b = 0; b < njobs && b < kMaxRRJobs; ++b) {
+      if (jobs[b].kx > 0) split[ns++] = jobs[b];
+      else whole[nw++] = jobs[b];
+   }
+   int st = kSuccess;
+   if (nw) st = rr_truncated_group(ctx, nw, whole);
+   if (!st && ns) st = rr_split_group(ctx, ns, split);
+   return st;
+}
+
+// ---------------------------------------------------------------------------
 // generalised problem H x = lambda (G + gtol I) x via Cholesky (one workgroup)
 __global__ __launch_bounds__(1024) void chol_transform_kernel(int m, const double *__restrict__ G,
                                                               int ldg, const double *__restrict__ H,
@@ -854,6 +1264,16 @@ int ofrr_rr_truncated(ofrr_ctx *ctx, int m, const double *G, int ldg, const doub
    if (m <= 0 || m > kJacobiMaxN || ldg < m || ldh < m || ldy < m || !G || !H || !theta || !Y)
       return set_error(ctx, kErrorInvalidArgument, "rr_truncated: bad arguments");
    return rr_truncated(ctx, m, G, ldg, H, ldh, tau, nkeep, theta, Y, ldy, rank);
+}
+
+int ofrr_rr_split(ofrr_ctx *ctx, int m, int kx, const double *G, int ldg, const double *H, int ldh,
+                  double tau, int nkeep, double *theta, double *Y, int ldy, int32_t *rank) {
+   OFRR_CHECK_CTX(ctx);
+   if (m <= 0 || m > kJacobiMaxN || kx <= 0 || kx > m || kx > 128 || nkeep <= 0 || nkeep > m ||
+       ldg < m || ldh < m || ldy < m || !G || !H || !theta || !Y || !rank)
+      return set_error(ctx, kErrorInvalidArgument, "rr_split: bad arguments");
+   const RRJob job{G, H, ldg, ldh, m, nkeep, tau, theta, Y, ldy, rank, kx};
+   return rr_split_group(ctx, 1, &job);
 }
 
 int ofrr_rr_generalized(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H, int ldh,

@@ -80,7 +80,9 @@ int rr_truncated(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H
                  double tau, int nkeep, double *theta, double *Y, int ldy, int32_t *rank);
 int rr_generalized(ofrr_ctx *ctx, int m, const double *G, int ldg, const double *H, int ldh,
                    double gtol, double *theta, double *Y, int ldy);
-// One job of a grouped truncated Rayleigh-Ritz: rr_truncated's arguments.
+// One job of a grouped truncated Rayleigh-Ritz: rr_truncated's arguments, and
+// kx > 0 for the split form (the first kx basis columns are the previous Ritz
+// block; rank -1 out when its Gram block is not positive definite).
 constexpr int kMaxRRJobs = 16;
 struct RRJob {
    const double *G, *H;
@@ -89,11 +91,18 @@ struct RRJob {
    double *theta, *Y;
    int ldy;
    int32_t *rank;
+   int kx;
 };
 // rr_truncated for each of njobs jobs (m <= 384 each) in one launch per stage on
 // ctx's stream; each job's results equal rr_truncated's on its own arguments
 int rr_truncated_group(ofrr_ctx *ctx, int njobs, const RRJob *jobs);
 size_t rr_group_workspace_bytes(int njobs, int mx);
+// the split form for jobs with kx > 0 (kx <= 128): results equal the oracle's
+// orc_rr_split on each job's own arguments
+int rr_split_group(ofrr_ctx *ctx, int njobs, const RRJob *jobs);
+size_t rr_split_workspace_bytes(int njobs, int mx, int kxm);
+// the jobs with kx = 0 through rr_truncated_group, the others through rr_split_group
+int rr_group_run(ofrr_ctx *ctx, int njobs, const RRJob *jobs);
 // argmax |x| (1-based index + signed value, device outputs)
 int iamax(ofrr_ctx *ctx, int dtype, int64_t n, const void *x, int64_t inc, int32_t *idx1,
           double *val);

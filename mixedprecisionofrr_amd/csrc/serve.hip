@@ -24,7 +24,7 @@
 // within a window) run as one batched launch per stage on the stream of the
 // solve that formed the group, ordered after each other job's inputs by an
 // event on its slot's stream and before that slot's next work by another.
-// Each job's arithmetic is the one it gets alone (rr_truncated_group).
+// Each job's arithmetic is the one it gets alone (rr_truncated_group, rr_split_group).
 // On request (ofrr_server_opts.graphs) the filter sweeps of every slot run as
 // captured HIP graphs, one per sweep shape, kept by the slot across requests.
 #include <atomic>
@@ -180,7 +180,7 @@ struct RRGroupExec {
          if (p->slot != lead) OFRR_HIP(lead, hipStreamWaitEvent(lead->stream, p->slot->ev_rr_ready, 0));
          jobs.push_back(p->job);
       }
-      int st = rr_truncated_group(lead, (int)jobs.size(), jobs.data());
+      int st = rr_group_run(lead, (int)jobs.size(), jobs.data());
       if (st) return st;
       for (Pending *p : batch)
          if (p->slot != lead) OFRR_HIP(lead, hipEventRecord(p->slot->ev_rr_done, lead->stream));
@@ -348,7 +348,9 @@ void ofrr_server::worker(int slot) {
    // the workspace every solve of this slot reuses, reserved once up front
    // (a slot that leads a grouped Rayleigh-Ritz runs the group in its workspace)
    size_t reserve = filtered_workspace_bound(ctx ? ctx->num_cus : 256);
-   if (rr) reserve = std::max(reserve, rr_group_workspace_bytes(rr->max_jobs, 384));
+   if (rr)
+      reserve = std::max(reserve, std::max(rr_group_workspace_bytes(rr->max_jobs, 384),
+                                           rr_split_workspace_bytes(rr->max_jobs, 384, 128)));
    if (!st && !workspace(ctx, reserve)) st = kErrorCUDAMemory;
    if (!st && rr) {
       ctx->rr_group = &RRGroupExec::entry;

@@ -975,8 +975,8 @@ size_t filtered_workspace_bound(int num_cus) {
    // gram / gram_multi partials, the Rayleigh-Ritz at basis 384, the
    // back-transform's repacked coefficients (3 blocks of 128, width 128)
    const size_t basis = (size_t)(3 * 128 / 16) * 128 * 16 * sizeof(double);
-   return std::max(std::max(gram_workspace_bound(num_cus), rr_workspace_bytes(384)),
-                   std::max(basis, (size_t)256));
+   const size_t rr = std::max(rr_workspace_bytes(384), rr_split_workspace_bytes(1, 384, 128));
+   return std::max(std::max(gram_workspace_bound(num_cus), rr), std::max(basis, (size_t)256));
 }
 
 // p[0 .. nwords) = 0 (4-byte words): a kernel rather than a memset, so the
@@ -1172,13 +1172,14 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    // ones: eigensolves on the compacted matrices when the blocks are padded
    // the truncated Rayleigh-Ritz itself, or (a serving slot) its job handed
    // to the server's grouped Rayleigh-Ritz
-   auto rr_run = [&](int mm, const double *Gm, const double *Hm, double *Y) -> int {
-      if (!ctx->rr_group) return rr_truncated(ctx, mm, Gm, mm, Hm, mm, o->tau, kr, th, Y, mm, rank);
-      const RRJob job{Gm, Hm, mm, mm, mm, kr, o->tau, th, Y, mm, rank};
+   // kx > 0: the split form with the first kx columns the previous Ritz block
+   auto rr_run = [&](int mm, const double *Gm, const double *Hm, double *Y, int kx) -> int {
+      const RRJob job{Gm, Hm, mm, mm, mm, kr, o->tau, th, Y, mm, rank, kx};
+      if (!ctx->rr_group) return rr_group_run(ctx, 1, &job);
       return ctx->rr_group(ctx->rr_group_arg, ctx, &job);
    };
-   auto rayleigh_ritz = [&](int me, int ka) -> int {
-      if (!padded) return rr_run(me, G, H, C);
+   auto rayleigh_ritz = [&](int me, int ka, int kx) -> int {
+      if (!padded) return rr_run(me, G, H, C, kx);
       const int kar = ka - (k - kr), nb = (me - k) / std::max(1, ka);
       int mr = 0;
       for (int p = 0; p < me; ++p) h_cinv[p] = -1;
@@ -1197,7 +1198,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       sym_gather_kernel<<<(mr * mr + 255) / 256, 256, 0, ctx->stream>>>(mr, fb.cmap, G, me, fb.Gc);
       sym_gather_kernel<<<(mr * mr + 255) / 256, 256, 0, ctx->stream>>>(mr, fb.cmap, H, me, fb.Hc);
       OFRR_LAUNCH(ctx);
-      int s2 = rr_run(mr, fb.Gc, fb.Hc, fb.Cc);
+      int s2 = rr_run(mr, fb.Gc, fb.Hc, fb.Cc, kx);
       if (s2) return s2;
       coef_scatter_kernel<<<(me * k + 255) / 256, 256, 0, ctx->stream>>>(me, k, kr, mr, fb.cinv, fb.Cc, C);
       OFRR_LAUNCH(ctx);
@@ -1208,7 +1209,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    for (int i = 0; i < 8; ++i) OFRR_HIP(ctx, hipEventCreate(&ev[i]));
    float t_acc[7] = {0, 0, 0, 0, 0, 0, 0};
    int filter_steps = 0, resid_passes = 0, filter_col_steps = 0, filter_first_cols = 0;
-   int graph_replays = 0, graph_captures = 0;
+   int graph_replays = 0, graph_captures = 0, rr_fallbacks = 0;
    auto rec = [&](int i) { return hipEventRecord(ev[i], ctx->stream); };
    OFRR_HIP(ctx, rec(0));
 
@@ -1229,7 +1230,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    ++spmm_calls;
    if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, X, OFRR_F64, k, 1, 1.0, 0.0, G, k))) return st;
    if ((st = gram(ctx, n, k, k, X, OFRR_F64, k, 1, W, OFRR_F64, k, 1, 1.0, 0.0, H, k))) return st;
-   if ((st = rayleigh_ritz(k, k))) return st;
+   if ((st = rayleigh_ritz(k, k, 0))) return st;
    {
       const void *vb[1] = {X};
       int dt[1] = {OFRR_F64}, kb[1] = {k}, ldc[1] = {k};
@@ -1564,20 +1565,26 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       mirror_upper_kernel<<<(me * me + 255) / 256, 256, 0, ctx->stream>>>(me, H, me);
       OFRR_LAUNCH(ctx);
       OFRR_HIP(ctx, hipEventRecord(ev[6], ctx->stream));
-      if ((st = rayleigh_ritz(me, ka))) return st;
-      OFRR_HIP(ctx, hipEventRecord(ev[7], ctx->stream));
-      {
-         const void *vb[3] = {X, Dd, Dmid};
-         int dt[3] = {OFRR_F64, bdt, bdt}, kb[3] = {k, ka, ka}, ldc[3] = {me, me, me};
-         int64_t rs[3] = {k, ka, ka}, cs[3] = {1, 1, 1};
-         const double *cb[3] = {C, C + k, C + k + ka};
-         if ((st = basis_update(ctx, n, k, 1 + nd, vb, dt, kb, rs, cs, cb, ldc, Xn, OFRR_F64, k, 1))) return st;
-         std::swap(X, Xn);
+      // the split Rayleigh-Ritz (X = the first kr columns); where X^T X is not
+      // positive definite it reports rank -1 and the whole form runs instead
+      for (int form = o->rr_whole ? 1 : 0; form < 2; ++form) {
+         if ((st = rayleigh_ritz(me, ka, form ? 0 : kr))) return st;
+         OFRR_HIP(ctx, hipEventRecord(ev[7], ctx->stream));
+         {
+            const void *vb[3] = {X, Dd, Dmid};
+            int dt[3] = {OFRR_F64, bdt, bdt}, kb[3] = {k, ka, ka}, ldc[3] = {me, me, me};
+            int64_t rs[3] = {k, ka, ka}, cs[3] = {1, 1, 1};
+            const double *cb[3] = {C, C + k, C + k + ka};
+            if ((st = basis_update(ctx, n, k, 1 + nd, vb, dt, kb, rs, cs, cb, ldc, Xn, OFRR_F64, k, 1))) return st;
+         }
+         OFRR_HIP(ctx, hipMemcpyAsync(h_th, th, (size_t)k * 8, hipMemcpyDeviceToHost, ctx->stream));
+         OFRR_HIP(ctx, hipMemcpyAsync(h_rank, rank, 4, hipMemcpyDeviceToHost, ctx->stream));
+         OFRR_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+         OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+         if (h_rank[0] >= 0) break;
+         ++rr_fallbacks;
       }
-      OFRR_HIP(ctx, hipMemcpyAsync(h_th, th, (size_t)k * 8, hipMemcpyDeviceToHost, ctx->stream));
-      OFRR_HIP(ctx, hipMemcpyAsync(h_rank, rank, 4, hipMemcpyDeviceToHost, ctx->stream));
-      OFRR_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
-      OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      std::swap(X, Xn);
       {
          float a, b, c, dd, e2;
          OFRR_HIP(ctx, hipEventElapsedTime(&a, ev[3], ev[4]));
@@ -1623,6 +1630,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       stats->filter_first_cols = filter_first_cols;
       stats->graph_replays = graph_replays;
       stats->graph_captures = graph_captures;
+      stats->rr_fallbacks = rr_fallbacks;
       stats->resid_passes = resid_passes;
       stats->t_total_ms = total;
       stats->t_filter_ms = t_acc[0];

@@ -650,6 +650,171 @@ int orc_rr_truncated(int m, const double *G, int ldg, const double *H, int ldh, 
    return r;
 }
 
+/* Split truncated Rayleigh-Ritz for a basis V = [X | D] whose first kx columns
+ * are the previous Ritz block (X^T X ~ I, so its Gram block is well
+ * conditioned).  Same pencil and truncation idea as orc_rr_truncated
+ * (eig_rr.m:48-57 with the shift replaced by truncation), reached through a
+ * congruence that keeps the eigendecompositions small:
+ *   G_XX = L L^T (Cholesky), C = G_XX^{-1} G_XD, S = G_DD - G_XD^T C (the Gram
+ *   matrix of D with its X component removed), s = diag(S)^{-1/2};
+ *   eig(s S s) = U Lam, kept Lam_i > tau max(1, max Lam) -> B_D = s U_r Lam_r^{-1/2};
+ *   W = [[L^{-T}, -C B_D], [0, B_D]] (W^T G W = I on the kept directions);
+ *   Hr = W^T H W; eig(Hr_DD) = Q hd;  Hr2 = [[Hr_XX, Hr_XD Q], [., diag(hd)]];
+ *   eig(Hr2) = Z th;  Y = W blkdiag(I, Q) Z (top nkeep, descending).
+ * Only the upper triangles of S, Hr_DD and Hr2 are read by the eigensolver.
+ * Every sum is a sequential fused multiply-add chain in increasing index order,
+ * except the back substitutions with L^T, whose terms run in decreasing index
+ * order (the order in which a right-looking substitution finalises them); the
+ * device kernels run the same chains.  Returns the kept dimension kx + r,
+ * or -1 when G_XX is not positive definite. */
+int orc_rr_split(int m, int kx, const double *G, int ldg, const double *H, int ldh, double tau,
+                 int nkeep, double *theta, double *Y, int ldy) {
+   const int md = m - kx;
+#define Gs_(i, j) (0.5 * (G[(i) + (int64_t)(j) * ldg] + G[(j) + (int64_t)(i) * ldg]))
+#define Hs_(i, j) (0.5 * (H[(i) + (int64_t)(j) * ldh] + H[(j) + (int64_t)(i) * ldh]))
+   double *L = (double *)calloc((size_t)kx * kx + 1, sizeof(double));
+   for (int j = 0; j < kx; ++j)
+      for (int i = j; i < kx; ++i) L[i + (size_t)j * kx] = Gs_(i, j);
+   if (cholesky(kx, L)) { free(L); return -1; }
+#define L_(i, j) L[(i) + (size_t)(j) * kx]
+   /* C = G_XX^{-1} G_XD: forward then backward substitution per column */
+   double *Cc = (double *)malloc(sizeof(double) * ((size_t)kx * md + 1));
+   for (int c = 0; c < md; ++c) {
+      double *x = Cc + (size_t)c * kx;
+      for (int i = 0; i < kx; ++i) {
+         double s2 = Gs_(i, kx + c);
+         for (int p = 0; p < i; ++p) s2 = fma(-L_(i, p), x[p], s2);
+         x[i] = s2 / L_(i, i);
+      }
+      for (int i = kx - 1; i >= 0; --i) {
+         double s2 = x[i];
+         for (int p = kx - 1; p > i; --p) s2 = fma(-L_(p, i), x[p], s2);
+         x[i] = s2 / L_(i, i);
+      }
+   }
+   /* S (upper) = G_DD - G_XD^T C, scaled by diag(S)^{-1/2} */
+   double *S = (double *)malloc(sizeof(double) * ((size_t)md * md + 1));
+   double *sd = (double *)malloc(sizeof(double) * (md + 1));
+   for (int j = 0; j < md; ++j)
+      for (int i = 0; i <= j; ++i) {
+         double s2 = Gs_(kx + i, kx + j);
+         for (int p = 0; p < kx; ++p) s2 = fma(-Gs_(p, kx + i), Cc[p + (size_t)j * kx], s2);
+         S[i + (size_t)j * md] = s2;
+      }
+   for (int i = 0; i < md; ++i) {
+      const double d = S[i + (size_t)i * md];
+      sd[i] = d > 0.0 ? 1.0 / sqrt(d) : 0.0;
+   }
+   for (int j = 0; j < md; ++j)
+      for (int i = 0; i <= j; ++i) {
+         const double v = S[i + (size_t)j * md] * sd[i] * sd[j];
+         S[i + (size_t)j * md] = v;
+         S[j + (size_t)i * md] = v;
+      }
+   double *lam = (double *)malloc(sizeof(double) * (md + 1));
+   double *U = (double *)malloc(sizeof(double) * ((size_t)md * md + 1));
+   if (md > 0) jacobi_eig(md, S, lam, U, md);
+   double lmax = (md > 0 && lam[md - 1] > 1.0) ? lam[md - 1] : 1.0;
+   int r = 0;
+   for (int i = 0; i < md; ++i)
+      if (lam[i] > tau * lmax) ++r;
+   const int mr = kx + r;
+   /* W (m x mr) */
+   double *W = (double *)calloc((size_t)m * mr + 1, sizeof(double));
+#define W_(i, j) W[(i) + (size_t)(j) * m]
+   for (int c = 0; c < kx; ++c) /* L^{-T}: solve L^T w = e_c */
+      for (int i = kx - 1; i >= 0; --i) {
+         double s2 = (i == c) ? 1.0 : 0.0;
+         for (int p = kx - 1; p > i; --p) s2 = fma(-L_(p, i), W_(p, c), s2);
+         W_(i, c) = s2 / L_(i, i);
+      }
+   for (int c = 0; c < r; ++c) {
+      const int src = md - r + c;
+      const double f = sqrt(lam[src]);
+      for (int i = 0; i < md; ++i) W_(kx + i, kx + c) = U[i + (size_t)src * md] / f * sd[i];
+   }
+   for (int c = 0; c < r; ++c)
+      for (int i = 0; i < kx; ++i) {
+         double s2 = 0.0;
+         for (int p = 0; p < md; ++p) s2 = fma(Cc[i + (size_t)p * kx], W_(kx + p, kx + c), s2);
+         W_(i, kx + c) = -s2;
+      }
+   /* Hr = W^T (H W), full */
+   double *T = (double *)malloc(sizeof(double) * ((size_t)m * mr + 1));
+   double *Hr = (double *)malloc(sizeof(double) * ((size_t)mr * mr + 1));
+   for (int c = 0; c < mr; ++c)
+      for (int i = 0; i < m; ++i) {
+         double s2 = 0.0;
+         for (int p = 0; p < m; ++p) s2 = fma(Hs_(i, p), W_(p, c), s2);
+         T[i + (size_t)c * m] = s2;
+      }
+   for (int b = 0; b < mr; ++b)
+      for (int a = 0; a < mr; ++a) {
+         double s2 = 0.0;
+         for (int p = 0; p < m; ++p) s2 = fma(W_(p, a), T[p + (size_t)b * m], s2);
+         Hr[a + (size_t)b * mr] = s2;
+      }
+   /* eig of the D block (upper triangle) */
+   double *Hd = (double *)malloc(sizeof(double) * ((size_t)r * r + 1));
+   double *hd = (double *)malloc(sizeof(double) * (r + 1));
+   double *Q = (double *)malloc(sizeof(double) * ((size_t)r * r + 1));
+   for (int j = 0; j < r; ++j)
+      for (int i = 0; i < r; ++i) {
+         const int a = i < j ? i : j, b = i < j ? j : i;
+         Hd[i + (size_t)j * r] = Hr[(kx + a) + (size_t)(kx + b) * mr];
+      }
+   if (r > 0) jacobi_eig(r, Hd, hd, Q, r);
+   /* Hr2 (full from its upper triangle) and W2 = W blkdiag(I, Q) */
+   double *H2 = (double *)malloc(sizeof(double) * ((size_t)mr * mr + 1));
+   for (int b = 0; b < mr; ++b)
+      for (int a = 0; a <= b; ++a) {
+         double v;
+         if (b < kx) v = Hr[a + (size_t)b * mr];
+         else if (a < kx) {
+            double s2 = 0.0;
+            for (int p = 0; p < r; ++p) s2 = fma(Hr[a + (size_t)(kx + p) * mr], Q[p + (size_t)(b - kx) * r], s2);
+            v = s2;
+         } else v = (a == b) ? hd[a - kx] : 0.0;
+         H2[a + (size_t)b * mr] = v;
+         H2[b + (size_t)a * mr] = v;
+      }
+   double *W2 = (double *)malloc(sizeof(double) * ((size_t)m * mr + 1));
+   for (int c = 0; c < mr; ++c)
+      for (int i = 0; i < m; ++i) {
+         double v = W_(i, c);
+         if (c >= kx) {
+            double s2 = 0.0;
+            for (int p = 0; p < r; ++p) s2 = fma(W_(i, kx + p), Q[p + (size_t)(c - kx) * r], s2);
+            v = s2;
+         }
+         W2[i + (size_t)c * m] = v;
+      }
+   double *th = (double *)malloc(sizeof(double) * (mr + 1));
+   double *Z = (double *)malloc(sizeof(double) * ((size_t)mr * mr + 1));
+   jacobi_eig(mr, H2, th, Z, mr);
+   const int nout = nkeep < mr ? nkeep : mr;
+   for (int c = 0; c < nout; ++c) {
+      const int src = mr - 1 - c;
+      theta[c] = th[src];
+      for (int i = 0; i < m; ++i) {
+         double s2 = 0.0;
+         for (int p = 0; p < mr; ++p) s2 = fma(W2[i + (size_t)p * m], Z[p + (size_t)src * mr], s2);
+         Y[i + (int64_t)c * ldy] = s2;
+      }
+   }
+   for (int c = nout; c < nkeep; ++c) {
+      theta[c] = NAN;
+      for (int i = 0; i < m; ++i) Y[i + (int64_t)c * ldy] = 0.0;
+   }
+#undef Gs_
+#undef Hs_
+#undef L_
+#undef W_
+   free(L); free(Cc); free(S); free(sd); free(lam); free(U); free(W); free(T); free(Hr);
+   free(Hd); free(hd); free(Q); free(H2); free(W2); free(th); free(Z);
+   return mr;
+}
+
 int orc_rr_generalized(int m, const double *G, int ldg, const double *H, int ldh, double gtol,
                        double *theta, double *Y, int ldy) {
    double *B = (double *)calloc((size_t)m * m + 1, sizeof(double));
@@ -1012,7 +1177,8 @@ int orc_filtered_ofrr(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
       orc_spmm_csr(n, row_ptr, col_idx, vals, me - k, Dd, n, W + (int64_t)k * n, n);
       orc_gemm_tn(n, me, me, V, n, V, n, G, me);
       orc_gemm_tn(n, me, me, V, n, W, n, H, me);
-      int r = orc_rr_truncated(me, G, me, H, me, o->tau, k, theta, C, me);
+      int r = o->rr_whole ? -1 : orc_rr_split(me, k, G, me, H, me, o->tau, k, theta, C, me);
+      if (r < 0) r = orc_rr_truncated(me, G, me, H, me, o->tau, k, theta, C, me);
       if (r < k) { it = -(it + 1); break; }
       orc_gemm_nn(n, k, me, V, n, C, me, T, n);
       memcpy(V, T, sizeof(double) * (size_t)n * k);

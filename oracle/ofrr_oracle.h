@@ -92,6 +92,13 @@ int orc_sygv(int n, const double *A, int lda, const double *B, int ldb, double *
 int orc_rr_truncated(int m, const double *G, int ldg, const double *H, int ldh, double tau,
                      int nkeep, double *theta, double *Y, int ldy);
 /* eig_rr.m:48-57 generalised form: eig(H, G + gtol I), sorted descending */
+/* Split truncated Rayleigh-Ritz over V = [X | D], X the first kx columns:
+ * G_XX = L L^T, S = G_DD - G_XD^T G_XX^{-1} G_XD scaled to unit diagonal,
+ * eig(S) truncated at tau max(1, max eig) -> whitened basis W; eig of the D block
+ * of W^T H W first, then of the coupled matrix.  theta descending, Y (m x nkeep).
+ * Returns kx + the kept D rank, or -1 (G_XX not positive definite). */
+int orc_rr_split(int m, int kx, const double *G, int ldg, const double *H, int ldh, double tau,
+                 int nkeep, double *theta, double *Y, int ldy);
 int orc_rr_generalized(int m, const double *G, int ldg, const double *H, int ldh, double gtol,
                        double *theta, double *Y, int ldy);
 
@@ -136,6 +143,10 @@ typedef struct {
    double normA;     /* ||A||_2 (0: use the largest Ritz value) */
    int lock;         /* 1: soft locking (a sweep filters the columns after the
                         leading converged wanted pairs only); 0: all k columns */
+   int rr_whole;     /* 1: every Rayleigh-Ritz eigendecomposes the Gram matrix of the
+                        whole basis (orc_rr_truncated); 0: the split form
+                        (orc_rr_split, X whitened by Cholesky), the whole form only
+                        where G_XX is not positive definite */
 } orc_ofrr_opts;
 int orc_filtered_ofrr(int64_t n, const int64_t *row_ptr, const int32_t *col_idx,
                       const double *vals, const orc_ofrr_opts *opt, double *X, int64_t ldx,

@@ -23,7 +23,8 @@ _u32 = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 class OfrrOpts(C.Structure):
     _fields_ = [("k", C.c_int), ("nwant", C.c_int), ("nd", C.c_int), ("degree", C.c_int),
                 ("maxiter", C.c_int), ("tol", C.c_double), ("tau", C.c_double),
-                ("lower", C.c_double), ("normA", C.c_double), ("lock", C.c_int)]
+                ("lower", C.c_double), ("normA", C.c_double), ("lock", C.c_int),
+                ("rr_whole", C.c_int)]
 
 
 def build():
@@ -61,6 +62,8 @@ def lib():
         L.orc_sygv.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _dp, _dp, C.c_int]
         L.orc_rr_truncated.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_int,
                                        _dp, _dp, C.c_int]
+        L.orc_rr_split.argtypes = [C.c_int, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double,
+                                   C.c_int, _dp, _dp, C.c_int]
         L.orc_rr_generalized.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double, _dp,
                                          _dp, C.c_int]
         L.orc_arnoldi_hess_eig.argtypes = [C.c_int64, _ip64, _ip32, _dp, C.c_int, C.c_int, C.c_int,
@@ -235,6 +238,15 @@ def rr_truncated(G, H, tau, nkeep):
     return r, th, Y
 
 
+def rr_split(G, H, kx, tau, nkeep):
+    G, H = _f(G), _f(H)
+    m = G.shape[0]
+    th = np.zeros(nkeep)
+    Y = np.zeros((m, nkeep), order="F")
+    r = lib().orc_rr_split(m, kx, G, m, H, m, tau, nkeep, th, Y, m)
+    return r, th, Y
+
+
 def rr_generalized(G, H, gtol):
     G, H = _f(G), _f(H)
     m = G.shape[0]
@@ -288,10 +300,11 @@ def subspace_iter_svd(A, V0, maxiter, step_size=1, select_tol=2.0 ** -52, genera
 
 
 def filtered_ofrr(A, X0, nwant, nd=1, degree=20, maxiter=50, tol=1e-8, tau=1e-12, lower=0.0,
-                  normA=0.0, lock=True):
+                  normA=0.0, lock=True, rr_whole=False):
     X = _f(X0).copy(order="F")
     k = X.shape[1]
-    o = OfrrOpts(k, nwant, nd, degree, maxiter, tol, tau, lower, normA, 1 if lock else 0)
+    o = OfrrOpts(k, nwant, nd, degree, maxiter, tol, tau, lower, normA, 1 if lock else 0,
+                 1 if rr_whole else 0)
     th = np.zeros(k)
     res = np.zeros(k)
     it = lib().orc_filtered_ofrr(A.n, A.row_ptr, A.col_idx, A.vals, C.byref(o), X, A.n, th, res)

@@ -331,6 +331,48 @@ def test_rr_truncated_vs_oracle(ctx, m, drop):
     assert np.all(res <= 10 * res_o + 1e-10)
 
 
+def _split_basis(n, kx, kd, seed):
+    """A solver-like basis [X | D]: X orthonormal, D correlated with X, its second
+    half nearly parallel to its first (the two filter snapshots)."""
+    rng = np.random.default_rng(seed)
+    X = np.linalg.qr(rng.standard_normal((n, kx)))[0]
+    D = rng.standard_normal((n, kd)) + X @ rng.standard_normal((kx, kd)) * 0.7
+    h = kd // 2
+    D[:, h:] = D[:, :kd - h] + 1e-4 * rng.standard_normal((n, kd - h))
+    return np.concatenate([X, D], axis=1)
+
+
+@pytest.mark.parametrize("kx,kd", [(64, 128), (64, 64), (32, 33), (16, 111), (128, 172), (7, 0)])
+def test_rr_split_bitwise_vs_oracle(ctx, kx, kd):
+    """The split Rayleigh-Ritz on the GPU runs the oracle's arithmetic: theta,
+    Y and the kept rank are bitwise equal (D-block eigensolves of even and odd
+    size, the LDS and global-memory eigensolvers, no D block at all)."""
+    from mixedprecisionofrr_amd import api
+    Ah = O.stencil(3, 12)
+    S = Ah.to_scipy()
+    V = _split_basis(Ah.n, kx, kd, kx * 1000 + kd)
+    G, H = V.T @ V, V.T @ (S @ V)
+    nk = min(kx, 64)
+    r_o, th_o, Y_o = O.rr_split(G, H, kx, 1e-12, nk)
+    print(f"rr_split kx={kx} kd={kd}", flush=True)
+    th, Y, r = api.rr_split(ctx, _dev(G.T).t(), _dev(H.T).t(), kx, 1e-12, nk)
+    assert r == r_o
+    assert np.array_equal(th.cpu().numpy(), th_o)
+    assert np.array_equal(Y.cpu().numpy(), Y_o)
+
+
+def test_rr_split_indefinite_block_reports_minus_one(ctx):
+    """An X block that is not positive definite: rank -1, outputs untouched."""
+    from mixedprecisionofrr_amd import api
+    V = _split_basis(500, 8, 8, 3)
+    G, H = V.T @ V, V.T @ V * 2.0
+    G[2, 2] = -1.0
+    print("rr_split indefinite X block", flush=True)
+    th, Y, r = api.rr_split(ctx, _dev(G.T).t(), _dev(H.T).t(), 8, 1e-12, 8)
+    assert r == -1
+    assert np.all(np.isnan(th.cpu().numpy()))
+
+
 def test_rr_generalized_vs_oracle(ctx):
     from mixedprecisionofrr_amd import api
     rng = np.random.default_rng(13)

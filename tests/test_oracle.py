@@ -242,6 +242,84 @@ def test_rr_truncated_exact_ritz_and_rank_drop():
     assert np.allclose(th2, th, rtol=1e-11)
 
 
+def _split_basis(S, kx, kd, seed, collinear=True):
+    """A solver-like basis [X | D] for a symmetric S: X orthonormal (previous
+    Ritz block), D correlated with X and, optionally, within itself."""
+    rng = np.random.default_rng(seed)
+    n = S.shape[0]
+    X = np.linalg.qr(rng.standard_normal((n, kx)))[0]
+    D = rng.standard_normal((n, kd)) + X @ rng.standard_normal((kx, kd)) * 0.7
+    if collinear:  # nearly parallel columns, as the two filter snapshots give
+        D[:, kd // 2:] = D[:, :kd - kd // 2] + 1e-3 * rng.standard_normal((n, kd - kd // 2))
+    return np.concatenate([X, D], axis=1)
+
+
+@pytest.mark.parametrize("kx,kd", [(6, 10), (16, 32), (7, 9)])
+def test_rr_split_matches_textbook_pencil(kx, kd):
+    """orc_rr_split against scipy's generalised eigensolver on a well-conditioned
+    pencil: the top Ritz values agree, Y^T G Y = I and Y^T H Y = diag(theta)."""
+    A = O.stencil(2, 14)
+    S = A.to_scipy().toarray()
+    V = _split_basis(S, kx, kd, kx + kd, collinear=False)
+    G, H = V.T @ V, V.T @ S @ V
+    nk = kx
+    r, th, Y = O.rr_split(G, H, kx, 1e-12, nk)
+    assert r == kx + kd
+    ref = sla.eigh(H, G, eigvals_only=True)[::-1][:nk]
+    assert np.abs(th - ref).max() < 1e-12 * abs(ref[0])
+    assert np.abs(Y.T @ G @ Y - np.eye(nk)).max() < 1e-12
+    assert np.abs(Y.T @ H @ Y - np.diag(th)).max() < 1e-12 * abs(th[0])
+
+
+def test_rr_split_exact_ritz_rank_drop_and_indefinite_block():
+    """Exact Ritz values from an invariant subspace; D columns inside span(X)
+    are truncated away; a non-positive-definite X block returns -1."""
+    N = 20
+    A = O.stencil(2, N)
+    S = A.to_scipy().toarray()
+    lam, U = np.linalg.eigh(S)
+    rng = np.random.default_rng(5)
+    idx = [399, 398, 396, 390, 380, 300, 250, 200]
+    X = U[:, idx[:4]] @ np.linalg.qr(rng.standard_normal((4, 4)))[0]
+    D = U[:, idx] @ rng.standard_normal((8, 6))
+    V = np.concatenate([X, D, X[:, :2] * 3.0], axis=1)  # two columns inside span(X)
+    G, H = V.T @ V, V.T @ S @ V
+    r, th, Y = O.rr_split(G, H, 4, 1e-12, 8)
+    assert r == 8  # X and the 4 directions of D outside span(X)
+    assert np.allclose(th, np.sort(lam[idx])[::-1], rtol=1e-11, atol=0)
+    Xr = V @ Y
+    assert np.abs(S @ Xr - Xr * th).max() < 1e-9
+    # the whole form agrees
+    r2, th2, _ = O.rr_truncated(G, H, 1e-12, 8)
+    assert np.abs(th2 - th).max() < 1e-11 * th[0]
+    # indefinite X block
+    G2 = G.copy()
+    G2[1, 1] = -1.0
+    r3, _, _ = O.rr_split(G2, H, 4, 1e-12, 4)
+    assert r3 == -1
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_rr_split_solver_like_basis(seed):
+    """Collinear correction columns (Gram condition ~6.5e6): the split form and
+    the whole form both meet the Ritz values of an orthonormal basis of the same
+    span (Householder QR) within the accuracy the Gram matrix carries,
+    eps cond(G) ||S||."""
+    A = O.stencil(3, 10)
+    S = A.to_scipy().toarray()
+    V = _split_basis(S, 16, 32, seed)
+    G, H = V.T @ V, V.T @ S @ V
+    r, th, Y = O.rr_split(G, H, 16, 1e-12, 16)
+    r2, th2, Y2 = O.rr_truncated(G, H, 1e-12, 16)
+    Q = np.linalg.qr(V)[0]
+    ref = np.linalg.eigvalsh(Q.T @ S @ Q)[::-1][:16]
+    bound = 2.2e-16 * np.linalg.cond(G) * ref[0]
+    assert r == r2 == 48
+    assert np.abs(th - ref).max() < bound
+    assert np.abs(th2 - ref).max() < bound
+    assert np.abs(Y.T @ G @ Y - np.eye(16)).max() < 1e-9
+
+
 def test_arnoldi_exact_termination():
     # v0 = sum of 3 sine modes: the Krylov space is invariant after 3 columns
     N = 16
