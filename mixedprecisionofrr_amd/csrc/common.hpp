@@ -72,6 +72,11 @@ struct ofrr_ctx {
    int (*rr_group)(void *arg, ofrr_ctx *slot, const void *job) = nullptr;
    void *rr_group_arg = nullptr;
    hipEvent_t ev_rr_ready = nullptr, ev_rr_done = nullptr;
+   // a second stream of the context (the Hessenberg process runs each panel's
+   // trailing update on it, beside the next panel's columns), ordered with
+   // `stream` by events only; created on first use
+   hipStream_t aux_stream = nullptr;
+   hipEvent_t ev_aux_fork = nullptr, ev_aux_join = nullptr;
    std::string last_error;
 };
 

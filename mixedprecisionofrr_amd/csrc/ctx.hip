@@ -129,6 +129,12 @@ int ofrr_ctx_destroy(ofrr_ctx *c) {
    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
    if (c->ev_rr_ready) (void)hipEventDestroy(c->ev_rr_ready);
    if (c->ev_rr_done) (void)hipEventDestroy(c->ev_rr_done);
+   if (c->aux_stream) {
+      (void)hipStreamSynchronize(c->aux_stream);
+      (void)hipStreamDestroy(c->aux_stream);
+   }
+   if (c->ev_aux_fork) (void)hipEventDestroy(c->ev_aux_fork);
+   if (c->ev_aux_join) (void)hipEventDestroy(c->ev_aux_join);
    if (c->ws) (void)hipFree(c->ws);
    if (c->pinned) (void)hipHostFree(c->pinned);
    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);

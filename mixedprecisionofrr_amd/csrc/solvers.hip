@@ -1547,18 +1547,47 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
             OFRR_LAUNCH(ctx);
             Dp2 = Db[b];
          }
-         struct Prod { const void *a; int adt, wa; const void *b; int bdt_, wb; double *dst; int ro, co; };
-         const Prod pr[6] = {{X, OFRR_F64, k, X, OFRR_F64, k, G, 0, 0},
-                             {X, OFRR_F64, k, Dp2, bdt, kd, G, 0, k},
-                             {Dp2, bdt, kd, Dp2, bdt, kd, G, k, k},
-                             {X, OFRR_F64, k, W, OFRR_F64, k, H, 0, 0},
-                             {W, OFRR_F64, k, Dp2, bdt, kd, H, 0, k},
-                             {Dp2, bdt, kd, Z, zdt, kd, H, k, k}};
-         for (int q = 0; q < 6; ++q) {
-            const Prod &P = pr[q];
-            if ((st = gram(ctx, n, P.wa, P.wb, P.a, P.adt, P.wa, 1, P.b, P.bdt_, P.wb, 1, 1.0, 0.0,
-                           P.dst + P.ro + (size_t)P.co * me, me)))
-               return st;
+         if (kd == k) {
+            // the packed correction block is as wide as X: the six products in
+            // one fused launch, as for one full-width correction block
+            GramMultiArgs ga{};
+            ga.op[0] = {X, (int64_t)k, 0};
+            ga.op[1] = {W, (int64_t)k, 0};
+            ga.op[2] = {Dp2, (int64_t)kd, 1};
+            ga.op[3] = {Z, (int64_t)kd, 2};
+            struct Blk { int a, b; double *dst; int br, bc; };
+            const Blk blks[6] = {{0, 0, G, 0, 0}, {0, 2, G, 0, 1}, {2, 2, G, 1, 1},
+                                 {0, 1, H, 0, 0}, {1, 2, H, 0, 1}, {2, 3, H, 1, 1}};
+            const int nt = (k + 63) / 64;
+            int t = 0;
+            for (int q = 0; q < 6; ++q)
+               for (int ti = 0; ti < nt; ++ti)
+                  for (int tj = 0; tj < nt; ++tj, ++t) {
+                     ga.ia[t] = blks[q].a;
+                     ga.ib[t] = blks[q].b;
+                     ga.ti[t] = ti;
+                     ga.tj[t] = tj;
+                     ga.dst[t] = blks[q].dst;
+                     ga.ld[t] = me;
+                     ga.ro[t] = blks[q].br * k;
+                     ga.co[t] = blks[q].bc * k;
+                  }
+            ga.nitems = t;
+            if ((st = gram_multi(ctx, n, k, bdt, zdt, ga))) return st;
+         } else {
+            struct Prod { const void *a; int adt, wa; const void *b; int bdt_, wb; double *dst; int ro, co; };
+            const Prod pr[6] = {{X, OFRR_F64, k, X, OFRR_F64, k, G, 0, 0},
+                                {X, OFRR_F64, k, Dp2, bdt, kd, G, 0, k},
+                                {Dp2, bdt, kd, Dp2, bdt, kd, G, k, k},
+                                {X, OFRR_F64, k, W, OFRR_F64, k, H, 0, 0},
+                                {W, OFRR_F64, k, Dp2, bdt, kd, H, 0, k},
+                                {Dp2, bdt, kd, Z, zdt, kd, H, k, k}};
+            for (int q = 0; q < 6; ++q) {
+               const Prod &P = pr[q];
+               if ((st = gram(ctx, n, P.wa, P.wb, P.a, P.adt, P.wa, 1, P.b, P.bdt_, P.wb, 1, 1.0, 0.0,
+                              P.dst + P.ro + (size_t)P.co * me, me)))
+                  return st;
+            }
          }
       }
       mirror_upper_kernel<<<(me * me + 255) / 256, 256, 0, ctx->stream>>>(me, G, me);

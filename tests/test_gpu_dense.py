@@ -123,15 +123,20 @@ def test_svd_filtered_lowrank(ctx, bdt):
     assert np.allclose(U[:, :nwant].norm(dim=0).cpu().numpy(), 1.0, atol=1e-8)
 
 
-@pytest.mark.parametrize("dt", [torch.float64, torch.float32, torch.float16])
-def test_hessenberg_v3_and_steps(ctx, dt):
+@pytest.mark.parametrize("dt,drops", [(torch.float64, (9,)), (torch.float32, (9,)), (torch.float16, (9,)),
+                                      (torch.float64, (15, 16, 31, 39)),
+                                      (torch.float16, (15, 16, 31, 39))])
+def test_hessenberg_v3_and_steps(ctx, dt, drops):
     """C++ v3 (dropped columns still eliminated) against the oracle's variant 1,
     and the column-by-column step entry point bit for bit equal to the full
-    factorizations of both variants."""
+    factorizations of both variants.  The full factorization runs in panels of
+    16 columns: the second case drops the last column of a panel, the first of
+    the next, and the last column of all."""
     from mixedprecisionofrr_amd import api
     rng = np.random.default_rng(44)
     A = rng.uniform(-1, 1, (700, 40))
-    A[:, 9] = A[:, 3] + 1e-9 * rng.uniform(-1, 1, 700)   # dropped at select_tol 1e-6
+    for d in drops:  # dropped at select_tol 1e-6
+        A[:, d] = A[:, d - 6] + 1e-9 * rng.uniform(-1, 1, 700)
     tol = 1e-6 if dt != torch.float16 else 1e-2
     Ad = torch.as_tensor(A, device="cuda:0").to(dt)
     for variant in (0, 1):
@@ -145,6 +150,6 @@ def test_hessenberg_v3_and_steps(ctx, dt):
         assert np.array_equal(perm.cpu().numpy()[kept], prow.cpu().numpy()[kept] - 1)
         if dt == torch.float64:
             Qo, Ro, so, po = O.hessenberg(A, tol, variant=variant)
-            assert list(np.nonzero(so == 0)[0]) == [9]
+            assert list(np.nonzero(so == 0)[0]) == list(drops)
             assert np.array_equal(Q.cpu().numpy(), Qo)
             assert np.array_equal(R.cpu().numpy(), Ro)
