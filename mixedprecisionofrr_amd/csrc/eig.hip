@@ -59,14 +59,15 @@ __device__ inline void pair_of(int M, int r, int s, int &p, int &q) {
 // Storage of the symmetric working matrix on nn = n + (n odd) indices; an odd
 // n gets a virtual index n whose row and column hold zeros (its pairs never
 // rotate and its entries are never written, so the arithmetic equals skipping
-// it).  FULL keeps both triangles with row stride nn + 1 (conflict-free LDS
-// rows): entry (i, j) at off(i) + j with off(i) = i (nn + 1).  Otherwise the
-// packed upper triangle: (i, j), i <= j, at off(j) + i with off(j) = j (j + 1) / 2.
+// it).  Only the upper triangle is stored and touched.  FULL lays it out in
+// square storage with row stride nn + 1 (conflict-free LDS rows): entry (i, j),
+// i <= j, at off(i) + j with off(i) = i (nn + 1).  Otherwise the packed upper
+// triangle: (i, j), i <= j, at off(j) + i with off(j) = j (j + 1) / 2.
 template <bool FULL> struct SymIdx;
 template <> struct SymIdx<true> {
    int ld;
    __device__ int off(int i) const { return i * ld; }
-   __device__ int at(int i, int j, int oi, int oj) const { (void)oj; return oi + j; }
+   __device__ int at(int i, int j, int oi, int oj) const { return (i <= j) ? oi + j : oj + i; }
 };
 template <> struct SymIdx<false> {
    int ld;
@@ -112,7 +113,6 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
       for (int i = 0; i <= j; ++i) {
          const double v = (j < n) ? Ab[i + (int64_t)j * lda] : 0.0;
          a[pk.at(i, j, pk.off(i), oj)] = v;
-         if (FULL) a[pk.at(j, i, oj, pk.off(i))] = v;
          lmax = fmax(lmax, fabs(v));
       }
    }
@@ -157,11 +157,14 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
             if (q < n) {
                const double apq = a[pk.at(p, q, op, oq)], app = a[pk.at(p, p, op, op)],
                             aqq = a[pk.at(q, q, oq, oq)];
+               // the rotation is formed whether or not it is taken, so its
+               // divide/square-root chain overlaps the threshold's square root
+               const double th = (aqq - app) / (2.0 * apq);
+               const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(fma(th, th, 1.0)));
+               const double cr = 1.0 / sqrt(fma(t, t, 1.0));
                if (fabs(apq) > fmax(1.1102230246251565e-16 * sqrt(fabs(app) * fabs(aqq)), tol_abs)) {
-                  const double th = (aqq - app) / (2.0 * apq);
-                  const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(fma(th, th, 1.0)));
-                  c = 1.0 / sqrt(fma(t, t, 1.0));
-                  sn = t * c;
+                  c = cr;
+                  sn = t * cr;
                   rotated = 1;
                }
             }
@@ -215,19 +218,9 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
                if (p1 == p2) b12 = b21 = 0.0;  // diagonal block
                const bool q1ok = q1 < n, q2ok = q2 < n;
                a[pk.at(p1, p2, op1, op2)] = b11;
-               if (FULL) a[pk.at(p2, p1, op2, op1)] = b11;
-               if (q2ok) {
-                  a[pk.at(p1, q2, op1, oq2)] = b12;
-                  if (FULL) a[pk.at(q2, p1, oq2, op1)] = b12;
-               }
-               if (q1ok) {
-                  a[pk.at(q1, p2, oq1, op2)] = b21;
-                  if (FULL) a[pk.at(p2, q1, op2, oq1)] = b21;
-               }
-               if (q1ok && q2ok) {
-                  a[pk.at(q1, q2, oq1, oq2)] = b22;
-                  if (FULL) a[pk.at(q2, q1, oq2, oq1)] = b22;
-               }
+               if (q2ok) a[pk.at(p1, q2, op1, oq2)] = b12;
+               if (q1ok) a[pk.at(q1, p2, oq1, op2)] = b21;
+               if (q1ok && q2ok) a[pk.at(q1, q2, oq1, oq2)] = b22;
             }
          }
          __syncthreads();
