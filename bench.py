@@ -224,7 +224,7 @@ def main():
     p.add_argument("--k", type=int, default=64)
     p.add_argument("--nd", type=int, default=2)
     p.add_argument("--nwant", type=int, default=50)
-    p.add_argument("--degree", type=int, default=120)
+    p.add_argument("--degree", type=int, default=110)
     p.add_argument("--maxiter", type=int, default=200)
     p.add_argument("--tol", type=float, default=1e-8)
     p.add_argument("--tau", type=float, default=1e-12)

@@ -237,10 +237,16 @@ inline int64_t xcd_grid(int64_t nblocks) { return ((nblocks + 7) / 8) * 8; }
 // measured slower on the 128^3 stencil: they lose more in-plane neighbours
 // across strip edges than they gain in reuse distance.  plane <= 0 selects
 // xcd_block.
-__device__ inline int64_t sched_block(int64_t nblocks_logical, int64_t plane) {
+__device__ inline int64_t sched_block(int64_t nblocks_logical, int64_t plane,
+                                      bool reverse = false) {
    if (plane <= 0) return xcd_block(nblocks_logical);
    const int64_t b = blockIdx.x, j = b >> 3, S = plane >> 3;
-   const int64_t t = (j / S) * plane + (b & 7) * S + j % S;
+   int64_t pl = j / S, w = j % S;
+   if (reverse) {  // the same strips, walked from the last plane back to the first
+      pl = (int64_t)(gridDim.x >> 3) / S - 1 - pl;
+      w = S - 1 - w;
+   }
+   const int64_t t = pl * plane + (b & 7) * S + w;
    return t < nblocks_logical ? t : -1;
 }
 inline int64_t sched_grid(int64_t nblocks, int64_t plane) {

@@ -34,6 +34,7 @@ struct ChebArgs {
    float *zero_next;          // cleared for the next step (nullable)
    double *sn_out;            // scale of the stored D_{j+1}
    int target_exp;
+   int reverse = 0;           // walk the row tiles from the last back to the first
 };
 int cheb_step(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, const void *va,
               int vdt, int k, const void *Dc, const void *Dp, const void *R, int bdt,

@@ -1455,6 +1455,9 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
             ca.zero_next = (s + 1 < d) ? cmax[(s + 1) % 3] : nullptr;
             ca.sn_out = scale[sb[s].out];
             ca.target_exp = tgt_step;
+            // alternate steps walk the row tiles backwards, so a step starts on
+            // the rows whose blocks the previous step left in the memory-side cache
+            ca.reverse = s & 1;
             if ((st2 = op.cheb(ctx, ka, sb[s].Dc, sb[s].Dp, Ra, bdt, ca, Db[sb[s].out]))) return st2;
          }
          return kSuccess;

@@ -423,7 +423,7 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
    __shared__ float redm[256][VEC];
    __shared__ __align__(16) ACC s_coef[3][kChebMaxK];
    __shared__ int s_se[kChebMaxK];
-   const int64_t blk = sched_block(nblk, plane);
+   const int64_t blk = sched_block(nblk, plane, ca_.reverse != 0);
    if (blk < 0) return;
    const int G = 1 << lg;
    const int GROUPS = 256 >> lg;
