@@ -77,10 +77,32 @@ struct ofrr_ctx {
    // `stream` by events only; created on first use
    hipStream_t aux_stream = nullptr;
    hipEvent_t ev_aux_fork = nullptr, ev_aux_join = nullptr;
+   // while ws_on_aux is set (work enqueued on aux_stream in place of stream),
+   // workspace() hands out ws_aux, so the two streams never share scratch
+   void *ws_aux = nullptr;
+   size_t ws_aux_bytes = 0;
+   bool ws_on_aux = false;
    std::string last_error;
 };
 
 namespace ofrr {
+
+// While in scope, the library's operations on ctx enqueue on its second stream
+// (created beforehand) and take their scratch from the second workspace.
+struct OnAuxStream {
+   ofrr_ctx *c;
+   hipStream_t main;
+   explicit OnAuxStream(ofrr_ctx *c_) : c(c_), main(c_->stream) {
+      c->stream = c->aux_stream;
+      c->ws_on_aux = true;
+   }
+   ~OnAuxStream() {
+      c->stream = main;
+      c->ws_on_aux = false;
+   }
+   OnAuxStream(const OnAuxStream &) = delete;
+   OnAuxStream &operator=(const OnAuxStream &) = delete;
+};
 
 int set_error(ofrr_ctx *ctx, int status, const std::string &msg);
 int hip_status(ofrr_ctx *ctx, hipError_t e, const char *what);

@@ -21,21 +21,23 @@ int hip_status(ofrr_ctx *ctx, hipError_t e, const char *what) {
 
 void *workspace(ofrr_ctx *ctx, size_t bytes) {
    if (bytes == 0) bytes = 256;
-   if (ctx->ws_bytes >= bytes) return ctx->ws;
-   if (ctx->ws) {
-      // the old workspace may still be read by enqueued work
+   void *&ws = ctx->ws_on_aux ? ctx->ws_aux : ctx->ws;
+   size_t &ws_bytes = ctx->ws_on_aux ? ctx->ws_aux_bytes : ctx->ws_bytes;
+   if (ws_bytes >= bytes) return ws;
+   if (ws) {
+      // the old workspace may still be read by work enqueued on its stream
       if (hipStreamSynchronize(ctx->stream) != hipSuccess) return nullptr;
-      (void)hipFree(ctx->ws);
-      ctx->ws = nullptr;
-      ctx->ws_bytes = 0;
+      (void)hipFree(ws);
+      ws = nullptr;
+      ws_bytes = 0;
    }
    size_t want = bytes + bytes / 4;
-   if (hipMalloc(&ctx->ws, want) != hipSuccess) {
-      ctx->ws = nullptr;
+   if (hipMalloc(&ws, want) != hipSuccess) {
+      ws = nullptr;
       return nullptr;
    }
-   ctx->ws_bytes = want;
-   return ctx->ws;
+   ws_bytes = want;
+   return ws;
 }
 
 void *pinned(ofrr_ctx *ctx, size_t bytes) {
@@ -136,6 +138,7 @@ int ofrr_ctx_destroy(ofrr_ctx *c) {
    if (c->ev_aux_fork) (void)hipEventDestroy(c->ev_aux_fork);
    if (c->ev_aux_join) (void)hipEventDestroy(c->ev_aux_join);
    if (c->ws) (void)hipFree(c->ws);
+   if (c->ws_aux) (void)hipFree(c->ws_aux);
    if (c->pinned) (void)hipHostFree(c->pinned);
    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
    delete c;

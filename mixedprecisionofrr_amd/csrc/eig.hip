@@ -695,6 +695,8 @@ int rr_truncated_group(ofrr_ctx *ctx, int njobs, const RRJob *jobs) {
    // Hr at sc + mx + 5 mm, th at lam + mx, Z over Gs (dead after its eigensolve)
    double *Gs0 = base + mx, *U0 = base + mx + 2 * mm, *lam0 = base + mx + 6 * mm;
    double *Hr0 = base + mx + 5 * mm, *th0 = lam0 + mx;
+   for (int b = 0; b < njobs; ++b)
+      if (jobs[b].h_ready) OFRR_HIP(ctx, hipStreamWaitEvent(ctx->stream, jobs[b].h_ready, 0));
    rrg_scale_kernel<<<dim3((unsigned)((mm + 255) / 256), njobs), 256, 0, ctx->stream>>>(a, base, mdev);
    OFRR_LAUNCH(ctx);
    int st = syev_batched(ctx, njobs, mx, mdev, Gs0, mx, a.slab, lam0, a.slab, U0, mx, a.slab, jwork);
@@ -1087,6 +1089,9 @@ int rr_split_group(ofrr_ctx *ctx, int njobs, const RRJob *jobs) {
           a, base, ints);
       OFRR_LAUNCH(ctx);
    }
+   // the first read of H (everything above needs G only)
+   for (int b = 0; b < njobs; ++b)
+      if (jobs[b].h_ready) OFRR_HIP(ctx, hipStreamWaitEvent(ctx->stream, jobs[b].h_ready, 0));
    for (int stage = 0; stage < 2; ++stage) {
       rrs_gemm_kernel<<<dim3((mx + 63) / 64, mx, njobs), 64, 0, ctx->stream>>>(a, base, ints, stage);
       OFRR_LAUNCH(ctx);

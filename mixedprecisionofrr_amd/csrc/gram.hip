@@ -462,8 +462,10 @@ int gram_multi(ofrr_ctx *ctx, int64_t n, int k, int tb, int tz, GramMultiArgs &a
    if ((1 << la.lk) != k) return set_error(ctx, kErrorInvalidArgument, "gram_multi: k a power of two");
    la.stride = k + ((16 - k % 32) + 32) % 32;  // row stride = 16 mod 32 doubles: two-pass LDS reads
    const size_t lds = (size_t)nop_max * la.R * la.stride * sizeof(double);
-   // about two workgroups per CU over the device
-   int64_t want = std::max<int64_t>(1, (2 * (int64_t)ctx->num_cus) / ngroups);
+   // about two workgroups per CU over the device (for the sizing item count)
+   const int size_groups =
+       ((args.size_halves > 0 ? std::max(args.size_halves, nh) : nh) + kGramWaves - 1) / kGramWaves;
+   int64_t want = std::max<int64_t>(1, (2 * (int64_t)ctx->num_cus) / size_groups);
    int64_t rows = (n + want - 1) / want;
    rows = std::max<int64_t>(la.R, (rows + la.R - 1) / la.R * la.R);
    const int nchunks = (int)((n + rows - 1) / rows);
@@ -471,11 +473,34 @@ int gram_multi(ofrr_ctx *ctx, int64_t n, int k, int tb, int tz, GramMultiArgs &a
    if (!part) return set_error(ctx, kErrorCUDAMemory, "gram_multi: workspace");
    const unsigned grid = (unsigned)(nchunks * ngroups);
    // operand patterns (FP64, basis format, Z format) of the filtered solver:
-   // [X W | D_d D_mid | Z_d Z_mid] and [X W | D_d | Z_d]
-   const int pattern = (cnt[0] == 2 && cnt[1] == 2 && cnt[2] == 2) ? 222
-                       : (cnt[0] == 2 && cnt[1] == 1 && cnt[2] == 1) ? 211 : 0;
-   if (!pattern) return set_error(ctx, kErrorInvalidArgument, "gram_multi: operand pattern");
-   int st = dispatch_dtype(tb, [&](auto b) -> int {
+   // [X W | D_d D_mid | Z_d Z_mid] and [X W | D_d | Z_d] (G and H together, or
+   // H alone), [X | D_d D_mid] and [X | D_d] (G alone)
+   const int pattern = 100 * cnt[0] + 10 * cnt[1] + cnt[2];
+   if (pattern != 222 && pattern != 211 && pattern != 120 && pattern != 110)
+      return set_error(ctx, kErrorInvalidArgument, "gram_multi: operand pattern");
+   int st;
+   if (cnt[2] == 0) {
+      // no operand in the Z format: one instantiation per basis format
+      st = dispatch_dtype(tb, [&](auto b) -> int {
+         using TB = decltype(b);
+         auto launch = [&](auto tag) -> int {
+            constexpr int P = decltype(tag)::value;
+            constexpr auto kern = (P == 120) ? gram_lds_kernel<TB, TB, 1, 2, 0> : gram_lds_kernel<TB, TB, 1, 1, 0>;
+            static std::once_flag once;
+            static hipError_t attr = hipSuccess;
+            std::call_once(once, [] {
+               attr = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          160 * 1024);
+            });
+            OFRR_HIP(ctx, attr);
+            kern<<<grid, kGramThreads, lds, ctx->stream>>>(n, k, la, rows, nchunks, ngroups, part);
+            OFRR_LAUNCH(ctx);
+            return kSuccess;
+         };
+         return pattern == 120 ? launch(std::integral_constant<int, 120>{})
+                               : launch(std::integral_constant<int, 110>{});
+      });
+   } else st = dispatch_dtype(tb, [&](auto b) -> int {
       using TB = decltype(b);
       return dispatch_dtype(tz, [&](auto z) -> int {
          using TZ = decltype(z);

@@ -60,6 +60,10 @@ struct GramMultiArgs {
    int ia[kMaxGramItems], ib[kMaxGramItems], ti[kMaxGramItems], tj[kMaxGramItems];
    double *dst[kMaxGramItems];
    int ld[kMaxGramItems], ro[kMaxGramItems], co[kMaxGramItems];
+   // > 0: the row chunks are sized as for a launch of this many 64 x 32 half
+   // items, so that a subset of a product set, launched on its own, sums each
+   // entry in the order the whole set's launch does (0: this launch's items)
+   int size_halves;
 };
 int gram_multi(ofrr_ctx *ctx, int64_t n, int k, int tb, int tz, GramMultiArgs &args);
 // Y = sum_b V_b C_b
@@ -93,6 +97,10 @@ struct RRJob {
    int ldy;
    int32_t *rank;
    int kx;
+   // nullable: an event H is complete at; the job's stream waits for it
+   // before its first read of H (the projected matrix may still be in flight
+   // on another stream while the Gram-only stages run)
+   hipEvent_t h_ready = nullptr;
 };
 // rr_truncated for each of njobs jobs (m <= 384 each) in one launch per stage on
 // ctx's stream; each job's results equal rr_truncated's on its own arguments

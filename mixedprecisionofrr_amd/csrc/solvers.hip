@@ -676,6 +676,9 @@ struct FilterOp {
    virtual void set_precise(bool) {}
    // whether a sweep may filter a trailing column range only (soft locking)
    virtual bool can_lock() const { return false; }
+   // whether apply_lo may run on the context's second stream (its launches
+   // take their stream from the context and it keeps no scratch of its own)
+   virtual bool can_overlap() const { return false; }
    // the narrowest block storage the operator's products take
    virtual int min_width() const { return 1; }
    // whether a sweep's launches may be recorded as a graph, and what of the
@@ -724,6 +727,7 @@ struct CsrOp final : FilterOp {
    bool can_refine() const override { return true; }
    void set_precise(bool p) override { precise = p; }
    bool can_lock() const override { return true; }
+   bool can_overlap() const override { return true; }
    bool can_graph() const override { return true; }
    void sweep_sig(std::vector<uint64_t> &v) const override {
       v.insert(v.end(), {(uint64_t)(uintptr_t)rp, (uint64_t)(uintptr_t)ci,
@@ -1173,8 +1177,11 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    // the truncated Rayleigh-Ritz itself, or (a serving slot) its job handed
    // to the server's grouped Rayleigh-Ritz
    // kx > 0: the split form with the first kx columns the previous Ritz block
+   // h_ready: set while the projected matrix H is still being formed on the
+   // second stream (the Rayleigh-Ritz waits for it before its first read of H)
+   hipEvent_t h_ready = nullptr;
    auto rr_run = [&](int mm, const double *Gm, const double *Hm, double *Y, int kx) -> int {
-      const RRJob job{Gm, Hm, mm, mm, mm, kr, o->tau, th, Y, mm, rank, kx};
+      const RRJob job{Gm, Hm, mm, mm, mm, kr, o->tau, th, Y, mm, rank, kx, h_ready};
       if (!ctx->rr_group) return rr_group_run(ctx, 1, &job);
       return ctx->rr_group(ctx->rr_group_arg, ctx, &job);
    };
@@ -1195,6 +1202,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       }
       OFRR_HIP(ctx, hipMemcpyAsync(fb.cmap, h_cmap, (size_t)mr * 4, hipMemcpyHostToDevice, ctx->stream));
       OFRR_HIP(ctx, hipMemcpyAsync(fb.cinv, h_cinv, (size_t)me * 4, hipMemcpyHostToDevice, ctx->stream));
+      if (h_ready) OFRR_HIP(ctx, hipStreamWaitEvent(ctx->stream, h_ready, 0));
       sym_gather_kernel<<<(mr * mr + 255) / 256, 256, 0, ctx->stream>>>(mr, fb.cmap, G, me, fb.Gc);
       sym_gather_kernel<<<(mr * mr + 255) / 256, 256, 0, ctx->stream>>>(mr, fb.cmap, H, me, fb.Hc);
       OFRR_LAUNCH(ctx);
@@ -1205,8 +1213,8 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       return kSuccess;
    };
 
-   hipEvent_t ev[8];
-   for (int i = 0; i < 8; ++i) OFRR_HIP(ctx, hipEventCreate(&ev[i]));
+   hipEvent_t ev[11];
+   for (int i = 0; i < 11; ++i) OFRR_HIP(ctx, hipEventCreate(&ev[i]));
    float t_acc[7] = {0, 0, 0, 0, 0, 0, 0};
    int filter_steps = 0, resid_passes = 0, filter_col_steps = 0, filter_first_cols = 0;
    int graph_replays = 0, graph_captures = 0, rr_fallbacks = 0;
@@ -1488,82 +1496,91 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       OFRR_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
       // ---- projection: V = [X | Dd (| Dmid)], AV = [W | Z]
       const void *Dd = Dc;
-      if ((st = op.apply_lo(ctx, ka, Dd, bdt, Z, zdt, (int64_t)ka * nd))) return st;
-      ++spmm_calls;
-      if (nd == 2) {
-         if ((st = op.apply_lo(ctx, ka, Dmid, bdt, (char *)Z + (size_t)ka * zsz, zdt, (int64_t)ka * nd))) return st;
-         ++spmm_calls;
-      }
-      OFRR_HIP(ctx, hipEventRecord(ev[5], ctx->stream));
       const int64_t zs = (int64_t)k * nd;
-      // G and H blocks (upper block triangles, column-major ld m) in one fused
-      // launch: G = [X Dd Dm]^T [X Dd Dm]; H = [X Dd Dm]^T [W Zd Zm] with the
-      // X-D blocks of H formed as W^T D (W = A X in FP64, D exact in its format)
-      if (c == 0) {
-         GramMultiArgs ga{};
-         ga.op[0] = {X, (int64_t)k, 0};
-         ga.op[1] = {W, (int64_t)k, 0};
-         ga.op[2] = {Dd, (int64_t)k, 1};
-         ga.op[3] = {Dmid, (int64_t)k, 1};
-         ga.op[4] = {Z, zs, 2};
-         ga.op[5] = {(const char *)Z + (size_t)k * zsz, zs, 2};
+      // Outside a serving slot (whose Rayleigh-Ritz is grouped) the products
+      // split over two streams: G = V^T V first on the solver's stream, where
+      // the Rayleigh-Ritz then starts (its Gram-only stages, one CU), while the
+      // second stream forms Z = A D and H = V^T A V on the other CUs; the
+      // Rayleigh-Ritz waits for them before it first reads H.
+      const bool overlap = !ctx->rr_group && op.can_overlap();
+      if (overlap) {
+         if (!ctx->aux_stream) OFRR_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+         if (!ctx->ev_aux_fork) OFRR_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_aux_fork, hipEventDisableTiming));
+         if (!ctx->ev_aux_join) OFRR_HIP(ctx, hipEventCreateWithFlags(&ctx->ev_aux_join, hipEventDisableTiming));
+      }
+      auto z_products = [&]() -> int {
+         int s2 = op.apply_lo(ctx, ka, Dd, bdt, Z, zdt, (int64_t)ka * nd);
+         if (s2) return s2;
+         ++spmm_calls;
+         if (nd == 2) {
+            if ((s2 = op.apply_lo(ctx, ka, Dmid, bdt, (char *)Z + (size_t)ka * zsz, zdt, (int64_t)ka * nd))) return s2;
+            ++spmm_calls;
+         }
+         return kSuccess;
+      };
+      // locked columns present with two correction blocks: [D_d | D_mid] packed
+      // row by row next to the Z layout [Z_d | Z_mid] (a free filter buffer:
+      // neither block, nor Db[0] whose second half may hold D_mid = D_1)
+      const int kd = nd * ka;
+      const void *Dp2 = Dd;
+      if (c > 0 && nd == 2) {
+         int b = -1;
+         for (int i = 1; i < 4 && b < 0; ++i)
+            if (i != curIdx && i != snap) b = i;
+         const int es = (int)dtype_size(bdt);
+         const int wr = ka * es / 8;
+         const unsigned g = (unsigned)std::min<int64_t>((n * 2 * wr + 255) / 256, 65536);
+         pack2_rows_kernel<<<g, 256, 0, ctx->stream>>>(n, wr, (const uint64_t *)Dd,
+                                                       (const uint64_t *)Dmid, (uint64_t *)Db[b]);
+         OFRR_LAUNCH(ctx);
+         Dp2 = Db[b];
+      }
+      // the G blocks (want & 1) and the H blocks (want & 2) of the upper block
+      // triangles (column-major, ld me), then their mirror images
+      auto products = [&](int want) -> int {
          // (rows operand, columns operand, destination, block row, block column)
          struct Blk { int a, b; double *dst; int br, bc; };
-         Blk blks[12] = {{0, 0, G, 0, 0}, {0, 2, G, 0, 1}, {2, 2, G, 1, 1},
-                         {0, 1, H, 0, 0}, {1, 2, H, 0, 1}, {2, 4, H, 1, 1},
-                         {0, 3, G, 0, 2}, {2, 3, G, 1, 2}, {3, 3, G, 2, 2},
-                         {1, 3, H, 0, 2}, {2, 5, H, 1, 2}, {3, 5, H, 2, 2}};
-         const int nblk = (nd == 2) ? 12 : 6, nt = (k + 63) / 64;
-         int t = 0;
-         for (int q = 0; q < nblk; ++q)
-            for (int ti = 0; ti < nt; ++ti)
-               for (int tj = 0; tj < nt; ++tj, ++t) {
-                  ga.ia[t] = blks[q].a;
-                  ga.ib[t] = blks[q].b;
-                  ga.ti[t] = ti;
-                  ga.tj[t] = tj;
-                  ga.dst[t] = blks[q].dst;
-                  ga.ld[t] = m;
-                  ga.ro[t] = blks[q].br * k;
-                  ga.co[t] = blks[q].bc * k;
-               }
-         ga.nitems = t;
-         if ((st = gram_multi(ctx, n, k, bdt, zdt, ga))) return st;
-      } else {
-         // locked columns present: X and W have width k, the correction blocks
-         // ka.  With two blocks, [D_d | D_mid] is packed row by row next to the
-         // Z layout [Z_d | Z_mid] (a free filter buffer: neither block, nor
-         // Db[0] whose second half may hold D_mid = D_1), so the six products
-         // X^T X, X^T D, D^T D, X^T W, W^T D, D^T Z each take one launch
-         // (their lower blocks are replaced by the mirror below)
-         const int kd = nd * ka;
-         const void *Dp2 = Dd;
-         if (nd == 2) {
-            int b = -1;
-            for (int i = 1; i < 4 && b < 0; ++i)
-               if (i != curIdx && i != snap) b = i;
-            const int es = (int)dtype_size(bdt);
-            const int wr = ka * es / 8;
-            const unsigned g = (unsigned)std::min<int64_t>((n * 2 * wr + 255) / 256, 65536);
-            pack2_rows_kernel<<<g, 256, 0, ctx->stream>>>(n, wr, (const uint64_t *)Dd,
-                                                          (const uint64_t *)Dmid, (uint64_t *)Db[b]);
-            OFRR_LAUNCH(ctx);
-            Dp2 = Db[b];
-         }
-         if (kd == k) {
-            // the packed correction block is as wide as X: the six products in
-            // one fused launch, as for one full-width correction block
+         int s2 = kSuccess;
+         if (c == 0 || kd == k) {
+            // fused launches: G = [X Dd Dm]^T [X Dd Dm]; H = [X Dd Dm]^T [W Zd Zm]
+            // with the X-D blocks of H formed as W^T D (W = A X in FP64, D exact in
+            // its format); with locked columns the packed correction block is as
+            // wide as X and takes the place of one full-width block
             GramMultiArgs ga{};
             ga.op[0] = {X, (int64_t)k, 0};
             ga.op[1] = {W, (int64_t)k, 0};
-            ga.op[2] = {Dp2, (int64_t)kd, 1};
-            ga.op[3] = {Z, (int64_t)kd, 2};
-            struct Blk { int a, b; double *dst; int br, bc; };
-            const Blk blks[6] = {{0, 0, G, 0, 0}, {0, 2, G, 0, 1}, {2, 2, G, 1, 1},
-                                 {0, 1, H, 0, 0}, {1, 2, H, 0, 1}, {2, 3, H, 1, 1}};
+            int nb;
+            const Blk *blks;
+            static const int kG = 0, kH = 1;
+            Blk b12[12], b6[6];
+            double *dst[2] = {G, H};
+            if (c == 0) {
+               ga.op[2] = {Dd, (int64_t)k, 1};
+               ga.op[3] = {Dmid, (int64_t)k, 1};
+               ga.op[4] = {Z, zs, 2};
+               ga.op[5] = {(const char *)Z + (size_t)k * zsz, zs, 2};
+               const int t12[12][5] = {{0, 0, kG, 0, 0}, {0, 2, kG, 0, 1}, {2, 2, kG, 1, 1},
+                                       {0, 1, kH, 0, 0}, {1, 2, kH, 0, 1}, {2, 4, kH, 1, 1},
+                                       {0, 3, kG, 0, 2}, {2, 3, kG, 1, 2}, {3, 3, kG, 2, 2},
+                                       {1, 3, kH, 0, 2}, {2, 5, kH, 1, 2}, {3, 5, kH, 2, 2}};
+               for (int q = 0; q < 12; ++q)
+                  b12[q] = Blk{t12[q][0], t12[q][1], dst[t12[q][2]], t12[q][3], t12[q][4]};
+               nb = (nd == 2) ? 12 : 6;
+               blks = b12;
+            } else {
+               ga.op[2] = {Dp2, (int64_t)kd, 1};
+               ga.op[3] = {Z, (int64_t)kd, 2};
+               const int t6[6][5] = {{0, 0, kG, 0, 0}, {0, 2, kG, 0, 1}, {2, 2, kG, 1, 1},
+                                     {0, 1, kH, 0, 0}, {1, 2, kH, 0, 1}, {2, 3, kH, 1, 1}};
+               for (int q = 0; q < 6; ++q)
+                  b6[q] = Blk{t6[q][0], t6[q][1], dst[t6[q][2]], t6[q][3], t6[q][4]};
+               nb = 6;
+               blks = b6;
+            }
             const int nt = (k + 63) / 64;
             int t = 0;
-            for (int q = 0; q < 6; ++q)
+            for (int q = 0; q < nb; ++q) {
+               if (!(want & (blks[q].dst == G ? 1 : 2))) continue;
                for (int ti = 0; ti < nt; ++ti)
                   for (int tj = 0; tj < nt; ++tj, ++t) {
                      ga.ia[t] = blks[q].a;
@@ -1575,9 +1592,15 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
                      ga.ro[t] = blks[q].br * k;
                      ga.co[t] = blks[q].bc * k;
                   }
+            }
             ga.nitems = t;
-            if ((st = gram_multi(ctx, n, k, bdt, zdt, ga))) return st;
+            // chunks sized for all nb blocks: G and H launched apart sum each
+            // entry as their fused launch does (64 x 32 halves: ceil(k / 32) per tile row)
+            ga.size_halves = nb * nt * ((k + 31) / 32);
+            if ((s2 = gram_multi(ctx, n, k, bdt, zdt, ga))) return s2;
          } else {
+            // the six products X^T X, X^T D, D^T D, X^T W, W^T D, D^T Z, one
+            // launch each
             struct Prod { const void *a; int adt, wa; const void *b; int bdt_, wb; double *dst; int ro, co; };
             const Prod pr[6] = {{X, OFRR_F64, k, X, OFRR_F64, k, G, 0, 0},
                                 {X, OFRR_F64, k, Dp2, bdt, kd, G, 0, k},
@@ -1587,20 +1610,44 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
                                 {Dp2, bdt, kd, Z, zdt, kd, H, k, k}};
             for (int q = 0; q < 6; ++q) {
                const Prod &P = pr[q];
-               if ((st = gram(ctx, n, P.wa, P.wb, P.a, P.adt, P.wa, 1, P.b, P.bdt_, P.wb, 1, 1.0, 0.0,
+               if (!(want & (P.dst == G ? 1 : 2))) continue;
+               if ((s2 = gram(ctx, n, P.wa, P.wb, P.a, P.adt, P.wa, 1, P.b, P.bdt_, P.wb, 1, 1.0, 0.0,
                               P.dst + P.ro + (size_t)P.co * me, me)))
-                  return st;
+                  return s2;
             }
          }
+         if (want & 1) mirror_upper_kernel<<<(me * me + 255) / 256, 256, 0, ctx->stream>>>(me, G, me);
+         if (want & 2) mirror_upper_kernel<<<(me * me + 255) / 256, 256, 0, ctx->stream>>>(me, H, me);
+         OFRR_LAUNCH(ctx);
+         return kSuccess;
+      };
+      if (!overlap) {
+         if ((st = z_products())) return st;
+         OFRR_HIP(ctx, hipEventRecord(ev[5], ctx->stream));
+         if ((st = products(3))) return st;
+      } else {
+         if ((st = products(1))) return st;
+         OFRR_HIP(ctx, hipEventRecord(ctx->ev_aux_fork, ctx->stream));
+         OFRR_HIP(ctx, hipStreamWaitEvent(ctx->aux_stream, ctx->ev_aux_fork, 0));
+         {
+            OnAuxStream aux(ctx);
+            OFRR_HIP(ctx, hipEventRecord(ev[8], ctx->stream));
+            if ((st = z_products())) return st;
+            OFRR_HIP(ctx, hipEventRecord(ev[9], ctx->stream));
+            if ((st = products(2))) return st;
+            OFRR_HIP(ctx, hipEventRecord(ev[10], ctx->stream));
+            OFRR_HIP(ctx, hipEventRecord(ctx->ev_aux_join, ctx->stream));
+         }
       }
-      mirror_upper_kernel<<<(me * me + 255) / 256, 256, 0, ctx->stream>>>(me, G, me);
-      mirror_upper_kernel<<<(me * me + 255) / 256, 256, 0, ctx->stream>>>(me, H, me);
-      OFRR_LAUNCH(ctx);
       OFRR_HIP(ctx, hipEventRecord(ev[6], ctx->stream));
       // the split Rayleigh-Ritz (X = the first kr columns); where X^T X is not
       // positive definite it reports rank -1 and the whole form runs instead
+      h_ready = overlap ? ctx->ev_aux_join : nullptr;
       for (int form = o->rr_whole ? 1 : 0; form < 2; ++form) {
          if ((st = rayleigh_ritz(me, ka, form ? 0 : kr))) return st;
+         // (already waited for inside; kept explicit so that nothing later on
+         // this stream can overtake the second stream's work)
+         if (h_ready) OFRR_HIP(ctx, hipStreamWaitEvent(ctx->stream, h_ready, 0));
          OFRR_HIP(ctx, hipEventRecord(ev[7], ctx->stream));
          {
             const void *vb[3] = {X, Dd, Dmid};
@@ -1620,8 +1667,17 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       {
          float a, b, c, dd, e2;
          OFRR_HIP(ctx, hipEventElapsedTime(&a, ev[3], ev[4]));
-         OFRR_HIP(ctx, hipEventElapsedTime(&b, ev[4], ev[5]));
-         OFRR_HIP(ctx, hipEventElapsedTime(&c, ev[5], ev[6]));
+         if (overlap) {
+            // Z and the H blocks ran on the second stream beside the Rayleigh-Ritz
+            float gh;
+            OFRR_HIP(ctx, hipEventElapsedTime(&b, ev[8], ev[9]));
+            OFRR_HIP(ctx, hipEventElapsedTime(&c, ev[4], ev[6]));
+            OFRR_HIP(ctx, hipEventElapsedTime(&gh, ev[9], ev[10]));
+            c += gh;
+         } else {
+            OFRR_HIP(ctx, hipEventElapsedTime(&b, ev[4], ev[5]));
+            OFRR_HIP(ctx, hipEventElapsedTime(&c, ev[5], ev[6]));
+         }
          OFRR_HIP(ctx, hipEventElapsedTime(&dd, ev[6], ev[7]));
          OFRR_HIP(ctx, hipEventElapsedTime(&e2, ev[7], ev[2]));
          t_acc[0] += a;
@@ -1647,7 +1703,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    OFRR_HIP(ctx, hipStreamSynchronize(ctx->stream));
    float total;
    OFRR_HIP(ctx, hipEventElapsedTime(&total, ev[0], ev[2]));
-   for (int i = 0; i < 8; ++i) (void)hipEventDestroy(ev[i]);
+   for (int i = 0; i < 11; ++i) (void)hipEventDestroy(ev[i]);
    for (int j = 0; j < kr; ++j) {
       theta_host[j] = theta[j];
       res_host[j] = res[j];
