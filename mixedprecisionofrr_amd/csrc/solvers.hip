@@ -1274,9 +1274,22 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    std::vector<double> worst_hist;
    for (int j = 0; j < k; ++j) h_sR[j] = 1.0;
    for (int j = 0; j < k; ++j) theta[j] = h_th[j];
+   // column maxima |R| of the last two iterations' final residual passes (FP16)
+   std::vector<double> cm_last(k, 0.0), cm_prev(k, 0.0);
    for (it = 0;; ++it) {
       // ---- FP64 operator application + residual, quantised R
       OFRR_HIP(ctx, rec(1));
+      if (bdt == OFRR_F16) {
+         // predicted scale of R: the last maximum times the column's last
+         // reduction factor (within [2^-6, 1]), so that the stored range stays
+         // near the top of the format and R is seldom formed twice
+         for (int j = 0; j < k; ++j) {
+            if (!(cm_last[j] > 0.0)) continue;
+            double f = cm_prev[j] > 0.0 ? cm_last[j] / cm_prev[j] : 1.0;
+            f = std::min(1.0, std::max(1.0 / 64.0, f));
+            h_sR[j] = std::exp2(std::ceil(std::log2(cm_last[j] * f)) - tgt_resid);
+         }
+      }
    resid_pass:
       for (int pass = 0; pass < 2; ++pass) {
          for (int j = 0; j < k; ++j) inv[j] = 1.0 / h_sR[j];
@@ -1303,6 +1316,11 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
          if (!redo || pass == 1) break;
          for (int j = 0; j < k; ++j) h_sR[j] = pred[j];
       }
+      if (bdt == OFRR_F16)
+         for (int j = 0; j < k; ++j) {
+            cm_prev[j] = cm_last[j];
+            cm_last[j] = (double)h_cmaxR[j];
+         }
       OFRR_HIP(ctx, rec(2));
       OFRR_HIP(ctx, hipEventSynchronize(ev[2]));
       const double normA = o->normA > 0 ? o->normA : theta[0];
