@@ -80,11 +80,19 @@ def measured_traffic(args):
         return None
 
 
-def run_solver(api, ctx, A, X0, args, basis, label):
+def wide_of(args):
+    """The wide format of the mixed solve: FP32 for FP16 growth blocks when
+    --wide-tol > 0 (None: every sweep in the growth format)."""
+    import torch
+    return torch.float32 if (args.wide_tol > 0 and args.basis == "f16") else None
+
+
+def run_solver(api, ctx, A, X0, args, basis, label, wide=None):
     import torch
     st, th, X, res, stats = api.filtered_eig(
         ctx, A, X0, args.nwant, nd=args.nd, degree=args.degree, maxiter=args.maxiter,
-        basis_dtype=basis, tol=args.tol, tau=args.tau)
+        basis_dtype=basis, tol=args.tol, tau=args.tau, wide_dtype=wide,
+        wide_tol=args.wide_tol if wide is not None else 0.0)
     torch.cuda.synchronize()
     return st, th, X, res, stats
 
@@ -230,6 +238,9 @@ def main():
     p.add_argument("--tau", type=float, default=1e-12)
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--fp64-steps", type=int, default=1)
+    p.add_argument("--wide-tol", type=float, default=0.0,
+                   help="FP16 basis: the sweeps after the first that start with a wanted residual "
+                        "above this build their blocks in FP32 (0, the default: every sweep in FP16)")
     p.add_argument("--basis", choices=["f16", "bf16", "fp8"], default="f16",
                    help="growth-block format of the mixed path (the workload names FP16)")
     p.add_argument("--no-cpu", action="store_true")
@@ -313,8 +324,9 @@ def main():
     torch.cuda.synchronize()
 
     # ---- mixed path: warmup + timed steps
+    wide = wide_of(args)
     for w in range(args.warmup):
-        st, th, X, res, stats = run_solver(api, ctx, A, X0, args, basis, "mixed")
+        st, th, X, res, stats = run_solver(api, ctx, A, X0, args, basis, "mixed", wide)
         log(f"warmup {w}: status {st} iters {stats['iterations']} {stats['t_total_ms']:.1f} ms")
     lat, pairs, last = [], 0, None
     filt_ms = filt_steps = 0.0
@@ -322,7 +334,7 @@ def main():
     t0 = time.perf_counter()
     for s in range(args.steps):
         ts = time.perf_counter()
-        st, th, X, res, stats = run_solver(api, ctx, A, X0, args, basis, "mixed")
+        st, th, X, res, stats = run_solver(api, ctx, A, X0, args, basis, "mixed", wide)
         lat.append((time.perf_counter() - ts) * 1e3)
         pairs += int(np.sum(res[:args.nwant] <= args.tol))
         filt_ms += stats["t_filter_ms"]
@@ -339,14 +351,28 @@ def main():
     # independent residual check of the returned block (FP64, on the GPU)
     res_chk = api.residuals(ctx, A, X, th, float(exact[0]))
 
-    # ---- roofline of the dominant kernel (Chebyshev step)
+    # ---- every sweep in the growth format (untimed in the value): its own
+    # figures, and the roofline of the growth-format step kernel
+    plain = None
+    rstats, rfilt_ms, rfilt_steps = stats, filt_ms, filt_steps
+    if wide is not None:
+        ts = time.perf_counter()
+        stp, thp, Xp, resp, sp = run_solver(api, ctx, A, X0, args, basis, "plain")
+        msp = (time.perf_counter() - ts) * 1e3
+        log(f"{args.basis} every sweep: status {stp} iters {sp['iterations']} {msp:.1f} ms")
+        plain = {"value": int(np.sum(resp[:args.nwant] <= args.tol)) / (msp * 1e-3),
+                 "unit": "eigenpairs/s", "ms_per_solve": msp, "iterations": sp["iterations"],
+                 "status": stp, "max_residual": float(np.max(resp[:args.nwant])),
+                 "t_filter_ms": sp["t_filter_ms"], "filter_steps": sp["filter_steps"]}
+        rstats, rfilt_ms, rfilt_steps = sp, sp["t_filter_ms"], sp["filter_steps"]
+
+    # ---- roofline of the dominant kernel (Chebyshev step, growth format)
     n, nnz, k = A.n, A.nnz, args.k
-    sweeps = stats["iterations"]
-    per_solve_steps = stats["filter_steps"]
+    per_solve_steps = rstats["filter_steps"]
     b_basis = 1 if args.basis == "fp8" else 2
-    bytes_solve = filter_bytes(n, nnz, 2, b_basis, per_solve_steps, stats["filter_col_steps"],
-                               stats["filter_first_cols"])
-    avg_launch_ms = filt_ms / max(1, filt_steps)
+    bytes_solve = filter_bytes(n, nnz, 2, b_basis, per_solve_steps, rstats["filter_col_steps"],
+                               rstats["filter_first_cols"])
+    avg_launch_ms = rfilt_ms / max(1, rfilt_steps)
     bytes_per_launch = bytes_solve / max(1, per_solve_steps)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
 
@@ -396,7 +422,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"{args.basis} storage / f32 accumulate (growth), f64 (Ritz block, Gram, RR, residual)",
+        "dtype": (f"{args.basis} storage" +
+                  (f" (f32 for the sweeps after the first that start above residual {args.wide_tol:g}:"
+                   f" {stats['wide_sweeps']} of {stats['iterations']})" if wide is not None else "") +
+                  " / f32 accumulate (growth), f64 (Ritz block, Gram, RR, residual)"),
         "data": "synthetic",
         "config": {
             "workload": (f"mixed-precision filtered block subspace iteration + OFRR, 3D 7-point "
@@ -405,10 +434,13 @@ def main():
                          f"{args.basis.upper()} basis"),
             "n": n, "nnz": nnz, "k": k, "basis_dim": k * (1 + args.nd), "nwant": args.nwant,
             "filter_degree": args.degree, "tol": args.tol, "seed": args.seed,
+            "wide_sweeps": ("f32 after the first sweep while the wanted residual > "
+                            f"{args.wide_tol:g}") if wide is not None else None,
         },
         "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99), "arrival": "back-to-back solves"},
         "speedup_vs_fp64_gpu": (value / fp64["value"]) if fp64 and fp64["value"] else None,
         "fp64_gpu": fp64,
+        "every_sweep_in_growth_format": plain,
         "solve": {"status": st, "iterations": stats["iterations"], "spmm_calls": stats["spmm_calls"],
                   "converged": stats["converged"], "max_residual": float(np.max(res[:args.nwant])),
                   "max_residual_recheck": float(np.max(res_chk[:args.nwant])),
@@ -416,7 +448,8 @@ def main():
                   "t_total_ms": stats["t_total_ms"], "t_filter_ms": stats["t_filter_ms"],
                   "t_zspmm_ms": stats["t_zspmm_ms"], "t_gram_ms": stats["t_gram_ms"],
                   "t_rr_ms": stats["t_rr_ms"], "t_update_ms": stats["t_update_ms"],
-                  "t_resid_ms": stats["t_resid_ms"], "rank_min": stats["rank_min"]},
+                  "t_resid_ms": stats["t_resid_ms"], "rank_min": stats["rank_min"],
+                  "wide_sweeps": stats["wide_sweeps"], "filter_steps": stats["filter_steps"]},
         "roofline": {"kernel": f"cheb_step_kernel (FP16 operator, {args.basis.upper()} blocks: CSR SpMM "
                                f"+ filter axpys)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -351,6 +351,13 @@ typedef struct {
                         its X component removed is eigendecomposed (size m - k), then
                         the D block of the projected matrix before the coupled one; the
                         whole form is used where X^T X is not positive definite */
+   int wide_dtype;   /* CSR operator only, with wide_tol > 0: every sweep after the
+                        first whose largest wanted residual exceeds wide_tol builds its
+                        growth blocks in wide_dtype, a format wider than basis_dtype
+                        (e.g. FP32 for FP16 blocks); the other sweeps use basis_dtype.
+                        FP16 rounding of the correction stalls the second sweep, whose
+                        residual block is still large against its useful part */
+   double wide_tol;  /* 0: off */
 } ofrr_filter_opts;
 
 typedef struct {
@@ -373,6 +380,7 @@ typedef struct {
    int graph_captures;   /* of those, sweeps recorded anew (first of a shape, or new buffers) */
    int rr_fallbacks;     /* split Rayleigh-Ritz steps redone in the whole form (X^T X
                             not positive definite) */
+   int wide_sweeps;      /* sweeps built in opts.wide_dtype */
 } ofrr_filter_stats;
 
 /* X_dev: n x k FP64 row-major (row stride k), start block in / Ritz block out;

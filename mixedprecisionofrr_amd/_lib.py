@@ -41,7 +41,7 @@ class FilterOpts(C.Structure):
                 ("tau", C.c_double), ("lower", C.c_double), ("normA", C.c_double),
                 ("refine", C.c_int), ("refine_dtype", C.c_int), ("refine_tol", C.c_double),
                 ("lock", C.c_int), ("mid_cols", C.c_int), ("graph", C.c_int),
-                ("rr_whole", C.c_int)]
+                ("rr_whole", C.c_int), ("wide_dtype", C.c_int), ("wide_tol", C.c_double)]
 
 
 class FilterStats(C.Structure):
@@ -52,7 +52,7 @@ class FilterStats(C.Structure):
                 ("t_update_ms", C.c_double), ("t_resid_ms", C.c_double),
                 ("filter_col_steps", C.c_int), ("filter_first_cols", C.c_int),
                 ("graph_replays", C.c_int), ("graph_captures", C.c_int),
-                ("rr_fallbacks", C.c_int)]
+                ("rr_fallbacks", C.c_int), ("wide_sweeps", C.c_int)]
 
 
 class ServerOpts(C.Structure):

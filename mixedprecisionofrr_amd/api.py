@@ -514,7 +514,7 @@ def subspace_iter_eig(ctx, A, V0, maxiter, step_size=1, select_tol=2.0 ** -52, g
 def filtered_eig(ctx, A, X0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=torch.float16,
                  tol=1e-8, tau=1e-12, lower=0.0, normA=0.0, use_lo_values=True,
                  refine_dtype=None, refine_tol=0.0, lock=True, mid_cols=0, graph=False,
-                 rr_whole=False):
+                 rr_whole=False, wide_dtype=None, wide_tol=0.0):
     """Filtered OFRR solve; X0 (n x k, FP64) is copied.  refine_dtype: the
     precision ladder's refinement format (None: off).  lock: soft locking (a
     sweep filters only the columns after the leading converged wanted pairs).
@@ -522,6 +522,9 @@ def filtered_eig(ctx, A, X0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=to
     shape repeats (same results as plain launches).  rr_whole: every
     Rayleigh-Ritz eigendecomposes the Gram matrix of the whole basis instead of
     the split form (X whitened by Cholesky, the D block's Schur complement).
+    wide_dtype / wide_tol: every sweep after the first that starts with a wanted
+    residual above wide_tol builds its blocks in wide_dtype (wider than
+    basis_dtype; None / 0: off).
     Returns (status, theta, X row-major, res, stats dict)."""
     A.require_square("filtered_eig")
     _check_rows(X0, A.n, "filtered_eig", "X0")
@@ -531,7 +534,9 @@ def filtered_eig(ctx, A, X0, nwant, nd=1, degree=20, maxiter=100, basis_dtype=to
     o = FilterOpts(k, nwant, nd, degree, maxiter, bdt, tol, tau, lower, normA,
                    0 if refine_dtype is None else 1,
                    0 if refine_dtype is None else dtype_code(refine_dtype), refine_tol,
-                   1 if lock else 0, mid_cols, 1 if graph else 0, 1 if rr_whole else 0)
+                   1 if lock else 0, mid_cols, 1 if graph else 0, 1 if rr_whole else 0,
+                   0 if wide_dtype is None else dtype_code(wide_dtype),
+                   0.0 if wide_dtype is None else float(wide_tol))
     th = (C.c_double * k)()
     res = (C.c_double * k)()
     stats = FilterStats()

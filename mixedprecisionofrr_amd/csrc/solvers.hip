@@ -902,9 +902,14 @@ static int filter_width(int k, int minw = 1) {
 // the narrowest storage the kernels take: the step kernels read 16 bytes per
 // lane (8 FP16 / 16 FP8 columns), the residual kernel two FP64 columns, and
 // the dense operators' MFMA products 16 columns (op_minw)
+// the format of the wide sweeps (the growth format itself when they are off)
+static int wide_format(const ofrr_filter_opts &o) {
+   return o.wide_tol > 0.0 ? o.wide_dtype : o.basis_dtype;
+}
 static int filter_min_width(const ofrr_filter_opts &o, int op_minw) {
    const int rdt = o.refine ? o.refine_dtype : o.basis_dtype;
-   const size_t es = std::min(dtype_size(o.basis_dtype), dtype_size(rdt));
+   const size_t es = std::min(std::min(dtype_size(o.basis_dtype), dtype_size(rdt)),
+                              dtype_size(wide_format(o)));
    return std::max(std::max(2, (int)(16 / std::max<size_t>(1, es))), op_minw);
 }
 
@@ -933,10 +938,12 @@ struct Carve {
 static void filtered_layout(Carve &cv, int64_t n, const ofrr_filter_opts &o, int op_minw, FiltBufs &b) {
    const int k = o.k, kp = filter_width(k, filter_min_width(o, op_minw)), nd = o.nd, d = o.degree;
    const int m = k * (1 + nd), mp = kp * (1 + nd);
-   const int bdt = o.basis_dtype, rdt = o.refine ? o.refine_dtype : bdt;
-   // blocks sized for the larger of the growth format and the refinement format
-   const size_t bsz = std::max(dtype_size(bdt), dtype_size(rdt));
-   const size_t zsz = (bdt == OFRR_F64 || rdt == OFRR_F64) ? 8 : 4;
+   const int bdt = o.basis_dtype, rdt = o.refine ? o.refine_dtype : bdt, wdt = wide_format(o);
+   // blocks sized for the largest of the growth, refinement and wide formats
+   const size_t bsz = std::max(std::max(dtype_size(bdt), dtype_size(rdt)), dtype_size(wdt));
+   // Z = A D in FP64 for the 4- and 8-byte formats (see set_format)
+   auto zbytes = [](int dt) { return (dt == OFRR_F64 || dt == OFRR_F32) ? 8 : 4; };
+   const size_t zsz = (size_t)std::max(std::max(zbytes(bdt), zbytes(rdt)), zbytes(wdt));
    const size_t nk = (size_t)n * kp, kq = (size_t)kp;
    b.X = (double *)cv.take(nk * 8);
    b.Xn = (double *)cv.take(nk * 8);
@@ -1130,6 +1137,11 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    if (o->refine && (!dtype_valid(o->refine_dtype) || !op.can_refine()))
       return set_error(ctx, kErrorInvalidArgument,
                        "filtered_eig: the refinement format is valid for a CSR operator only");
+   const bool wide_on = o->wide_tol > 0.0;
+   if (wide_on && (!dtype_valid(o->wide_dtype) || !op.can_refine() ||
+                   dtype_size(o->wide_dtype) <= dtype_size(o->basis_dtype)))
+      return set_error(ctx, kErrorInvalidArgument,
+                       "filtered_eig: the wide format must be wider than the growth format (CSR operator only)");
    const int k = filter_width(kr, minw);
    // mid: the trailing columns of the mid-filter block kept in the basis (0: all)
    const int mid_keep = (nd == 2 && o->mid_cols > 0 && o->mid_cols < kr) ? o->mid_cols : 0;
@@ -1137,7 +1149,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    const int m = k * (1 + nd);
    if (kr * nd + (mid_keep ? mid_keep : (nd == 2 ? kr : 0)) > 384)
       return set_error(ctx, kErrorInvalidArgument, "filtered_eig: basis dimension > 384");
-   int zdt = (bdt == OFRR_F64) ? OFRR_F64 : OFRR_F32;
+   int zdt = (bdt == OFRR_F64 || bdt == OFRR_F32) ? OFRR_F64 : OFRR_F32;  // as set_format
    size_t zsz = dtype_size(zdt);
    // ---- buffers
    const size_t state_bytes = filtered_state_bytes(n, *o, op.min_width());
@@ -1217,7 +1229,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    for (int i = 0; i < 11; ++i) OFRR_HIP(ctx, hipEventCreate(&ev[i]));
    float t_acc[7] = {0, 0, 0, 0, 0, 0, 0};
    int filter_steps = 0, resid_passes = 0, filter_col_steps = 0, filter_first_cols = 0;
-   int graph_replays = 0, graph_captures = 0, rr_fallbacks = 0;
+   int graph_replays = 0, graph_captures = 0, rr_fallbacks = 0, wide_sweeps = 0;
    auto rec = [&](int i) { return hipEventRecord(ev[i], ctx->stream); };
    OFRR_HIP(ctx, rec(0));
 
@@ -1262,7 +1274,13 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    double lo_ratio = 1.0;
    auto set_format = [&](int dt) {
       bdt = dt;
-      zdt = (bdt == OFRR_F64) ? OFRR_F64 : OFRR_F32;
+      // Z = A D accumulated and kept in FP64 for FP32 blocks too: their Gram
+      // matrix resolves directions down to about 1e-15 of its largest
+      // eigenvalue, above the truncation, and an FP32 Z (relative error ~6e-8)
+      // would put errors of order 1e-10 / lambda into those directions of H
+      // (spurious Ritz values above the wanted ones); the FP16 blocks' own
+      // rounding keeps their Gram spectrum above ~1e-7, where FP32 Z suffices
+      zdt = (bdt == OFRR_F64 || bdt == OFRR_F32) ? OFRR_F64 : OFRR_F32;
       zsz = dtype_size(zdt);
       scaled = (bdt == OFRR_F16 || bdt == OFRR_FP8);
       tgt_resid = (bdt == OFRR_FP8) ? 8 : 0;
@@ -1330,6 +1348,22 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
          res[j] = std::sqrt(h_sums[2 * j]) / (normA * std::sqrt(h_sums[2 * j + 1]));
          if (j < o->nwant && res[j] <= o->tol) ++nconv;
          if (j < o->nwant) worst = std::max(worst, res[j]);
+      }
+      // wide sweeps: after the first, a sweep that starts above wide_tol builds
+      // its blocks in the wide format, the others in the growth format (until
+      // the precision ladder, if on, has switched); R is then formed again in
+      // the sweep's format, with the scale measured in this pass
+      if (wide_on && nconv < o->nwant && !(o->refine && bdt == o->refine_dtype)) {
+         const int tf = (it >= 1 && worst > o->wide_tol) ? o->wide_dtype : o->basis_dtype;
+         if (tf != bdt) {
+            set_format(tf);
+            op.set_precise(tf == OFRR_F64);
+            for (int j = 0; j < k; ++j)
+               h_sR[j] = (scaled && h_cmaxR[j] > 0.f)
+                             ? std::exp2(std::ceil(std::log2((double)h_cmaxR[j])) - tgt_resid)
+                             : 1.0;
+            goto resid_pass;
+         }
       }
       // precision ladder: continue in the refinement format once the wanted
       // residuals are below refine_tol or have not halved in three iterations;
@@ -1509,6 +1543,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       }
       spmm_calls += d - 1;
       filter_steps += d - 1;
+      if (wide_on && bdt == o->wide_dtype) ++wide_sweeps;
       filter_col_steps += ka * (d - 1);
       filter_first_cols += ka;
       OFRR_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
@@ -1737,6 +1772,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       stats->graph_replays = graph_replays;
       stats->graph_captures = graph_captures;
       stats->rr_fallbacks = rr_fallbacks;
+      stats->wide_sweeps = wide_sweeps;
       stats->resid_passes = resid_passes;
       stats->t_total_ms = total;
       stats->t_filter_ms = t_acc[0];

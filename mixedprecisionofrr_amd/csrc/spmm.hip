@@ -187,7 +187,9 @@ __global__ __launch_bounds__(256) void spmm_rm_kernel(
     const TV *__restrict__ va, int k, const TX *__restrict__ X, int64_t ldx, int64_t nx,
     double alpha, double beta, TY *__restrict__ Y, int64_t ldy, int lg, int64_t nblk,
     int64_t plane) {
-   using ACC = typename std::conditional<std::is_same<TX, double>::value, double, float>::type;
+   // FP64 accumulation for FP64 operands or an FP64 result
+   using ACC = typename std::conditional<std::is_same<TX, double>::value || std::is_same<TY, double>::value,
+                                         double, float>::type;
    __shared__ CsrTile<TV> tile;
    const int64_t blk = sched_block(nblk, plane);
    if (blk < 0) return;
@@ -493,7 +495,9 @@ __global__ void spmm_strided_kernel(int64_t n, const int64_t *__restrict__ rp,
                                     int k, const TX *__restrict__ X, int64_t xrs, int64_t xcs,
                                     double alpha, double beta, TY *__restrict__ Y, int64_t yrs,
                                     int64_t ycs) {
-   using ACC = typename std::conditional<std::is_same<TX, double>::value, double, float>::type;
+   // FP64 accumulation for FP64 operands or an FP64 result
+   using ACC = typename std::conditional<std::is_same<TX, double>::value || std::is_same<TY, double>::value,
+                                         double, float>::type;
    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
    if (t >= n * (int64_t)k) return;
    int64_t row = t % n;

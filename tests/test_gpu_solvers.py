@@ -263,6 +263,73 @@ def test_filtered_mixed_reaches_1e8(ctx, bdt):
     assert res_o[:nwant].max() <= 1.05e-8
 
 
+def test_filtered_wide_sweeps(ctx):
+    """FP16 growth blocks with FP32 for the sweeps after the first that start
+    above residual 1e-3 (wide_dtype / wide_tol): the wide sweeps are counted,
+    the solve reaches residual 1e-8 and the closed-form spectrum, and it takes
+    fewer iterations than with every sweep in FP16 (whose second sweep stalls on
+    the rounding of the correction blocks; the FP64 path's count is the floor)."""
+    from mixedprecisionofrr_amd import api
+    N, k, nwant = 40, 64, 50
+    A = api.CSRMatrix.stencil(ctx, 3, N).with_low_precision(torch.float16)
+    X0 = torch.empty(A.n, k, dtype=torch.float64, device="cuda:0")
+    api.fill_randn(ctx, X0, seed=0)
+    print("filtered wide sweeps N=40 k=64", flush=True)
+    runs = {}
+    for name, bdt, wide in [("f16", torch.float16, None), ("wide", torch.float16, torch.float32),
+                            ("f64", torch.float64, None)]:
+        runs[name] = api.filtered_eig(ctx, A, X0, nwant, nd=2, degree=120, maxiter=60,
+                                      basis_dtype=bdt, tol=1e-8, wide_dtype=wide,
+                                      wide_tol=1e-3 if wide is not None else 0.0)
+        assert runs[name][0] == 0, (name, runs[name][4])
+    st, th, X, res, stats = runs["wide"]
+    assert stats["wide_sweeps"] >= 1 and runs["f16"][4]["wide_sweeps"] == 0
+    ex = O.laplacian_eigs(3, N)[:nwant]
+    assert np.max(np.abs(th[:nwant] - ex) / ex) < 1e-10
+    res_o = O.residuals(O.stencil(3, N), X.cpu().numpy(), th, ex[0])
+    assert res_o[:nwant].max() <= 1.05e-8
+    its = {name: r[4]["iterations"] for name, r in runs.items()}
+    assert its["f64"] <= its["wide"] < its["f16"], its
+
+
+@pytest.mark.parametrize("N", [40, 64])
+def test_filtered_fp32_basis_high_degree(ctx, N):
+    """FP32 growth blocks at degree 120: their Gram matrix resolves directions
+    down to ~1e-15 of its largest eigenvalue, so Z = A D must be FP64 (an FP32 Z
+    put spurious Ritz values above the wanted ones: residual ~1 from the second
+    iteration on).  The solve reaches 1e-8 and the closed-form spectrum in no
+    more iterations than the FP16 one."""
+    from mixedprecisionofrr_amd import api
+    k, nwant = 64, 50
+    A = api.CSRMatrix.stencil(ctx, 3, N).with_low_precision(torch.float16)
+    X0 = torch.empty(A.n, k, dtype=torch.float64, device="cuda:0")
+    api.fill_randn(ctx, X0, seed=0)
+    print(f"filtered fp32 basis degree 120 N={N}", flush=True)
+    its = {}
+    for bdt in (torch.float32, torch.float16):
+        st, th, X, res, stats = api.filtered_eig(ctx, A, X0, nwant, nd=2, degree=120, maxiter=40,
+                                                 basis_dtype=bdt, tol=1e-8)
+        assert st == 0, (bdt, stats)
+        its[bdt] = stats["iterations"]
+        if bdt == torch.float32:
+            ex = O.laplacian_eigs(3, N)[:nwant]
+            assert np.max(np.abs(th[:nwant] - ex) / ex) < 1e-10
+            res_o = O.residuals(O.stencil(3, N), X.cpu().numpy(), th, ex[0])
+            assert res_o[:nwant].max() <= 1.05e-8
+    assert its[torch.float32] <= its[torch.float16], its
+
+
+def test_filtered_wide_format_must_be_wider(ctx):
+    """The wide format must be wider than the growth format."""
+    from mixedprecisionofrr_amd import api
+    A = api.CSRMatrix.stencil(ctx, 3, 8).with_low_precision(torch.float16)
+    X0 = torch.empty(A.n, 16, dtype=torch.float64, device="cuda:0")
+    api.fill_randn(ctx, X0, seed=0)
+    with pytest.raises(Exception, match="wide format"):
+        api.filtered_eig(ctx, A, X0, 8, nd=1, degree=8, maxiter=4, basis_dtype=torch.float16,
+                         wide_dtype=torch.bfloat16, wide_tol=1e-3)
+
+
 def test_filtered_same_alone_and_repeated(ctx):
     """Deterministic reductions: the same solve twice gives bitwise equal results."""
     from mixedprecisionofrr_amd import api
