@@ -393,6 +393,7 @@ def main():
                 "iterations": stats64["iterations"], "status": st64,
                 "max_residual": float(np.max(res64[:args.nwant])),
                 "t_filter_ms": stats64["t_filter_ms"], "t_gram_ms": stats64["t_gram_ms"],
+                "t_aux_ms": stats64["t_aux_ms"],
                 "t_rr_ms": stats64["t_rr_ms"], "t_update_ms": stats64["t_update_ms"],
                 "t_resid_ms": stats64["t_resid_ms"], "t_zspmm_ms": stats64["t_zspmm_ms"]}
     # ---- CPU baseline
@@ -447,6 +448,7 @@ def main():
                   "max_eig_rel_err_vs_closed_form": api_exact,
                   "t_total_ms": stats["t_total_ms"], "t_filter_ms": stats["t_filter_ms"],
                   "t_zspmm_ms": stats["t_zspmm_ms"], "t_gram_ms": stats["t_gram_ms"],
+                  "t_aux_ms": stats["t_aux_ms"],
                   "t_rr_ms": stats["t_rr_ms"], "t_update_ms": stats["t_update_ms"],
                   "t_resid_ms": stats["t_resid_ms"], "rank_min": stats["rank_min"],
                   "wide_sweeps": stats["wide_sweeps"], "filter_steps": stats["filter_steps"]},

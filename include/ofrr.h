@@ -370,7 +370,8 @@ typedef struct {
    double t_total_ms;    /* device time of the solve (events on the stream) */
    double t_filter_ms;   /* Chebyshev steps only */
    double t_zspmm_ms;    /* operator applied to the growth blocks for V^T A V */
-   double t_gram_ms;     /* projection products */
+   double t_gram_ms;     /* projection products on the solver's stream (all of them,
+                            or G = V^T V alone when t_aux_ms > 0) */
    double t_rr_ms;       /* Rayleigh-Ritz */
    double t_update_ms;   /* back-transform */
    double t_resid_ms;    /* FP64 operator application + residual (incl. host checks) */
@@ -381,6 +382,9 @@ typedef struct {
    int rr_fallbacks;     /* split Rayleigh-Ritz steps redone in the whole form (X^T X
                             not positive definite) */
    int wide_sweeps;      /* sweeps built in opts.wide_dtype */
+   double t_aux_ms;      /* Z = A D and the H = V^T A V products on the context's second
+                            stream, run beside the Rayleigh-Ritz (overlaps t_rr_ms; 0
+                            when the products all run on the solver's stream) */
 } ofrr_filter_stats;
 
 /* X_dev: n x k FP64 row-major (row stride k), start block in / Ritz block out;

@@ -52,7 +52,7 @@ class FilterStats(C.Structure):
                 ("t_update_ms", C.c_double), ("t_resid_ms", C.c_double),
                 ("filter_col_steps", C.c_int), ("filter_first_cols", C.c_int),
                 ("graph_replays", C.c_int), ("graph_captures", C.c_int),
-                ("rr_fallbacks", C.c_int), ("wide_sweeps", C.c_int)]
+                ("rr_fallbacks", C.c_int), ("wide_sweeps", C.c_int), ("t_aux_ms", C.c_double)]
 
 
 class ServerOpts(C.Structure):

@@ -1227,7 +1227,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
 
    hipEvent_t ev[11];
    for (int i = 0; i < 11; ++i) OFRR_HIP(ctx, hipEventCreate(&ev[i]));
-   float t_acc[7] = {0, 0, 0, 0, 0, 0, 0};
+   float t_acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
    int filter_steps = 0, resid_passes = 0, filter_col_steps = 0, filter_first_cols = 0;
    int graph_replays = 0, graph_captures = 0, rr_fallbacks = 0, wide_sweeps = 0;
    auto rec = [&](int i) { return hipEventRecord(ev[i], ctx->stream); };
@@ -1721,12 +1721,13 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
          float a, b, c, dd, e2;
          OFRR_HIP(ctx, hipEventElapsedTime(&a, ev[3], ev[4]));
          if (overlap) {
-            // Z and the H blocks ran on the second stream beside the Rayleigh-Ritz
-            float gh;
+            // Z and the H blocks ran on the second stream beside the Rayleigh-Ritz:
+            // c is G on the solver's stream, the second stream's time goes to t_aux
+            float aux;
             OFRR_HIP(ctx, hipEventElapsedTime(&b, ev[8], ev[9]));
             OFRR_HIP(ctx, hipEventElapsedTime(&c, ev[4], ev[6]));
-            OFRR_HIP(ctx, hipEventElapsedTime(&gh, ev[9], ev[10]));
-            c += gh;
+            OFRR_HIP(ctx, hipEventElapsedTime(&aux, ev[8], ev[10]));
+            t_acc[7] += aux;
          } else {
             OFRR_HIP(ctx, hipEventElapsedTime(&b, ev[4], ev[5]));
             OFRR_HIP(ctx, hipEventElapsedTime(&c, ev[5], ev[6]));
@@ -1773,6 +1774,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       stats->graph_captures = graph_captures;
       stats->rr_fallbacks = rr_fallbacks;
       stats->wide_sweeps = wide_sweeps;
+      stats->t_aux_ms = t_acc[7];
       stats->resid_passes = resid_passes;
       stats->t_total_ms = total;
       stats->t_filter_ms = t_acc[0];
