@@ -493,7 +493,8 @@ __global__ __launch_bounds__(256) void cheb_combine_kernel(
    using ACC = typename std::conditional<std::is_same<TB, double>::value, double, float>::type;
    __shared__ float sm[256];
    const int c = threadIdx.x % k, rstep = 256 / k;
-   const float mc = ca_.maxc[c];
+   float mc = 0.f;
+   for (int sl = 0; sl < kMaxSlots; ++sl) mc = fmaxf(mc, ca_.maxc[sl * ca_.slot_ld + c]);
    const int se = (mc > 0.f) ? ceil_log2_2m_d(mc) - ca_.target_exp : 0;
    const double alpha = ca_.step ? ca_.step[0] : ca_.alpha;
    const double beta = ca_.step ? ca_.step[1] : ca_.beta;
@@ -503,7 +504,8 @@ __global__ __launch_bounds__(256) void cheb_combine_kernel(
    const ACC cshift = (ACC)(ca_.step ? ca_.step[2] : ca_.c);
    if (blockIdx.x == 0 && threadIdx.x < k) {
       ca_.sn_out[c] = ldexp(1.0, se);
-      if (ca_.zero_next) ca_.zero_next[c] = 0.f;
+      if (ca_.zero_next)
+         for (int sl = 0; sl < kMaxSlots; ++sl) ca_.zero_next[sl * ca_.slot_ld + c] = 0.f;
    }
    const int64_t r0 = (int64_t)blockIdx.x * rows_per_block, r1 = min(n, r0 + rows_per_block);
    float m = 0.f;
@@ -520,7 +522,8 @@ __global__ __launch_bounds__(256) void cheb_combine_kernel(
    if (threadIdx.x < k) {
       float pm = 0.f;
       for (int g = 0; g < rstep; ++g) pm = fmaxf(pm, sm[g * k + threadIdx.x]);
-      atomic_max_nonneg(ca_.maxn + threadIdx.x, ldexpf(pm, se));
+      atomic_max_nonneg(ca_.maxn + (blockIdx.x & (kMaxSlots - 1)) * ca_.slot_ld + threadIdx.x,
+                        ldexpf(pm, se));
    }
 }
 

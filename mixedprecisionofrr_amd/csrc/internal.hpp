@@ -23,15 +23,21 @@ int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, c
 // Dn = alpha (A Dc - c Dc) - beta Dp + w . R with stored blocks Dc = Dch sc,
 // Dp = Dph sp, R = Rq sR; the output is stored as Dn / sn with
 // sn = 2^(ceil(log2(2 max|Dc|)) - target_exp) written to sn_out
+// The column maxima of a Chebyshev step are folded into kMaxSlots partial
+// maxima per column (slot = workgroup index & 7, the workgroup's XCD in the
+// round-robin dispatch), slot_ld floats apart; a reader takes the largest.
+// One address per column drew every workgroup's atomic: 10 us of a 235 us step.
+constexpr int kMaxSlots = 8;
 struct ChebArgs {
    double alpha, beta, c;     // step scalars and the centre of the damped interval
    const double *step;        // nullable: the same three scalars in device memory (a
                               // captured sweep replays with new values)
    const double *w;           // per-column weight of the residual block (k values)
    const double *sR, *sc, *sp;// scales of R, Dc, Dp (sp may be null: no Dp)
-   const float *maxc;         // max |D_j| per column
-   float *maxn;               // max |D_{j+1}| per column (zeroed beforehand)
-   float *zero_next;          // cleared for the next step (nullable)
+   const float *maxc;         // max |D_j| per column (kMaxSlots partial maxima)
+   float *maxn;               // max |D_{j+1}| per column (partial maxima, zeroed beforehand)
+   float *zero_next;          // cleared for the next step (nullable; every slot)
+   int slot_ld = 0;           // floats between the partial-maximum slots (0: one shared slot)
    double *sn_out;            // scale of the stored D_{j+1}
    int target_exp;
    int reverse = 0;           // walk the row tiles from the last back to the first

@@ -971,7 +971,7 @@ static void filtered_layout(Carve &cv, int64_t n, const ofrr_filter_opts &o, int
    b.sD1 = (double *)cv.take(kq * 8);
    b.zall = (double *)cv.take((size_t)d * kq * 8);
    b.steps = (double *)cv.take((size_t)3 * d * 8);
-   for (int i = 0; i < 3; ++i) b.cmax[i] = (float *)cv.take(kq * 4);
+   for (int i = 0; i < 3; ++i) b.cmax[i] = (float *)cv.take(kMaxSlots * kq * 4);  // partial maxima
    b.cmaxR = (float *)cv.take(kq * 4);
    b.rank = (int32_t *)cv.take(64);
 }
@@ -1266,7 +1266,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
    std::vector<double> Lth(k), res(k, NAN), theta(k);
    // host staging kept alive across the asynchronous uploads of an iteration
    std::vector<double> inv(k), s1(k), pred(k, 1.0);
-   std::vector<float> m1(k);
+   std::vector<float> m1((size_t)kMaxSlots * k, 0.f);  // slot 0: the maxima, the others 0
    // quantisation targets: the stored maximum of R sits near 2^tgt_resid and
    // the filter blocks near 2^tgt_step (FP8 E4M3 uses the top of its range)
    bool scaled = false;
@@ -1445,7 +1445,8 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
       for (int jj = 0; jj < ka; ++jj) s1[jj] = h_sR[c + jj] / (e * Lb);
       OFRR_HIP(ctx, hipMemcpyAsync(sD1, s1.data(), (size_t)ka * 8, hipMemcpyHostToDevice, ctx->stream));
       for (int jj = 0; jj < ka; ++jj) m1[jj] = (float)((double)h_cmaxR[c + jj] / (e * Lb));
-      OFRR_HIP(ctx, hipMemcpyAsync(cmax[0], m1.data(), (size_t)ka * 4, hipMemcpyHostToDevice, ctx->stream));
+      OFRR_HIP(ctx, hipMemcpyAsync(cmax[0], m1.data(), (size_t)kMaxSlots * k * 4, hipMemcpyHostToDevice,
+                                   ctx->stream));
       OFRR_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
       // the filtered columns of the quantised residual, n x ka row-major: Rq
       // itself, or (locked columns present) a compact copy in the second half
@@ -1498,7 +1499,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
                                                          (const uint64_t *)Rq, (uint64_t *)Ra);
             OFRR_LAUNCH(ctx);
          }
-         int st2 = zero_bytes(ctx, cmax[1], (size_t)k * 4);
+         int st2 = zero_bytes(ctx, cmax[1], (size_t)kMaxSlots * k * 4);
          if (st2) return st2;
          for (int s = 1; s < d; ++s) {
             ChebArgs ca;
@@ -1513,6 +1514,7 @@ static int filtered_core(ofrr_ctx *ctx, FilterOp &op, const ofrr_filter_opts *o,
             ca.maxc = cmax[(s - 1) % 3];
             ca.maxn = cmax[s % 3];
             ca.zero_next = (s + 1 < d) ? cmax[(s + 1) % 3] : nullptr;
+            ca.slot_ld = k;
             ca.sn_out = scale[sb[s].out];
             ca.target_exp = tgt_step;
             // alternate steps walk the row tiles backwards, so a step starts on

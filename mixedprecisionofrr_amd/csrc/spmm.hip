@@ -442,7 +442,9 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
    const double alpha = ca_.step ? ca_.step[0] : ca_.alpha;
    const double beta = ca_.step ? ca_.step[1] : ca_.beta;
    for (int j = threadIdx.x; j < k; j += 256) {
-      const float m = ca_.maxc[j];
+      float m = 0.f;
+#pragma unroll
+      for (int sl = 0; sl < kMaxSlots; ++sl) m = fmaxf(m, ca_.maxc[sl * ca_.slot_ld + j]);
       // stored scale s = 2^(ceil(log2(2 max|T|)) - target): the stored maximum lands in [2^(t-1), 2^t]
       const int se = (m > 0.f) ? ceil_log2_2m(m) - ca_.target_exp : 0;
       s_coef[0][j] = (ACC)ldexp(alpha * ca_.sc[j], -se);
@@ -451,7 +453,8 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
       s_se[j] = se;
       if (blk == 0) {
          ca_.sn_out[j] = ldexp(1.0, se);
-         if (ca_.zero_next) ca_.zero_next[j] = 0.f;
+         if (ca_.zero_next)
+            for (int sl = 0; sl < kMaxSlots; ++sl) ca_.zero_next[sl * ca_.slot_ld + j] = 0.f;
       }
    }
    float mx[VEC];
@@ -484,7 +487,7 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
       const int lane = t / VEC, v = t % VEC;
       float m = 0.f;
       for (int g = 0; g < GROUPS; ++g) m = fmaxf(m, redm[g * G + lane][v]);
-      atomic_max_nonneg(ca_.maxn + t, m);
+      atomic_max_nonneg(ca_.maxn + (blockIdx.x & (kMaxSlots - 1)) * ca_.slot_ld + t, m);
    }
 }
 
