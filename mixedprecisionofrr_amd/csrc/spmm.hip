@@ -119,23 +119,81 @@ template <typename TV> struct CsrTile {
    int32_t ci[kTileCap + 8];
    TV va[kTileCap + 8];
 };
-// stage rows [r0, r0 + nr); false when they hold more than kTileCap entries
-// (the offsets are staged anyway; indices then come from ci + base, va + base)
+// The copy runs in two phases, loads into registers and then LDS stores, over a
+// fixed number of passes (workgroups of kCsrThreads threads), so a workgroup
+// waits for one memory round trip.  A loop of load / wait / store per pass of
+// 256 entries waited for nine of them in a row (2,056 entries at most) before
+// its first row could start.
+constexpr int kCsrThreads = 256;
+template <typename TV> struct TileRegs {
+   static constexpr int kRp = (kTileRows + 1 + kCsrThreads - 1) / kCsrThreads;
+   static constexpr int kNz = (kTileCap + 8 + kCsrThreads - 1) / kCsrThreads;
+   int64_t base, nnzb;
+   bool in_lds;
+   int64_t rp[kRp];
+   int32_t ci[kNz];
+   TV va[kNz];
+};
+// load phase for rows [r0, r0 + nr); in_lds is false when they hold more than
+// kTileCap entries (the offsets are staged anyway; indices then come from
+// ci + base, va + base).  Every address is clamped into the tile's rows.
+template <typename TV>
+__device__ inline void tile_load(TileRegs<TV> &g, const int64_t *__restrict__ rp,
+                                 const int32_t *__restrict__ ci, const TV *__restrict__ va,
+                                 int64_t r0, int nr) {
+   g.base = rp[r0];
+   g.nnzb = rp[r0 + nr] - g.base;
+   g.in_lds = g.nnzb <= kTileCap;
+   if (g.in_lds && g.nnzb > 0) {  // uniform across the workgroup
+      const int last = (int)g.nnzb - 1;
+#pragma unroll
+      for (int u = 0; u < TileRegs<TV>::kNz; ++u) {
+         const int j = min((int)threadIdx.x + u * kCsrThreads, last);
+         g.ci[u] = ci[g.base + j];
+         g.va[u] = va[g.base + j];
+      }
+   } else {
+#pragma unroll
+      for (int u = 0; u < TileRegs<TV>::kNz; ++u) {
+         g.ci[u] = 0;
+         g.va[u] = TV(0);
+      }
+   }
+#pragma unroll
+   for (int u = 0; u < TileRegs<TV>::kRp; ++u)
+      g.rp[u] = rp[r0 + min((int)threadIdx.x + u * kCsrThreads, nr)];
+}
+// store phase: row offsets relative to the first entry, indices and values
+// padded by 8 zero entries
+template <typename TV>
+__device__ inline void tile_store(CsrTile<TV> &t, const TileRegs<TV> &g, int nr) {
+#pragma unroll
+   for (int u = 0; u < TileRegs<TV>::kRp; ++u) {
+      const int i = (int)threadIdx.x + u * kCsrThreads;
+      if (i <= nr) t.rp[i] = (int32_t)(g.rp[u] - g.base);
+   }
+   if (g.in_lds) {
+      const int nz = (int)g.nnzb;
+#pragma unroll
+      for (int u = 0; u < TileRegs<TV>::kNz; ++u) {
+         const int i = (int)threadIdx.x + u * kCsrThreads;
+         if (i < nz + 8) {
+            const bool ok = i < nz;
+            t.ci[i] = ok ? g.ci[u] : 0;
+            t.va[i] = ok ? g.va[u] : TV(0);
+         }
+      }
+   }
+}
 template <typename TV>
 __device__ inline bool stage_tile(CsrTile<TV> &t, const int64_t *__restrict__ rp,
                                   const int32_t *__restrict__ ci, const TV *__restrict__ va,
                                   int64_t r0, int nr, int64_t &base) {
-   base = rp[r0];
-   const int64_t nnzb = rp[r0 + nr] - base;
-   const bool in_lds = nnzb <= kTileCap;
-   for (int i = threadIdx.x; i <= nr; i += blockDim.x) t.rp[i] = (int32_t)(rp[r0 + i] - base);
-   if (in_lds)
-      for (int i = threadIdx.x; i < (int)nnzb + 8; i += blockDim.x) {
-         const bool ok = i < (int)nnzb;
-         t.ci[i] = ok ? ci[base + i] : 0;
-         t.va[i] = ok ? va[base + i] : TV(0);
-      }
-   return in_lds;
+   TileRegs<TV> g;
+   tile_load(g, rp, ci, va, r0, nr);
+   tile_store(t, g, nr);
+   base = g.base;
+   return g.in_lds;
 }
 
 // ---------------------------------------------------------------------------
