@@ -256,6 +256,24 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
 // into registers while the current one is applied.  S = 2 serves n <= 256,
 // S = 3 n <= 384.
 constexpr int kApplyChunk = 16;
+// Whole-wave lane shifts of a double (DPP wave_shl:1 / wave_shr:1 on each
+// 32-bit half): register moves, where a general shuffle is an LDS round trip.
+// The lane with no source (63, resp. 0) receives 0.
+__device__ inline double wave_next(double v) {  // lane i <- lane i + 1
+   const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, true);
+   const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, true);
+   return __hiloint2double(hi, lo);
+}
+__device__ inline double wave_prev(double v) {  // lane i <- lane i - 1
+   const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, true);
+   const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, true);
+   return __hiloint2double(hi, lo);
+}
+__device__ inline double lane_value(double v, int l) {  // lane l's v, in every lane
+   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+   return __hiloint2double(hi, lo);
+}
 template <int S>
 __global__ __launch_bounds__(64) void jacobi_apply_kernel(
     int n_max, const int32_t *__restrict__ n_dev, const double2 *__restrict__ rotlog,
@@ -311,21 +329,31 @@ __global__ __launch_bounds__(64) void jacobi_apply_kernel(
       for (int u = 0; u < kPer; ++u) lg[lane + 64 * u] = pre[u];
       __syncthreads();
       if (c0 + kApplyChunk < rounds) fetch(c0 + kApplyChunk);  // in flight during this chunk
+      // each round's (c, s) is read from LDS one round ahead, so the read is
+      // off the round's dependence chain (rotate, lane shifts, selects)
+      double2 csn[S];
+#pragma unroll
+      for (int u = 0; u < S; ++u) csn[u] = lg[lane + 64 * u];
       for (int k = 0; k < nr; ++k) {
-         const double2 *lr = lg + k * np;
+         double2 csc[S];
+#pragma unroll
+         for (int u = 0; u < S; ++u) csc[u] = csn[u];
+         const double2 *lr = lg + min(k + 1, nr - 1) * np;
+#pragma unroll
+         for (int u = 0; u < S; ++u) csn[u] = lr[lane + 64 * u];
 #pragma unroll
          for (int u = 0; u < S; ++u)
-            if (has[u]) rotate(lr[lane + 64 * u], ia[u], ib[u], av[u], bv[u]);
+            if (has[u]) rotate(csc[u], ia[u], ib[u], av[u], bv[u]);
          // move to the next round's slots
          double an[S], bp[S], af[S], bl[S];
 #pragma unroll
          for (int u = 0; u < S; ++u) {
-            an[u] = __shfl(av[u], lane + 1, 64);  // a[s + 1], same set
-            bp[u] = __shfl(bv[u], lane - 1, 64);  // b[s - 1], same set
-            af[u] = __shfl(av[u], 0, 64);         // a[64 u]: a[s + 1] of lane 63 of set u - 1
-            bl[u] = __shfl(bv[u], 63, 64);        // b[64 u + 63]: b[s - 1] of lane 0 of set u + 1
+            an[u] = wave_next(av[u]);             // a[s + 1], same set (lane 63: unused)
+            bp[u] = wave_prev(bv[u]);             // b[s - 1], same set (lane 0: unused)
+            af[u] = (u > 0) ? lane_value(av[u], 0) : 0.0;       // a[64 u]: a[s + 1] of lane 63 of set u - 1
+            bl[u] = (u + 1 < S) ? lane_value(bv[u], 63) : 0.0;  // b[64 u + 63]: b[s - 1] of lane 0 of set u + 1
          }
-         const double a01 = __shfl(av[0], 1, 64);  // a[1] for slot 0
+         const double a01 = lane_value(av[0], 1);  // a[1] for slot 0
          double na[S], nb[S];
 #pragma unroll
          for (int u = 0; u < S; ++u) {
