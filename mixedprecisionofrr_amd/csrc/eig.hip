@@ -62,18 +62,27 @@ __device__ inline void pair_of(int M, int r, int s, int &p, int &q) {
 // it).  Only the upper triangle is stored and touched.  FULL lays it out in
 // square storage with row stride nn + 1 (conflict-free LDS rows): entry (i, j),
 // i <= j, at off(i) + j with off(i) = i (nn + 1).  Otherwise the packed upper
-// triangle: (i, j), i <= j, at off(j) + i with off(j) = j (j + 1) / 2.
+// triangle: (i, j), i <= j, at off(j) + i with off(j) = j (j + 1) / 2.  Both
+// offsets grow with the index, so the entry of (i, j) in either order is
+// min(oi, oj) + max(i, j) (full) or max(oi, oj) + min(i, j) (packed): three
+// integer operations, and the same on byte-scaled arguments (atb: 8 i, 8 oi).
 template <bool FULL> struct SymIdx;
 template <> struct SymIdx<true> {
    int ld;
    __device__ int off(int i) const { return i * ld; }
-   __device__ int at(int i, int j, int oi, int oj) const { return (i <= j) ? oi + j : oj + i; }
+   __device__ int at(int i, int j, int oi, int oj) const { return min(oi, oj) + max(i, j); }
+   __device__ int atb(int ib, int jb, int oib, int ojb) const { return min(oib, ojb) + max(ib, jb); }
 };
 template <> struct SymIdx<false> {
    int ld;
    __device__ int off(int i) const { return (i * (i + 1)) >> 1; }
-   __device__ int at(int i, int j, int oi, int oj) const { return (i <= j) ? oj + i : oi + j; }
+   __device__ int at(int i, int j, int oi, int oj) const { return max(oi, oj) + min(i, j); }
+   __device__ int atb(int ib, int jb, int oib, int ojb) const { return max(oib, ojb) + min(ib, jb); }
 };
+// the double at byte offset ob of a
+__device__ inline double &at_byte(double *a, int ob) {
+   return *reinterpret_cast<double *>(reinterpret_cast<char *>(a) + ob);
+}
 
 // Rotations: A is diagonalised in place (LDS or global) and the (c, s) of every
 // pair of every round is logged; the eigenvectors are formed afterwards by
@@ -91,7 +100,7 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
     int64_t strideInts, double *__restrict__ work, int64_t strideWork, int max_sweeps) {
    extern __shared__ double lds[];
    __shared__ double2 rcs[kMaxPairs];  // (c, s) of pair slot s in this round
-   __shared__ int4 rpq[kMaxPairs];     // (p, q, off(p), off(q)) of pair slot s, p < q
+   __shared__ int4 rpq[kMaxPairs];     // 8 (p, q, off(p), off(q)) of pair slot s, p < q (bytes)
    __shared__ int rotated, s_n;
    __shared__ double s_amax[16];
    const int b = blockIdx.x;
@@ -145,6 +154,10 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
       }
    }
    int done = 0;
+   // the sweeps, with the virtual-index tests compiled in only for odd sizes
+   // (ODD = true is correct for every size)
+   auto sweeps = [&](auto odd_tag) {
+   constexpr bool ODD = decltype(odd_tag)::value;
    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
       if (threadIdx.x == 0) rotated = 0;
       __syncthreads();
@@ -154,7 +167,7 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
             pair_of(M, r, s, p, q);
             const int op = pk.off(p), oq = pk.off(q);
             double c = 1.0, sn = 0.0;
-            if (q < n) {
+            if (!ODD || q < n) {
                const double apq = a[pk.at(p, q, op, oq)], app = a[pk.at(p, p, op, op)],
                             aqq = a[pk.at(q, q, oq, oq)];
                // the rotation is formed whether or not it is taken, so its
@@ -169,7 +182,7 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
                }
             }
             rcs[s] = make_double2(c, sn);
-            rpq[s] = make_int4(p, q, op, oq);
+            rpq[s] = make_int4(8 * p, 8 * q, 8 * op, 8 * oq);
             log[((int64_t)sweep * M + r) * np + s] = make_double2(c, sn);
          }
          __syncthreads();
@@ -197,10 +210,10 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
                if (act[e]) {
                   // rows p1 < q1 (pair I), columns p2 < q2 (pair J); a virtual
                   // index holds zeros, so every read is of stored memory
-                  v11[e] = a[pk.at(qI[e].x, qJ[e].x, qI[e].z, qJ[e].z)];
-                  v12[e] = a[pk.at(qI[e].x, qJ[e].y, qI[e].z, qJ[e].w)];
-                  v21[e] = a[pk.at(qI[e].y, qJ[e].x, qI[e].w, qJ[e].z)];
-                  v22[e] = a[pk.at(qI[e].y, qJ[e].y, qI[e].w, qJ[e].w)];
+                  v11[e] = at_byte(a, pk.atb(qI[e].x, qJ[e].x, qI[e].z, qJ[e].z));
+                  v12[e] = at_byte(a, pk.atb(qI[e].x, qJ[e].y, qI[e].z, qJ[e].w));
+                  v21[e] = at_byte(a, pk.atb(qI[e].y, qJ[e].x, qI[e].w, qJ[e].z));
+                  v22[e] = at_byte(a, pk.atb(qI[e].y, qJ[e].y, qI[e].w, qJ[e].w));
                }
             }
 #pragma unroll
@@ -216,11 +229,12 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
                double b21 = fma(s1, m11, c1 * m21);
                const double b22 = fma(s1, m12, c1 * m22);
                if (p1 == p2) b12 = b21 = 0.0;  // diagonal block
-               const bool q1ok = q1 < n, q2ok = q2 < n;
-               a[pk.at(p1, p2, op1, op2)] = b11;
-               if (q2ok) a[pk.at(p1, q2, op1, oq2)] = b12;
-               if (q1ok) a[pk.at(q1, p2, oq1, op2)] = b21;
-               if (q1ok && q2ok) a[pk.at(q1, q2, oq1, oq2)] = b22;
+               // (byte-scaled indices) only an odd size has a virtual index
+               const bool q1ok = !ODD || q1 < 8 * n, q2ok = !ODD || q2 < 8 * n;
+               at_byte(a, pk.atb(p1, p2, op1, op2)) = b11;
+               if (q2ok) at_byte(a, pk.atb(p1, q2, op1, oq2)) = b12;
+               if (q1ok) at_byte(a, pk.atb(q1, p2, oq1, op2)) = b21;
+               if (q1ok && q2ok) at_byte(a, pk.atb(q1, q2, oq1, oq2)) = b22;
             }
          }
          __syncthreads();
@@ -228,6 +242,15 @@ __global__ __launch_bounds__(kJacobiThreads) void jacobi_rot_kernel(
       if (!rotated) break;
       done = sweep + 1;
       __syncthreads();
+   }
+   };
+   if constexpr (LDS) {
+      if (n & 1)
+         sweeps(std::true_type{});
+      else
+         sweeps(std::false_type{});
+   } else {  // the global-memory forms keep one copy (registers are at their limit)
+      sweeps(std::true_type{});
    }
    if (threadIdx.x == 0) nsweeps[b * strideInts] = done;
    double *wb = w + b * stridew;
