@@ -703,6 +703,83 @@ __global__ __launch_bounds__(64) void basis_update_mfma_kernel(int64_t n, int k,
          }
 }
 
+// The same product with the repacked coefficients staged once per workgroup in
+// LDS.  Read from L2, every wave fetched all of them for its 32 rows (about
+// 98 KB against 10 KB of basis rows); here a persistent workgroup of kUpdWaves
+// waves (one per CU, 4 waves per SIMD) copies them once and walks row tiles of
+// 16 rows per wave (launched with RT = 1: with the coefficients in LDS, two
+// tiles per wave only cost registers).
+// Column col of group G sits at (G k + col) kUpdLdsStride + 4 g + s: the 2
+// doubles of padding put the 16 lanes of one g on distinct banks.  The MFMA
+// sequence of every output is the one of basis_update_mfma_kernel.
+constexpr int kUpdWaves = 16;
+constexpr int kUpdLdsStride = 18;
+constexpr size_t kUpdLdsMax = 160 * 1024;
+template <typename TY, int NCT, int RT>
+__global__ __launch_bounds__(64 * kUpdWaves) void basis_update_lds_kernel(
+    int64_t n, int k, int groups, UpdMfmaArgs a, const double *__restrict__ Cp, TY *__restrict__ Y,
+    int64_t yrs) {
+   extern __shared__ __align__(16) double sC[];
+   const int total = groups * k * 16;  // Cp: [((G k + col) 4 + g) 4 + s], 16 doubles per column
+   for (int t = 2 * (int)threadIdx.x; t < total; t += 2 * (int)blockDim.x) {
+      const int col = t >> 4, w = t & 15;
+      *reinterpret_cast<double2 *>(sC + col * kUpdLdsStride + w) = *reinterpret_cast<const double2 *>(Cp + t);
+   }
+   __syncthreads();
+   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, g = lane >> 4;
+   const int64_t tiles = (n + 16 * RT - 1) / (16 * RT);
+   for (int64_t tile = (int64_t)blockIdx.x * kUpdWaves + wave; tile < tiles;
+        tile += (int64_t)gridDim.x * kUpdWaves) {
+      const int64_t row0 = tile * 16 * RT;
+      dbl4 acc[RT][NCT];
+#pragma unroll
+      for (int r = 0; r < RT; ++r)
+#pragma unroll
+         for (int c = 0; c < NCT; ++c) acc[r][c] = dbl4{0.0, 0.0, 0.0, 0.0};
+      int G = 0;
+      for (int b = 0; b < a.nb; ++b) {
+         const int ng = a.kb[b] / 16, dt = a.dtype[b];
+         for (int gb = 0; gb < ng; ++gb, ++G) {
+            double av[RT][4];
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+               const int64_t row = row0 + 16 * r + li;
+               const bool rok = row < n;
+               const int64_t off = row * a.rs[b] + 16 * gb + 4 * g;
+               switch (dt) {  // wave-uniform
+                  case OFRR_F64: upd_load4<double>(a.V[b], off, rok, av[r]); break;
+                  case OFRR_F32: upd_load4<float>(a.V[b], off, rok, av[r]); break;
+                  case OFRR_F16: upd_load4<__half>(a.V[b], off, rok, av[r]); break;
+                  case OFRR_BF16: upd_load4<__hip_bfloat16>(a.V[b], off, rok, av[r]); break;
+                  default: upd_load4<fp8_t>(a.V[b], off, rok, av[r]); break;
+               }
+            }
+#pragma unroll
+            for (int c = 0; c < NCT; ++c) {
+               const double *cb = sC + (G * k + 16 * c + li) * kUpdLdsStride + 4 * g;
+               const double b0 = cb[0], b1 = cb[1], b2 = cb[2], b3 = cb[3];
+#pragma unroll
+               for (int r = 0; r < RT; ++r) {
+                  acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[r][0], b0, acc[r][c], 0, 0, 0);
+                  acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[r][1], b1, acc[r][c], 0, 0, 0);
+                  acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[r][2], b2, acc[r][c], 0, 0, 0);
+                  acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[r][3], b3, acc[r][c], 0, 0, 0);
+               }
+            }
+         }
+      }
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+         for (int c = 0; c < NCT; ++c)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+               const int64_t r = row0 + 16 * rt + 4 * t + g;
+               if (r < n) Y[r * yrs + 16 * c + li] = Elem<TY>::store(acc[rt][c][t]);
+            }
+   }
+}
+
 int basis_update(ofrr_ctx *ctx, int64_t n, int k, int nb, const void *const *V, const int *vdt,
                  const int *kb, const int64_t *vrs, const int64_t *vcs, const double *const *C,
                  const int *ldc, void *Y, int ydt, int64_t yrs, int64_t ycs) {
@@ -737,10 +814,38 @@ int basis_update(ofrr_ctx *ctx, int64_t n, int k, int nb, const void *const *V, 
       // of back-transforms per 128^3 solve against one tile (four tiles drop
       // to one wave per SIMD and are slower)
       constexpr int kUpdRT = 2;
+      const size_t lds = (size_t)groups * k * kUpdLdsStride * sizeof(double);
+      int lds_st = kSuccess;
       auto launch = [&](auto nct_tag) {
          constexpr int NCT = decltype(nct_tag)::value;
          constexpr int RT = kUpdRT;
-         const unsigned gr = (unsigned)((n + 16 * RT - 1) / (16 * RT));
+         const int64_t tiles = (n + 16 * RT - 1) / (16 * RT);
+         // (wider outputs need more accumulators than 3 waves per SIMD leave)
+         if constexpr (NCT <= 4) if (lds <= kUpdLdsMax) {
+            // coefficients in LDS, persistent workgroups (at most one per CU),
+            // one 16-row tile per wave
+            const int64_t tiles1 = (n + 15) / 16;
+            const unsigned grl = (unsigned)std::max<int64_t>(
+                1, std::min<int64_t>(ctx->num_cus, (tiles1 + kUpdWaves - 1) / kUpdWaves));
+            auto go = [&](auto ytag) {
+               using TYk = decltype(ytag);
+               constexpr auto kern = basis_update_lds_kernel<TYk, NCT, 1>;
+               static std::once_flag once;
+               static hipError_t attr = hipSuccess;
+               std::call_once(once, [] {
+                  attr = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)kUpdLdsMax);
+               });
+               if (attr != hipSuccess) {
+                  lds_st = set_error(ctx, kErrorCUDAGeneral, "basis_update: LDS size attribute");
+                  return;
+               }
+               kern<<<grl, 64 * kUpdWaves, lds, ctx->stream>>>(n, k, groups, m, Cp, (TYk *)Y, yrs);
+            };
+            if (ydt == OFRR_F64) go(double{}); else go(float{});
+            return;
+         }
+         const unsigned gr = (unsigned)tiles;
          if (ydt == OFRR_F64)
             basis_update_mfma_kernel<double, NCT, RT><<<gr, 64, 0, ctx->stream>>>(n, k, m, Cp, (double *)Y, yrs);
          else
@@ -756,6 +861,7 @@ int basis_update(ofrr_ctx *ctx, int64_t n, int k, int nb, const void *const *V, 
          case 7: launch(std::integral_constant<int, 7>{}); break;
          default: launch(std::integral_constant<int, 8>{}); break;
       }
+      if (lds_st) return lds_st;
       OFRR_LAUNCH(ctx);
       return kSuccess;
    }
