@@ -649,7 +649,13 @@ int spmm_resid(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, c
       dim3 grid((unsigned)sched_grid(nblk, pt));
       int lg = 0;
       while ((1 << lg) < G) ++lg;
-      spmm_resid_kernel<TB><<<grid, 256, 0, ctx->stream>>>(n, rp, ci, va, k, X, theta, W, (TB *)Rq,
+      // 20 KiB of dynamic LDS the kernel does not touch hold it to three
+      // workgroups per CU (56 KiB each) instead of four: the 512-byte FP64 rows
+      // a CU has in flight then leave the XCD's L2 room for their +-N^2
+      // neighbours (6.85 -> 6.30 ms of residual passes per 128^3 mixed solve;
+      // two workgroups per CU: 8.2 ms)
+      constexpr int kResidLdsPad = 20 * 1024;
+      spmm_resid_kernel<TB><<<grid, 256, kResidLdsPad, ctx->stream>>>(n, rp, ci, va, k, X, theta, W, (TB *)Rq,
                                                            inv_sR, colmax, part, lg, nblk, pt);
       OFRR_LAUNCH(ctx);
       return kSuccess;
