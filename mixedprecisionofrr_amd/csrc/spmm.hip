@@ -114,10 +114,11 @@ __device__ inline void row_gather8(CI cs, VA vs, int64_t b, int64_t e, const TX 
 // gathers may read past a row's end (row_gather8<..., true>).
 constexpr int kTileRows = 256;
 constexpr int kTileCap = 8 * kTileRows;
-template <typename TV> struct CsrTile {
-   int32_t rp[kTileRows + 1];
-   int32_t ci[kTileCap + 8];
-   TV va[kTileCap + 8];
+template <typename TV, int ROWS = kTileRows> struct CsrTile {
+   static constexpr int kRows = ROWS, kCap = 8 * ROWS;
+   int32_t rp[ROWS + 1];
+   int32_t ci[kCap + 8];
+   TV va[kCap + 8];
 };
 // The copy runs in two phases, loads into registers and then LDS stores, over a
 // fixed number of passes (workgroups of kCsrThreads threads), so a workgroup
@@ -125,9 +126,10 @@ template <typename TV> struct CsrTile {
 // 256 entries waited for nine of them in a row (2,056 entries at most) before
 // its first row could start.
 constexpr int kCsrThreads = 256;
-template <typename TV> struct TileRegs {
-   static constexpr int kRp = (kTileRows + 1 + kCsrThreads - 1) / kCsrThreads;
-   static constexpr int kNz = (kTileCap + 8 + kCsrThreads - 1) / kCsrThreads;
+template <typename TV, int ROWS = kTileRows> struct TileRegs {
+   static constexpr int kCap = 8 * ROWS;
+   static constexpr int kRp = (ROWS + 1 + kCsrThreads - 1) / kCsrThreads;
+   static constexpr int kNz = (kCap + 8 + kCsrThreads - 1) / kCsrThreads;
    int64_t base, nnzb;
    bool in_lds;
    int64_t rp[kRp];
@@ -135,47 +137,47 @@ template <typename TV> struct TileRegs {
    TV va[kNz];
 };
 // load phase for rows [r0, r0 + nr); in_lds is false when they hold more than
-// kTileCap entries (the offsets are staged anyway; indices then come from
+// 8 entries per row (the offsets are staged anyway; indices then come from
 // ci + base, va + base).  Every address is clamped into the tile's rows.
-template <typename TV>
-__device__ inline void tile_load(TileRegs<TV> &g, const int64_t *__restrict__ rp,
+template <typename TV, int ROWS>
+__device__ inline void tile_load(TileRegs<TV, ROWS> &g, const int64_t *__restrict__ rp,
                                  const int32_t *__restrict__ ci, const TV *__restrict__ va,
                                  int64_t r0, int nr) {
    g.base = rp[r0];
    g.nnzb = rp[r0 + nr] - g.base;
-   g.in_lds = g.nnzb <= kTileCap;
+   g.in_lds = g.nnzb <= TileRegs<TV, ROWS>::kCap;
    if (g.in_lds && g.nnzb > 0) {  // uniform across the workgroup
       const int last = (int)g.nnzb - 1;
 #pragma unroll
-      for (int u = 0; u < TileRegs<TV>::kNz; ++u) {
+      for (int u = 0; u < TileRegs<TV, ROWS>::kNz; ++u) {
          const int j = min((int)threadIdx.x + u * kCsrThreads, last);
          g.ci[u] = ci[g.base + j];
          g.va[u] = va[g.base + j];
       }
    } else {
 #pragma unroll
-      for (int u = 0; u < TileRegs<TV>::kNz; ++u) {
+      for (int u = 0; u < TileRegs<TV, ROWS>::kNz; ++u) {
          g.ci[u] = 0;
          g.va[u] = TV(0);
       }
    }
 #pragma unroll
-   for (int u = 0; u < TileRegs<TV>::kRp; ++u)
+   for (int u = 0; u < TileRegs<TV, ROWS>::kRp; ++u)
       g.rp[u] = rp[r0 + min((int)threadIdx.x + u * kCsrThreads, nr)];
 }
 // store phase: row offsets relative to the first entry, indices and values
 // padded by 8 zero entries
-template <typename TV>
-__device__ inline void tile_store(CsrTile<TV> &t, const TileRegs<TV> &g, int nr) {
+template <typename TV, int ROWS>
+__device__ inline void tile_store(CsrTile<TV, ROWS> &t, const TileRegs<TV, ROWS> &g, int nr) {
 #pragma unroll
-   for (int u = 0; u < TileRegs<TV>::kRp; ++u) {
+   for (int u = 0; u < TileRegs<TV, ROWS>::kRp; ++u) {
       const int i = (int)threadIdx.x + u * kCsrThreads;
       if (i <= nr) t.rp[i] = (int32_t)(g.rp[u] - g.base);
    }
    if (g.in_lds) {
       const int nz = (int)g.nnzb;
 #pragma unroll
-      for (int u = 0; u < TileRegs<TV>::kNz; ++u) {
+      for (int u = 0; u < TileRegs<TV, ROWS>::kNz; ++u) {
          const int i = (int)threadIdx.x + u * kCsrThreads;
          if (i < nz + 8) {
             const bool ok = i < nz;
@@ -185,11 +187,11 @@ __device__ inline void tile_store(CsrTile<TV> &t, const TileRegs<TV> &g, int nr)
       }
    }
 }
-template <typename TV>
-__device__ inline bool stage_tile(CsrTile<TV> &t, const int64_t *__restrict__ rp,
+template <typename TV, int ROWS>
+__device__ inline bool stage_tile(CsrTile<TV, ROWS> &t, const int64_t *__restrict__ rp,
                                   const int32_t *__restrict__ ci, const TV *__restrict__ va,
                                   int64_t r0, int nr, int64_t &base) {
-   TileRegs<TV> g;
+   TileRegs<TV, ROWS> g;
    tile_load(g, rp, ci, va, r0, nr);
    tile_store(t, g, nr);
    base = g.base;
@@ -479,7 +481,7 @@ __global__ __launch_bounds__(256) void cheb_step_kernel(
     const TV *__restrict__ va, int k, const TB *__restrict__ Dc, const TB *__restrict__ Dp,
     const TB *__restrict__ R, ChebArgs ca_, TB *__restrict__ Dn, int lg, int64_t nblk, int64_t plane) {
    using ACC = typename std::conditional<std::is_same<TB, double>::value, double, float>::type;
-   __shared__ CsrTile<TV> tile;
+   __shared__ CsrTile<TV, kChebRows> tile;
    __shared__ float redm[256][VEC];
    __shared__ __align__(16) ACC s_coef[3][kChebMaxK];
    __shared__ int s_se[kChebMaxK];
@@ -693,7 +695,7 @@ int cheb_step(ofrr_ctx *ctx, int64_t n, const int64_t *rp, const int32_t *ci, co
          // workgroup to about 36 KiB, four per CU instead of the five its
          // registers allow (FP16 step 195.9 -> 193.7 us; three per CU: 199.1 us)
          using ACC = typename std::conditional<std::is_same<TB, double>::value, double, float>::type;
-         constexpr size_t kStatic = sizeof(CsrTile<TV>) + 256 * VEC * sizeof(float) +
+         constexpr size_t kStatic = sizeof(CsrTile<TV, kChebRows>) + 256 * VEC * sizeof(float) +
                                     3 * kChebMaxK * sizeof(ACC) + kChebMaxK * sizeof(int);
          constexpr size_t kPad = (sizeof(TB) == 2 && kStatic < 36 * 1024) ? 36 * 1024 - kStatic : 0;
          auto go = [&](auto has_dp, auto o32) {
