@@ -225,10 +225,14 @@ def test_basis_update_mixed_blocks(ctx):
     (4099, 128, (128, 128), (torch.float64, torch.float16), torch.float64),
     (777, 48, (48, 48, 48), (torch.float64, torch.float8_e4m3fn, torch.float32), torch.float32),
     (513, 64, (64, 40), (torch.float64, torch.float16), torch.float64),  # width 40: LDS kernel
+    # more 16-row tiles than one pass of the persistent workgroups covers (256 CUs x 16 waves)
+    (150001, 64, (64, 64, 64), (torch.float64, torch.float16, torch.float16), torch.float64),
 ])
 def test_basis_update_shapes(ctx, n, k, kbs, dts, out):
-    """Back-transform Y = sum_b V_b C_b (FP64 MFMA path for row-major blocks of
-    widths that are multiples of 16, the LDS kernel otherwise) against FP64."""
+    """Back-transform Y = sum_b V_b C_b against FP64.  Row-major blocks of widths
+    that are multiples of 16 take the FP64 MFMA path (coefficients staged in LDS by
+    persistent workgroups for k <= 64, read from L2 above); other layouts take the
+    FMA kernel with LDS tiles."""
     from mixedprecisionofrr_amd import api
     rng = np.random.default_rng(n + k)
     blocks, ref = [], 0.0
